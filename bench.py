@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Throughput of tree-approach incremental checkpointing on one MI355X.
+
+Workload (default): a 10-checkpoint chain of a 1 GiB uint8 buffer, 128-byte
+chunks, Murmur3-x64-128; checkpoint 0 is uniform random bytes and each later
+checkpoint changes 1 % of the chunks as runs of geometric length (mean 16
+chunks), half fresh bytes and half copies of earlier runs.  The whole chain
+is generated in HBM before timing.
+
+A step is one checkpoint call (dd_checkpoint_device: input resident in HBM,
+diff assembled in HBM); steps walk the chain in order, so every 10th step is
+a baseline.  `value` = bytes deduplicated / wall time over the timed steps.
+
+`roofline` covers the dominant kernel (leaf hashing), timed with HIP events
+on the library's stream during the timed steps; its algorithmic bytes per
+launch are C * (chunk + 33): the chunk, the previous 16-byte leaf digest, the
+new digest and the label.  `cpu_baseline` times the serial CPU oracle (one
+core) on the same recipe at 256 MiB.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=1 << 30)
+    ap.add_argument("--chunk", type=int, default=128)
+    ap.add_argument("--frac", type=float, default=0.01)
+    ap.add_argument("--nckpt", type=int, default=10)
+    ap.add_argument("--hash", choices=["murmur3", "md5"], default="murmur3")
+    ap.add_argument("--sweep", action="store_true", help="also run chunk {128,512,4096} x change {0.1,1,10}%%")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-size", type=int, default=256 << 20)
+    return ap.parse_args()
+
+
+def traffic_from_profiles(kernel, workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def run_config(torch, dd, W, size, chunk, frac, nckpt, steps, warmup, kind, seed=1931):
+    chain = W.chain_runs_gpu(torch, seed, size, chunk, nckpt, frac)
+    torch.cuda.synchronize()
+    g = dd.TreeDeduplicator(chunk, kind)
+    g.set_profiling(True)
+    cap = dd.max_diff_size(size, chunk, warmup + steps + 1)
+    out = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    diff_bytes = []
+
+    def step(i):
+        k = i % nckpt
+        n = g.checkpoint_device(chain[k].data_ptr(), size, out.data_ptr(), cap, k == 0)
+        return k, n
+
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    stages = []
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        k, n = step(i)
+        diff_bytes.append(n)
+        stages.append((k, g.stage_times()))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    el = t1 - t0
+    leaf_ms = sum(s["leaf_hash"] for _, s in stages) / len(stages)
+    C = (size + chunk - 1) // chunk
+    leaf_bytes = C * (chunk + 33)
+    avg = {key: sum(s[key] for _, s in stages) / len(stages) for key in stages[0][1]}
+    base = [sum(s.values()) for k, s in stages if k == 0]
+    incr = [sum(s.values()) for k, s in stages if k != 0]
+    res = dict(
+        gbs=steps * size / el / 1e9,
+        ms_per_step=el * 1e3 / steps,
+        leaf_ms=leaf_ms,
+        leaf_gbs=leaf_bytes / (leaf_ms * 1e-3) / 1e9,
+        leaf_bytes=leaf_bytes,
+        stage_ms=avg,
+        gpu_ms_baseline=(sum(base) / len(base)) if base else None,
+        gpu_ms_incremental=(sum(incr) / len(incr)) if incr else None,
+        diff_bytes_avg=sum(diff_bytes) / len(diff_bytes),
+    )
+    del chain, out, g
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(W, size, chunk, frac, nckpt):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    chain = W.chain_runs(1931, size, chunk, nckpt, frac)
+    o = O.OracleDeduplicator(chunk)
+    t0 = time.perf_counter()
+    for i, x in enumerate(chain):
+        o.checkpoint_array(x, i == 0)
+    el = time.perf_counter() - t0
+    return dict(value=round(nckpt * size / el / 1e9, 4), unit="GB/s", cores=1, kind="port",
+                sample=f"serial CPU oracle, {nckpt}-checkpoint chain of {size >> 20} MiB, chunk {chunk} B, "
+                       f"{frac * 100:g}% changed per checkpoint ({el:.1f} s)")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (buffers of different ranks are "
+              "independent, so N GPUs run N copies of this workload).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import reproducibility_icpp23_scalable_gpu_deduplication_amd as dd
+    from reproducibility_icpp23_scalable_gpu_deduplication_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    kind = dd.HASH_MD5 if args.hash == "md5" else dd.HASH_MURMUR3
+    r = run_config(torch, dd, W, args.size, args.chunk, args.frac, args.nckpt, args.steps, args.warmup, kind)
+    workload = f"plain_{args.size >> 20}MiB_c{args.chunk}_{args.frac * 100:g}pct_{args.hash}"
+    traffic = traffic_from_profiles("leaf_hash", workload)
+    line = {
+        "metric": "input GB/s deduped (plain tree approach), bit-exact",
+        "value": round(r["gbs"], 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": workload,
+            "buffer_bytes": args.size,
+            "chunk_size": args.chunk,
+            "hash": args.hash,
+            "checkpoints_in_chain": args.nckpt,
+            "change_fraction": args.frac,
+            "change_pattern": "geometric runs (mean 16 chunks), half fresh, half copies",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(r["leaf_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(r["leaf_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "k_leaf_hash",
+            "algorithmic_bytes_per_launch": r["leaf_bytes"],
+            "avg_launch_ms": round(r["leaf_ms"], 4),
+        },
+        "stage_ms": {k: round(v, 4) for k, v in r["stage_ms"].items()},
+        "gpu_ms_baseline_ckpt": r["gpu_ms_baseline"] and round(r["gpu_ms_baseline"], 4),
+        "gpu_ms_incremental_ckpt": r["gpu_ms_incremental"] and round(r["gpu_ms_incremental"], 4),
+        "diff_bytes_avg": int(r["diff_bytes_avg"]),
+    }
+    if args.sweep:
+        sweep = []
+        for chunk in (128, 512, 4096):
+            for frac in (0.001, 0.01, 0.1):
+                s = run_config(torch, dd, W, args.size, chunk, frac, args.nckpt, args.nckpt, args.nckpt, kind)
+                sweep.append(dict(chunk=chunk, frac=frac, gbs=round(s["gbs"], 2), leaf_gbs=round(s["leaf_gbs"], 1),
+                                  ms_baseline=s["gpu_ms_baseline"], ms_incremental=s["gpu_ms_incremental"]))
+                print(json.dumps({"sweep_point": sweep[-1]}), file=sys.stderr, flush=True)
+        line["sweep"] = sweep
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(W, args.cpu_size, args.chunk, args.frac, args.nckpt)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

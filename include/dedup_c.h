@@ -1,0 +1,137 @@
+/*
+ * dedup_c.h -- C ABI of the MI355X tree-approach incremental-checkpoint
+ * deduplicator (libdedup_hip.so).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses on
+ * the handle's device; host buffers are ordinary process memory.  Every entry
+ * point returns a dd_status; none prints.
+ *
+ * Each function names the reference interface it replaces
+ * (TauferLab ICPP'23 GPU deduplication; paths relative to that repository).
+ *
+ * Threading: calls on one handle must be serialized.  Different handles may be
+ * used concurrently from different host threads; each handle owns one HIP
+ * stream and its own device-resident tree and digest map.
+ */
+#ifndef DEDUP_C_H
+#define DEDUP_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dd_status {
+  DD_OK = 0,
+  DD_EINVAL = 1,    /* bad argument (length mismatch, zero length, bad handle) */
+  DD_ENOMEM = 2,    /* host or device allocation failed */
+  DD_ECORRUPT = 3,  /* a diff in the chain is inconsistent or the chain is short */
+  DD_EHIP = 4,      /* a HIP runtime call failed */
+  DD_ESTATE = 5     /* call order violates the checkpoint chain (see dd_checkpoint) */
+} dd_status;
+
+typedef enum dd_hash_kind {
+  DD_HASH_MURMUR3 = 0, /* include/kokkos_murmur3.hpp:213-292 incl. the truncating rotl64 (:30-31) */
+  DD_HASH_MD5 = 1      /* include/kokkos_md5.hpp:287-323 */
+} dd_hash_kind;
+
+/* header_t, include/utils.hpp:29-37 (32 bytes, little-endian). */
+typedef struct dd_header {
+  uint32_t ref_id;
+  uint32_t chkpt_id;
+  uint64_t datalen;
+  uint32_t chunk_size;
+  uint32_t num_first_ocur;
+  uint32_t num_prior_chkpts;
+  uint32_t num_shift_dupl;
+} dd_header;
+
+/* A host byte buffer; buffers returned by the library are released with
+ * dd_buf_free (Kokkos::View<uint8_t*>::HostMirror in the reference). */
+typedef struct dd_buf {
+  uint8_t* data;
+  uint64_t len;
+} dd_buf;
+
+typedef struct dd_dedup dd_dedup;
+
+/* TreeDeduplicator(uint32_t bytes_per_chunk), include/tree_approach.hpp:16-205,
+ * with the hash chosen at run time instead of at compile time
+ * (include/hash_functions.hpp:16-20).  `hip_stream` may be NULL: the handle
+ * then creates its own stream on the current device. */
+dd_status dd_create(uint32_t chunk_size, int hash_kind, void* hip_stream, dd_dedup** out);
+dd_status dd_destroy(dd_dedup* h);
+
+/* TreeDeduplicator::checkpoint(uint8_t* data_ptr, size_t len, HostMirror& diff_h,
+ * bool make_baseline), src/tree_approach.cpp:1740-1749: deduplicates the device
+ * buffer against the handle's previous checkpoint, returns the diff on the
+ * host in *out (header + metadata + data), and advances the checkpoint id.
+ * An incremental checkpoint (make_baseline == 0 after the first call) must
+ * keep the buffer length of the chain; otherwise DD_ESTATE. */
+dd_status dd_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline,
+                        dd_buf* out);
+
+/* TreeDeduplicator::checkpoint(header_t&, uint8_t*, size_t, HostMirror&, bool),
+ * src/tree_approach.cpp:1577-1689: as dd_checkpoint but also returns the
+ * header and does NOT advance the checkpoint id (callers advance it with
+ * dd_set_current_id, as the reference's wrapper overloads do). */
+dd_status dd_checkpoint_header(dd_dedup* h, dd_header* header, const void* dev_data, uint64_t len,
+                               int make_baseline, dd_buf* out);
+
+/* Device-resident variant of dd_checkpoint (the reference's collect_diff
+ * output buffer_d before its D2H copy, src/tree_approach.cpp:1668-1684): the
+ * diff is written to `dev_out` (capacity `out_cap`, at least
+ * dd_max_diff_size(len, chunk_size, current id)) and its length to *out_len.
+ * Advances the checkpoint id. */
+dd_status dd_checkpoint_device(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline,
+                               void* dev_out, uint64_t out_cap, uint64_t* out_len);
+
+/* Upper bound of a diff's size: header, metadata for every chunk, the prior
+ * checkpoint table (ckpt_id+1 entries) and every chunk's data. */
+uint64_t dd_max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
+
+/* num_first_ocur() / num_shift_dupl() of the last checkpoint,
+ * include/tree_approach.hpp:53-59. */
+dd_status dd_counts(dd_dedup* h, uint64_t* num_first_ocur, uint64_t* num_shift_dupl);
+
+/* BaseDeduplicator::current_id / baseline_id, include/deduplicator_interface.hpp:16-18. */
+dd_status dd_get_ids(dd_dedup* h, uint32_t* current_id, uint32_t* baseline_id);
+dd_status dd_set_current_id(dd_dedup* h, uint32_t current_id);
+
+/* BaseDeduplicator::timers[4] (setup, dedup, gather, copy-to-host, seconds) and
+ * restart_timers[2] (copy, total), include/deduplicator_interface.hpp:22-23. */
+dd_status dd_get_timers(dd_dedup* h, double timers[4], double restart_timers[2]);
+
+/* TreeDeduplicator::restart(uint8_t* data_ptr, size_t len,
+ * std::vector<HostMirror>& chkpts, std::string& logname, uint32_t chkpt_id),
+ * src/tree_approach.cpp:1809-1816 -> restart_chkpt :712-1128: rebuilds
+ * checkpoint `chkpt_id` from host diffs indexed by checkpoint id into the
+ * device buffer `dev_out`, whose length must equal the header's datalen
+ * (DD_EINVAL otherwise). */
+dd_status dd_restart(dd_dedup* h, const dd_buf* chain, uint32_t nchain, uint32_t chkpt_id,
+                     void* dev_out, uint64_t len);
+
+/* Same as dd_restart with the diffs already in device memory
+ * (dev_chain[i] holds chain_lens[i] bytes; both arrays are host arrays). */
+dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uint64_t* chain_lens,
+                            uint32_t nchain, uint32_t chkpt_id, void* dev_out, uint64_t len);
+
+/* Diagnostics (no counterpart in the reference; its equivalent is the
+ * Kokkos::Profiling regions of src/tree_approach.cpp): when enabled, the
+ * next checkpoints record HIP events on the handle's stream between stages,
+ * and dd_get_stage_times returns the last checkpoint's GPU milliseconds for
+ * {leaf hash, leaf inserts, first-occurrence forest, interior inserts,
+ * duplicate forest, roots + grouping, metadata + gather}. */
+#define DD_NUM_STAGES 7
+dd_status dd_set_profiling(dd_dedup* h, int enable);
+dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]);
+
+void dd_buf_free(dd_buf* b);
+const char* dd_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

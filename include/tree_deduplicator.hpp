@@ -1,0 +1,72 @@
+// tree_deduplicator.hpp -- C++ host interface mirroring the reference's
+// TreeDeduplicator (include/tree_approach.hpp:16-205, include/
+// deduplicator_interface.hpp:14-184), implemented over the C ABI in
+// dedup_c.h.  Kokkos views are replaced by raw device pointers plus lengths
+// (device data) and std::vector<uint8_t> (host diffs).  Errors raise
+// std::runtime_error where the reference would print or abort.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "dedup_c.h"
+
+namespace dd {
+
+using header_t = dd_header;
+using HostBuf = std::vector<uint8_t>;
+
+enum class HashKind : int { Murmur3 = DD_HASH_MURMUR3, MD5 = DD_HASH_MD5 };
+
+class TreeDeduplicator {
+ public:
+  // TreeDeduplicator(uint32_t bytes_per_chunk), src/tree_approach.cpp.
+  explicit TreeDeduplicator(uint32_t chunk_size, HashKind kind = HashKind::Murmur3, void* hip_stream = nullptr);
+  ~TreeDeduplicator();
+  TreeDeduplicator(const TreeDeduplicator&) = delete;
+  TreeDeduplicator& operator=(const TreeDeduplicator&) = delete;
+
+  // src/tree_approach.cpp:1577 -- does not advance current_id.
+  void checkpoint(header_t& header, uint8_t* data_ptr, size_t data_len, HostBuf& diff_h, bool make_baseline);
+  // src/tree_approach.cpp:1705 -- writes `filename` and the logs, advances current_id.
+  void checkpoint(uint8_t* data_ptr, size_t len, std::string& filename, std::string& logname, bool make_baseline);
+  // src/tree_approach.cpp:1740 -- advances current_id.
+  void checkpoint(uint8_t* data_ptr, size_t len, HostBuf& diff_h, bool make_baseline);
+  // src/tree_approach.cpp:1765 -- writes the logs, advances current_id.
+  void checkpoint(uint8_t* data_ptr, size_t len, HostBuf& diff_h, std::string& logname, bool make_baseline);
+
+  // src/tree_approach.cpp:1809 -- rebuilds checkpoint `chkpt_id` into the
+  // device buffer data_ptr (len must equal the header's datalen).
+  void restart(uint8_t* data_ptr, size_t len, std::vector<HostBuf>& chkpts, std::string& logname, uint32_t chkpt_id);
+  // src/tree_approach.cpp:1827 -- reads "<name>.hashtree.incr_chkpt" files.
+  void restart(uint8_t* data_ptr, size_t len, std::vector<std::string>& chkpt_filenames, std::string& logname,
+               uint32_t chkpt_id);
+
+  // include/tree_approach.hpp:53-59
+  size_t num_first_ocur();
+  size_t num_shift_dupl();
+
+  // src/tree_approach.cpp:1853-1915
+  void write_chkpt_log(header_t& header, HostBuf& diff_h, std::string& logname);
+  void write_restart_log(uint32_t select_chkpt, std::string& logname);
+
+  uint32_t current_id() const;
+  uint32_t baseline_id() const;
+  void set_current_id(uint32_t id);
+  dd_dedup* handle() const { return h_; }
+
+ private:
+  dd_dedup* h_ = nullptr;
+  uint32_t chunk_size_;
+  uint64_t data_len_ = 0;
+  std::pair<uint64_t, uint64_t> datasizes_{0, 0};  // (data incl. header, metadata), src/tree_approach.cpp:706-709
+};
+
+// src/utils.cpp:38-124: header, first-occurrence metadata and per-source-
+// checkpoint shifted metadata sizes of one diff, as one CSV row.
+std::string metadata_breakdown(const header_t& header, const HostBuf& diff, uint32_t num_chkpts);
+
+}  // namespace dd

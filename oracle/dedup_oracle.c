@@ -330,10 +330,11 @@ int or_checkpoint(or_dedup* d, const uint8_t* data, uint64_t len, int make_basel
   uint64_t N = 2 * C64 - 1;
   const uint32_t K = d->current_id;
 
-  /* Allocation policy, src/tree_approach.cpp:1600-1623: a baseline starts a
-   * fresh tree (all-zero digests) and a fresh map.  An incremental checkpoint
-   * must keep the buffer length of the chain. */
-  int baseline = make_baseline || !d->have_tree;
+  /* Allocation policy, src/tree_approach.cpp:1600-1623,1644-1653: a baseline
+   * (requested, the first call, or checkpoint id 0) starts a fresh tree
+   * (all-zero digests) and a fresh map.  An incremental checkpoint must keep
+   * the buffer length of the chain. */
+  int baseline = make_baseline || !d->have_tree || K == 0;
   if (!baseline && len != d->data_len) return OR_EINVAL;
   if (baseline) {
     free(d->tree);
@@ -344,7 +345,7 @@ int or_checkpoint(or_dedup* d, const uint8_t* data, uint64_t len, int make_basel
     d->data_len = len;
     d->C = C;
   }
-  if (K == 0 || make_baseline) d->baseline_id = K; /* :1644-1653 */
+  if (baseline) d->baseline_id = K;
 
   /* --- dedup_low_offset_ref, include/reference_impl.hpp:15-132 --- */
   uint8_t* prev = d->tree;

@@ -1,0 +1,208 @@
+"""ctypes binding of libdedup_hip.so (the C ABI declared in include/dedup_c.h).
+
+There is no fallback: if the HIP library is missing or fails to load, every
+entry point raises.  Device buffers are passed as raw HIP device addresses
+(for example ``tensor.data_ptr()`` of a ROCm torch tensor).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdedup_hip.so")
+
+DD_OK, DD_EINVAL, DD_ENOMEM, DD_ECORRUPT, DD_EHIP, DD_ESTATE = range(6)
+HASH_MURMUR3 = 0
+HASH_MD5 = 1
+
+# Every symbol include/dedup_c.h declares.
+EXPORTED = [
+    "dd_create", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
+    "dd_max_diff_size", "dd_counts", "dd_get_ids", "dd_set_current_id", "dd_get_timers",
+    "dd_restart", "dd_restart_device", "dd_set_profiling", "dd_get_stage_times", "dd_buf_free", "dd_strerror",
+]
+STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
+          "meta_gather"]
+
+
+class DedupError(RuntimeError):
+    def __init__(self, status, what):
+        super().__init__(f"{what}: {strerror(status)} ({status})")
+        self.status = status
+
+
+class dd_buf(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(ctypes.c_uint8)), ("len", ctypes.c_uint64)]
+
+
+class dd_header(ctypes.Structure):
+    _fields_ = [("ref_id", ctypes.c_uint32), ("chkpt_id", ctypes.c_uint32), ("datalen", ctypes.c_uint64),
+                ("chunk_size", ctypes.c_uint32), ("num_first_ocur", ctypes.c_uint32),
+                ("num_prior_chkpts", ctypes.c_uint32), ("num_shift_dupl", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def load():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    P = ctypes.POINTER
+    sig = {
+        "dd_create": ([u32, i32, vp, P(vp)], i32),
+        "dd_destroy": ([vp], i32),
+        "dd_checkpoint": ([vp, vp, u64, i32, P(dd_buf)], i32),
+        "dd_checkpoint_header": ([vp, P(dd_header), vp, u64, i32, P(dd_buf)], i32),
+        "dd_checkpoint_device": ([vp, vp, u64, i32, vp, u64, P(u64)], i32),
+        "dd_max_diff_size": ([u64, u32, u32], u64),
+        "dd_counts": ([vp, P(u64), P(u64)], i32),
+        "dd_get_ids": ([vp, P(u32), P(u32)], i32),
+        "dd_set_current_id": ([vp, u32], i32),
+        "dd_get_timers": ([vp, P(ctypes.c_double), P(ctypes.c_double)], i32),
+        "dd_restart": ([vp, P(dd_buf), u32, u32, vp, u64], i32),
+        "dd_restart_device": ([vp, P(vp), P(u64), u32, u32, vp, u64], i32),
+        "dd_set_profiling": ([vp, i32], i32),
+        "dd_get_stage_times": ([vp, P(ctypes.c_double)], i32),
+        "dd_buf_free": ([P(dd_buf)], None),
+        "dd_strerror": ([i32], ctypes.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def strerror(status):
+    return load().dd_strerror(status).decode()
+
+
+def _check(rc, what):
+    if rc != DD_OK:
+        raise DedupError(rc, what)
+
+
+def max_diff_size(length, chunk_size, ckpt_id):
+    return load().dd_max_diff_size(length, chunk_size, ckpt_id)
+
+
+class TreeDeduplicator:
+    """Python view of the C ABI, shaped like the reference's TreeDeduplicator
+    (include/tree_approach.hpp:16-205).  Data arguments are device addresses
+    (int) plus lengths; diffs come back as ``bytes``."""
+
+    def __init__(self, chunk_size, hash_kind=HASH_MURMUR3, stream=None):
+        L = load()
+        h = ctypes.c_void_p()
+        _check(L.dd_create(chunk_size, hash_kind, stream, ctypes.byref(h)), "dd_create")
+        self._h = h
+        self.chunk_size = chunk_size
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- checkpoint -------------------------------------------------------
+    def checkpoint(self, dev_ptr, length, make_baseline=False):
+        """dd_checkpoint: diff bytes on the host; advances the checkpoint id."""
+        b = dd_buf()
+        _check(load().dd_checkpoint(self._h, dev_ptr, length, int(make_baseline), ctypes.byref(b)), "dd_checkpoint")
+        try:
+            return ctypes.string_at(b.data, b.len)
+        finally:
+            load().dd_buf_free(ctypes.byref(b))
+
+    def checkpoint_header(self, dev_ptr, length, make_baseline=False):
+        """dd_checkpoint_header: (header dict, diff bytes); id not advanced."""
+        b = dd_buf()
+        hd = dd_header()
+        _check(load().dd_checkpoint_header(self._h, ctypes.byref(hd), dev_ptr, length, int(make_baseline),
+                                           ctypes.byref(b)), "dd_checkpoint_header")
+        try:
+            data = ctypes.string_at(b.data, b.len)
+        finally:
+            load().dd_buf_free(ctypes.byref(b))
+        return {k: getattr(hd, k) for k, _ in dd_header._fields_}, data
+
+    def checkpoint_device(self, dev_ptr, length, out_ptr, out_cap, make_baseline=False):
+        """dd_checkpoint_device: diff left in device memory; returns its length."""
+        n = ctypes.c_uint64()
+        _check(load().dd_checkpoint_device(self._h, dev_ptr, length, int(make_baseline), out_ptr, out_cap,
+                                           ctypes.byref(n)), "dd_checkpoint_device")
+        return n.value
+
+    # -- restart ----------------------------------------------------------
+    def restart(self, chain, ckpt_id, out_ptr, length):
+        keep = [bytes(c) for c in chain]
+        arr = (dd_buf * len(keep))()
+        for i, b in enumerate(keep):
+            arr[i].data = ctypes.cast(ctypes.c_char_p(b), ctypes.POINTER(ctypes.c_uint8))
+            arr[i].len = len(b)
+        _check(load().dd_restart(self._h, arr, len(keep), ckpt_id, out_ptr, length), "dd_restart")
+
+    def restart_rc(self, chain, ckpt_id, out_ptr, length):
+        keep = [bytes(c) for c in chain]
+        arr = (dd_buf * len(keep))()
+        for i, b in enumerate(keep):
+            arr[i].data = ctypes.cast(ctypes.c_char_p(b), ctypes.POINTER(ctypes.c_uint8))
+            arr[i].len = len(b)
+        return load().dd_restart(self._h, arr, len(keep), ckpt_id, out_ptr, length)
+
+    def restart_device(self, dev_ptrs, lens, ckpt_id, out_ptr, length):
+        n = len(dev_ptrs)
+        ptrs = (ctypes.c_void_p * n)(*dev_ptrs)
+        ls = (ctypes.c_uint64 * n)(*lens)
+        _check(load().dd_restart_device(self._h, ptrs, ls, n, ckpt_id, out_ptr, length), "dd_restart_device")
+
+    # -- state ------------------------------------------------------------
+    def counts(self):
+        f, s = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(load().dd_counts(self._h, ctypes.byref(f), ctypes.byref(s)), "dd_counts")
+        return f.value, s.value
+
+    def num_first_ocur(self):
+        return self.counts()[0]
+
+    def num_shift_dupl(self):
+        return self.counts()[1]
+
+    @property
+    def current_id(self):
+        c, b = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(load().dd_get_ids(self._h, ctypes.byref(c), ctypes.byref(b)), "dd_get_ids")
+        return c.value
+
+    @current_id.setter
+    def current_id(self, v):
+        _check(load().dd_set_current_id(self._h, v), "dd_set_current_id")
+
+    @property
+    def baseline_id(self):
+        c, b = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(load().dd_get_ids(self._h, ctypes.byref(c), ctypes.byref(b)), "dd_get_ids")
+        return b.value
+
+    def set_profiling(self, on=True):
+        _check(load().dd_set_profiling(self._h, int(on)), "dd_set_profiling")
+
+    def stage_times(self):
+        """GPU milliseconds per stage of the last checkpoint (profiling on)."""
+        ms = (ctypes.c_double * len(STAGES))()
+        _check(load().dd_get_stage_times(self._h, ms), "dd_get_stage_times")
+        return dict(zip(STAGES, list(ms)))
+
+    def timers(self):
+        t = (ctypes.c_double * 4)()
+        r = (ctypes.c_double * 2)()
+        _check(load().dd_get_timers(self._h, t, r), "dd_get_timers")
+        return list(t), list(r)

@@ -1,0 +1,793 @@
+// dd_engine.cpp -- host orchestration and the C ABI (include/dedup_c.h).
+//
+// Reference orchestration: TreeDeduplicator::checkpoint
+// (src/tree_approach.cpp:1577-1689) and restart_chkpt (:712-1128).
+#include "dd_engine.hpp"
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <new>
+
+#define DD_HIP(x)                         \
+  do {                                    \
+    if ((x) != hipSuccess) return DD_EHIP; \
+  } while (0)
+#define DD_TRY(x)                    \
+  do {                               \
+    dd_status _rc = (x);             \
+    if (_rc != DD_OK) return _rc;    \
+  } while (0)
+
+namespace dd {
+
+// ---------------------------------------------------------------------------
+// DevBuf
+// ---------------------------------------------------------------------------
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+dd_status DevBuf::ensure(size_t n, bool zero) {
+  if (n == 0) n = 16;
+  if (n <= bytes) return DD_OK;
+  release();
+  if (hipMalloc(&p, n) != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return DD_ENOMEM;
+  }
+  bytes = n;
+  if (zero && hipMemset(p, 0, n) != hipSuccess) return DD_EHIP;
+  return DD_OK;
+}
+
+static uint64_t next_pow2(uint64_t x) {
+  uint64_t c = 1;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id) {
+  if (chunk_size == 0) return 0;
+  const uint64_t C = ceil_div(len, chunk_size);
+  // first-occurrence and shifted roots are disjoint subtrees: 4F + 8S <= 8C
+  return 32 + 8 * C + 8 * ((uint64_t)ckpt_id + 1) + C * (uint64_t)chunk_size;
+}
+
+static MapView make_view(void* slots, uint64_t cap) {
+  MapView m;
+  m.slots = reinterpret_cast<Slot*>(slots);
+  m.mask = cap - 1;
+  m.shift = 64 - ilog2_64(cap);
+  return m;
+}
+
+static uint64_t map_limit_of(uint64_t cap) { return cap / 2 + cap / 4; }
+
+// ---------------------------------------------------------------------------
+// StreamState
+// ---------------------------------------------------------------------------
+dd_status StreamState::grow_map(uint64_t need, hipStream_t s) {
+  uint64_t cap = next_pow2(std::max<uint64_t>(2 * need, 1024));
+  if (cap <= map_cap) cap = map_cap * 2;
+  DevBuf fresh;
+  DD_TRY(fresh.ensure(cap * sizeof(Slot)));
+  launch_map_clear(fresh.as<Slot>(), cap, s);
+  DevBuf flag;
+  DD_TRY(flag.ensure(4, true));
+  launch_map_rehash(map.as<Slot>(), map_cap, make_view(fresh.p, cap), flag.as<uint32_t>(), s);
+  uint32_t f = 0;
+  DD_HIP(hipMemcpyAsync(&f, flag.p, 4, hipMemcpyDeviceToHost, s));
+  DD_HIP(hipStreamSynchronize(s));
+  if (f) return DD_ENOMEM;
+  std::swap(map.p, fresh.p);
+  std::swap(map.bytes, fresh.bytes);
+  map_cap = cap;
+  return DD_OK;
+}
+
+dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline, uint8_t* out, uint64_t out_cap,
+                               JobDev& job, hipStream_t s) {
+  const uint64_t cs = chunk_size;
+  const uint64_t Cn = ceil_div(len, cs);
+  if (len == 0 || Cn > 0x7fffffffull) return DD_EINVAL;
+  const uint64_t Nn = 2 * Cn - 1;
+  const uint32_t K = current_id;
+  if (!baseline) {
+    if (len != data_len) return DD_ESTATE;
+    if ((int64_t)K <= last_map_ckpt) return DD_ESTATE;
+  }
+  if (baseline) {
+    DD_TRY(tree.ensure(Nn * sizeof(Digest)));
+    DD_TRY(labels.ensure(Nn));
+    DD_TRY(slot_of.ensure(Nn * 4));
+    const uint64_t cap = next_pow2(std::max<uint64_t>(2 * Nn, 1024));
+    if (map.bytes > 4 * cap * sizeof(Slot)) map.release();
+    DD_TRY(map.ensure(cap * sizeof(Slot)));
+    map_cap = cap;
+    launch_map_clear(map.as<Slot>(), cap, s);
+    map_count = 0;
+    have_tree = true;
+    data_len = len;
+    C = Cn;
+    N = Nn;
+    last_map_ckpt = -1;
+  }
+  const uint64_t nblk = ceil_div(N, kNodesPerBlock);
+  DD_TRY(blk.ensure(3 * nblk * 4));
+  DD_TRY(f_list.ensure((C + 1) * 4));
+  DD_TRY(f_off.ensure((C + 1) * 4));
+  DD_TRY(s_node.ensure(C * 4));
+  DD_TRY(s_prev.ensure(C * 4));
+  DD_TRY(s_tree.ensure(C * 4));
+  DD_TRY(ss_node.ensure(C * 4));
+  DD_TRY(ss_prev.ensure(C * 4));
+  DD_TRY(s_hist.ensure(((uint64_t)K + 1) * ceil_div(C, kSTile) * 4));
+  DD_TRY(ptab.ensure(((uint64_t)K + 1) * 8));
+  DD_TRY(ctr.ensure(sizeof(JobCounters), true));
+  const uint64_t need = max_diff_size(len, chunk_size, K);
+  if (!out) {
+    DD_TRY(diff.ensure(need));
+    out = diff.as<uint8_t>();
+    out_cap = diff.bytes;
+  } else if (out_cap < need) {
+    return DD_EINVAL;
+  }
+
+  memset(&job, 0, sizeof job);
+  job.data = data;
+  job.len = len;
+  job.chunk_size = chunk_size;
+  job.hash_kind = hash_kind;
+  job.C = C;
+  job.N = N;
+  job.D = make_geom(C).D;
+  job.ckpt_id = K;
+  job.baseline_id = baseline ? K : baseline_id;
+  job.is_baseline = baseline ? 1 : 0;
+  job.tree = tree.as<Digest>();
+  job.labels = labels.as<uint8_t>();
+  job.slot_of = slot_of.as<uint32_t>();
+  job.map = make_view(map.p, map_cap);
+  job.map_count_at_start = map_count;
+  job.map_limit = map_limit_of(map_cap);
+  job.ctr = ctr.as<JobCounters>();
+  job.blk_f = blk.as<uint32_t>();
+  job.blk_fc = job.blk_f + nblk;
+  job.blk_s = job.blk_fc + nblk;
+  job.nblk = (uint32_t)nblk;
+  job.f_list = f_list.as<uint32_t>();
+  job.f_off = f_off.as<uint32_t>();
+  job.s_node = s_node.as<uint32_t>();
+  job.s_prev = s_prev.as<uint32_t>();
+  job.s_tree = s_tree.as<uint32_t>();
+  job.ss_node = ss_node.as<uint32_t>();
+  job.ss_prev = ss_prev.as<uint32_t>();
+  job.s_hist = s_hist.as<uint32_t>();
+  job.ptab = ptab.as<uint32_t>();
+  job.diff = out;
+  job.diff_cap = out_cap;
+  return DD_OK;
+}
+
+void StreamState::commit(const JobCounters& c, bool baseline) {
+  if (baseline) baseline_id = current_id;
+  nfirst = c.nfirst;
+  nshift = c.nshift;
+  map_count = c.map_count;
+  last_diff_len = c.diff_len;
+  last_map_ckpt = current_id;
+}
+
+// ---------------------------------------------------------------------------
+// Pipeline
+// ---------------------------------------------------------------------------
+enum Stage { S_LEAF = 0, S_INTERIOR, S_NODEBLK, S_STILE, S_META, S_GATHER, S_TILE0 };
+
+static int groups_of(uint32_t D) { return D == 0 ? 1 : (int)((D + kTileLevels - 1) / kTileLevels); }
+
+static uint64_t tiles_in_group(uint32_t D, int g) {
+  if (g >= groups_of(D)) return 0;
+  const int Lb = (int)D - kTileLevels * g;
+  const int Lr = Lb > kTileLevels ? Lb - kTileLevels : 0;
+  return 1ull << Lr;
+}
+
+Pipeline::~Pipeline() {
+  if (host_jobs_) (void)hipHostFree(host_jobs_);
+  if (host_prefix_) (void)hipHostFree(host_prefix_);
+  if (host_ctr) (void)hipHostFree(host_ctr);
+}
+
+dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
+  const size_t nj = jobs.size();
+  max_groups_ = 0;
+  aligned16_ = vec16_ = true;
+  hash_kind_ = jobs[0].hash_kind;
+  for (const JobDev& J : jobs) {
+    if (J.hash_kind != hash_kind_) return DD_EINVAL;
+    max_groups_ = std::max(max_groups_, groups_of(J.D));
+    const bool al = (reinterpret_cast<uintptr_t>(J.data) & 15) == 0 && (J.chunk_size & 15) == 0;
+    aligned16_ = aligned16_ && al;
+    vec16_ = vec16_ && al;
+  }
+  const size_t nstages = S_TILE0 + max_groups_;
+  const size_t need_prefix = nstages * (nj + 1);
+  if (need_prefix > host_prefix_cap_) {
+    if (host_prefix_) (void)hipHostFree(host_prefix_);
+    DD_HIP(hipHostMalloc((void**)&host_prefix_, need_prefix * 4, hipHostMallocDefault));
+    host_prefix_cap_ = need_prefix;
+  }
+  if (nj > host_jobs_cap_) {
+    if (host_jobs_) (void)hipHostFree(host_jobs_);
+    DD_HIP(hipHostMalloc((void**)&host_jobs_, nj * sizeof(JobDev), hipHostMallocDefault));
+    host_jobs_cap_ = nj;
+  }
+  if (nj > host_ctr_cap_) {
+    if (host_ctr) (void)hipHostFree(host_ctr);
+    DD_HIP(hipHostMalloc((void**)&host_ctr, nj * sizeof(JobCounters), hipHostMallocDefault));
+    host_ctr_cap_ = nj;
+  }
+  off_.assign(nstages, 0);
+  grid_.assign(nstages, 0);
+  for (size_t st = 0; st < nstages; st++) {
+    off_[st] = (uint32_t)(st * (nj + 1));
+    uint64_t acc = 0;
+    for (size_t j = 0; j < nj; j++) {
+      const JobDev& J = jobs[j];
+      host_prefix_[off_[st] + j] = (uint32_t)acc;
+      uint64_t n = 0;
+      switch (st) {
+        case S_LEAF: n = ceil_div(J.C, 256); break;
+        case S_INTERIOR: n = ceil_div(J.C - 1, 256); break;
+        case S_NODEBLK: n = J.nblk; break;
+        case S_STILE: n = ceil_div(J.C, kSTile); break;
+        case S_META: n = std::min<uint64_t>(ceil_div(J.C, 256), 256); break;
+        case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
+        default: n = tiles_in_group(J.D, (int)st - S_TILE0); break;
+      }
+      acc += n;
+    }
+    if (acc > 0x7fffffffull) return DD_EINVAL;
+    host_prefix_[off_[st] + nj] = (uint32_t)acc;
+    grid_[st] = (uint32_t)acc;
+  }
+  memcpy(host_jobs_, jobs.data(), nj * sizeof(JobDev));
+  DD_TRY(dev_jobs_.ensure(nj * sizeof(JobDev)));
+  DD_TRY(dev_prefix_.ensure(need_prefix * 4));
+  DD_HIP(hipMemcpyAsync(dev_jobs_.p, host_jobs_, nj * sizeof(JobDev), hipMemcpyHostToDevice, s));
+  DD_HIP(hipMemcpyAsync(dev_prefix_.p, host_prefix_, need_prefix * 4, hipMemcpyHostToDevice, s));
+  return DD_OK;
+}
+
+dd_status Pipeline::run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, hipEvent_t ev_dedup,
+                        hipEvent_t* stage_ev) {
+  if (jobs.empty()) return DD_EINVAL;
+  DD_TRY(upload(jobs, s));
+  const uint32_t nj = (uint32_t)jobs.size();
+  const JobDev* J = dev_jobs_.as<JobDev>();
+  auto P = [&](int st) { return dev_prefix_.as<uint32_t>() + off_[st]; };
+  auto mark = [&](int i) -> dd_status {
+    if (stage_ev) DD_HIP(hipEventRecord(stage_ev[i], s));
+    return DD_OK;
+  };
+  launch_begin(J, nullptr, nj, rerun, s);
+  DD_TRY(mark(0));
+  if (!rerun) launch_leaf_hash(J, P(S_LEAF), nj, grid_[S_LEAF], aligned16_, hash_kind_, s);
+  DD_TRY(mark(1));
+  launch_insert_leaves(J, P(S_LEAF), nj, grid_[S_LEAF], s);
+  DD_TRY(mark(2));
+  for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
+  DD_TRY(mark(3));
+  launch_insert_interior(J, P(S_INTERIOR), nj, grid_[S_INTERIOR], s);
+  DD_TRY(mark(4));
+  for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
+  DD_TRY(mark(5));
+  if (ev_dedup) DD_HIP(hipEventRecord(ev_dedup, s));
+  launch_roots_count(J, P(S_NODEBLK), nj, grid_[S_NODEBLK], s);
+  launch_roots_scan(J, nj, s);
+  launch_roots_write(J, P(S_NODEBLK), nj, grid_[S_NODEBLK], s);
+  launch_sbin_count(J, P(S_STILE), nj, grid_[S_STILE], s);
+  launch_sbin_scan(J, nj, s);
+  launch_sbin_scatter(J, P(S_STILE), nj, grid_[S_STILE], s);
+  DD_TRY(mark(6));
+  launch_write_meta(J, P(S_META), nj, grid_[S_META], s);
+  launch_gather(J, P(S_GATHER), nj, grid_[S_GATHER], vec16_, s);
+  DD_TRY(mark(7));
+  DD_HIP(hipGetLastError());
+  return DD_OK;
+}
+
+dd_status Pipeline::fetch_counters(const std::vector<JobDev>& jobs, hipStream_t s) {
+  for (size_t j = 0; j < jobs.size(); j++)
+    DD_HIP(hipMemcpyAsync(&host_ctr[j], jobs[j].ctr, sizeof(JobCounters), hipMemcpyDeviceToHost, s));
+  DD_HIP(hipStreamSynchronize(s));
+  return DD_OK;
+}
+
+}  // namespace dd
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace dd;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// One checkpoint of a handle: runs the pipeline, grows the map and re-runs the
+// inserts when the map would pass its load limit, and commits the state.
+dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline, uint8_t* out,
+                         uint64_t out_cap, JobCounters* result) {
+  const double t0 = now_s();
+  DD_HIP(hipSetDevice(h->device));
+  StreamState& st = h->st;
+  const bool baseline = make_baseline || !st.have_tree || st.current_id == 0;
+  JobDev job;
+  DD_TRY(st.prepare(reinterpret_cast<const uint8_t*>(dev_data), len, baseline, out, out_cap, job, h->stream));
+  std::vector<JobDev> jobs(1, job);
+  st.timers[0] = now_s() - t0;
+  DD_HIP(hipEventRecord(h->ev[0], h->stream));
+  DD_TRY(h->pipe.run(jobs, h->stream, false, h->ev[1], h->profile ? h->pev : nullptr));
+  DD_HIP(hipEventRecord(h->ev[2], h->stream));
+  DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
+  JobCounters c = h->pipe.host_ctr[0];
+  for (int attempt = 0; (c.flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) && attempt < 4; attempt++) {
+    const uint64_t need = c.map_count + 2 * c.pending;
+    DD_TRY(st.grow_map(need, h->stream));
+    jobs[0].map = make_view(st.map.p, st.map_cap);
+    jobs[0].map_count_at_start = c.map_count;
+    jobs[0].map_limit = map_limit_of(st.map_cap);
+    DD_TRY(h->pipe.run(jobs, h->stream, true));
+    DD_HIP(hipEventRecord(h->ev[2], h->stream));
+    DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
+    c = h->pipe.host_ctr[0];
+  }
+  if (c.flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return DD_ENOMEM;
+  if (c.flags & FLAG_BAD_SOURCE) return DD_ECORRUPT;
+  float ms1 = 0, ms2 = 0;
+  (void)hipEventElapsedTime(&ms1, h->ev[0], h->ev[1]);
+  (void)hipEventElapsedTime(&ms2, h->ev[1], h->ev[2]);
+  st.timers[1] = ms1 * 1e-3;
+  st.timers[2] = ms2 * 1e-3;
+  if (h->profile) {
+    for (int i = 0; i < DD_NUM_STAGES; i++) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, h->pev[i], h->pev[i + 1]);
+      h->stage_ms[i] = ms;
+    }
+  }
+  st.commit(c, baseline);
+  *result = c;
+  return DD_OK;
+}
+
+void fill_header(const dd_dedup* h, const JobCounters& c, uint64_t len, dd_header* hd) {
+  hd->ref_id = h->st.baseline_id;
+  hd->chkpt_id = h->st.current_id;
+  hd->datalen = len;
+  hd->chunk_size = h->st.chunk_size;
+  hd->num_first_ocur = c.nfirst;
+  hd->num_prior_chkpts = c.nprior;
+  hd->num_shift_dupl = c.nshift;
+}
+
+dd_status checkpoint_to_host(dd_dedup* h, dd_header* header, const void* dev_data, uint64_t len, int make_baseline,
+                             dd_buf* out) {
+  if (!h || !out || !dev_data) return DD_EINVAL;
+  out->data = nullptr;
+  out->len = 0;
+  JobCounters c;
+  DD_TRY(run_checkpoint(h, dev_data, len, make_baseline, nullptr, 0, &c));
+  const double t0 = now_s();
+  uint8_t* buf = static_cast<uint8_t*>(std::malloc(c.diff_len));
+  if (!buf) return DD_ENOMEM;
+  if (hipMemcpy(buf, h->st.diff.p, c.diff_len, hipMemcpyDeviceToHost) != hipSuccess) {
+    std::free(buf);
+    return DD_EHIP;
+  }
+  h->st.timers[3] = now_s() - t0;
+  out->data = buf;
+  out->len = c.diff_len;
+  if (header) fill_header(h, c, len, header);
+  return DD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Restart
+// ---------------------------------------------------------------------------
+uint32_t rd32(const uint8_t* p) {
+  uint32_t v;
+  memcpy(&v, p, 4);
+  return v;
+}
+
+struct HostCkpt {
+  dd_header hd;
+  uint64_t meta_bytes, data_bytes, total;
+  std::vector<uint8_t> meta;   // first list, table, shift list
+  std::vector<uint32_t> f_off, s_off, s_tree;
+  uint64_t nfl = 0, nsl = 0;   // leaves in first / shifted regions
+};
+
+// Validates one diff of the chain and builds its region prefixes.  Every node
+// id, leaf count and data range the kernels will touch is checked here.
+dd_status parse_ckpt(const uint8_t* hdr32, const uint8_t* meta, uint64_t total, uint32_t idx, uint64_t len,
+                     uint32_t cs, HostCkpt& hc) {
+  if (total < 32) return DD_ECORRUPT;
+  memcpy(&hc.hd, hdr32, 32);
+  const dd_header& hd = hc.hd;
+  if (hd.datalen != len || hd.chunk_size != cs || hd.chkpt_id != idx || hd.ref_id > hd.chkpt_id) return DD_ECORRUPT;
+  hc.meta_bytes = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
+  if (32 + hc.meta_bytes > total) return DD_ECORRUPT;
+  hc.total = total;
+  hc.data_bytes = total - 32 - hc.meta_bytes;
+  hc.meta.assign(meta, meta + hc.meta_bytes);
+  const TreeGeom g = make_geom(ceil_div(len, cs));
+  const uint8_t* fl = hc.meta.data();
+  const uint8_t* tb = fl + 4ull * hd.num_first_ocur;
+  const uint8_t* sl = tb + 8ull * hd.num_prior_chkpts;
+  hc.f_off.resize((size_t)hd.num_first_ocur + 1);
+  uint64_t acc = 0;
+  for (uint32_t i = 0; i < hd.num_first_ocur; i++) {
+    const uint32_t node = rd32(fl + 4ull * i);
+    if (node >= g.N) return DD_ECORRUPT;
+    hc.f_off[i] = (uint32_t)acc;
+    const LeafRuns r = subtree_runs(g, node);
+    acc += r.n0 + r.n1;
+    if (acc > g.C) return DD_ECORRUPT;
+  }
+  hc.f_off[hd.num_first_ocur] = (uint32_t)acc;
+  hc.nfl = acc;
+  if (acc * cs > hc.data_bytes) return DD_ECORRUPT;
+  // Prior table -> source checkpoint of each shifted entry (cumulative
+  // counts, src/tree_approach.cpp:868-898).
+  std::vector<uint64_t> cnt((size_t)hd.chkpt_id + 1, 0);
+  uint64_t tot = 0;
+  for (uint32_t i = 0; i < hd.num_prior_chkpts; i++) {
+    const uint32_t ck = rd32(tb + 8ull * i), n = rd32(tb + 8ull * i + 4);
+    if (ck > hd.chkpt_id) return DD_ECORRUPT;
+    cnt[ck] = n;
+  }
+  for (uint64_t v : cnt) tot += v;
+  if (tot != hd.num_shift_dupl) return DD_ECORRUPT;
+  hc.s_tree.resize(hd.num_shift_dupl);
+  hc.s_off.resize((size_t)hd.num_shift_dupl + 1);
+  {
+    uint64_t k = 0;
+    for (uint32_t t = 0; t <= hd.chkpt_id; t++)
+      for (uint64_t q = 0; q < cnt[t]; q++) hc.s_tree[k++] = t;
+  }
+  acc = 0;
+  for (uint32_t i = 0; i < hd.num_shift_dupl; i++) {
+    const uint32_t node = rd32(sl + 8ull * i), prev = rd32(sl + 8ull * i + 4);
+    if (node >= g.N || prev >= g.N) return DD_ECORRUPT;
+    const LeafRuns a = subtree_runs(g, node), b = subtree_runs(g, prev);
+    if (a.n0 + a.n1 != b.n0 + b.n1) return DD_ECORRUPT;
+    hc.s_off[i] = (uint32_t)acc;
+    acc += a.n0 + a.n1;
+    if (acc > g.C) return DD_ECORRUPT;
+  }
+  hc.s_off[hd.num_shift_dupl] = (uint32_t)acc;
+  hc.nsl = acc;
+  return DD_OK;
+}
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+// Walks the chain newest to oldest on the device.  `dev_diffs[k]` is the
+// device copy of ckpts[k]'s diff (or null: it is uploaded from host_diffs).
+dd_status restart_walk(dd_dedup* h, std::vector<HostCkpt>& ckpts, const std::vector<const uint8_t*>& host_diffs,
+                       const std::vector<const uint8_t*>& dev_diffs, uint32_t chkpt_id, void* dev_out, uint64_t len) {
+  const double t0 = now_s();
+  hipStream_t s = h->stream;
+  const uint32_t cs = ckpts[0].hd.chunk_size;
+  const TreeGeom g = make_geom(ceil_div(len, cs));
+  // Arena: [diff copies][f_off][s_off][s_tree] per checkpoint.
+  uint64_t arena = 0;
+  for (size_t k = 0; k < ckpts.size(); k++) {
+    if (!dev_diffs[k]) arena += align256(ckpts[k].total);
+    arena += align256(ckpts[k].f_off.size() * 4) + align256(ckpts[k].s_off.size() * 4) +
+             align256(ckpts[k].s_tree.size() * 4 + 4);
+  }
+  DD_TRY(h->rs_arena.ensure(arena));
+  DD_TRY(h->rs_desc.ensure(ckpts.size() * sizeof(RestartCkpt)));
+  DD_TRY(h->ref.ensure(g.C * 8));
+  if (h->cover.bytes < g.C * sizeof(CoverEntry)) {
+    h->cover.release();
+    DD_TRY(h->cover.ensure(g.C * sizeof(CoverEntry), true));
+    h->stamp = 0;
+  }
+  DD_TRY(h->rs_ctr.ensure(sizeof(JobCounters), true));
+  if (!h->rs_ctr_host) DD_HIP(hipHostMalloc((void**)&h->rs_ctr_host, sizeof(JobCounters), hipHostMallocDefault));
+  if (h->stamp > 0xfffff000u) {  // stamps are about to wrap: start over from a zeroed array
+    DD_HIP(hipMemsetAsync(h->cover.p, 0, h->cover.bytes, s));
+    h->stamp = 0;
+  }
+  std::vector<RestartCkpt> desc(ckpts.size());
+  uint8_t* base = h->rs_arena.as<uint8_t>();
+  uint64_t off = 0;
+  const double c0 = now_s();
+  for (size_t k = 0; k < ckpts.size(); k++) {
+    HostCkpt& hc = ckpts[k];
+    RestartCkpt& rc = desc[k];
+    if (dev_diffs[k]) {
+      rc.diff = dev_diffs[k];
+    } else {
+      DD_HIP(hipMemcpyAsync(base + off, host_diffs[k], hc.total, hipMemcpyHostToDevice, s));
+      rc.diff = base + off;
+      off += align256(hc.total);
+    }
+    DD_HIP(hipMemcpyAsync(base + off, hc.f_off.data(), hc.f_off.size() * 4, hipMemcpyHostToDevice, s));
+    rc.f_off = reinterpret_cast<const uint32_t*>(base + off);
+    off += align256(hc.f_off.size() * 4);
+    DD_HIP(hipMemcpyAsync(base + off, hc.s_off.data(), hc.s_off.size() * 4, hipMemcpyHostToDevice, s));
+    rc.s_off = reinterpret_cast<const uint32_t*>(base + off);
+    off += align256(hc.s_off.size() * 4);
+    if (!hc.s_tree.empty())
+      DD_HIP(hipMemcpyAsync(base + off, hc.s_tree.data(), hc.s_tree.size() * 4, hipMemcpyHostToDevice, s));
+    rc.s_tree = reinterpret_cast<const uint32_t*>(base + off);
+    off += align256(hc.s_tree.size() * 4 + 4);
+    rc.nfirst = hc.hd.num_first_ocur;
+    rc.nshift = hc.hd.num_shift_dupl;
+    rc.ckpt_id = hc.hd.chkpt_id;
+    rc.first_off = 32;
+    rc.shift_off = 32 + 4ull * hc.hd.num_first_ocur + 8ull * hc.hd.num_prior_chkpts;
+    rc.data_off = 32 + hc.meta_bytes;
+  }
+  DD_HIP(hipMemcpyAsync(h->rs_desc.p, desc.data(), desc.size() * sizeof(RestartCkpt), hipMemcpyHostToDevice, s));
+  DD_HIP(hipStreamSynchronize(s));
+  const double c1 = now_s();
+  JobCounters* ctr = h->rs_ctr.as<JobCounters>();
+  launch_restart_init(h->ref.as<uint64_t>(), g.C, chkpt_id, ctr, s);
+  const RestartCkpt* dd = h->rs_desc.as<RestartCkpt>();
+  for (size_t k = 0; k < ckpts.size(); k++) {
+    const uint32_t stamp = ++h->stamp;
+    launch_restart_cover(dd + k, g, h->cover.as<CoverEntry>(), stamp, ckpts[k].nfl, ckpts[k].nsl, cs, s);
+    launch_restart_resolve(dd + k, g, h->cover.as<CoverEntry>(), stamp, h->ref.as<uint64_t>(),
+                           reinterpret_cast<uint8_t*>(dev_out), len, cs, ctr, s);
+  }
+  launch_restart_check(h->ref.as<uint64_t>(), g.C, ctr, s);
+  DD_HIP(hipGetLastError());
+  DD_HIP(hipMemcpyAsync(h->rs_ctr_host, ctr, sizeof(JobCounters), hipMemcpyDeviceToHost, s));
+  DD_HIP(hipStreamSynchronize(s));
+  h->st.restart_timers[0] = c1 - c0;
+  h->st.restart_timers[1] = now_s() - t0;
+  if (h->rs_ctr_host->unresolved || (h->rs_ctr_host->flags & FLAG_BAD_SOURCE)) return DD_ECORRUPT;
+  return DD_OK;
+}
+
+// Collects the checkpoints the walk visits: chkpt_id down to the reference id
+// named by each visited header (src/tree_approach.cpp:943).
+template <typename GetHdr>
+dd_status collect_chain(uint32_t nchain, uint32_t chkpt_id, uint64_t len, GetHdr get, std::vector<HostCkpt>& out,
+                        std::vector<uint32_t>& ids) {
+  if (chkpt_id >= nchain) return DD_EINVAL;
+  uint8_t hdr[32];
+  uint64_t total = 0;
+  std::vector<uint8_t> meta;
+  DD_TRY(get(chkpt_id, hdr, meta, total));
+  dd_header h0;
+  if (total < 32) return DD_ECORRUPT;
+  memcpy(&h0, hdr, 32);
+  if (h0.datalen != len) return DD_EINVAL;
+  if (h0.chunk_size == 0) return DD_ECORRUPT;
+  const uint32_t cs = h0.chunk_size;
+  int64_t idx = chkpt_id;
+  int64_t ref = h0.ref_id;
+  bool first = true;
+  while (idx >= ref) {
+    if (!first) DD_TRY(get((uint32_t)idx, hdr, meta, total));
+    HostCkpt hc;
+    DD_TRY(parse_ckpt(hdr, meta.data(), total, (uint32_t)idx, len, cs, hc));
+    ref = hc.hd.ref_id;
+    out.push_back(std::move(hc));
+    ids.push_back((uint32_t)idx);
+    first = false;
+    idx--;
+  }
+  return DD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+dd_status dd_create(uint32_t chunk_size, int hash_kind, void* hip_stream, dd_dedup** out) {
+  if (!out) return DD_EINVAL;
+  *out = nullptr;
+  if (chunk_size == 0 || (hash_kind != DD_HASH_MURMUR3 && hash_kind != DD_HASH_MD5)) return DD_EINVAL;
+  dd_dedup* h = new (std::nothrow) dd_dedup;
+  if (!h) return DD_ENOMEM;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return DD_EHIP;
+  }
+  h->st.chunk_size = chunk_size;
+  h->st.hash_kind = hash_kind;
+  if (hip_stream) {
+    h->stream = reinterpret_cast<hipStream_t>(hip_stream);
+  } else {
+    if (hipStreamCreate(&h->stream) != hipSuccess) {
+      delete h;
+      return DD_EHIP;
+    }
+    h->own_stream = true;
+  }
+  for (auto& e : h->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      dd_destroy(h);
+      return DD_EHIP;
+    }
+  }
+  for (auto& e : h->pev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      dd_destroy(h);
+      return DD_EHIP;
+    }
+  }
+  *out = h;
+  return DD_OK;
+}
+
+dd_status dd_destroy(dd_dedup* h) {
+  if (!h) return DD_EINVAL;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : h->pev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->rs_ctr_host) (void)hipHostFree(h->rs_ctr_host);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return DD_OK;
+}
+
+dd_status dd_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline, dd_buf* out) {
+  DD_TRY(checkpoint_to_host(h, nullptr, dev_data, len, make_baseline, out));
+  h->st.current_id += 1;
+  return DD_OK;
+}
+
+dd_status dd_checkpoint_header(dd_dedup* h, dd_header* header, const void* dev_data, uint64_t len, int make_baseline,
+                               dd_buf* out) {
+  if (!header) return DD_EINVAL;
+  return checkpoint_to_host(h, header, dev_data, len, make_baseline, out);
+}
+
+dd_status dd_checkpoint_device(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline, void* dev_out,
+                               uint64_t out_cap, uint64_t* out_len) {
+  if (!h || !dev_data || !dev_out || !out_len) return DD_EINVAL;
+  JobCounters c;
+  DD_TRY(run_checkpoint(h, dev_data, len, make_baseline, reinterpret_cast<uint8_t*>(dev_out), out_cap, &c));
+  h->st.timers[3] = 0;
+  *out_len = c.diff_len;
+  h->st.current_id += 1;
+  return DD_OK;
+}
+
+uint64_t dd_max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id) {
+  return max_diff_size(len, chunk_size, ckpt_id);
+}
+
+dd_status dd_counts(dd_dedup* h, uint64_t* nf, uint64_t* ns) {
+  if (!h) return DD_EINVAL;
+  if (nf) *nf = h->st.nfirst;
+  if (ns) *ns = h->st.nshift;
+  return DD_OK;
+}
+
+dd_status dd_get_ids(dd_dedup* h, uint32_t* cur, uint32_t* base) {
+  if (!h) return DD_EINVAL;
+  if (cur) *cur = h->st.current_id;
+  if (base) *base = h->st.baseline_id;
+  return DD_OK;
+}
+
+dd_status dd_set_current_id(dd_dedup* h, uint32_t id) {
+  if (!h) return DD_EINVAL;
+  h->st.current_id = id;
+  return DD_OK;
+}
+
+dd_status dd_get_timers(dd_dedup* h, double t[4], double rt[2]) {
+  if (!h) return DD_EINVAL;
+  if (t) memcpy(t, h->st.timers, sizeof h->st.timers);
+  if (rt) memcpy(rt, h->st.restart_timers, sizeof h->st.restart_timers);
+  return DD_OK;
+}
+
+dd_status dd_restart(dd_dedup* h, const dd_buf* chain, uint32_t nchain, uint32_t chkpt_id, void* dev_out,
+                     uint64_t len) {
+  if (!h || !chain || !dev_out) return DD_EINVAL;
+  if (hipSetDevice(h->device) != hipSuccess) return DD_EHIP;
+  auto get = [&](uint32_t i, uint8_t* hdr, std::vector<uint8_t>& meta, uint64_t& total) -> dd_status {
+    total = chain[i].len;
+    if (!chain[i].data || total < 32) return DD_ECORRUPT;
+    memcpy(hdr, chain[i].data, 32);
+    dd_header hd;
+    memcpy(&hd, hdr, 32);
+    const uint64_t mb = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
+    if (32 + mb > total) return DD_ECORRUPT;
+    meta.assign(chain[i].data + 32, chain[i].data + 32 + mb);
+    return DD_OK;
+  };
+  std::vector<HostCkpt> ck;
+  std::vector<uint32_t> ids;
+  DD_TRY(collect_chain(nchain, chkpt_id, len, get, ck, ids));
+  std::vector<const uint8_t*> hd(ck.size()), dv(ck.size(), nullptr);
+  for (size_t k = 0; k < ck.size(); k++) hd[k] = chain[ids[k]].data;
+  return restart_walk(h, ck, hd, dv, chkpt_id, dev_out, len);
+}
+
+dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uint64_t* lens, uint32_t nchain,
+                            uint32_t chkpt_id, void* dev_out, uint64_t len) {
+  if (!h || !dev_chain || !lens || !dev_out) return DD_EINVAL;
+  if (hipSetDevice(h->device) != hipSuccess) return DD_EHIP;
+  auto get = [&](uint32_t i, uint8_t* hdr, std::vector<uint8_t>& meta, uint64_t& total) -> dd_status {
+    total = lens[i];
+    if (!dev_chain[i] || total < 32) return DD_ECORRUPT;
+    if (hipMemcpy(hdr, dev_chain[i], 32, hipMemcpyDeviceToHost) != hipSuccess) return DD_EHIP;
+    dd_header hd;
+    memcpy(&hd, hdr, 32);
+    const uint64_t mb = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
+    if (32 + mb > total) return DD_ECORRUPT;
+    meta.resize(mb);
+    if (mb && hipMemcpy(meta.data(), static_cast<const uint8_t*>(dev_chain[i]) + 32, mb, hipMemcpyDeviceToHost) !=
+                  hipSuccess)
+      return DD_EHIP;
+    return DD_OK;
+  };
+  std::vector<HostCkpt> ck;
+  std::vector<uint32_t> ids;
+  DD_TRY(collect_chain(nchain, chkpt_id, len, get, ck, ids));
+  std::vector<const uint8_t*> hd(ck.size(), nullptr), dv(ck.size());
+  for (size_t k = 0; k < ck.size(); k++) dv[k] = static_cast<const uint8_t*>(dev_chain[ids[k]]);
+  return restart_walk(h, ck, hd, dv, chkpt_id, dev_out, len);
+}
+
+dd_status dd_set_profiling(dd_dedup* h, int enable) {
+  if (!h) return DD_EINVAL;
+  h->profile = enable != 0;
+  return DD_OK;
+}
+
+dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]) {
+  if (!h || !ms) return DD_EINVAL;
+  memcpy(ms, h->stage_ms, sizeof h->stage_ms);
+  return DD_OK;
+}
+
+void dd_buf_free(dd_buf* b) {
+  if (!b) return;
+  std::free(b->data);
+  b->data = nullptr;
+  b->len = 0;
+}
+
+const char* dd_strerror(int s) {
+  switch (s) {
+    case DD_OK: return "ok";
+    case DD_EINVAL: return "invalid argument";
+    case DD_ENOMEM: return "out of memory";
+    case DD_ECORRUPT: return "corrupt or incomplete checkpoint chain";
+    case DD_EHIP: return "HIP runtime error";
+    case DD_ESTATE: return "call violates the checkpoint chain";
+    default: return "unknown status";
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// dd_engine.hpp -- host-side orchestration shared by the C ABI entry points:
+// device buffers, the per-checkpoint kernel sequence over a job table, digest
+// map growth, and the restart chain walk.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/dedup_c.h"
+#include "dd_kernels.hpp"
+
+namespace dd {
+
+// A grow-only device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf();
+  dd_status ensure(size_t n, bool zero = false);
+  void release();
+  template <typename T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Persistent per-stream deduplication state: the previous tree's leaf
+// digests, the digest map and the scratch arrays of one checkpoint buffer.
+struct StreamState {
+  uint32_t chunk_size = 0;
+  int hash_kind = 0;
+  uint32_t current_id = 0, baseline_id = 0;
+  bool have_tree = false;
+  uint64_t data_len = 0, C = 0, N = 0;
+  int64_t last_map_ckpt = -1;  // checkpoint id of the last insert into the map
+
+  DevBuf tree, labels, slot_of;
+  DevBuf map;
+  uint64_t map_cap = 0, map_count = 0;
+  DevBuf blk, f_list, f_off, s_node, s_prev, s_tree, ss_node, ss_prev, s_hist, ptab;
+  DevBuf diff;  // internal diff buffer (host-output path)
+  DevBuf ctr;   // JobCounters
+
+  uint64_t nfirst = 0, nshift = 0, last_diff_len = 0;
+  double timers[4] = {0, 0, 0, 0};
+  double restart_timers[2] = {0, 0};
+
+  // Sizes the state for a checkpoint of `len` bytes (fresh map and zero tree
+  // on a baseline) and fills `job`.  `out`/`out_cap` are the diff buffer.
+  dd_status prepare(const uint8_t* data, uint64_t len, bool baseline, uint8_t* out, uint64_t out_cap, JobDev& job,
+                    hipStream_t s);
+  // Called after the checkpoint's counters are back on the host.
+  void commit(const JobCounters& c, bool baseline);
+  // Grows the map so that `need` entries stay under the load limit.
+  dd_status grow_map(uint64_t need, hipStream_t s);
+};
+
+// Runs the checkpoint kernels over a table of jobs.  Owns the device copies
+// of the job table and of the per-stage workgroup prefixes.
+class Pipeline {
+ public:
+  ~Pipeline();
+  // Uploads the job table and launches every stage on `s`.  `host_ctr`
+  // receives the counters (pinned, njobs entries) once `s` is synchronized.
+  // `stage_ev`, when given, holds DD_NUM_STAGES + 1 events recorded around
+  // the stages; `ev_dedup` is recorded after the duplicate forest.
+  dd_status run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, hipEvent_t ev_dedup = nullptr,
+                hipEvent_t* stage_ev = nullptr);
+  dd_status fetch_counters(const std::vector<JobDev>& jobs, hipStream_t s);
+  JobCounters* host_ctr = nullptr;
+
+ private:
+  dd_status upload(const std::vector<JobDev>& jobs, hipStream_t s);
+  DevBuf dev_jobs_, dev_prefix_;
+  JobDev* host_jobs_ = nullptr;
+  uint32_t* host_prefix_ = nullptr;
+  size_t host_jobs_cap_ = 0, host_prefix_cap_ = 0, host_ctr_cap_ = 0;
+  std::vector<uint32_t> off_;  // offset of each stage's prefix in dev_prefix_
+  std::vector<uint32_t> grid_;
+  int max_groups_ = 0;
+  bool aligned16_ = true, vec16_ = true;
+  int hash_kind_ = 0;
+};
+
+uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
+
+}  // namespace dd
+
+struct dd_dedup {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  dd::StreamState st;
+  dd::Pipeline pipe;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool profile = false;
+  hipEvent_t pev[DD_NUM_STAGES + 1] = {};
+  double stage_ms[DD_NUM_STAGES] = {};
+  // restart scratch
+  dd::DevBuf ref, cover, rs_arena, rs_desc;
+  uint32_t stamp = 0;
+  dd::DevBuf rs_ctr;
+  dd::JobCounters* rs_ctr_host = nullptr;
+};

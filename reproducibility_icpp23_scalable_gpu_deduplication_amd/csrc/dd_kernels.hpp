@@ -1,0 +1,122 @@
+// dd_kernels.hpp -- host-visible declarations of the deduplication kernels.
+//
+// Every kernel works on a table of jobs (one job = one checkpoint buffer with
+// its own tree, digest map and output diff).  A launch covers all jobs at once:
+// `prefix[j]` is the first workgroup of job j and prefix[njobs] the grid size,
+// so the single-buffer path is a table of one job and the multi-buffer path
+// runs the same kernels over many jobs in one launch per stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dd_device.hpp"
+
+namespace dd {
+
+constexpr int kTileLevels = 8;     // levels per tree tile (256 bottom nodes)
+constexpr int kNodesPerBlock = 4096;  // root scan: 256 threads x 16 nodes
+constexpr int kSTile = 4096;          // shifted-root sort tile
+
+// Per-job counters written by the device and read back by the host.
+struct JobCounters {
+  unsigned long long pending;     // leaves that needed a map insert
+  unsigned long long map_count;   // entries in the digest map
+  uint32_t flags;                 // FLAG_*
+  uint32_t nfirst;                // first-occurrence roots
+  uint32_t nshift;                // shifted-duplicate roots
+  uint32_t nprior;                // prior-checkpoint table entries
+  uint64_t dchunks;               // chunks in first-occurrence regions
+  uint64_t diff_len;              // bytes of the diff
+  uint32_t unresolved;            // restart: chunks left without a source
+  uint32_t pad;
+};
+
+struct JobDev {
+  // input
+  const uint8_t* data;
+  uint64_t len;
+  uint32_t chunk_size;
+  int32_t hash_kind;
+  uint64_t C, N;
+  uint32_t D;
+  uint32_t ckpt_id;
+  uint32_t baseline_id;
+  uint32_t is_baseline;
+  // persistent state
+  Digest* tree;        // N digests, heap order
+  uint8_t* labels;     // N labels
+  uint32_t* slot_of;   // N: map slot of pending leaves and shifted interior nodes
+  MapView map;
+  uint64_t map_count_at_start;  // entries before this checkpoint
+  uint64_t map_limit;           // max entries the map may hold
+  JobCounters* ctr;
+  // scratch
+  uint32_t* blk_f;    // per node block: first roots
+  uint32_t* blk_fc;   // per node block: chunks under first roots
+  uint32_t* blk_s;    // per node block: shifted roots
+  uint32_t nblk;      // node blocks = ceil(N / kNodesPerBlock)
+  uint32_t* f_list;   // first roots, ascending
+  uint32_t* f_off;    // chunk offset of each first region in the data section
+  uint32_t* s_node;   // shifted roots, ascending
+  uint32_t* s_prev;   // source node of each shifted root
+  uint32_t* s_tree;   // source checkpoint of each shifted root
+  uint32_t* ss_node;  // shifted roots grouped by source checkpoint
+  uint32_t* ss_prev;
+  uint32_t* s_hist;   // (ckpt_id+1) x ceil(C / kSTile) histogram / bases
+  uint32_t* ptab;     // prior table {ckpt, count} x (ckpt_id+1)
+  // output
+  uint8_t* diff;
+  uint64_t diff_cap;
+};
+
+// Restart descriptors -------------------------------------------------------
+struct RestartCkpt {
+  const uint8_t* diff;      // device copy of the diff
+  const uint32_t* f_off;    // exclusive prefix of first-region leaf counts (nfirst+1)
+  const uint32_t* s_off;    // exclusive prefix of shifted-region leaf counts (nshift+1)
+  const uint32_t* s_tree;   // source checkpoint per shifted entry
+  uint32_t nfirst, nshift;
+  uint32_t ckpt_id;
+  uint64_t first_off;       // byte offset of the first-occurrence list
+  uint64_t shift_off;       // byte offset of the shifted list
+  uint64_t data_off;        // byte offset of the data section
+};
+
+struct CoverEntry {
+  uint32_t stamp;     // generation that wrote the entry
+  uint32_t src_ckpt;  // kCoverFirst for data in this checkpoint
+  uint64_t payload;   // data byte offset, or source leaf node
+};
+constexpr uint32_t kCoverFirst = 0xffffffffu;
+
+// Launch helpers (dd_kernels.hip) -------------------------------------------
+void launch_begin(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, bool rerun, hipStream_t s);
+void launch_leaf_hash(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool all_aligned16,
+                      int hash_kind, hipStream_t s);
+void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
+void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
+void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_roots_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s);
+void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_sbin_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_sbin_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s);
+void launch_sbin_scatter(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_write_meta(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_gather(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool vec16,
+                   hipStream_t s);
+
+void launch_map_clear(Slot* slots, uint64_t cap, hipStream_t s);
+void launch_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView nmap, uint32_t* flags, hipStream_t s);
+
+void launch_restart_init(uint64_t* ref, uint64_t C, uint32_t ckpt, JobCounters* ctr, hipStream_t s);
+void launch_restart_cover(const RestartCkpt* rc, TreeGeom g, CoverEntry* cover, uint32_t stamp, uint64_t nfirst_leaves,
+                          uint64_t nshift_leaves, uint32_t chunk_size, hipStream_t s);
+void launch_restart_resolve(const RestartCkpt* rc, TreeGeom g, const CoverEntry* cover, uint32_t stamp,
+                            uint64_t* ref, uint8_t* out, uint64_t len, uint32_t chunk_size, JobCounters* ctr,
+                            hipStream_t s);
+void launch_restart_check(const uint64_t* ref, uint64_t C, JobCounters* ctr, hipStream_t s);
+
+}  // namespace dd

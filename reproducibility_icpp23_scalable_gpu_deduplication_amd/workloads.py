@@ -1,0 +1,134 @@
+"""Synthetic checkpoint chains for the tree-approach workloads.
+
+Each generator is seeded and returns the full chain (checkpoint 0 first).
+Host versions return numpy uint8 arrays; ``*_gpu`` versions build the chain
+directly in device memory with torch (ROCm) so multi-GiB chains never cross
+PCIe.  Changes are chunk-aligned so they can deduplicate.
+"""
+import numpy as np
+
+
+def _pick_runs(rng, nchunks, target, mean_len):
+    """Contiguous runs of geometric length (mean `mean_len`) at uniform starts
+    until `target` chunks are covered; returns a list of (start, length)."""
+    runs, covered = [], 0
+    p = 1.0 / mean_len
+    while covered < target:
+        ln = int(min(rng.geometric(p), nchunks))
+        st = int(rng.integers(0, nchunks - ln + 1))
+        runs.append((st, ln))
+        covered += ln
+    return runs
+
+
+def chain_uniform_changes(seed, size, chunk, nckpt, fresh_frac=0.01, copy_frac=0.02, zero_bytes=1 << 20):
+    """64 MiB-style chain: per checkpoint, `fresh_frac` of the chunks get fresh
+    random bytes, `copy_frac` become copies of other random chunks of the same
+    buffer, and one `zero_bytes`-aligned region is zero-filled."""
+    rng = np.random.default_rng(seed)
+    cur = rng.integers(0, 256, size, dtype=np.uint8)
+    out = [cur.copy()]
+    C = size // chunk
+    for _ in range(1, nckpt):
+        prev = out[-1]
+        cur = prev.copy()
+        v = cur[: C * chunk].reshape(C, chunk)
+        pv = prev[: C * chunk].reshape(C, chunk)
+        nf = int(round(fresh_frac * C))
+        nc = int(round(copy_frac * C))
+        idx = rng.permutation(C)
+        fresh = idx[:nf]
+        dst = idx[nf:nf + nc]
+        src = rng.integers(0, C, nc)
+        v[fresh] = rng.integers(0, 256, (nf, chunk), dtype=np.uint8)
+        v[dst] = pv[src]
+        if zero_bytes and size >= zero_bytes:
+            z = int(rng.integers(0, size // zero_bytes)) * zero_bytes
+            cur[z:z + zero_bytes] = 0
+        out.append(cur)
+    return out
+
+
+def chain_runs(seed, size, chunk, nckpt, change_frac, mean_run=16):
+    """1 GiB-style chain (host): per checkpoint `change_frac` of the chunks
+    change as runs of geometric length; half of the runs get fresh bytes,
+    half copy a run of the previous checkpoint at a random offset."""
+    rng = np.random.default_rng(seed)
+    cur = rng.integers(0, 256, size, dtype=np.uint8)
+    out = [cur.copy()]
+    C = size // chunk
+    for _ in range(1, nckpt):
+        prev = out[-1]
+        cur = prev.copy()
+        v = cur[: C * chunk].reshape(C, chunk)
+        pv = prev[: C * chunk].reshape(C, chunk)
+        for i, (st, ln) in enumerate(_pick_runs(rng, C, max(1, int(change_frac * C)), mean_run)):
+            if i % 2 == 0:
+                v[st:st + ln] = rng.integers(0, 256, (ln, chunk), dtype=np.uint8)
+            else:
+                s = int(rng.integers(0, C - ln + 1))
+                v[st:st + ln] = pv[s:s + ln]
+        out.append(cur)
+    return out
+
+
+def chain_runs_gpu(torch, seed, size, chunk, nckpt, change_frac, mean_run=16, device="cuda"):
+    """Same recipe as chain_runs, generated on the GPU (torch RNG, so bytes
+    differ from the host version for the same seed)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    C = size // chunk
+    cur = torch.randint(0, 256, (size,), dtype=torch.uint8, device=device, generator=gen)
+    out = [cur]
+    for _ in range(1, nckpt):
+        prev = out[-1]
+        cur = prev.clone()
+        v = cur[: C * chunk].view(C, chunk)
+        pv = prev[: C * chunk].view(C, chunk)
+        fresh_idx, dst_idx, src_idx = [], [], []
+        for i, (st, ln) in enumerate(_pick_runs(rng, C, max(1, int(change_frac * C)), mean_run)):
+            if i % 2 == 0:
+                fresh_idx.append(np.arange(st, st + ln))
+            else:
+                s = int(rng.integers(0, C - ln + 1))
+                dst_idx.append(np.arange(st, st + ln))
+                src_idx.append(np.arange(s, s + ln))
+        if fresh_idx:
+            fi = torch.from_numpy(np.concatenate(fresh_idx)).to(device)
+            v[fi] = torch.randint(0, 256, (fi.numel(), chunk), dtype=torch.uint8, device=device, generator=gen)
+        if dst_idx:
+            di = torch.from_numpy(np.concatenate(dst_idx)).to(device)
+            si = torch.from_numpy(np.concatenate(src_idx)).to(device)
+            v[di] = pv[si]
+        out.append(cur)
+    return out
+
+
+def chain_float_field(seed, n_values, nckpt, zero_frac=0.6, block_bytes=4096, perturb_frac=0.05, toggle_frac=0.01):
+    """4 GiB-style float32 field (host): `zero_frac` of the values are exact
+    zeros in blocks of `block_bytes`, the rest N(0,1); per checkpoint
+    `perturb_frac` of the nonzero values get +N(0,1e-3) and `toggle_frac` of
+    the blocks flip between zero and nonzero."""
+    rng = np.random.default_rng(seed)
+    per_block = block_bytes // 4
+    nblk = n_values // per_block
+    zero = rng.random(nblk) < zero_frac
+    x = rng.standard_normal(n_values).astype(np.float32)
+    x.reshape(nblk, per_block)[zero] = 0
+    out = [x.view(np.uint8).copy()]
+    for _ in range(1, nckpt):
+        x = x.copy()
+        nz = np.flatnonzero(x)
+        pick = rng.choice(nz, size=int(perturb_frac * nz.size), replace=False) if nz.size else nz
+        x[pick] += rng.normal(0, 1e-3, pick.size).astype(np.float32)
+        tog = rng.choice(nblk, size=max(1, int(toggle_frac * nblk)), replace=False)
+        xb = x.reshape(nblk, per_block)
+        for b in tog:
+            if zero[b]:
+                xb[b] = rng.standard_normal(per_block).astype(np.float32)
+            else:
+                xb[b] = 0
+            zero[b] = ~zero[b]
+        out.append(x.view(np.uint8).copy())
+    return out

@@ -1,0 +1,168 @@
+"""HIP path vs the CPU oracle, bit-exact, through the C ABI.
+
+Every diff the GPU returns must equal the oracle's byte for byte, and every
+checkpoint of every chain must restart byte-identical (restart on the GPU).
+"""
+import random
+import struct
+
+import numpy as np
+import pytest
+
+import oracle as O
+import reproducibility_icpp23_scalable_gpu_deduplication_amd as dd
+from reproducibility_icpp23_scalable_gpu_deduplication_amd import workloads as W
+from gpu_util import first_mismatch, gpu_chain, gpu_restart, oracle_chain, to_dev
+from test_oracle import HAND_DIFFS, KATS, _make_chain
+
+pytestmark = pytest.mark.gpu
+
+
+def check_chain(torch, chunk, datas, kind=dd.HASH_MURMUR3, baselines=(0,), restart_ids=None):
+    g, gd, gc = gpu_chain(torch, chunk, datas, kind, baselines)
+    od, oc = oracle_chain(chunk, datas, kind, baselines)
+    for k, (a, b) in enumerate(zip(gd, od)):
+        assert a == b, f"ckpt {k}: {first_mismatch(a, b)}"
+    assert gc == oc
+    ids = range(len(datas)) if restart_ids is None else restart_ids
+    for k in ids:
+        want = datas[k].tobytes() if isinstance(datas[k], np.ndarray) else bytes(datas[k])
+        got = gpu_restart(torch, g, gd, k, len(want))
+        assert got == want, f"restart {k}: {first_mismatch(got, want)}"
+    return g, gd
+
+
+@pytest.mark.parametrize("name", sorted(KATS))
+def test_kats(torch_gpu, name):
+    a, b, ef, es = KATS[name]
+    g, gd = check_chain(torch_gpu, 1, [a.encode(), b.encode()])
+    assert g.counts() == (ef, es)
+    if name in HAND_DIFFS:
+        assert gd[1] == HAND_DIFFS[name]
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_small_chains(torch_gpu, seed):
+    rng = random.Random(1000 + seed)
+    C = rng.choice([1, 2, 3, 5, 8, 13, 32, 33, 100, 255, 256, 257])
+    cs = rng.choice([1, 3, 4, 16, 32, 64])
+    tail = rng.randrange(0, cs) if cs > 1 else 0
+    kind = rng.choice([dd.HASH_MURMUR3, dd.HASH_MD5])
+    datas = _make_chain(rng, C, cs, 6, tail)
+    check_chain(torch_gpu, cs, datas, kind)
+
+
+@pytest.mark.parametrize("C,cs", [(1 << 12, 128), (5000, 128), (1 << 14, 16), (3 << 11, 512), (777, 4096)])
+def test_medium_chains_with_dups(torch_gpu, C, cs):
+    datas = W.chain_uniform_changes(C + cs, C * cs, cs, 6, fresh_frac=0.02, copy_frac=0.05, zero_bytes=4 * cs)
+    check_chain(torch_gpu, cs, datas)
+
+
+def test_md5_float_field(torch_gpu):
+    # 60 % zero 4 KiB blocks: intra-checkpoint duplicates from the baseline on.
+    datas = W.chain_float_field(3, 1 << 16, 5)
+    check_chain(torch_gpu, 64, datas, dd.HASH_MD5)
+
+
+def test_partial_last_chunk(torch_gpu):
+    rng = np.random.default_rng(9)
+    base = rng.integers(0, 256, 128 * 37 + 77, dtype=np.uint8)
+    d1 = base.copy()
+    d1[-50:] = 0
+    d2 = d1.copy()
+    d2[:128] = d1[128 * 5:128 * 6]
+    check_chain(torch_gpu, 128, [base, d1, d2])
+
+
+def test_fresh_data_every_checkpoint_grows_map(torch_gpu):
+    # The reference's round-trip test (test/tree_chkpt.cpp): new random bytes
+    # each checkpoint, so every checkpoint inserts about 2C digests and the
+    # map must grow.
+    rng = np.random.default_rng(2)
+    datas = [rng.integers(0, 256, 1 << 20, dtype=np.uint8) for _ in range(6)]
+    check_chain(torch_gpu, 128, datas)
+
+
+def test_rebaseline_mid_chain(torch_gpu):
+    datas = W.chain_uniform_changes(5, 1 << 18, 64, 8, zero_bytes=4096)
+    check_chain(torch_gpu, 64, datas, baselines=(0, 3))
+
+
+def test_whole_buffer_repeat_matches_oracle(torch_gpu):
+    # Root 0 SHIFT is not emitted (include/reference_impl.hpp:105-106).
+    datas = [b"ABCDEFGH", b"IJKLMNOP", b"ABCDEFGH"]
+    g, gd, _ = gpu_chain(torch_gpu, 1, datas)
+    od, _ = oracle_chain(1, datas)
+    assert gd == od
+    assert gpu_restart(torch_gpu, g, gd, 2, 8) == b"IJKLMNOP"
+
+
+def test_config1_shape_64mib(torch_gpu):
+    # 64 MiB, 128 B chunks, 2^19 leaves, 10 checkpoints: 1 % fresh chunks,
+    # 2 % copied chunks, one 1 MiB zero region per checkpoint.
+    datas = W.chain_uniform_changes(1931, 64 << 20, 128, 10)
+    check_chain(torch_gpu, 128, datas, restart_ids=[0, 4, 9])
+
+
+def test_device_output_matches_host_output(torch_gpu):
+    torch = torch_gpu
+    datas = W.chain_uniform_changes(8, 1 << 20, 128, 4, zero_bytes=8192)
+    g1 = dd.TreeDeduplicator(128)
+    g2 = dd.TreeDeduplicator(128)
+    cap = dd.max_diff_size(1 << 20, 128, 3)
+    out = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    dev_diffs = []
+    for i, x in enumerate(datas):
+        t = to_dev(torch, x)
+        host = g1.checkpoint(t.data_ptr(), t.numel(), i == 0)
+        n = g2.checkpoint_device(t.data_ptr(), t.numel(), out.data_ptr(), cap, i == 0)
+        torch.cuda.synchronize()
+        assert out[:n].cpu().numpy().tobytes() == host
+        dev_diffs.append(out[:n].clone())
+    torch.cuda.synchronize()
+    res = torch.zeros(1 << 20, dtype=torch.uint8, device="cuda")
+    for k in range(len(datas)):
+        g2.restart_device([d.data_ptr() for d in dev_diffs], [d.numel() for d in dev_diffs], k, res.data_ptr(),
+                          1 << 20)
+        torch.cuda.synchronize()
+        assert res.cpu().numpy().tobytes() == datas[k].tobytes()
+
+
+def test_header_overload_does_not_advance(torch_gpu):
+    torch = torch_gpu
+    g = dd.TreeDeduplicator(1)
+    t = to_dev(torch, b"ABCDEFGH")
+    hd, diff = g.checkpoint_header(t.data_ptr(), 8, True)
+    assert g.current_id == 0
+    assert hd == dict(ref_id=0, chkpt_id=0, datalen=8, chunk_size=1, num_first_ocur=1, num_prior_chkpts=0,
+                      num_shift_dupl=0)
+    assert diff[:32] == struct.pack("<IIQIIII", 0, 0, 8, 1, 1, 0, 0)
+    g.current_id = 1
+    t2 = to_dev(torch, b"IJABCDEK")
+    hd2, diff2 = g.checkpoint_header(t2.data_ptr(), 8, False)
+    assert diff2 == HAND_DIFFS["04"]
+    assert hd2["num_shift_dupl"] == 3 and g.current_id == 1
+
+
+def test_error_paths(torch_gpu):
+    torch = torch_gpu
+    g = dd.TreeDeduplicator(16)
+    t = to_dev(torch, bytes(range(256)) * 4)
+    d0 = g.checkpoint(t.data_ptr(), 1024, True)
+    # an incremental checkpoint must keep the chain's length
+    with pytest.raises(dd.DedupError) as e:
+        g.checkpoint(t.data_ptr(), 512, False)
+    assert e.value.status == dd.DD_ESTATE
+    out = torch.zeros(1024, dtype=torch.uint8, device="cuda")
+    assert g.restart_rc([d0], 0, out.data_ptr(), 1000) == dd.DD_EINVAL
+    assert g.restart_rc([d0[:40]], 0, out.data_ptr(), 1024) == dd.DD_ECORRUPT
+    bad = bytearray(d0)
+    struct.pack_into("<I", bad, 32, 1 << 30)
+    assert g.restart_rc([bytes(bad)], 0, out.data_ptr(), 1024) == dd.DD_ECORRUPT
+    assert g.restart_rc([d0], 3, out.data_ptr(), 1024) == dd.DD_EINVAL
+    # a missing middle checkpoint makes the chain incomplete
+    d1 = g.checkpoint(t.data_ptr(), 1024, False)
+    assert g.restart_rc([d0[:10], d1], 1, out.data_ptr(), 1024) == dd.DD_ECORRUPT
+    g.restart([d0, d1], 1, out.data_ptr(), 1024)
+    torch.cuda.synchronize()
+    assert out.cpu().numpy().tobytes() == bytes(range(256)) * 4
