@@ -118,14 +118,34 @@ dd_status dd_restart(dd_dedup* h, const dd_buf* chain, uint32_t nchain, uint32_t
 dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uint64_t* chain_lens,
                             uint32_t nchain, uint32_t chkpt_id, void* dev_out, uint64_t len);
 
+/* Batched checkpointing (no counterpart in the reference, whose multi-rank
+ * driver src/dedup_files_mpi.cpp:54-120 runs one TreeDeduplicator per MPI
+ * rank and checkpoints each buffer on its own).  A batch groups handles; one
+ * batch call makes one checkpoint of every handle (same semantics and bytes as
+ * dd_checkpoint_device on each), running each stage as a single launch over
+ * all buffers.  With graphs enabled the launch sequence is captured once per
+ * buffer-shape set and replayed.  The handles must not be used elsewhere
+ * while a batch call runs; the batch has its own HIP stream. */
+typedef struct dd_batch dd_batch;
+dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out);
+dd_status dd_batch_destroy(dd_batch* b);
+dd_status dd_batch_use_graph(dd_batch* b, int enable);
+/* Arrays have one entry per handle.  dev_out[i] must hold at least
+ * dd_max_diff_size(lens[i], chunk_size, current id of handle i) bytes. */
+dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, const uint64_t* lens,
+                                     const int* make_baseline, void* const* dev_out, const uint64_t* out_caps,
+                                     uint64_t* out_lens);
+
 /* Diagnostics (no counterpart in the reference; its equivalent is the
- * Kokkos::Profiling regions of src/tree_approach.cpp): when enabled, the
- * next checkpoints record HIP events on the handle's stream between stages,
- * and dd_get_stage_times returns the last checkpoint's GPU milliseconds for
- * {leaf hash, leaf inserts, first-occurrence forest, interior inserts,
- * duplicate forest, roots + grouping, metadata + gather}. */
+ * Kokkos::Profiling regions of src/tree_approach.cpp).  Level 1 records HIP
+ * events on the handle's stream around the leaf-hash kernel only; level 2
+ * around every stage (each event adds a few microseconds).  Level 0 turns
+ * them off.  dd_get_stage_times returns the last checkpoint's GPU
+ * milliseconds for {leaf hash, leaf inserts, first-occurrence forest,
+ * interior inserts, duplicate forest, roots + grouping, metadata + gather};
+ * stages not recorded read 0. */
 #define DD_NUM_STAGES 7
-dd_status dd_set_profiling(dd_dedup* h, int enable);
+dd_status dd_set_profiling(dd_dedup* h, int level);
 dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]);
 
 void dd_buf_free(dd_buf* b);

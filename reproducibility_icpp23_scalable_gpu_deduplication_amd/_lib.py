@@ -18,7 +18,8 @@ HASH_MD5 = 1
 EXPORTED = [
     "dd_create", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
     "dd_max_diff_size", "dd_counts", "dd_get_ids", "dd_set_current_id", "dd_get_timers",
-    "dd_restart", "dd_restart_device", "dd_set_profiling", "dd_get_stage_times", "dd_buf_free", "dd_strerror",
+    "dd_restart", "dd_restart_device", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
+    "dd_batch_checkpoint_device", "dd_set_profiling", "dd_get_stage_times", "dd_buf_free", "dd_strerror",
 ]
 STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
           "meta_gather"]
@@ -66,6 +67,10 @@ def load():
         "dd_get_timers": ([vp, P(ctypes.c_double), P(ctypes.c_double)], i32),
         "dd_restart": ([vp, P(dd_buf), u32, u32, vp, u64], i32),
         "dd_restart_device": ([vp, P(vp), P(u64), u32, u32, vp, u64], i32),
+        "dd_batch_create": ([P(vp), u32, P(vp)], i32),
+        "dd_batch_destroy": ([vp], i32),
+        "dd_batch_use_graph": ([vp, i32], i32),
+        "dd_batch_checkpoint_device": ([vp, P(vp), P(u64), P(i32), P(vp), P(u64), P(u64)], i32),
         "dd_set_profiling": ([vp, i32], i32),
         "dd_get_stage_times": ([vp, P(ctypes.c_double)], i32),
         "dd_buf_free": ([P(dd_buf)], None),
@@ -98,6 +103,7 @@ class TreeDeduplicator:
     (int) plus lengths; diffs come back as ``bytes``."""
 
     def __init__(self, chunk_size, hash_kind=HASH_MURMUR3, stream=None):
+        self._h = None
         L = load()
         h = ctypes.c_void_p()
         _check(L.dd_create(chunk_size, hash_kind, stream, ctypes.byref(h)), "dd_create")
@@ -192,8 +198,9 @@ class TreeDeduplicator:
         _check(load().dd_get_ids(self._h, ctypes.byref(c), ctypes.byref(b)), "dd_get_ids")
         return b.value
 
-    def set_profiling(self, on=True):
-        _check(load().dd_set_profiling(self._h, int(on)), "dd_set_profiling")
+    def set_profiling(self, level=1):
+        """0 off, 1 leaf-hash kernel events only, 2 events around every stage."""
+        _check(load().dd_set_profiling(self._h, int(level)), "dd_set_profiling")
 
     def stage_times(self):
         """GPU milliseconds per stage of the last checkpoint (profiling on)."""
@@ -206,3 +213,37 @@ class TreeDeduplicator:
         r = (ctypes.c_double * 2)()
         _check(load().dd_get_timers(self._h, t, r), "dd_get_timers")
         return list(t), list(r)
+
+
+class Batch:
+    """dd_batch: one checkpoint of every handle per call, each stage launched
+    once over all buffers (optionally replayed from a captured hipGraph)."""
+
+    def __init__(self, dedups, graph=True):
+        L = load()
+        self._dedups = list(dedups)
+        arr = (ctypes.c_void_p * len(self._dedups))(*[d._h.value for d in self._dedups])
+        h = ctypes.c_void_p()
+        _check(L.dd_batch_create(arr, len(self._dedups), ctypes.byref(h)), "dd_batch_create")
+        self._h = h
+        _check(L.dd_batch_use_graph(self._h, int(graph)), "dd_batch_use_graph")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def checkpoint_device(self, dev_ptrs, lens, make_baseline, out_ptrs, out_caps):
+        n = len(self._dedups)
+        vp, u64 = ctypes.c_void_p, ctypes.c_uint64
+        d = (vp * n)(*dev_ptrs)
+        ln = (u64 * n)(*lens)
+        mk = (ctypes.c_int * n)(*[int(bool(x)) for x in make_baseline])
+        o = (vp * n)(*out_ptrs)
+        oc = (u64 * n)(*out_caps)
+        res = (u64 * n)()
+        _check(load().dd_batch_checkpoint_device(self._h, d, ln, mk, o, oc, res), "dd_batch_checkpoint_device")
+        return list(res)

@@ -73,6 +73,17 @@ static MapView make_view(void* slots, uint64_t cap) {
 
 static uint64_t map_limit_of(uint64_t cap) { return cap / 2 + cap / 4; }
 
+// Tree tiles: group g of a tree of depth D covers levels D-8g down to
+// max(D-8(g+1), 0); it has one tile per node of its top level.
+static int groups_of(uint32_t D) { return D == 0 ? 1 : (int)((D + kTileLevels - 1) / kTileLevels); }
+
+static uint64_t tiles_in_group(uint32_t D, int g) {
+  if (g >= groups_of(D)) return 0;
+  const int Lb = (int)D - kTileLevels * g;
+  const int Lr = Lb > kTileLevels ? Lb - kTileLevels : 0;
+  return 1ull << Lr;
+}
+
 // ---------------------------------------------------------------------------
 // StreamState
 // ---------------------------------------------------------------------------
@@ -108,7 +119,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   }
   if (baseline) {
     DD_TRY(tree.ensure(Nn * sizeof(Digest)));
-    DD_TRY(labels.ensure(Nn));
+    DD_TRY(labels.ensure(ceil_div(Nn, kNodesPerBlock) * kNodesPerBlock + 64));  // root scans read 16 at a time
     DD_TRY(slot_of.ensure(Nn * 4));
     const uint64_t cap = next_pow2(std::max<uint64_t>(2 * Nn, 1024));
     if (map.bytes > 4 * cap * sizeof(Slot)) map.release();
@@ -123,7 +134,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     last_map_ckpt = -1;
   }
   const uint64_t nblk = ceil_div(N, kNodesPerBlock);
-  DD_TRY(blk.ensure(3 * nblk * 4));
+  DD_TRY(blk.ensure(3 * kMaxStageWG * 4));
   DD_TRY(f_list.ensure((C + 1) * 4));
   DD_TRY(f_off.ensure((C + 1) * 4));
   DD_TRY(s_node.ensure(C * 4));
@@ -133,7 +144,29 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   DD_TRY(ss_prev.ensure(C * 4));
   DD_TRY(s_hist.ensure(((uint64_t)K + 1) * ceil_div(C, kSTile) * 4));
   DD_TRY(ptab.ensure(((uint64_t)K + 1) * 8));
-  DD_TRY(ctr.ensure(sizeof(JobCounters), true));
+  DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * 8, true));
+  const uint64_t nlb = ceil_div(C, 256);
+  DD_TRY(leafblk.ensure((nlb + kMaxStageWG) * 4));
+  DD_TRY(pend_list.ensure(nlb * 256 * 4));
+  const TreeGeom gm = make_geom(C);
+  uint64_t ntiles = 0;
+  uint32_t tile_base[4] = {0, 0, 0, 0};
+  for (int g = 0; g < 4; g++) {
+    tile_base[g] = (uint32_t)ntiles;
+    ntiles += tiles_in_group(gm.D, g);
+  }
+  DD_TRY(tiles.ensure((ntiles + kMaxStageWG) * 4));
+  // Per group-0 tile stamp of the last checkpoint that changed one of its
+  // leaves; zero-filled when (re)allocated, epochs start at 1.
+  const uint64_t ntiles0 = tiles_in_group(gm.D, 0);
+  if (tile_epoch.bytes < ntiles0 * 4 || epoch == 0xffffffffu) {
+    tile_epoch.release();
+    DD_TRY(tile_epoch.ensure(ntiles0 * 4, true));
+    epoch = 0;
+  }
+  epoch += 1;
+  const uint64_t dirty_cap = ceil_div(ntiles0, 32), act_cap = ceil_div(nlb, 32);
+  DD_TRY(lists.ensure((64 + 32 * dirty_cap + 32 * act_cap) * 4));
   const uint64_t need = max_diff_size(len, chunk_size, K);
   if (!out) {
     DD_TRY(diff.ensure(need));
@@ -161,9 +194,26 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.map_count_at_start = map_count;
   job.map_limit = map_limit_of(map_cap);
   job.ctr = ctr.as<JobCounters>();
+  job.host_ctr = nullptr;  // set by the pipeline
+  job.blk_pend = leafblk.as<uint32_t>();
+  job.blk_claim = job.blk_pend + nlb;
+  job.pend_acc = reinterpret_cast<unsigned long long*>(ctr.as<uint8_t>() + sizeof(JobCounters));
+  job.pend_list = pend_list.as<uint32_t>();
+  for (int g = 0; g < 4; g++) job.tile_base[g] = tile_base[g];
+  job.ntiles = (uint32_t)ntiles;
+  job.tile_fint = tiles.as<uint32_t>();
+  job.tile_claim = job.tile_fint + ntiles;
+  job.epoch = epoch;
+  job.tile_epoch = tile_epoch.as<uint32_t>();
+  job.dirty_cnt = lists.as<uint32_t>();
+  job.act_cnt = job.dirty_cnt + 32;
+  job.dirty_cap = (uint32_t)dirty_cap;
+  job.act_cap = (uint32_t)act_cap;
+  job.dirty_list = job.act_cnt + 32;
+  job.act_list = job.dirty_list + 32 * dirty_cap;
   job.blk_f = blk.as<uint32_t>();
-  job.blk_fc = job.blk_f + nblk;
-  job.blk_s = job.blk_fc + nblk;
+  job.blk_fc = job.blk_f + kMaxStageWG;
+  job.blk_s = job.blk_fc + kMaxStageWG;
   job.nblk = (uint32_t)nblk;
   job.f_list = f_list.as<uint32_t>();
   job.f_off = f_off.as<uint32_t>();
@@ -191,25 +241,18 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
 // ---------------------------------------------------------------------------
 // Pipeline
 // ---------------------------------------------------------------------------
-enum Stage { S_LEAF = 0, S_INTERIOR, S_NODEBLK, S_STILE, S_META, S_GATHER, S_TILE0 };
+enum Stage { S_LEAF = 0, S_INSL, S_TILES, S_NODEBLK, S_STILE, S_META, S_GATHER, S_TILE0 };
 
-static int groups_of(uint32_t D) { return D == 0 ? 1 : (int)((D + kTileLevels - 1) / kTileLevels); }
-
-static uint64_t tiles_in_group(uint32_t D, int g) {
-  if (g >= groups_of(D)) return 0;
-  const int Lb = (int)D - kTileLevels * g;
-  const int Lr = Lb > kTileLevels ? Lb - kTileLevels : 0;
-  return 1ull << Lr;
-}
 
 Pipeline::~Pipeline() {
-  if (host_jobs_) (void)hipHostFree(host_jobs_);
-  if (host_prefix_) (void)hipHostFree(host_prefix_);
+  if (host_table_) (void)hipHostFree(host_table_);
   if (host_ctr) (void)hipHostFree(host_ctr);
 }
 
 dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   const size_t nj = jobs.size();
+  if (nj == 0) return DD_EINVAL;
+  njobs_ = (uint32_t)nj;
   max_groups_ = 0;
   aligned16_ = vec16_ = true;
   hash_kind_ = jobs[0].hash_kind;
@@ -220,23 +263,37 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
     aligned16_ = aligned16_ && al;
     vec16_ = vec16_ && al;
   }
+  // Leaf kernel mapping.  The lane-strided kernel measured faster than the
+  // LDS-staged one on MI355X for 128-byte chunks (4.97 vs 4.50 TB/s), so it
+  // is the default; DD_LEAF_KERNEL=lds selects the LDS-staged kernel, which
+  // needs 16-byte aligned buffers and chunk sizes that are multiples of its
+  // segment.
+  leaf_seg_ = 0;
+  if (const char* e = std::getenv("DD_LEAF_KERNEL")) {
+    if (!strcmp(e, "lds") && aligned16_) {
+      int seg = 256;
+      for (const JobDev& J : jobs) {
+        while (seg >= 64 && (J.chunk_size % seg) != 0) seg >>= 1;
+      }
+      if (seg >= 64) leaf_seg_ = seg;
+    }
+  }
   const size_t nstages = S_TILE0 + max_groups_;
   const size_t need_prefix = nstages * (nj + 1);
-  if (need_prefix > host_prefix_cap_) {
-    if (host_prefix_) (void)hipHostFree(host_prefix_);
-    DD_HIP(hipHostMalloc((void**)&host_prefix_, need_prefix * 4, hipHostMallocDefault));
-    host_prefix_cap_ = need_prefix;
-  }
-  if (nj > host_jobs_cap_) {
-    if (host_jobs_) (void)hipHostFree(host_jobs_);
-    DD_HIP(hipHostMalloc((void**)&host_jobs_, nj * sizeof(JobDev), hipHostMallocDefault));
-    host_jobs_cap_ = nj;
+  const size_t jobs_bytes = (nj * sizeof(JobDev) + 255) & ~size_t(255);
+  const size_t table_bytes = jobs_bytes + need_prefix * 4;
+  if (table_bytes > host_table_cap_) {
+    if (host_table_) (void)hipHostFree(host_table_);
+    DD_HIP(hipHostMalloc((void**)&host_table_, table_bytes, hipHostMallocDefault));
+    host_table_cap_ = table_bytes;
   }
   if (nj > host_ctr_cap_) {
     if (host_ctr) (void)hipHostFree(host_ctr);
     DD_HIP(hipHostMalloc((void**)&host_ctr, nj * sizeof(JobCounters), hipHostMallocDefault));
     host_ctr_cap_ = nj;
   }
+  JobDev* hjobs = reinterpret_cast<JobDev*>(host_table_);
+  uint32_t* hprefix = reinterpret_cast<uint32_t*>(host_table_ + jobs_bytes);
   off_.assign(nstages, 0);
   grid_.assign(nstages, 0);
   for (size_t st = 0; st < nstages; st++) {
@@ -244,57 +301,83 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
     uint64_t acc = 0;
     for (size_t j = 0; j < nj; j++) {
       const JobDev& J = jobs[j];
-      host_prefix_[off_[st] + j] = (uint32_t)acc;
+      hprefix[off_[st] + j] = (uint32_t)acc;
       uint64_t n = 0;
       switch (st) {
         case S_LEAF: n = ceil_div(J.C, 256); break;
-        case S_INTERIOR: n = ceil_div(J.C - 1, 256); break;
-        case S_NODEBLK: n = J.nblk; break;
+        case S_INSL: n = std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
+        case S_TILES: n = std::min<uint64_t>(J.ntiles, kMaxStageWG); break;
+        case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kMaxStageWG); break;
         case S_STILE: n = ceil_div(J.C, kSTile); break;
         case S_META: n = std::min<uint64_t>(ceil_div(J.C, 256), 256); break;
         case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
-        default: n = tiles_in_group(J.D, (int)st - S_TILE0); break;
+        default: n = std::min<uint64_t>(tiles_in_group(J.D, (int)st - S_TILE0), kMaxStageWG); break;
       }
       acc += n;
     }
     if (acc > 0x7fffffffull) return DD_EINVAL;
-    host_prefix_[off_[st] + nj] = (uint32_t)acc;
+    hprefix[off_[st] + nj] = (uint32_t)acc;
     grid_[st] = (uint32_t)acc;
   }
-  memcpy(host_jobs_, jobs.data(), nj * sizeof(JobDev));
-  DD_TRY(dev_jobs_.ensure(nj * sizeof(JobDev)));
-  DD_TRY(dev_prefix_.ensure(need_prefix * 4));
-  DD_HIP(hipMemcpyAsync(dev_jobs_.p, host_jobs_, nj * sizeof(JobDev), hipMemcpyHostToDevice, s));
-  DD_HIP(hipMemcpyAsync(dev_prefix_.p, host_prefix_, need_prefix * 4, hipMemcpyHostToDevice, s));
+  for (size_t j = 0; j < nj; j++) {
+    hjobs[j] = jobs[j];
+    hjobs[j].host_ctr = &host_ctr[j];
+    hjobs[j].nwg_ins_leaf = hprefix[off_[S_INSL] + j + 1] - hprefix[off_[S_INSL] + j];
+    hjobs[j].nwg_ins_int = hprefix[off_[S_TILES] + j + 1] - hprefix[off_[S_TILES] + j];
+  }
+  DD_TRY(dev_table_.ensure(table_bytes));
+  dev_jobs_ = dev_table_.as<JobDev>();
+  dev_prefix_ = reinterpret_cast<uint32_t*>(dev_table_.as<uint8_t>() + jobs_bytes);
+  DD_HIP(hipMemcpyAsync(dev_table_.p, host_table_, table_bytes, hipMemcpyHostToDevice, s));
   return DD_OK;
+}
+
+std::vector<uint64_t> Pipeline::signature() const {
+  std::vector<uint64_t> sig;
+  sig.push_back(njobs_);
+  sig.push_back((uint64_t)max_groups_);
+  sig.push_back((uint64_t)leaf_seg_);
+  sig.push_back((uint64_t)aligned16_ | ((uint64_t)vec16_ << 1));
+  sig.push_back((uint64_t)hash_kind_);
+  sig.push_back(reinterpret_cast<uintptr_t>(dev_table_.p));
+  for (uint32_t o : off_) sig.push_back(o);
+  for (uint32_t g : grid_) sig.push_back(g);
+  return sig;
 }
 
 dd_status Pipeline::run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, hipEvent_t ev_dedup,
                         hipEvent_t* stage_ev) {
   if (jobs.empty()) return DD_EINVAL;
   DD_TRY(upload(jobs, s));
-  const uint32_t nj = (uint32_t)jobs.size();
-  const JobDev* J = dev_jobs_.as<JobDev>();
-  auto P = [&](int st) { return dev_prefix_.as<uint32_t>() + off_[st]; };
+  return launch_all(s, rerun, ev_dedup, stage_ev);
+}
+
+dd_status Pipeline::launch_all(hipStream_t s, bool rerun, hipEvent_t ev_dedup, hipEvent_t* stage_ev) {
+  const uint32_t nj = njobs_;
+  const JobDev* J = dev_jobs_;
+  auto P = [&](int st) { return dev_prefix_ + off_[st]; };
   auto mark = [&](int i) -> dd_status {
-    if (stage_ev) DD_HIP(hipEventRecord(stage_ev[i], s));
+    if (stage_ev && (profile_level_ >= 2 || i <= 1)) DD_HIP(hipEventRecord(stage_ev[i], s));
     return DD_OK;
   };
   launch_begin(J, nullptr, nj, rerun, s);
   DD_TRY(mark(0));
-  if (!rerun) launch_leaf_hash(J, P(S_LEAF), nj, grid_[S_LEAF], aligned16_, hash_kind_, s);
+  if (!rerun) {
+    if (leaf_seg_) launch_leaf_hash_lds(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_seg_, hash_kind_, s);
+    else launch_leaf_hash(J, P(S_LEAF), nj, grid_[S_LEAF], aligned16_, hash_kind_, s);
+  }
   DD_TRY(mark(1));
-  launch_insert_leaves(J, P(S_LEAF), nj, grid_[S_LEAF], s);
+  launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], s);
   DD_TRY(mark(2));
   for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
   DD_TRY(mark(3));
-  launch_insert_interior(J, P(S_INTERIOR), nj, grid_[S_INTERIOR], s);
+  launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], s);
   DD_TRY(mark(4));
   for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
   DD_TRY(mark(5));
   if (ev_dedup) DD_HIP(hipEventRecord(ev_dedup, s));
   launch_roots_count(J, P(S_NODEBLK), nj, grid_[S_NODEBLK], s);
-  launch_roots_scan(J, nj, s);
+  launch_roots_scan(J, P(S_NODEBLK), nj, s);
   launch_roots_write(J, P(S_NODEBLK), nj, grid_[S_NODEBLK], s);
   launch_sbin_count(J, P(S_STILE), nj, grid_[S_STILE], s);
   launch_sbin_scan(J, nj, s);
@@ -307,9 +390,7 @@ dd_status Pipeline::run(const std::vector<JobDev>& jobs, hipStream_t s, bool rer
   return DD_OK;
 }
 
-dd_status Pipeline::fetch_counters(const std::vector<JobDev>& jobs, hipStream_t s) {
-  for (size_t j = 0; j < jobs.size(); j++)
-    DD_HIP(hipMemcpyAsync(&host_ctr[j], jobs[j].ctr, sizeof(JobCounters), hipMemcpyDeviceToHost, s));
+dd_status Pipeline::fetch_counters(const std::vector<JobDev>&, hipStream_t s) {
   DD_HIP(hipStreamSynchronize(s));
   return DD_OK;
 }
@@ -340,6 +421,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   std::vector<JobDev> jobs(1, job);
   st.timers[0] = now_s() - t0;
   DD_HIP(hipEventRecord(h->ev[0], h->stream));
+  h->pipe.profile_level_ = h->profile;
   DD_TRY(h->pipe.run(jobs, h->stream, false, h->ev[1], h->profile ? h->pev : nullptr));
   DD_HIP(hipEventRecord(h->ev[2], h->stream));
   DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
@@ -365,7 +447,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   if (h->profile) {
     for (int i = 0; i < DD_NUM_STAGES; i++) {
       float ms = 0;
-      (void)hipEventElapsedTime(&ms, h->pev[i], h->pev[i + 1]);
+      if (h->profile >= 2 || i == 0) (void)hipEventElapsedTime(&ms, h->pev[i], h->pev[i + 1]);
       h->stage_ms[i] = ms;
     }
   }
@@ -759,9 +841,116 @@ dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uin
   return restart_walk(h, ck, hd, dv, chkpt_id, dev_out, len);
 }
 
+dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out) {
+  if (!handles || n == 0 || !out) return DD_EINVAL;
+  *out = nullptr;
+  for (uint32_t i = 0; i < n; i++) {
+    if (!handles[i]) return DD_EINVAL;
+    if (handles[i]->device != handles[0]->device || handles[i]->st.hash_kind != handles[0]->st.hash_kind)
+      return DD_EINVAL;
+    for (uint32_t k = 0; k < i; k++)
+      if (handles[k] == handles[i]) return DD_EINVAL;
+  }
+  dd_batch* b = new (std::nothrow) dd_batch;
+  if (!b) return DD_ENOMEM;
+  b->device = handles[0]->device;
+  b->hs.assign(handles, handles + n);
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) {
+    delete b;
+    return DD_EHIP;
+  }
+  *out = b;
+  return DD_OK;
+}
+
+dd_status dd_batch_destroy(dd_batch* b) {
+  if (!b) return DD_EINVAL;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return DD_OK;
+}
+
+dd_status dd_batch_use_graph(dd_batch* b, int enable) {
+  if (!b) return DD_EINVAL;
+  b->use_graph = enable != 0;
+  return DD_OK;
+}
+
+dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, const uint64_t* lens,
+                                     const int* make_baseline, void* const* dev_out, const uint64_t* out_caps,
+                                     uint64_t* out_lens) {
+  if (!b || !dev_data || !lens || !make_baseline || !dev_out || !out_caps || !out_lens) return DD_EINVAL;
+  DD_HIP(hipSetDevice(b->device));
+  const size_t n = b->hs.size();
+  std::vector<JobDev> jobs(n);
+  std::vector<char> base(n);
+  for (size_t i = 0; i < n; i++) {
+    if (!dev_data[i] || !dev_out[i]) return DD_EINVAL;
+    StreamState& st = b->hs[i]->st;
+    base[i] = make_baseline[i] || !st.have_tree || st.current_id == 0;
+    DD_TRY(st.prepare(static_cast<const uint8_t*>(dev_data[i]), lens[i], base[i], static_cast<uint8_t*>(dev_out[i]),
+                      out_caps[i], jobs[i], b->stream));
+  }
+  DD_TRY(b->pipe.upload(jobs, b->stream));
+  if (b->use_graph) {
+    std::vector<uint64_t> sig = b->pipe.signature();
+    if (!b->exec || sig != b->sig) {
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      DD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const dd_status rc = b->pipe.launch_all(b->stream, false);
+      const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+      if (rc != DD_OK) return rc;
+      if (ec != hipSuccess) return DD_EHIP;
+      DD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->sig = sig;
+    }
+    DD_HIP(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    DD_TRY(b->pipe.launch_all(b->stream, false));
+  }
+  DD_HIP(hipStreamSynchronize(b->stream));
+  std::vector<JobCounters> c(b->pipe.host_ctr, b->pipe.host_ctr + n);
+  for (int attempt = 0; attempt < 4; attempt++) {
+    std::vector<size_t> redo;
+    for (size_t i = 0; i < n; i++)
+      if (c[i].flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) redo.push_back(i);
+    if (redo.empty()) break;
+    std::vector<JobDev> sub;
+    for (size_t i : redo) {
+      StreamState& st = b->hs[i]->st;
+      DD_TRY(st.grow_map(c[i].map_count + 2 * c[i].pending, b->stream));
+      jobs[i].map = make_view(st.map.p, st.map_cap);
+      jobs[i].map_count_at_start = c[i].map_count;
+      jobs[i].map_limit = map_limit_of(st.map_cap);
+      sub.push_back(jobs[i]);
+    }
+    DD_TRY(b->pipe_rerun.run(sub, b->stream, true));
+    DD_HIP(hipStreamSynchronize(b->stream));
+    for (size_t k = 0; k < redo.size(); k++) c[redo[k]] = b->pipe_rerun.host_ctr[k];
+  }
+  for (size_t i = 0; i < n; i++) {
+    if (c[i].flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return DD_ENOMEM;
+    if (c[i].flags & FLAG_BAD_SOURCE) return DD_ECORRUPT;
+  }
+  for (size_t i = 0; i < n; i++) {
+    StreamState& st = b->hs[i]->st;
+    st.commit(c[i], base[i]);
+    out_lens[i] = c[i].diff_len;
+    st.current_id += 1;
+  }
+  return DD_OK;
+}
+
 dd_status dd_set_profiling(dd_dedup* h, int enable) {
   if (!h) return DD_EINVAL;
-  h->profile = enable != 0;
+  h->profile = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
   return DD_OK;
 }
 

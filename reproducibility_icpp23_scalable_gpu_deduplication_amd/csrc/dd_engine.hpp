@@ -38,6 +38,9 @@ struct StreamState {
   DevBuf map;
   uint64_t map_cap = 0, map_count = 0;
   DevBuf blk, f_list, f_off, s_node, s_prev, s_tree, ss_node, ss_prev, s_hist, ptab;
+  DevBuf leafblk, pend_list, tiles, tile_epoch, lists;
+  uint32_t epoch = 0;     // stamp of the current checkpoint (tile_epoch)
+  uint64_t tile_epoch_n = 0;
   DevBuf diff;  // internal diff buffer (host-output path)
   DevBuf ctr;   // JobCounters
 
@@ -66,20 +69,33 @@ class Pipeline {
   // the stages; `ev_dedup` is recorded after the duplicate forest.
   dd_status run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, hipEvent_t ev_dedup = nullptr,
                 hipEvent_t* stage_ev = nullptr);
+  // The two halves of run(): upload the job table (a copy on `s`), then
+  // enqueue the kernels.  launch_all() enqueues nothing but kernels and event
+  // records, so it can be captured into a hipGraph.
+  dd_status upload(const std::vector<JobDev>& jobs, hipStream_t s);
+  dd_status launch_all(hipStream_t s, bool rerun, hipEvent_t ev_dedup = nullptr, hipEvent_t* stage_ev = nullptr);
+  // Identifies the launch sequence of the last upload (grids, kernel
+  // variants, table address): a captured graph is valid while it is equal.
+  std::vector<uint64_t> signature() const;
+  // Waits for the stream; the kernels have written every job's counters to
+  // host_ctr[j] (host-mapped pinned memory).
   dd_status fetch_counters(const std::vector<JobDev>& jobs, hipStream_t s);
   JobCounters* host_ctr = nullptr;
+  int profile_level_ = 0;
 
  private:
-  dd_status upload(const std::vector<JobDev>& jobs, hipStream_t s);
-  DevBuf dev_jobs_, dev_prefix_;
-  JobDev* host_jobs_ = nullptr;
-  uint32_t* host_prefix_ = nullptr;
-  size_t host_jobs_cap_ = 0, host_prefix_cap_ = 0, host_ctr_cap_ = 0;
+  uint32_t njobs_ = 0;
+  DevBuf dev_table_;           // job table followed by the stage prefixes
+  uint8_t* host_table_ = nullptr;  // pinned staging of dev_table_
+  size_t host_table_cap_ = 0, host_ctr_cap_ = 0;
+  JobDev* dev_jobs_ = nullptr;
+  uint32_t* dev_prefix_ = nullptr;
   std::vector<uint32_t> off_;  // offset of each stage's prefix in dev_prefix_
   std::vector<uint32_t> grid_;
   int max_groups_ = 0;
   bool aligned16_ = true, vec16_ = true;
   int hash_kind_ = 0;
+  int leaf_seg_ = 0;  // 0: lane-strided leaf kernel; 64/128/256: LDS-staged with that segment
 };
 
 uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
@@ -93,7 +109,7 @@ struct dd_dedup {
   dd::StreamState st;
   dd::Pipeline pipe;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  bool profile = false;
+  int profile = 0;  // 0 off, 1 leaf-hash events only, 2 every stage
   hipEvent_t pev[DD_NUM_STAGES + 1] = {};
   double stage_ms[DD_NUM_STAGES] = {};
   // restart scratch
@@ -101,4 +117,15 @@ struct dd_dedup {
   uint32_t stamp = 0;
   dd::DevBuf rs_ctr;
   dd::JobCounters* rs_ctr_host = nullptr;
+};
+
+struct dd_batch {
+  int device = 0;
+  std::vector<dd_dedup*> hs;
+  hipStream_t stream = nullptr;
+  dd::Pipeline pipe, pipe_rerun;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> sig;
 };

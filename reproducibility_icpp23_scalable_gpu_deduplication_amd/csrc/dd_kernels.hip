@@ -75,6 +75,37 @@ __device__ __forceinline__ T block_excl_scan(T v, T* sh, T& total) {
 
 __device__ __forceinline__ void store_u32(uint8_t* p, uint32_t v) { *reinterpret_cast<uint32_t*>(p) = v; }
 
+// Rank of this lane among the lanes of the block whose `pred` holds (-1 if
+// it does not hold); `total` gets their number.  Every lane must call it.
+__device__ __forceinline__ int block_compact(bool pred, uint32_t* sh, uint32_t& total) {
+  const unsigned long long b = __ballot(pred);
+  const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane_id() == 0) sh[w] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (uint32_t i = 0; i < nw; i++) {
+    const uint32_t c = sh[i];
+    base += i < w ? c : 0;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return pred ? (int)(base + __popcll(b & lanemask_lt())) : -1;
+}
+
+// Sum over the block (every lane must call it).
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
+  v = wave_sum(v);
+  const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane_id() == 0) sh[w] = v;
+  __syncthreads();
+  T t = 0;
+  for (uint32_t i = 0; i < nw; i++) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
 // ---------------------------------------------------------------------------
 // begin: zero the counters of every job (or only the flags, for a re-run of
 // the map inserts after the map has grown).
@@ -85,7 +116,12 @@ __global__ void k_begin(const JobDev* jobs, uint32_t njobs, int rerun) {
   JobCounters* c = jobs[j].ctr;
   c->map_count = jobs[j].map_count_at_start;
   c->flags = 0;
-  if (rerun) return;  // keep `pending` from leaf_hash when re-running the inserts
+  if (rerun) return;  // keep the changed-leaf counts of leaf_hash when re-running the inserts
+  for (uint32_t i = 0; i < 32; i++) {
+    jobs[j].pend_acc[i] = 0;
+    jobs[j].dirty_cnt[i] = 0;
+    jobs[j].act_cnt[i] = 0;
+  }
   c->pending = 0;
   c->nfirst = c->nshift = c->nprior = 0;
   c->dchunks = 0;
@@ -99,6 +135,80 @@ __global__ void k_begin(const JobDev* jobs, uint32_t njobs, int rerun) {
 // Reference: "Checkpoint K: Leaves", src/tree_approach.cpp:305-351 and
 // include/reference_impl.hpp:43-71.
 // ---------------------------------------------------------------------------
+// The changed leaves of a leaf block (256 chunks) go to its segment of
+// pend_list and their number to blk_pend[b].  The job's total is accumulated
+// in 32 counters picked by block index: one counter hit by every workgroup
+// would serialize (32K same-address atomics per GiB at 128-byte chunks).
+constexpr uint32_t kPendSpread = 32;
+
+// Group-0 tile of a leaf (leaves sit on the deepest level D or on D-1).
+__device__ __forceinline__ uint32_t leaf_tile(uint32_t D, uint64_t leaf) {
+  const uint32_t Lr0 = D > (uint32_t)kTileLevels ? D - kTileLevels : 0;
+  const uint32_t lv = ilog2_64(leaf + 1);
+  return (uint32_t)(((leaf + 1) >> (lv - Lr0)) - (1ull << Lr0));
+}
+
+__device__ __forceinline__ void list_append(uint32_t* cnt, uint32_t* list, uint32_t cap, uint32_t item) {
+  const uint32_t r = item % kPendSpread;
+  const uint32_t k = atomicAdd(&cnt[r], 1u);
+  list[(uint64_t)r * cap + k] = item;
+}
+
+__device__ __forceinline__ void record_pending(const JobDev& J, uint32_t b, uint64_t k, bool pending) {
+  __shared__ uint32_t sh[4];
+  uint32_t tot;
+  const int pos = block_compact(pending, sh, tot);
+  if (pos >= 0) J.pend_list[(uint64_t)b * 256 + pos] = (uint32_t)k;
+  if (threadIdx.x == 0) {
+    J.blk_pend[b] = tot;
+    if (tot) {
+      atomicAdd(&J.pend_acc[b % kPendSpread], (unsigned long long)tot);
+      list_append(J.act_cnt, J.act_list, J.act_cap, b);
+    }
+  }
+  // Mark the group-0 tiles of changed leaves: the first marker of a tile in
+  // this checkpoint appends it to the dirty list.
+  uint32_t T = pending ? leaf_tile(J.D, J.C - 1 + k) : 0;
+  bool todo = pending;
+  while (true) {
+    const unsigned long long act = __ballot(todo);
+    if (!act) break;
+    const uint32_t leader = __ffsll((long long)act) - 1;
+    const uint32_t lt = __shfl(T, leader, 64);
+    if (lane_id() == leader) {
+      if (atomicExch(&J.tile_epoch[lt], J.epoch) != J.epoch) list_append(J.dirty_cnt, J.dirty_list, J.dirty_cap, lt);
+    }
+    if (todo && T == lt) todo = false;
+  }
+}
+
+// Item i of a 32-way split work list: sub-list counts are prefix-summed in
+// LDS by the caller (pre[r] = items before sub-list r, pre[32] = total).
+__device__ __forceinline__ void load_list_prefix(const uint32_t* cnt, uint32_t* pre) {
+  if (threadIdx.x == 0) {
+    uint32_t a = 0;
+    for (uint32_t r = 0; r < kPendSpread; r++) {
+      pre[r] = a;
+      a += cnt[r];
+    }
+    pre[kPendSpread] = a;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t list_item(const uint32_t* list, uint32_t cap, const uint32_t* pre, uint32_t i) {
+  uint32_t r = 0;
+  while (r + 1 < kPendSpread && pre[r + 1] <= i) r++;
+  return list[(uint64_t)r * cap + (i - pre[r])];
+}
+
+__device__ __forceinline__ uint64_t pending_total(const JobDev& J) {
+  uint64_t t = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kPendSpread; i++) t += J.pend_acc[i];
+  return t;
+}
+
 template <int Kind, bool Aligned16>
 __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
@@ -134,8 +244,111 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* jobs, const uin
     if (pending) J.tree[leaf] = d;
     J.labels[leaf] = lab;
   }
-  unsigned long long b = __ballot(pending);
-  if (lane_id() == 0 && b) atomicAdd(&J.ctr->pending, (unsigned long long)__popcll(b));
+  record_pending(J, blockIdx.x - prefix[j], k, pending);
+}
+
+// Shared tail of both leaf kernels: compare with the previous digest, record
+// the label, write the new digest when it changed.
+__device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const Digest& d) {
+  const uint64_t leaf = J.C - 1 + k;
+  Digest prev{0, 0};
+  if (!J.is_baseline) prev = J.tree[leaf];
+  bool pending;
+  uint8_t lab;
+  if (digest_eq(d, prev)) {
+    lab = J.is_baseline ? (uint8_t)(FIXED_DUPL | PENDING) : (uint8_t)FIXED_DUPL;
+    pending = J.is_baseline;
+  } else {
+    lab = PENDING;
+    pending = true;
+  }
+  if (pending) J.tree[leaf] = d;
+  J.labels[leaf] = lab;
+  return pending;
+}
+
+// leaf_hash, LDS-staged: each wavefront owns 64 consecutive chunks.  Per
+// round it loads SEG bytes of every chunk with contiguous 16-byte loads
+// (1 KiB per instruction when SEG >= 128), stores them to LDS with a 16-byte
+// pad per chunk (conflict-free 128-bit reads), and each lane then hashes the
+// next SEG bytes of its own chunk out of LDS.  Needs chunk % SEG == 0 and a
+// 16-byte aligned buffer; the partial last chunk is hashed from global memory.
+template <int Kind, int SEG>
+__global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+  constexpr int PITCH = SEG + 16;
+  constexpr int VPC = SEG / 16;        // 16-byte vectors per chunk per round
+  constexpr int LOADS = VPC;           // 64 chunks x VPC vectors spread over 64 lanes
+  __shared__ __attribute__((aligned(16))) uint8_t lds[4][64 * PITCH];
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobDev& J = jobs[j];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t k0 = ((uint64_t)(blockIdx.x - prefix[j]) * 4 + wave) * 64;  // first chunk of this wave
+  const uint64_t k = k0 + lane;
+  const uint64_t cs = J.chunk_size;
+  const uint64_t full = J.len / cs;  // chunks that are complete
+  uint8_t* L = lds[wave];
+  const uint8_t* base = J.data + k0 * cs;
+  const uint32_t rounds = (uint32_t)(cs / SEG);
+  Murmur m;
+  uint32_t st[4];
+  md5_init(st);
+  for (uint32_t r = 0; r < rounds; r++) {
+    // cooperative load: vector v of this round covers chunk v / VPC, piece v % VPC
+#pragma unroll
+    for (int i = 0; i < LOADS; i++) {
+      const uint32_t v = i * 64 + lane;
+      const uint32_t c = v / VPC, piece = v % VPC;
+      if (k0 + c < full) {
+        const uint4 x = *reinterpret_cast<const uint4*>(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16);
+        *reinterpret_cast<uint4*>(L + c * PITCH + piece * 16) = x;
+      }
+    }
+    __syncthreads();
+    if (k < full) {
+      const uint8_t* my = L + lane * PITCH;
+      if (Kind == HASH_MD5) {
+#pragma unroll
+        for (int b = 0; b < SEG / 64; b++) {
+          uint32_t X[16];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const uint4 w = *reinterpret_cast<const uint4*>(my + 64 * b + 16 * q);
+            X[4 * q] = w.x; X[4 * q + 1] = w.y; X[4 * q + 2] = w.z; X[4 * q + 3] = w.w;
+          }
+          md5_compress(st, X);
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < VPC; b++) {
+          const uint4 w = *reinterpret_cast<const uint4*>(my + 16 * b);
+          m.block(w.x | ((uint64_t)w.y << 32), w.z | ((uint64_t)w.w << 32));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  bool pending = false;
+  if (k < full) {
+    Digest d;
+    if (Kind == HASH_MD5) {
+      uint32_t X[16] = {0x80u, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      const uint64_t bits = cs * 8;
+      X[14] = (uint32_t)bits;
+      X[15] = (uint32_t)(bits >> 32);
+      md5_compress(st, X);
+      d = md5_digest(st);
+    } else {
+      d = m.finish(cs);
+    }
+    pending = leaf_finish(J, k, d);
+  } else if (k < J.C) {  // the partial last chunk
+    const uint64_t off = k * cs;
+    const uint8_t* p = J.data + off;
+    const uint64_t nb = J.len - off;
+    const Digest d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
+    pending = leaf_finish(J, k, d);
+  }
+  record_pending(J, blockIdx.x - prefix[j], k, pending);
 }
 
 // ---------------------------------------------------------------------------
@@ -150,89 +363,186 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* jobs, const uin
 // highest (interior nodes, UseMax=true), which is the first one the serial
 // reference visits (include/reference_impl.hpp:43-82).
 // ---------------------------------------------------------------------------
-template <bool UseMax>
-__device__ inline bool map_insert(const JobDev& J, const Digest& k, uint32_t node, uint64_t* slot_out, bool* claimed) {
+struct InsertResult {
+  bool ok;                  // found or inserted
+  bool claimed;             // this call created the entry
+  uint64_t slot;
+  unsigned long long prev;  // entry value before this call's atomicMin/Max (EMPTY when claimed)
+};
+
+// Leaf = true: leaf inserts (lowest node wins); false: interior inserts
+// (highest node wins).  During the leaf kernel every entry of this
+// checkpoint was claimed by a leaf of the same kernel, so its key is read from
+// the tree; during the interior kernel, leaf entries already have their key
+// in the slot and only interior entries need the tree.
+template <bool Leaf>
+__device__ inline InsertResult map_insert(const JobDev& J, const Digest& k, uint32_t node) {
   const MapView& m = J.map;
   const unsigned long long mine = pack_node(J.ckpt_id, node);
   uint64_t s = map_start(m, k) & m.mask;
-  *claimed = false;
   for (uint64_t it = 0; it <= m.mask; it++) {
     Slot* sl = &m.slots[s];
     unsigned long long v = __hip_atomic_load(&sl->val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (v == SLOT_EMPTY) {
-      unsigned long long old = atomicCAS(&sl->val, SLOT_EMPTY, mine);
+      const unsigned long long old = atomicCAS(&sl->val, SLOT_EMPTY, mine);
       if (old == SLOT_EMPTY) {
         sl->k0 = k.k0;
         sl->k1 = k.k1;
-        *slot_out = s;
-        *claimed = true;
-        return true;
+        return InsertResult{true, true, s, SLOT_EMPTY};
       }
       v = old;
     }
-    const uint32_t vt = (uint32_t)(v >> 32);
+    const uint32_t vt = (uint32_t)(v >> 32), vn = (uint32_t)v;
+    const bool this_kernel = vt == J.ckpt_id && (Leaf || vn < J.C - 1);
     Digest e;
-    if (vt == J.ckpt_id) {
-      e = J.tree[(uint32_t)v];
+    if (this_kernel) {
+      e = J.tree[vn];
     } else {
       e.k0 = sl->k0;
       e.k1 = sl->k1;
     }
     if (digest_eq(e, k)) {
-      if (vt == J.ckpt_id) {
-        if (UseMax) atomicMax(&sl->val, mine);
-        else atomicMin(&sl->val, mine);
-      }
-      *slot_out = s;
-      return true;
+      unsigned long long prev = v;
+      if (vt == J.ckpt_id) prev = Leaf ? atomicMin(&sl->val, mine) : atomicMax(&sl->val, mine);
+      return InsertResult{true, false, s, prev};
     }
     s = (s + 1) & m.mask;
   }
-  return false;
+  return InsertResult{false, false, 0, 0};
 }
 
 __device__ __forceinline__ bool map_guard_tripped(const JobDev& J) {
-  // Worst case of this checkpoint: every pending leaf and one interior node
-  // per pending leaf become new entries.
-  return J.map_count_at_start + 2 * J.ctr->pending > J.map_limit;
+  // Worst case of this checkpoint: one entry per changed leaf plus one per
+  // first-occurrence interior node (fewer than the changed leaves).
+  return J.map_count_at_start + 2 * pending_total(J) > J.map_limit;
 }
 
+// Workgroups stride over the leaf blocks that hold changed leaves.  Labels
+// settle without a later map read: a changed leaf stays PENDING
+// (first-occurrence candidate) unless it meets an older entry or a lower leaf
+// of this checkpoint (then it labels itself SHIFT), and a leaf that lowers an
+// entry demotes the previous lowest leaf to SHIFT.  Every leaf that is ever
+// the lowest is demoted by the next lower one, so exactly the serial
+// reference's first occurrence stays PENDING (include/reference_impl.hpp:49-67).
+// FIXED|PENDING leaves (baseline, zero digest) take part in the ordering but
+// keep their label.
 __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
-  if (map_guard_tripped(J)) {
-    if (blockIdx.x == prefix[j] && threadIdx.x == 0) atomicOr(&J.ctr->flags, FLAG_NEED_GROW);
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
+  uint32_t claims = 0;
+  __shared__ uint32_t sh[4];
+  __shared__ uint32_t pre[kPendSpread + 1];
+  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) {
+    if (threadIdx.x == 0) J.blk_claim[wg] = 0;
     return;
   }
-  const uint64_t k = (uint64_t)(blockIdx.x - prefix[j]) * blockDim.x + threadIdx.x;
-  bool claimed = false;
-  if (k < J.C) {
-    const uint64_t leaf = J.C - 1 + k;
-    if (J.labels[leaf] & PENDING) {
-      uint64_t s;
-      if (map_insert<false>(J, J.tree[leaf], (uint32_t)leaf, &s, &claimed)) {
-        J.slot_of[leaf] = (uint32_t)s;
-      } else {
-        atomicOr(&J.ctr->flags, FLAG_MAP_FULL);
+  if (map_guard_tripped(J)) {
+    if (threadIdx.x == 0) {
+      atomicOr(&J.ctr->flags, FLAG_NEED_GROW);
+      J.blk_claim[wg] = 0;
+    }
+    return;
+  }
+  load_list_prefix(J.act_cnt, pre);
+  const uint32_t nitems = pre[kPendSpread];
+  const uint32_t K = J.ckpt_id;
+  for (uint32_t it = wg; it < nitems; it += nwg) {
+    const uint32_t b = list_item(J.act_list, J.act_cap, pre, it);
+    const uint32_t n = J.blk_pend[b];
+    if (threadIdx.x >= n) continue;
+    const uint64_t leaf = J.C - 1 + J.pend_list[(uint64_t)b * 256 + threadIdx.x];
+    const uint8_t own = J.labels[leaf];
+    const InsertResult r = map_insert<true>(J, J.tree[leaf], (uint32_t)leaf);
+    if (!r.ok) {
+      atomicOr(&J.ctr->flags, FLAG_MAP_FULL);
+      continue;
+    }
+    J.slot_of[leaf] = (uint32_t)r.slot;
+    if (r.claimed) {
+      claims++;
+      continue;
+    }
+    const uint32_t pt = (uint32_t)(r.prev >> 32), pn = (uint32_t)r.prev;
+    bool self_shift = true;
+    if (pt == K) {
+      if (pn > leaf) {
+        self_shift = false;  // this leaf is now the lowest; the previous lowest is not
+        if (J.labels[pn] == PENDING) J.labels[pn] = SHIFT_DUPL;
+      } else if (pn == leaf) {
+        self_shift = false;  // re-run after a map growth: own entry
       }
     }
+    if (self_shift && own == PENDING) J.labels[leaf] = SHIFT_DUPL;
   }
-  unsigned long long b = __ballot(claimed);
-  if (lane_id() == 0 && b) atomicAdd(&J.ctr->map_count, (unsigned long long)__popcll(b));
+  const uint32_t n = block_sum<uint32_t>(claims, sh);
+  if (threadIdx.x == 0) J.blk_claim[wg] = n;
 }
 
+__device__ __forceinline__ int tile_groups(uint32_t D) { return D == 0 ? 1 : (int)((D + kTileLevels - 1) / kTileLevels); }
+
+// Tile geometry shared by the tile kernels: global tile index T -> group,
+// bottom/top levels and root node.
+struct TileRef {
+  int g, Lb, Lr, H;
+  uint64_t r;
+};
+
+__device__ __forceinline__ TileRef tile_ref(const JobDev& J, int g, uint64_t t) {
+  TileRef tr;
+  tr.g = g;
+  tr.Lb = (int)J.D - kTileLevels * g;
+  tr.Lr = tr.Lb > kTileLevels ? tr.Lb - kTileLevels : 0;
+  tr.H = tr.Lb - tr.Lr;
+  tr.r = ((1ull << tr.Lr) - 1) + t;
+  return tr;
+}
+
+// Interior node i (level-major, 0 = tile root) of a tile with root r.
+__device__ __forceinline__ uint64_t tile_node(uint64_t r, uint32_t i) {
+  const uint32_t q = 31 - __clz(i + 1);
+  return ((r + 1) << q) - 1 + (i + 1 - (1u << q));
+}
+
+// Work items of the interior inserts and of the upper tile groups: the dirty
+// group-0 tiles, then every tile of the groups above.
 __global__ __launch_bounds__(256) void k_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
-  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  const uint64_t x = (uint64_t)(blockIdx.x - prefix[j]) * blockDim.x + threadIdx.x;
-  bool claimed = false;
-  if (x + 1 < J.C && J.labels[x] == FIRST_OCUR) {
-    uint64_t s;
-    if (!map_insert<true>(J, J.tree[x], (uint32_t)x, &s, &claimed)) atomicOr(&J.ctr->flags, FLAG_MAP_FULL);
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
+  __shared__ uint32_t sh[4];
+  __shared__ uint32_t pre[kPendSpread + 1];
+  uint32_t claims = 0;
+  if (!(J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
+    load_list_prefix(J.dirty_cnt, pre);
+    const uint32_t nd = pre[kPendSpread];
+    const uint32_t ntiles0 = 1u << (J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0);
+    const uint32_t nup = J.ntiles - ntiles0;  // tiles of the groups above group 0
+    for (uint32_t it = wg; it < nd + nup; it += nwg) {
+      uint32_t T;
+      int g = 0;
+      uint64_t t;
+      if (it < nd) {
+        T = list_item(J.dirty_list, J.dirty_cap, pre, it);
+        t = T;
+      } else {
+        T = ntiles0 + (it - nd);
+        g = 1;
+        while (g + 1 < 4 && g + 1 < tile_groups(J.D) && J.tile_base[g + 1] <= T) g++;
+        t = T - J.tile_base[g];
+      }
+      if (J.tile_fint[T] == 0) continue;
+      const TileRef tr = tile_ref(J, g, t);
+      if (threadIdx.x + 1 >= (1u << tr.H)) continue;  // 2^H - 1 interior nodes
+      const uint64_t x = tile_node(tr.r, threadIdx.x);
+      if (x + 1 >= J.C || J.labels[x] != FIRST_OCUR) continue;
+      const InsertResult r = map_insert<false>(J, J.tree[x], (uint32_t)x);
+      if (!r.ok) atomicOr(&J.ctr->flags, FLAG_MAP_FULL);
+      claims += r.claimed ? 1 : 0;
+    }
   }
-  unsigned long long b = __ballot(claimed);
-  if (lane_id() == 0 && b) atomicAdd(&J.ctr->map_count, (unsigned long long)__popcll(b));
+  const uint32_t n = block_sum<uint32_t>(claims, sh);
+  if (threadIdx.x == 0) J.tile_claim[wg] = n;
 }
 
 // ---------------------------------------------------------------------------
@@ -242,75 +552,98 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobDev* jobs, con
 // except leaves one level above the deepest (chunk counts that are not a power
 // of two), which are loaded too.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int tile_groups(uint32_t D) { return D == 0 ? 1 : (int)((D + kTileLevels - 1) / kTileLevels); }
 
-// Label of a leaf after the map inserts: FIXED, FIRST when the map names this
-// leaf as the checkpoint's first occurrence, SHIFT otherwise.  Written back.
+// Label of a leaf after the map inserts: a leaf still PENDING is the
+// checkpoint's first occurrence of its digest; FIXED|PENDING is FIXED.
+// Written back.
 __device__ __forceinline__ uint8_t resolve_leaf(const JobDev& J, uint64_t leaf) {
-  uint8_t raw = J.labels[leaf];
+  const uint8_t raw = J.labels[leaf];
   if (!(raw & PENDING)) return raw;
-  uint8_t lab;
-  if (raw == (uint8_t)(FIXED_DUPL | PENDING)) {
-    lab = FIXED_DUPL;
-  } else {
-    unsigned long long v = J.map.slots[J.slot_of[leaf]].val;
-    lab = (v == pack_node(J.ckpt_id, (uint32_t)leaf)) ? FIRST_OCUR : SHIFT_DUPL;
-  }
+  const uint8_t lab = raw == PENDING ? FIRST_OCUR : FIXED_DUPL;
   J.labels[leaf] = lab;
   return lab;
 }
 
+// A group-1 tile's bottom nodes are group-0 tile roots.  When no leaf of
+// such a group-0 tile changed, the tile was not visited this checkpoint and
+// its stored labels are stale: every node of it is FIXED.
+__device__ __forceinline__ bool child_tile_clean(const JobDev& J, int g, uint64_t x) {
+  if (g != 1) return false;
+  const uint32_t Lr0 = J.D - kTileLevels;  // group 1 exists only when D > kTileLevels
+  return J.tile_epoch[x - ((1ull << Lr0) - 1)] != J.epoch;
+}
+
+__device__ __forceinline__ uint32_t group_items(const JobDev& J, int g, uint32_t* pre) {
+  if (g == 0) {
+    load_list_prefix(J.dirty_cnt, pre);
+    return pre[kPendSpread];
+  }
+  const int Lb = (int)J.D - kTileLevels * g;
+  return 1u << (Lb > kTileLevels ? Lb - kTileLevels : 0);
+}
+
 // Pass 1, the first-occurrence forest (include/reference_impl.hpp:73-82).
+// Group 0 visits the dirty tiles only; the groups above visit every tile.
 __global__ __launch_bounds__(256) void k_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   if (g >= tile_groups(J.D)) return;
-  const int Lb = (int)J.D - kTileLevels * g;
-  const int Lr = Lb > kTileLevels ? Lb - kTileLevels : 0;
-  const int H = Lb - Lr;
-  const uint64_t r = ((1ull << Lr) - 1) + (blockIdx.x - prefix[j]);
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint8_t lab[2][256];
   __shared__ Digest dig[2][256];
+  __shared__ uint32_t pre[kPendSpread + 1];
+  const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
-  const uint32_t width = 1u << H;
-  if (i < width) {
-    const uint64_t x = ((r + 1) << H) - 1 + i;
-    uint8_t L = 0xff;
-    Digest d{0, 0};
-    if (x < J.N) {
-      L = (g == 0) ? resolve_leaf(J, x) : J.labels[x];
-      if (L == FIRST_OCUR) d = J.tree[x];
-    }
-    lab[0][i] = L;
-    dig[0][i] = d;
-  }
-  int cur = 0;
-  for (int st = 1; st <= H; st++) {
-    __syncthreads();
-    const uint32_t w = width >> st;
-    if (i < w) {
-      const uint64_t x = ((r + 1) << (H - st)) - 1 + i;
-      uint8_t L;
+  for (uint32_t it = wg; it < nitems; it += nwg) {
+    const uint64_t t = g == 0 ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
+    const TileRef tr = tile_ref(J, g, t);
+    const int H = tr.H;
+    const uint64_t r = tr.r;
+    const uint32_t width = 1u << H;
+    if (i < width) {
+      const uint64_t x = ((r + 1) << H) - 1 + i;
+      uint8_t L = 0xff;
       Digest d{0, 0};
-      if (x >= J.C - 1) {  // a leaf one level above the deepest
-        L = resolve_leaf(J, x);
+      if (x < J.N) {
+        if (g == 0) L = resolve_leaf(J, x);
+        else L = child_tile_clean(J, g, x) ? (uint8_t)DONE : J.labels[x];
         if (L == FIRST_OCUR) d = J.tree[x];
-      } else {
-        const uint8_t a = lab[cur][2 * i], b = lab[cur][2 * i + 1];
-        if (a == FIRST_OCUR && b == FIRST_OCUR) {
-          L = FIRST_OCUR;
-          d = hash_pair(J.hash_kind, dig[cur][2 * i], dig[cur][2 * i + 1]);
-          J.tree[x] = d;
-        } else {
-          L = DONE;
-        }
-        J.labels[x] = L;
       }
-      lab[cur ^ 1][i] = L;
-      dig[cur ^ 1][i] = d;
+      lab[0][i] = L;
+      dig[0][i] = d;
     }
-    cur ^= 1;
+    bool any_first = false;
+    int cur = 0;
+    for (int st = 1; st <= H; st++) {
+      __syncthreads();
+      const uint32_t w = width >> st;
+      if (i < w) {
+        const uint64_t x = ((r + 1) << (H - st)) - 1 + i;
+        uint8_t L;
+        Digest d{0, 0};
+        if (x >= J.C - 1) {  // a leaf one level above the deepest
+          L = resolve_leaf(J, x);
+          if (L == FIRST_OCUR) d = J.tree[x];
+        } else {
+          const uint8_t a = lab[cur][2 * i], b = lab[cur][2 * i + 1];
+          if (a == FIRST_OCUR && b == FIRST_OCUR) {
+            L = FIRST_OCUR;
+            d = hash_pair(J.hash_kind, dig[cur][2 * i], dig[cur][2 * i + 1]);
+            J.tree[x] = d;
+            any_first = true;
+          } else {
+            L = DONE;
+          }
+          J.labels[x] = L;
+        }
+        lab[cur ^ 1][i] = L;
+        dig[cur ^ 1][i] = d;
+      }
+      cur ^= 1;
+    }
+    const bool tf = __syncthreads_or(any_first);
+    if (i == 0) J.tile_fint[J.tile_base[g] + t] = tf ? 1u : 0u;
   }
 }
 
@@ -320,59 +653,65 @@ __global__ __launch_bounds__(256) void k_pass2(const JobDev* jobs, const uint32_
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   if (g >= tile_groups(J.D)) return;
-  const int Lb = (int)J.D - kTileLevels * g;
-  const int Lr = Lb > kTileLevels ? Lb - kTileLevels : 0;
-  const int H = Lb - Lr;
-  const uint64_t r = ((1ull << Lr) - 1) + (blockIdx.x - prefix[j]);
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint8_t lab[2][256];
   __shared__ Digest dig[2][256];
+  __shared__ uint32_t pre[kPendSpread + 1];
+  const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
-  const uint32_t width = 1u << H;
-  if (i < width) {
-    const uint64_t x = ((r + 1) << H) - 1 + i;
-    uint8_t L = 0xff;
-    Digest d{0, 0};
-    if (x < J.N) {
-      L = J.labels[x];
-      if (L == SHIFT_DUPL) d = J.tree[x];
-    }
-    lab[0][i] = L;
-    dig[0][i] = d;
-  }
-  int cur = 0;
-  for (int st = 1; st <= H; st++) {
-    __syncthreads();
-    const uint32_t w = width >> st;
-    if (i < w) {
-      const uint64_t x = ((r + 1) << (H - st)) - 1 + i;
-      uint8_t L;
+  for (uint32_t it = wg; it < nitems; it += nwg) {
+    const uint64_t t = g == 0 ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
+    const TileRef tr = tile_ref(J, g, t);
+    const int H = tr.H;
+    const uint64_t r = tr.r;
+    const uint32_t width = 1u << H;
+    if (i < width) {
+      const uint64_t x = ((r + 1) << H) - 1 + i;
+      uint8_t L = 0xff;
       Digest d{0, 0};
-      if (x >= J.C - 1) {
-        L = J.labels[x];
+      if (x < J.N) {
+        L = child_tile_clean(J, g, x) ? (uint8_t)FIXED_DUPL : J.labels[x];
         if (L == SHIFT_DUPL) d = J.tree[x];
-      } else {
-        const uint8_t a = lab[cur][2 * i], b = lab[cur][2 * i + 1];
-        if (a != b) {
-          L = DONE;
-        } else if (a == SHIFT_DUPL) {
-          d = hash_pair(J.hash_kind, dig[cur][2 * i], dig[cur][2 * i + 1]);
-          const uint64_t s = map_find(J.map, d);
-          if (s != ~0ull) {
-            L = SHIFT_DUPL;
-            J.tree[x] = d;
-            J.slot_of[x] = (uint32_t)s;
-          } else {
-            L = DONE;
-          }
-        } else {
-          L = a;  // both FIXED, both FIRST or both DONE
-        }
-        J.labels[x] = L;
       }
-      lab[cur ^ 1][i] = L;
-      dig[cur ^ 1][i] = d;
+      lab[0][i] = L;
+      dig[0][i] = d;
     }
-    cur ^= 1;
+    int cur = 0;
+    for (int st = 1; st <= H; st++) {
+      __syncthreads();
+      const uint32_t w = width >> st;
+      if (i < w) {
+        const uint64_t x = ((r + 1) << (H - st)) - 1 + i;
+        uint8_t L;
+        Digest d{0, 0};
+        if (x >= J.C - 1) {
+          L = J.labels[x];
+          if (L == SHIFT_DUPL) d = J.tree[x];
+        } else {
+          const uint8_t a = lab[cur][2 * i], b = lab[cur][2 * i + 1];
+          if (a != b) {
+            L = DONE;
+          } else if (a == SHIFT_DUPL) {
+            d = hash_pair(J.hash_kind, dig[cur][2 * i], dig[cur][2 * i + 1]);
+            const uint64_t s = map_find(J.map, d);
+            if (s != ~0ull) {
+              L = SHIFT_DUPL;
+              J.tree[x] = d;
+              J.slot_of[x] = (uint32_t)s;
+            } else {
+              L = DONE;
+            }
+          } else {
+            L = a;  // both FIXED, both FIRST or both DONE
+          }
+          J.labels[x] = L;
+        }
+        lab[cur ^ 1][i] = L;
+        dig[cur ^ 1][i] = d;
+      }
+      cur ^= 1;
+    }
+    __syncthreads();
   }
 }
 
@@ -392,18 +731,51 @@ __device__ __forceinline__ uint64_t leaf_count(const TreeGeom& g, uint64_t v) {
   return r.n0 + r.n1;
 }
 
-__global__ __launch_bounds__(256) void k_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
-  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
-  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  const uint32_t b = blockIdx.x - prefix[j];
-  const TreeGeom g = make_geom(J.C);
-  const uint64_t base = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
-  uint32_t f = 0, fc = 0, s = 0;
+// Node blocks of kNodesPerBlock ids; a workgroup owns a contiguous run of
+// them so that its counts are contiguous in node order.
+__device__ __forceinline__ void node_block_range(const JobDev& J, uint32_t wg, uint32_t nwg, uint32_t& b0,
+                                                 uint32_t& b1) {
+  const uint32_t per = (J.nblk + nwg - 1) / nwg;
+  b0 = min(J.nblk, wg * per);
+  b1 = min(J.nblk, b0 + per);
+}
+
+// Nodes of group-0 tiles with no changed leaf were not visited this
+// checkpoint: all of them are FIXED whatever their stored label says.
+__device__ __forceinline__ bool node_in_clean_tile(const JobDev& J, uint64_t v) {
+  const uint32_t Lr0 = J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0;
+  const uint32_t lv = ilog2_64(v + 1);
+  if (lv < Lr0) return false;
+  const uint64_t T0 = ((v + 1) >> (lv - Lr0)) - (1ull << Lr0);
+  return J.tile_epoch[T0] != J.epoch;
+}
+
+// True when some byte of the 16 labels is FIRST (0) or SHIFT (2).
+__device__ __forceinline__ bool any_root_candidate(const uint4 w) {
+  auto z = [](uint32_t x) {
+    x &= 0xfdfdfdfdu;  // FIRST and SHIFT become zero bytes
+    return ((x - 0x01010101u) & ~x & 0x80808080u) != 0;
+  };
+  return z(w.x) || z(w.y) || z(w.z) || z(w.w);
+}
+
+// Counts (first roots, chunks under them, shifted roots) of the 16 nodes
+// starting at v0; `kinds` receives each node's root kind (0xff: none).
+__device__ __forceinline__ void scan16(const JobDev& J, const TreeGeom& g, uint64_t v0, uint8_t kinds[16],
+                                       uint32_t& f, uint32_t& fc, uint32_t& s) {
+  f = fc = s = 0;
+#pragma unroll
+  for (int t = 0; t < 16; t++) kinds[t] = 0xff;
+  if (v0 >= J.N) return;
+  const uint4 w = *reinterpret_cast<const uint4*>(J.labels + v0);  // labels are padded past N
+  if (!any_root_candidate(w)) return;
+#pragma unroll
   for (int t = 0; t < 16; t++) {
-    const uint64_t v = base + t;
+    const uint64_t v = v0 + t;
     if (v >= J.N) break;
-    const uint8_t rk = root_kind(J.labels, v);
+    uint8_t rk = root_kind(J.labels, v);
+    if (rk != 0xff && node_in_clean_tile(J, v)) rk = 0xff;  // stale label of an unvisited tile
+    kinds[t] = rk;
     if (rk == FIRST_OCUR) {
       f++;
       fc += (uint32_t)leaf_count(g, v);
@@ -411,39 +783,53 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobDev* jobs, const u
       s++;
     }
   }
-  __shared__ uint32_t red[3][4];
-  f = wave_sum(f);
-  fc = wave_sum(fc);
-  s = wave_sum(s);
-  if (lane_id() == 0) {
-    red[0][threadIdx.x >> 6] = f;
-    red[1][threadIdx.x >> 6] = fc;
-    red[2][threadIdx.x >> 6] = s;
+}
+
+__global__ __launch_bounds__(256) void k_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobDev& J = jobs[j];
+  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
+  uint32_t b0, b1;
+  node_block_range(J, wg, nwg, b0, b1);
+  const TreeGeom g = make_geom(J.C);
+  uint32_t f = 0, fc = 0, s = 0;
+  for (uint32_t b = b0; b < b1; b++) {
+    uint8_t kinds[16];
+    uint32_t a, c, d;
+    scan16(J, g, (uint64_t)b * kNodesPerBlock + threadIdx.x * 16, kinds, a, c, d);
+    f += a;
+    fc += c;
+    s += d;
   }
-  __syncthreads();
+  __shared__ uint32_t sh[4];
+  f = block_sum(f, sh);
+  fc = block_sum(fc, sh);
+  s = block_sum(s, sh);
   if (threadIdx.x == 0) {
-    J.blk_f[b] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    J.blk_fc[b] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-    J.blk_s[b] = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+    J.blk_f[wg] = f;
+    J.blk_fc[wg] = fc;
+    J.blk_s[wg] = s;
   }
 }
 
-// One workgroup per job: exclusive scans of the per-block counts.
-__global__ __launch_bounds__(1024) void k_roots_scan(const JobDev* jobs) {
+// One workgroup per job: exclusive scans of the per-workgroup counts.
+__global__ __launch_bounds__(1024) void k_roots_scan(const JobDev* jobs, const uint32_t* prefix) {
   const JobDev& J = jobs[blockIdx.x];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
+  const uint32_t nwg = prefix[blockIdx.x + 1] - prefix[blockIdx.x];
   __shared__ uint32_t sh[17];
   uint32_t carry_f = 0, carry_fc = 0, carry_s = 0;
-  for (uint32_t base = 0; base < J.nblk; base += 1024) {
+  for (uint32_t base = 0; base < nwg; base += 1024) {
     const uint32_t i = base + threadIdx.x;
-    uint32_t f = i < J.nblk ? J.blk_f[i] : 0;
-    uint32_t fc = i < J.nblk ? J.blk_fc[i] : 0;
-    uint32_t s = i < J.nblk ? J.blk_s[i] : 0;
+    const uint32_t f = i < nwg ? J.blk_f[i] : 0;
+    const uint32_t fc = i < nwg ? J.blk_fc[i] : 0;
+    const uint32_t s = i < nwg ? J.blk_s[i] : 0;
     uint32_t tf, tfc, ts;
-    uint32_t ef = block_excl_scan<16>(f, sh, tf);
-    uint32_t efc = block_excl_scan<16>(fc, sh, tfc);
-    uint32_t es = block_excl_scan<16>(s, sh, ts);
-    if (i < J.nblk) {
+    const uint32_t ef = block_excl_scan<16>(f, sh, tf);
+    const uint32_t efc = block_excl_scan<16>(fc, sh, tfc);
+    const uint32_t es = block_excl_scan<16>(s, sh, ts);
+    if (i < nwg) {
       J.blk_f[i] = carry_f + ef;
       J.blk_fc[i] = carry_fc + efc;
       J.blk_s[i] = carry_s + es;
@@ -463,43 +849,40 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobDev* jobs, const u
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  const uint32_t b = blockIdx.x - prefix[j];
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
+  uint32_t b0, b1;
+  node_block_range(J, wg, nwg, b0, b1);
   const TreeGeom g = make_geom(J.C);
-  const uint64_t base = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
-  uint8_t kinds[16];
-  uint32_t f = 0, fc = 0, s = 0;
-#pragma unroll
-  for (int t = 0; t < 16; t++) {
-    const uint64_t v = base + t;
-    uint8_t rk = 0xff;
-    if (v < J.N) rk = root_kind(J.labels, v);
-    kinds[t] = rk;
-    if (rk == FIRST_OCUR) {
-      f++;
-      fc += (uint32_t)leaf_count(g, v);
-    } else if (rk == SHIFT_DUPL) {
-      s++;
-    }
-  }
+  uint32_t run_f = J.blk_f[wg], run_fc = J.blk_fc[wg], run_s = J.blk_s[wg];
   __shared__ uint32_t sh[5];
-  uint32_t tot;
-  uint32_t pf = block_excl_scan<4>(f, sh, tot) + J.blk_f[b];
-  uint32_t pfc = block_excl_scan<4>(fc, sh, tot) + J.blk_fc[b];
-  uint32_t ps = block_excl_scan<4>(s, sh, tot) + J.blk_s[b];
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint64_t v0 = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
+    uint8_t kinds[16];
+    uint32_t f, fc, s;
+    scan16(J, g, v0, kinds, f, fc, s);
+    uint32_t tf, tfc, ts;
+    uint32_t pf = block_excl_scan<4>(f, sh, tf) + run_f;
+    uint32_t pfc = block_excl_scan<4>(fc, sh, tfc) + run_fc;
+    uint32_t ps = block_excl_scan<4>(s, sh, ts) + run_s;
+    run_f += tf;
+    run_fc += tfc;
+    run_s += ts;
+    if (f + s == 0) continue;
 #pragma unroll
-  for (int t = 0; t < 16; t++) {
-    const uint64_t v = base + t;
-    if (kinds[t] == FIRST_OCUR) {
-      J.f_list[pf] = (uint32_t)v;
-      J.f_off[pf] = pfc;
-      pf++;
-      pfc += (uint32_t)leaf_count(g, v);
-    } else if (kinds[t] == SHIFT_DUPL) {
-      const unsigned long long val = J.map.slots[J.slot_of[v]].val;
-      J.s_node[ps] = (uint32_t)v;
-      J.s_prev[ps] = (uint32_t)val;
-      J.s_tree[ps] = (uint32_t)(val >> 32);
-      ps++;
+    for (int t = 0; t < 16; t++) {
+      const uint64_t v = v0 + t;
+      if (kinds[t] == FIRST_OCUR) {
+        J.f_list[pf] = (uint32_t)v;
+        J.f_off[pf] = pfc;
+        pf++;
+        pfc += (uint32_t)leaf_count(g, v);
+      } else if (kinds[t] == SHIFT_DUPL) {
+        const unsigned long long val = J.map.slots[J.slot_of[v]].val;
+        J.s_node[ps] = (uint32_t)v;
+        J.s_prev[ps] = (uint32_t)val;
+        J.s_tree[ps] = (uint32_t)(val >> 32);
+        ps++;
+      }
     }
   }
 }
@@ -522,34 +905,89 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobDev* jobs, const ui
   for (uint32_t bn = threadIdx.x; bn < nbins; bn += blockDim.x) J.s_hist[(uint64_t)bn * nt + t] = 0;
   __syncthreads();
   const uint32_t S = J.ctr->nshift;
-  for (uint32_t e = t * kSTile + threadIdx.x; e < S && e < (t + 1) * kSTile; e += blockDim.x) {
-    uint32_t bn = J.s_tree[e];
-    if (bn >= nbins) {
-      atomicOr(&J.ctr->flags, FLAG_BAD_SOURCE);
-      bn = nbins - 1;
+  const uint32_t e1 = min(S, (t + 1) * kSTile);
+  for (uint32_t base = t * kSTile; base < e1; base += blockDim.x) {
+    const uint32_t e = base + threadIdx.x;
+    uint32_t bn = 0;
+    bool todo = e < e1;
+    if (todo) {
+      bn = J.s_tree[e];
+      if (bn >= nbins) {
+        atomicOr(&J.ctr->flags, FLAG_BAD_SOURCE);
+        bn = nbins - 1;
+      }
     }
-    atomicAdd(&J.s_hist[(uint64_t)bn * nt + t], 1u);
+    // one atomic per distinct bin of the wavefront (shifted roots mostly
+    // share one or two source checkpoints)
+    while (true) {
+      const unsigned long long act = __ballot(todo);
+      if (!act) break;
+      const uint32_t leader = __ffsll((long long)act) - 1;
+      const uint32_t lb = __shfl(bn, leader, 64);
+      const unsigned long long same = __ballot(todo && bn == lb);
+      if (lane_id() == leader) atomicAdd(&J.s_hist[(uint64_t)lb * nt + t], (uint32_t)__popcll(same));
+      if (todo && bn == lb) todo = false;
+    }
   }
 }
 
-// One workgroup per job: scan of the histogram and the prior table.
+// Copies the final counters into host-mapped memory (read by the host once
+// the stream is synchronized; no copy command needed).
+__device__ __forceinline__ void export_counters(const JobDev& J) {
+  const JobCounters c = *J.ctr;
+  *J.host_ctr = c;
+}
+
+// One workgroup per job: new map entries of the checkpoint, scan of the
+// shifted-root histogram, the prior table, the diff length; then exports the
+// counters to the host.
 __global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* jobs) {
   const JobDev& J = jobs[blockIdx.x];
-  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   __shared__ uint32_t sh[17];
+  __shared__ unsigned long long shl[16];
+  unsigned long long claims = 0;
+  for (uint32_t i = threadIdx.x; i < J.nwg_ins_leaf; i += blockDim.x) claims += J.blk_claim[i];
+  for (uint32_t i = threadIdx.x; i < J.nwg_ins_int; i += blockDim.x) claims += J.tile_claim[i];
+  claims = block_sum(claims, shl);
+  const bool skip = (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0;
+  __syncthreads();
+  if (skip) {
+    if (threadIdx.x == 0) {
+      J.ctr->map_count = J.map_count_at_start + claims;
+      J.ctr->pending = pending_total(J);
+      export_counters(J);
+    }
+    return;
+  }
   const uint32_t nt = stiles_of(J);
   const uint32_t nbins = J.ckpt_id + 1;
+  const uint32_t S = J.ctr->nshift;
   uint32_t nprior = 0;
   if (nt > 0) {
-    // Per-bin totals -> prior table entries in ascending checkpoint order.
+    // Exclusive scan of the histogram in bin-major order.
+    const uint64_t total = (uint64_t)nbins * nt;
+    uint32_t run = 0;
+    for (uint64_t base = 0; base < total; base += blockDim.x) {
+      const uint64_t i = base + threadIdx.x;
+      const uint32_t v = i < total ? J.s_hist[i] : 0;
+      uint32_t tot;
+      const uint32_t e = block_excl_scan<16>(v, sh, tot);
+      if (i < total) J.s_hist[i] = run + e;
+      run += tot;
+    }
+    __syncthreads();
+    // Per-bin totals from the scanned bases -> prior table, ascending.
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < nbins; base += 1024) {
+    for (uint32_t base = 0; base < nbins; base += blockDim.x) {
       const uint32_t bn = base + threadIdx.x;
       uint32_t cnt = 0;
-      if (bn < nbins)
-        for (uint32_t t = 0; t < nt; t++) cnt += J.s_hist[(uint64_t)bn * nt + t];
+      if (bn < nbins) {
+        const uint32_t lo = J.s_hist[(uint64_t)bn * nt];
+        const uint32_t hi = bn + 1 < nbins ? J.s_hist[(uint64_t)(bn + 1) * nt] : S;
+        cnt = hi - lo;
+      }
       uint32_t tot;
-      uint32_t pos = block_excl_scan<16>(cnt ? 1u : 0u, sh, tot) + carry;
+      const uint32_t pos = block_excl_scan<16>(cnt ? 1u : 0u, sh, tot) + carry;
       if (bn < nbins && cnt) {
         J.ptab[2 * pos] = bn;
         J.ptab[2 * pos + 1] = cnt;
@@ -557,22 +995,14 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* jobs) {
       carry += tot;
     }
     nprior = carry;
-    // Exclusive scan of the whole histogram in bin-major order.
-    const uint64_t total = (uint64_t)nbins * nt;
-    uint32_t run = 0;
-    for (uint64_t base = 0; base < total; base += 1024) {
-      const uint64_t i = base + threadIdx.x;
-      uint32_t v = i < total ? J.s_hist[i] : 0;
-      uint32_t tot;
-      uint32_t e = block_excl_scan<16>(v, sh, tot);
-      if (i < total) J.s_hist[i] = run + e;
-      run += tot;
-    }
   }
   if (threadIdx.x == 0) {
     J.ctr->nprior = nprior;
     const uint64_t meta = 4ull * J.ctr->nfirst + 8ull * nprior + 8ull * J.ctr->nshift;
     J.ctr->diff_len = 32 + meta + J.ctr->dchunks * (uint64_t)J.chunk_size;
+    J.ctr->map_count = J.map_count_at_start + claims;
+    J.ctr->pending = pending_total(J);
+    export_counters(J);
   }
 }
 
@@ -909,6 +1339,20 @@ void launch_begin(const JobDev* jobs, const uint32_t*, uint32_t njobs, bool reru
   DD_LAUNCH(k_begin, (njobs + 63) / 64, 64, 0, s, jobs, njobs, rerun ? 1 : 0);
 }
 
+void launch_leaf_hash_lds(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int seg, int kind,
+                          hipStream_t s) {
+  if (grid == 0) return;
+  if (kind == HASH_MD5) {
+    if (seg == 64) DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>), grid, 256, 0, s, jobs, prefix, njobs);
+    else if (seg == 128) DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>), grid, 256, 0, s, jobs, prefix, njobs);
+    else DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>), grid, 256, 0, s, jobs, prefix, njobs);
+  } else {
+    if (seg == 64) DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>), grid, 256, 0, s, jobs, prefix, njobs);
+    else if (seg == 128) DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>), grid, 256, 0, s, jobs, prefix, njobs);
+    else DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>), grid, 256, 0, s, jobs, prefix, njobs);
+  }
+}
+
 void launch_leaf_hash(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool al16, int kind,
                       hipStream_t s) {
   if (grid == 0) return;
@@ -936,8 +1380,8 @@ void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, ui
 void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   if (grid) DD_LAUNCH(k_roots_count, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_roots_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s) {
-  DD_LAUNCH(k_roots_scan, njobs, 1024, 0, s, jobs);
+void launch_roots_scan(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, hipStream_t s) {
+  DD_LAUNCH(k_roots_scan, njobs, 1024, 0, s, jobs, prefix);
 }
 void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   if (grid) DD_LAUNCH(k_roots_write, grid, 256, 0, s, jobs, prefix, njobs);

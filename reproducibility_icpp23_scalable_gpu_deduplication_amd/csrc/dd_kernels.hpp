@@ -14,9 +14,10 @@
 
 namespace dd {
 
-constexpr int kTileLevels = 8;     // levels per tree tile (256 bottom nodes)
-constexpr int kNodesPerBlock = 4096;  // root scan: 256 threads x 16 nodes
-constexpr int kSTile = 4096;          // shifted-root sort tile
+constexpr int kTileLevels = 8;          // levels per tree tile (256 bottom nodes)
+constexpr int kNodesPerBlock = 4096;    // root scan: 256 threads x 16 nodes
+constexpr int kSTile = 1024;            // shifted-root sort tile
+constexpr uint32_t kMaxStageWG = 4096;  // workgroups per job of a grid-stride stage
 
 // Per-job counters written by the device and read back by the host.
 struct JobCounters {
@@ -45,16 +46,38 @@ struct JobDev {
   uint32_t is_baseline;
   // persistent state
   Digest* tree;        // N digests, heap order
-  uint8_t* labels;     // N labels
+  uint8_t* labels;     // N labels (padded for 16-byte reads)
   uint32_t* slot_of;   // N: map slot of pending leaves and shifted interior nodes
   MapView map;
   uint64_t map_count_at_start;  // entries before this checkpoint
   uint64_t map_limit;           // max entries the map may hold
-  JobCounters* ctr;
-  // scratch
-  uint32_t* blk_f;    // per node block: first roots
-  uint32_t* blk_fc;   // per node block: chunks under first roots
-  uint32_t* blk_s;    // per node block: shifted roots
+  JobCounters* ctr;             // device counters
+  JobCounters* host_ctr;        // host-mapped copy written at the end of the pipeline
+  // per leaf block (256 chunks): changed leaves and new map entries
+  uint32_t* blk_pend;            // ceil(C/256)
+  unsigned long long* pend_acc;  // 32 partial sums of the changed leaves
+  uint32_t* pend_list;           // 256 per leaf block: chunk indices of changed leaves
+  uint32_t* blk_claim;           // per insert_leaves workgroup: new map entries
+  // Work lists built by the leaf kernel (32 sub-lists each, picked by index
+  // modulo 32, so no single counter is hit by every workgroup).
+  uint32_t epoch;        // stamp of this checkpoint (never 0)
+  uint32_t* tile_epoch;  // per group-0 tile: epoch of the last checkpoint that changed one of its leaves
+  uint32_t* dirty_cnt;   // 32 counters
+  uint32_t* dirty_list;  // 32 x dirty_cap: group-0 tiles with a changed leaf
+  uint32_t dirty_cap;
+  uint32_t* act_cnt;     // 32 counters
+  uint32_t* act_list;    // 32 x act_cap: leaf blocks with a changed leaf
+  uint32_t act_cap;
+  // per tree tile (all groups): whether it holds first-occurrence interior nodes
+  uint32_t tile_base[4];  // first tile index of each group
+  uint32_t ntiles;        // tiles over all groups
+  uint32_t* tile_fint;    // ntiles
+  uint32_t* tile_claim;   // per insert_interior workgroup: new map entries
+  uint32_t nwg_ins_leaf, nwg_ins_int;  // workgroups of the two insert stages
+  // root scan
+  uint32_t* blk_f;    // per root-scan workgroup: first roots
+  uint32_t* blk_fc;   // per root-scan workgroup: chunks under first roots
+  uint32_t* blk_s;    // per root-scan workgroup: shifted roots
   uint32_t nblk;      // node blocks = ceil(N / kNodesPerBlock)
   uint32_t* f_list;   // first roots, ascending
   uint32_t* f_off;    // chunk offset of each first region in the data section
@@ -94,12 +117,18 @@ constexpr uint32_t kCoverFirst = 0xffffffffu;
 void launch_begin(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, bool rerun, hipStream_t s);
 void launch_leaf_hash(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool all_aligned16,
                       int hash_kind, hipStream_t s);
+// LDS-staged variant: `seg` in {64, 128, 256} bytes per chunk per round;
+// every job needs chunk_size % seg == 0 and a 16-byte aligned buffer.  One
+// workgroup covers 256 chunks, like launch_leaf_hash.
+void launch_leaf_hash_lds(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int seg,
+                          int hash_kind, hipStream_t s);
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
 void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
 void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
-void launch_roots_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s);
+// `prefix` is the workgroup prefix of the roots_count/roots_write stage.
+void launch_roots_scan(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, hipStream_t s);
 void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_sbin_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_sbin_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s);

@@ -1,0 +1,67 @@
+"""Batched checkpointing (dd_batch_*) is byte-identical to one handle per
+buffer on the plain path and to the oracle, with and without hipGraph replay."""
+import numpy as np
+import pytest
+
+import reproducibility_icpp23_scalable_gpu_deduplication_amd as dd
+from reproducibility_icpp23_scalable_gpu_deduplication_amd import workloads as W
+from gpu_util import first_mismatch, gpu_restart, oracle_chain
+
+pytestmark = pytest.mark.gpu
+
+
+def run_batch(torch, chains, chunk, graph, baselines=(0,)):
+    n = len(chains)
+    handles = [dd.TreeDeduplicator(chunk) for _ in range(n)]
+    b = dd.Batch(handles, graph=graph)
+    nck = len(chains[0])
+    diffs = [[] for _ in range(n)]
+    for k in range(nck):
+        bufs = [torch.from_numpy(c[k]).to("cuda") for c in chains]
+        caps = [dd.max_diff_size(c[k].nbytes, chunk, k) for c in chains]
+        outs = [torch.empty(cap, dtype=torch.uint8, device="cuda") for cap in caps]
+        torch.cuda.synchronize()
+        lens = b.checkpoint_device([t.data_ptr() for t in bufs], [t.numel() for t in bufs], [k in baselines] * n,
+                                   [o.data_ptr() for o in outs], caps)
+        torch.cuda.synchronize()
+        for i in range(n):
+            diffs[i].append(outs[i][:lens[i]].cpu().numpy().tobytes())
+    return handles, diffs
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_oracle_uneven_sizes(torch_gpu, graph):
+    sizes = [1 << 20, 3 << 18, 777 * 128, 5000 * 128 + 40, 1 << 16]
+    chains = [W.chain_uniform_changes(11 + i, s, 128, 5, fresh_frac=0.02, copy_frac=0.03, zero_bytes=4096)
+              for i, s in enumerate(sizes)]
+    handles, diffs = run_batch(torch_gpu, chains, 128, graph)
+    for i, c in enumerate(chains):
+        od, _ = oracle_chain(128, c)
+        for k in range(len(c)):
+            assert diffs[i][k] == od[k], f"buffer {i} ckpt {k}: {first_mismatch(diffs[i][k], od[k])}"
+        for k in (0, len(c) - 1):
+            assert gpu_restart(torch_gpu, handles[i], diffs[i], k, c[k].nbytes) == c[k].tobytes()
+
+
+def test_batch_map_growth_rerun(torch_gpu):
+    # Fresh random data each checkpoint forces map growth and the re-run of
+    # the inserts for the grown buffers only.
+    rng = np.random.default_rng(4)
+    chains = [[rng.integers(0, 256, s, dtype=np.uint8) for _ in range(4)] for s in (1 << 18, 3 << 16, 1 << 17)]
+    _, diffs = run_batch(torch_gpu, chains, 128, True)
+    for i, c in enumerate(chains):
+        od, _ = oracle_chain(128, c)
+        assert diffs[i] == od
+
+
+def test_batch_equals_plain_path(torch_gpu):
+    torch = torch_gpu
+    sizes = [1 << 19, 9 << 16]
+    chains = [W.chain_runs(21 + i, s, 128, 4, 0.05) for i, s in enumerate(sizes)]
+    _, bdiffs = run_batch(torch, chains, 128, True)
+    for i, c in enumerate(chains):
+        g = dd.TreeDeduplicator(128)
+        for k, x in enumerate(c):
+            t = torch.from_numpy(x).to("cuda")
+            torch.cuda.synchronize()
+            assert g.checkpoint(t.data_ptr(), t.numel(), k == 0) == bdiffs[i][k]
