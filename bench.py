@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--sweep", action="store_true", help="also run chunk {128,512,4096} x change {0.1,1,10}%%")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-size", type=int, default=256 << 20)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-buffer batched comparison")
+    ap.add_argument("--batch-buffers", type=int, default=64)
     return ap.parse_args()
 
 
@@ -108,6 +110,64 @@ def run_config(torch, dd, W, size, chunk, frac, nckpt, steps, warmup, kind, seed
     del chain, out, g
     torch.cuda.empty_cache()
     return res
+
+
+def run_batched(torch, dd, W, nbuf, nckpt, chunk, kind, seed=1931):
+    """64 checkpoint buffers of log-uniform sizes in [4 MiB, 512 MiB] (one per
+    simulated rank), each with its own 10-checkpoint chain (1/3 fresh and 2/3
+    copied chunks at a per-buffer change rate uniform in [0.1 %, 5 %], plus a
+    1 MiB zero region).  Plain: 64 sequential dd_checkpoint_device calls per
+    checkpoint index.  Batched: one dd_batch call (hipGraph replay) per
+    checkpoint index.  Outputs are compared byte for byte."""
+    import numpy as np
+
+    sizes = W.batch_sizes(seed, nbuf)
+    rates = np.random.default_rng(seed).uniform(0.001, 0.05, nbuf)
+    chains = [W.chain_uniform_changes_gpu(torch, seed + 1 + i, sz, chunk, nckpt, r / 3, 2 * r / 3)
+              for i, (sz, r) in enumerate(zip(sizes, rates))]
+    torch.cuda.synchronize()
+    caps = [dd.max_diff_size(sz, chunk, 4 * nckpt) for sz in sizes]
+    outs_p = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    outs_b = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    plain = [dd.TreeDeduplicator(chunk, kind) for _ in sizes]
+    hs = [dd.TreeDeduplicator(chunk, kind) for _ in sizes]
+    batch = dd.Batch(hs, graph=True)
+    ptrs = [[c[k].data_ptr() for c in chains] for k in range(nckpt)]
+
+    def plain_step(k):
+        return [g.checkpoint_device(ptrs[k][i], sizes[i], outs_p[i].data_ptr(), caps[i], k == 0)
+                for i, g in enumerate(plain)]
+
+    def batch_step(k):
+        return batch.checkpoint_device(ptrs[k], sizes, [k == 0] * nbuf, [o.data_ptr() for o in outs_b], caps)
+
+    # Untimed chain: both paths, every diff compared byte for byte.
+    mismatches = 0
+    for k in range(nckpt):
+        lp = plain_step(k)
+        lb = batch_step(k)
+        torch.cuda.synchronize()
+        for i in range(nbuf):
+            if lp[i] != lb[i] or not torch.equal(outs_p[i][:lp[i]], outs_b[i][:lb[i]]):
+                mismatches += 1
+    res = {}
+    for name, fn in (("plain", plain_step), ("batched", batch_step)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nckpt):
+            fn(k)
+        torch.cuda.synchronize()
+        res[name] = time.perf_counter() - t0
+    total = sum(sizes) * nckpt
+    out = dict(
+        buffers=nbuf, total_bytes=sum(sizes), checkpoints=nckpt, chunk_size=chunk,
+        plain_gbs=round(total / res["plain"] / 1e9, 2), batched_gbs=round(total / res["batched"] / 1e9, 2),
+        speedup=round(res["plain"] / res["batched"], 3), bit_exact=mismatches == 0, mismatches=mismatches,
+        ms_per_checkpoint_index=dict(plain=round(res["plain"] * 1e3 / nckpt, 3),
+                                     batched=round(res["batched"] * 1e3 / nckpt, 3)))
+    del chains, outs_p, outs_b, plain, hs, batch
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(W, size, chunk, frac, nckpt):
@@ -191,6 +251,9 @@ def main():
                                   ms_baseline=s["gpu_ms_baseline"], ms_incremental=s["gpu_ms_incremental"]))
                 print(json.dumps({"sweep_point": sweep[-1]}), file=sys.stderr, flush=True)
         line["sweep"] = sweep
+    if not args.no_batched:
+        line["batched"] = run_batched(torch, dd, W, args.batch_buffers, args.nckpt, args.chunk, kind)
+        print(json.dumps({"batched": line["batched"]}), file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(W, args.cpu_size, args.chunk, args.frac, args.nckpt)
     print(json.dumps(line), flush=True)

@@ -380,9 +380,14 @@ __device__ inline InsertResult map_insert(const JobDev& J, const Digest& k, uint
   const MapView& m = J.map;
   const unsigned long long mine = pack_node(J.ckpt_id, node);
   uint64_t s = map_start(m, k) & m.mask;
+  // In a baseline the map starts empty, so the first probe goes straight to
+  // the CAS (a failed CAS returns the slot like a load would).
+  const bool cas_first = J.is_baseline != 0;
   for (uint64_t it = 0; it <= m.mask; it++) {
     Slot* sl = &m.slots[s];
-    unsigned long long v = __hip_atomic_load(&sl->val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    unsigned long long v = (cas_first && it == 0)
+                               ? SLOT_EMPTY
+                               : __hip_atomic_load(&sl->val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (v == SLOT_EMPTY) {
       const unsigned long long old = atomicCAS(&sl->val, SLOT_EMPTY, mine);
       if (old == SLOT_EMPTY) {

@@ -11,13 +11,17 @@ import numpy as np
 def _pick_runs(rng, nchunks, target, mean_len):
     """Contiguous runs of geometric length (mean `mean_len`) at uniform starts
     until `target` chunks are covered; returns a list of (start, length)."""
-    runs, covered = [], 0
     p = 1.0 / mean_len
+    n = int(target / mean_len * 1.3) + 8
+    runs, covered = [], 0
     while covered < target:
-        ln = int(min(rng.geometric(p), nchunks))
-        st = int(rng.integers(0, nchunks - ln + 1))
-        runs.append((st, ln))
-        covered += ln
+        lens = np.minimum(rng.geometric(p, n), nchunks)
+        starts = rng.integers(0, nchunks - lens + 1)
+        for st, ln in zip(starts.tolist(), lens.tolist()):
+            runs.append((st, ln))
+            covered += ln
+            if covered >= target:
+                break
     return runs
 
 
@@ -132,3 +136,38 @@ def chain_float_field(seed, n_values, nckpt, zero_frac=0.6, block_bytes=4096, pe
             zero[b] = ~zero[b]
         out.append(x.view(np.uint8).copy())
     return out
+
+
+def chain_uniform_changes_gpu(torch, seed, size, chunk, nckpt, fresh_frac=0.01, copy_frac=0.02,
+                              zero_bytes=1 << 20, device="cuda"):
+    """chain_uniform_changes generated in device memory (torch RNG)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    C = size // chunk
+    cur = torch.randint(0, 256, (size,), dtype=torch.uint8, device=device, generator=gen)
+    out = [cur]
+    for _ in range(1, nckpt):
+        prev = out[-1]
+        cur = prev.clone()
+        v = cur[: C * chunk].view(C, chunk)
+        pv = prev[: C * chunk].view(C, chunk)
+        nf = int(round(fresh_frac * C))
+        nc = int(round(copy_frac * C))
+        perm = torch.randperm(C, device=device, generator=gen)
+        if nf:
+            v[perm[:nf]] = torch.randint(0, 256, (nf, chunk), dtype=torch.uint8, device=device, generator=gen)
+        if nc:
+            src = torch.randint(0, C, (nc,), device=device, generator=gen)
+            v[perm[nf:nf + nc]] = pv[src]
+        if zero_bytes and size >= zero_bytes:
+            z = int(torch.randint(0, size // zero_bytes, (1,), generator=gen, device=device).item()) * zero_bytes
+            cur[z:z + zero_bytes] = 0
+        out.append(cur)
+    return out
+
+
+def batch_sizes(seed, n=64, lo=4 << 20, hi=512 << 20, align=4096):
+    """`n` buffer sizes log-uniform in [lo, hi], rounded to `align` bytes."""
+    rng = np.random.default_rng(seed)
+    x = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+    return [int(max(align, round(v / align) * align)) for v in x]
