@@ -44,6 +44,12 @@ def parse():
     ap.add_argument("--cpu-size", type=int, default=256 << 20)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-buffer batched comparison")
     ap.add_argument("--batch-buffers", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-loop (serving) measurement")
+    ap.add_argument("--serve-ranks", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals, requests/s")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-budget-gib", type=float, default=200.0)
     return ap.parse_args()
 
 
@@ -170,6 +176,144 @@ def run_batched(torch, dd, W, nbuf, nckpt, chunk, kind, seed=1931):
     return out
 
 
+def _pct(xs, q):
+    import numpy as np
+
+    return round(float(np.percentile(np.asarray(xs), q)) * 1e3, 3) if xs else None
+
+
+def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, seed=2024, versions=10):
+    """Request loop over `ranks` simulated ranks (dd_server).  Each rank has a
+    buffer of log-uniform size in [16 MiB, 1 GiB] and its own handle, whose
+    tree and digest map stay in HBM between requests.  Checkpoint data comes
+    from a pool of `versions` 1 GiB versions (the 64 MiB chain's recipe: 1 %
+    fresh chunks, 2 % copied chunks, one zeroed 1 MiB region per 64 MiB); rank r
+    reads version n % versions at its own offset, and re-baselines every
+    `versions` checkpoints.  Arrivals are Poisson at `rate` req/s for
+    `seconds` (85 % checkpoints, 15 % restarts of a random checkpoint of the
+    rank's current chain); a second, saturated pass submits a fixed number of
+    requests at once.  Every restart is compared byte for byte with the data
+    it restores.  Latency = completion - submission (steady clock of the
+    library)."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    sizes = W.batch_sizes(seed, ranks, 16 << 20, 1 << 30, 4096)
+    pool_size = max(sizes)
+    pool = W.chain_uniform_changes_gpu(torch, seed, pool_size, chunk, versions, 0.01, 0.02, 1 << 20,
+                                       zero_count=max(1, pool_size >> 26))
+    offs = [int(rng.integers(0, (pool_size - sz) // 4096 + 1)) * 4096 for sz in sizes]
+    streams = [torch.cuda.Stream() for _ in range(ranks)]
+    handles = [dd.TreeDeduplicator(chunk, kind, stream=st.cuda_stream) for st in streams]
+    kmax = 1 << 16
+    caps = [dd.max_diff_size(sz, chunk, kmax) for sz in sizes]
+    outs = [[torch.empty(caps[r], dtype=torch.uint8, device="cuda") for _ in range(versions)] for r in range(ranks)]
+    torch.cuda.synchronize()
+    srv = dd.Server(budget, slots)
+    nck = [0] * ranks  # checkpoints submitted per rank (= next checkpoint id)
+
+    def ckpt_args(r):
+        n = nck[r]
+        assert n < kmax
+        nck[r] += 1
+        v = n % versions
+        return (handles[r], pool[v].data_ptr() + offs[r], sizes[r], outs[r][v].data_ptr(), caps[r], v == 0)
+
+    def restart_args(r):
+        last = nck[r] - 1
+        base = last - last % versions
+        j = int(rng.integers(base, last + 1))
+        ptrs = [0] * (j + 1)
+        lens = [0] * (j + 1)
+        for i in range(base, j + 1):
+            ptrs[i] = outs[r][i % versions].data_ptr()
+            lens[i] = caps[r]
+        o = torch.empty(sizes[r], dtype=torch.uint8, device="cuda")
+        return j, o, (handles[r], ptrs, lens, j, o.data_ptr(), sizes[r])
+
+    # Warm-up (untimed): the first checkpoint of every rank allocates its state.
+    for t in [srv.submit_checkpoint(*ckpt_args(r)[:5], make_baseline=True) for r in range(ranks)]:
+        assert srv.wait(t)["status"] == dd.DD_OK
+    torch.cuda.synchronize()
+
+    def run_pass(arrivals):
+        """arrivals: list of (time offset s or None, rank, is_restart)."""
+        done = queue.Queue()
+        stats = dict(lat_ck=[], lat_rs=[], wait=[], ck_bytes=0, rs_bytes=0, bad=0, errors=0, t0=None, t1=0.0)
+
+        def collector(nreq):
+            for _ in range(nreq):
+                item = done.get()
+                res = srv.wait(item[0])
+                if res["status"] != dd.DD_OK:
+                    stats["errors"] += 1
+                    continue
+                lat = res["t_end"] - res["t_submit"]
+                stats["wait"].append(res["t_start"] - res["t_submit"])
+                stats["t1"] = max(stats["t1"], res["t_end"])
+                if item[1] is None:
+                    stats["lat_ck"].append(lat)
+                    stats["ck_bytes"] += item[2]
+                else:
+                    r, j, o = item[1], item[2], item[3]
+                    v = j % versions
+                    if not torch.equal(o, pool[v][offs[r]:offs[r] + sizes[r]]):
+                        stats["bad"] += 1
+                    stats["lat_rs"].append(lat)
+                    stats["rs_bytes"] += sizes[r]
+
+        th = threading.Thread(target=collector, args=(len(arrivals),))
+        th.start()
+        start = time.perf_counter()
+        stats["t0"] = srv.now()
+        for when, r, is_rs in arrivals:
+            if when is not None:
+                d = start + when - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            if is_rs and nck[r] > 0:
+                j, o, a = restart_args(r)
+                done.put((srv.submit_restart(*a), r, j, o))
+            else:
+                a = ckpt_args(r)
+                done.put((srv.submit_checkpoint(*a[:5], make_baseline=a[5]), None, sizes[r]))
+        th.join()
+        el = stats["t1"] - stats["t0"]
+        lat_all = stats["lat_ck"] + stats["lat_rs"]
+        return dict(
+            requests=len(arrivals), checkpoints=len(stats["lat_ck"]), restarts=len(stats["lat_rs"]),
+            elapsed_s=round(el, 3), req_per_s=round(len(lat_all) / el, 1),
+            checkpoint_gbs=round(stats["ck_bytes"] / el / 1e9, 2),
+            served_gbs=round((stats["ck_bytes"] + stats["rs_bytes"]) / el / 1e9, 2),
+            p50_ms=_pct(lat_all, 50), p99_ms=_pct(lat_all, 99),
+            checkpoint_p50_ms=_pct(stats["lat_ck"], 50), checkpoint_p99_ms=_pct(stats["lat_ck"], 99),
+            restart_p50_ms=_pct(stats["lat_rs"], 50), restart_p99_ms=_pct(stats["lat_rs"], 99),
+            queue_wait_p50_ms=_pct(stats["wait"], 50), queue_wait_p99_ms=_pct(stats["wait"], 99),
+            restarts_verified=len(stats["lat_rs"]) - stats["bad"], restart_mismatches=stats["bad"],
+            errors=stats["errors"])
+
+    n = int(rate * seconds)
+    gaps = rng.exponential(1.0 / rate, n)
+    poisson = [(float(t), int(rng.integers(0, ranks)), bool(rng.random() < 0.15)) for t in np.cumsum(gaps)]
+    out = dict(ranks=ranks, buffer_bytes_total=sum(sizes), buffer_bytes_min=min(sizes), buffer_bytes_max=max(sizes),
+               chunk_size=chunk, slots=slots, hbm_budget_bytes=budget, rebaseline_every=versions)
+    out["poisson"] = dict(rate_req_per_s=rate, **run_pass(poisson))
+    print(json.dumps({"serving_poisson": out["poisson"]}), file=sys.stderr, flush=True)
+    nsat = 12 * ranks
+    sat = [(None, int(rng.integers(0, ranks)), bool(rng.random() < 0.15)) for _ in range(nsat)]
+    out["saturated"] = run_pass(sat)
+    print(json.dumps({"serving_saturated": out["saturated"]}), file=sys.stderr, flush=True)
+    in_use, peak = srv.usage()
+    out["hbm_reserved_peak_bytes"] = peak
+    srv.close()
+    del pool, outs, handles
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(W, size, chunk, frac, nckpt):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -254,6 +398,9 @@ def main():
     if not args.no_batched:
         line["batched"] = run_batched(torch, dd, W, args.batch_buffers, args.nckpt, args.chunk, kind)
         print(json.dumps({"batched": line["batched"]}), file=sys.stderr, flush=True)
+    if not args.no_serve:
+        line["serving"] = run_serving(torch, dd, W, args.serve_ranks, args.serve_rate, args.serve_seconds,
+                                      int(args.serve_budget_gib * (1 << 30)), args.serve_slots, args.chunk, kind)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(W, args.cpu_size, args.chunk, args.frac, args.nckpt)
     print(json.dumps(line), flush=True)

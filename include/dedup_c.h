@@ -136,6 +136,41 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
                                      const int* make_baseline, void* const* dev_out, const uint64_t* out_caps,
                                      uint64_t* out_lens);
 
+/* Serving (no counterpart in the reference): a request loop over many
+ * checkpoint streams (one dd_dedup handle per simulated rank).  Requests are
+ * admitted in arrival order while an execution slot is free, the rank has no
+ * request in flight, and the request's device-memory footprint fits the HBM
+ * budget; a request that does not fit waits for completions and holds back
+ * the requests behind it.  With nothing in flight the first eligible request
+ * always starts, so a request larger than the budget runs alone.  Admitted
+ * requests run concurrently, each on its handle's HIP stream, so their
+ * kernels overlap on the device.  Results are the same bytes
+ * dd_checkpoint_device / dd_restart_device return.  While a handle has a
+ * request queued or in flight, the caller must not use it directly. */
+typedef struct dd_server dd_server;
+typedef struct dd_request_result {
+  int32_t status;      /* dd_status of the request */
+  uint32_t kind;       /* 0 checkpoint, 1 restart */
+  uint64_t out_len;    /* diff length (checkpoint) or restored bytes (restart) */
+  double t_submit;     /* seconds, steady clock */
+  double t_start;      /* admitted and started */
+  double t_end;        /* finished */
+} dd_request_result;
+/* `footprint` of each request: its input and output buffers plus, on a
+ * handle's first request, the handle's state estimate. */
+dd_status dd_server_create(uint64_t hbm_budget_bytes, uint32_t slots, dd_server** out);
+dd_status dd_server_destroy(dd_server* s); /* waits for every request */
+dd_status dd_server_submit_checkpoint(dd_server* s, dd_dedup* h, const void* dev_data, uint64_t len,
+                                      int make_baseline, void* dev_out, uint64_t out_cap, uint64_t* ticket);
+dd_status dd_server_submit_restart(dd_server* s, dd_dedup* h, const void* const* dev_chain,
+                                   const uint64_t* chain_lens, uint32_t nchain, uint32_t chkpt_id,
+                                   void* dev_out, uint64_t len, uint64_t* ticket);
+/* Blocks until the request is done; each ticket can be waited once. */
+dd_status dd_server_wait(dd_server* s, uint64_t ticket, dd_request_result* out);
+/* Bytes currently reserved against the budget, and the peak so far. */
+dd_status dd_server_usage(dd_server* s, uint64_t* in_use, uint64_t* peak);
+double dd_now(void); /* the steady clock of dd_request_result */
+
 /* Diagnostics (no counterpart in the reference; its equivalent is the
  * Kokkos::Profiling regions of src/tree_approach.cpp).  Level 1 records HIP
  * events on the handle's stream around the leaf-hash kernel only; level 2

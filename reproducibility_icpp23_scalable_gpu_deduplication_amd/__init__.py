@@ -7,5 +7,5 @@ Python callers (tests, bench).
 """
 from ._lib import (  # noqa: F401
     DD_ECORRUPT, DD_EHIP, DD_EINVAL, DD_ENOMEM, DD_ESTATE, DD_OK, EXPORTED, HASH_MD5, HASH_MURMUR3, LIB_PATH,
-    Batch, DedupError, TreeDeduplicator, load, max_diff_size, strerror,
+    Batch, DedupError, Server, TreeDeduplicator, load, max_diff_size, strerror,
 )

@@ -19,7 +19,9 @@ EXPORTED = [
     "dd_create", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
     "dd_max_diff_size", "dd_counts", "dd_get_ids", "dd_set_current_id", "dd_get_timers",
     "dd_restart", "dd_restart_device", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
-    "dd_batch_checkpoint_device", "dd_set_profiling", "dd_get_stage_times", "dd_buf_free", "dd_strerror",
+    "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
+    "dd_server_submit_restart", "dd_server_wait", "dd_server_usage", "dd_now", "dd_set_profiling",
+    "dd_get_stage_times", "dd_buf_free", "dd_strerror",
 ]
 STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
           "meta_gather"]
@@ -39,6 +41,11 @@ class dd_header(ctypes.Structure):
     _fields_ = [("ref_id", ctypes.c_uint32), ("chkpt_id", ctypes.c_uint32), ("datalen", ctypes.c_uint64),
                 ("chunk_size", ctypes.c_uint32), ("num_first_ocur", ctypes.c_uint32),
                 ("num_prior_chkpts", ctypes.c_uint32), ("num_shift_dupl", ctypes.c_uint32)]
+
+
+class dd_request_result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("kind", ctypes.c_uint32), ("out_len", ctypes.c_uint64),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double)]
 
 
 _lib = None
@@ -71,6 +78,13 @@ def load():
         "dd_batch_destroy": ([vp], i32),
         "dd_batch_use_graph": ([vp, i32], i32),
         "dd_batch_checkpoint_device": ([vp, P(vp), P(u64), P(i32), P(vp), P(u64), P(u64)], i32),
+        "dd_server_create": ([u64, u32, P(vp)], i32),
+        "dd_server_destroy": ([vp], i32),
+        "dd_server_submit_checkpoint": ([vp, vp, vp, u64, i32, vp, u64, P(u64)], i32),
+        "dd_server_submit_restart": ([vp, vp, P(vp), P(u64), u32, u32, vp, u64, P(u64)], i32),
+        "dd_server_wait": ([vp, u64, P(dd_request_result)], i32),
+        "dd_server_usage": ([vp, P(u64), P(u64)], i32),
+        "dd_now": ([], ctypes.c_double),
         "dd_set_profiling": ([vp, i32], i32),
         "dd_get_stage_times": ([vp, P(ctypes.c_double)], i32),
         "dd_buf_free": ([P(dd_buf)], None),
@@ -247,3 +261,55 @@ class Batch:
         res = (u64 * n)()
         _check(load().dd_batch_checkpoint_device(self._h, d, ln, mk, o, oc, res), "dd_batch_checkpoint_device")
         return list(res)
+
+
+class Server:
+    """dd_server: checkpoint and restart requests over many handles, admitted
+    in arrival order by execution slots and an HBM budget, run concurrently on
+    the handles' streams.  ``submit_*`` return a ticket; ``wait`` returns the
+    request's result as a dict (status, kind, out_len, t_submit, t_start,
+    t_end; times in seconds of the steady clock ``now()``)."""
+
+    def __init__(self, hbm_budget_bytes, slots=8):
+        h = ctypes.c_void_p()
+        _check(load().dd_server_create(int(hbm_budget_bytes), int(slots), ctypes.byref(h)), "dd_server_create")
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dd_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.close()
+
+    def submit_checkpoint(self, dedup, dev_ptr, length, out_ptr, out_cap, make_baseline=False):
+        t = ctypes.c_uint64()
+        _check(load().dd_server_submit_checkpoint(self._h, dedup._h, dev_ptr, length, int(bool(make_baseline)),
+                                                  out_ptr, out_cap, ctypes.byref(t)), "dd_server_submit_checkpoint")
+        return t.value
+
+    def submit_restart(self, dedup, dev_ptrs, lens, ckpt_id, out_ptr, length):
+        n = len(dev_ptrs)
+        ptrs = (ctypes.c_void_p * n)(*dev_ptrs)
+        ls = (ctypes.c_uint64 * n)(*lens)
+        t = ctypes.c_uint64()
+        _check(load().dd_server_submit_restart(self._h, dedup._h, ptrs, ls, n, ckpt_id, out_ptr, length,
+                                               ctypes.byref(t)), "dd_server_submit_restart")
+        return t.value
+
+    def wait(self, ticket):
+        r = dd_request_result()
+        _check(load().dd_server_wait(self._h, ticket, ctypes.byref(r)), "dd_server_wait")
+        return {k: getattr(r, k) for k, _ in dd_request_result._fields_}
+
+    def usage(self):
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(load().dd_server_usage(self._h, ctypes.byref(a), ctypes.byref(b)), "dd_server_usage")
+        return a.value, b.value
+
+    @staticmethod
+    def now():
+        return load().dd_now()

@@ -34,7 +34,7 @@ void DevBuf::release() {
   bytes = 0;
 }
 
-dd_status DevBuf::ensure(size_t n, bool zero) {
+dd_status DevBuf::ensure(size_t n, bool zero, hipStream_t zs) {
   if (n == 0) n = 16;
   if (n <= bytes) return DD_OK;
   release();
@@ -44,7 +44,7 @@ dd_status DevBuf::ensure(size_t n, bool zero) {
     return DD_ENOMEM;
   }
   bytes = n;
-  if (zero && hipMemset(p, 0, n) != hipSuccess) return DD_EHIP;
+  if (zero && hipMemsetAsync(p, 0, n, zs) != hipSuccess) return DD_EHIP;
   return DD_OK;
 }
 
@@ -94,7 +94,7 @@ dd_status StreamState::grow_map(uint64_t need, hipStream_t s) {
   DD_TRY(fresh.ensure(cap * sizeof(Slot)));
   launch_map_clear(fresh.as<Slot>(), cap, s);
   DevBuf flag;
-  DD_TRY(flag.ensure(4, true));
+  DD_TRY(flag.ensure(4, true, s));
   launch_map_rehash(map.as<Slot>(), map_cap, make_view(fresh.p, cap), flag.as<uint32_t>(), s);
   uint32_t f = 0;
   DD_HIP(hipMemcpyAsync(&f, flag.p, 4, hipMemcpyDeviceToHost, s));
@@ -144,9 +144,11 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   DD_TRY(ss_prev.ensure(C * 4));
   DD_TRY(s_hist.ensure(((uint64_t)K + 1) * ceil_div(C, kSTile) * 4));
   DD_TRY(ptab.ensure(((uint64_t)K + 1) * 8));
-  DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * 8, true));
+  DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * 8, true, s));
   const uint64_t nlb = ceil_div(C, 256);
-  DD_TRY(leafblk.ensure((nlb + kMaxStageWG) * 4));
+  // blk_pend (one per leaf block) then blk_claim (one per insert_leaves
+  // workgroup: a baseline launches one per leaf block).
+  DD_TRY(leafblk.ensure((nlb + std::max<uint64_t>(nlb, kMaxStageWG)) * 4));
   DD_TRY(pend_list.ensure(nlb * 256 * 4));
   const TreeGeom gm = make_geom(C);
   uint64_t ntiles = 0;
@@ -155,13 +157,15 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     tile_base[g] = (uint32_t)ntiles;
     ntiles += tiles_in_group(gm.D, g);
   }
-  DD_TRY(tiles.ensure((ntiles + kMaxStageWG) * 4));
+  // tile_fint (one per tile) then tile_claim (one per insert_interior
+  // workgroup: a baseline launches one per tile).
+  DD_TRY(tiles.ensure((ntiles + std::max<uint64_t>(ntiles, kMaxStageWG)) * 4));
   // Per group-0 tile stamp of the last checkpoint that changed one of its
   // leaves; zero-filled when (re)allocated, epochs start at 1.
   const uint64_t ntiles0 = tiles_in_group(gm.D, 0);
   if (tile_epoch.bytes < ntiles0 * 4 || epoch == 0xffffffffu) {
     tile_epoch.release();
-    DD_TRY(tile_epoch.ensure(ntiles0 * 4, true));
+    DD_TRY(tile_epoch.ensure(ntiles0 * 4, true, s));
     epoch = 0;
   }
   epoch += 1;
@@ -305,13 +309,19 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
       uint64_t n = 0;
       switch (st) {
         case S_LEAF: n = ceil_div(J.C, 256); break;
-        case S_INSL: n = std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
-        case S_TILES: n = std::min<uint64_t>(J.ntiles, kMaxStageWG); break;
+        // A baseline visits every leaf block and tile (one workgroup each);
+        // an incremental checkpoint strides over its work lists.
+        case S_INSL: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
+        case S_TILES: n = J.is_baseline ? J.ntiles : std::min<uint64_t>(J.ntiles, kMaxStageWG); break;
         case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kMaxStageWG); break;
         case S_STILE: n = ceil_div(J.C, kSTile); break;
         case S_META: n = std::min<uint64_t>(ceil_div(J.C, 256), 256); break;
         case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
-        default: n = std::min<uint64_t>(tiles_in_group(J.D, (int)st - S_TILE0), kMaxStageWG); break;
+        default: {
+          const uint64_t tg = tiles_in_group(J.D, (int)st - S_TILE0);
+          n = J.is_baseline ? tg : std::min<uint64_t>(tg, kMaxStageWG);
+          break;
+        }
       }
       acc += n;
     }
@@ -476,7 +486,8 @@ dd_status checkpoint_to_host(dd_dedup* h, dd_header* header, const void* dev_dat
   const double t0 = now_s();
   uint8_t* buf = static_cast<uint8_t*>(std::malloc(c.diff_len));
   if (!buf) return DD_ENOMEM;
-  if (hipMemcpy(buf, h->st.diff.p, c.diff_len, hipMemcpyDeviceToHost) != hipSuccess) {
+  if (hipMemcpyAsync(buf, h->st.diff.p, c.diff_len, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess) {
     std::free(buf);
     return DD_EHIP;
   }
@@ -589,10 +600,10 @@ dd_status restart_walk(dd_dedup* h, std::vector<HostCkpt>& ckpts, const std::vec
   DD_TRY(h->ref.ensure(g.C * 8));
   if (h->cover.bytes < g.C * sizeof(CoverEntry)) {
     h->cover.release();
-    DD_TRY(h->cover.ensure(g.C * sizeof(CoverEntry), true));
+    DD_TRY(h->cover.ensure(g.C * sizeof(CoverEntry), true, s));
     h->stamp = 0;
   }
-  DD_TRY(h->rs_ctr.ensure(sizeof(JobCounters), true));
+  DD_TRY(h->rs_ctr.ensure(sizeof(JobCounters), true, s));
   if (!h->rs_ctr_host) DD_HIP(hipHostMalloc((void**)&h->rs_ctr_host, sizeof(JobCounters), hipHostMallocDefault));
   if (h->stamp > 0xfffff000u) {  // stamps are about to wrap: start over from a zeroed array
     DD_HIP(hipMemsetAsync(h->cover.p, 0, h->cover.bytes, s));
@@ -822,14 +833,19 @@ dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uin
   auto get = [&](uint32_t i, uint8_t* hdr, std::vector<uint8_t>& meta, uint64_t& total) -> dd_status {
     total = lens[i];
     if (!dev_chain[i] || total < 32) return DD_ECORRUPT;
-    if (hipMemcpy(hdr, dev_chain[i], 32, hipMemcpyDeviceToHost) != hipSuccess) return DD_EHIP;
+    // Reads go through the handle's stream so a restart does not wait for
+    // work of other streams (the legacy null stream would).
+    if (hipMemcpyAsync(hdr, dev_chain[i], 32, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      return DD_EHIP;
     dd_header hd;
     memcpy(&hd, hdr, 32);
     const uint64_t mb = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
     if (32 + mb > total) return DD_ECORRUPT;
     meta.resize(mb);
-    if (mb && hipMemcpy(meta.data(), static_cast<const uint8_t*>(dev_chain[i]) + 32, mb, hipMemcpyDeviceToHost) !=
-                  hipSuccess)
+    if (mb && (hipMemcpyAsync(meta.data(), static_cast<const uint8_t*>(dev_chain[i]) + 32, mb,
+                              hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+               hipStreamSynchronize(h->stream) != hipSuccess))
       return DD_EHIP;
     return DD_OK;
   };

@@ -18,7 +18,9 @@ struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   ~DevBuf();
-  dd_status ensure(size_t n, bool zero = false);
+  // Grows to at least n bytes; a fresh allocation is zero-filled on stream
+  // `zs` when `zero` is set (ordered before later work on that stream).
+  dd_status ensure(size_t n, bool zero = false, hipStream_t zs = nullptr);
   void release();
   template <typename T>
   T* as() const { return reinterpret_cast<T*>(p); }

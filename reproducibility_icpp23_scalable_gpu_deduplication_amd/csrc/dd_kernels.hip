@@ -163,13 +163,14 @@ __device__ __forceinline__ void record_pending(const JobDev& J, uint32_t b, uint
     J.blk_pend[b] = tot;
     if (tot) {
       atomicAdd(&J.pend_acc[b % kPendSpread], (unsigned long long)tot);
-      list_append(J.act_cnt, J.act_list, J.act_cap, b);
+      if (!J.is_baseline) list_append(J.act_cnt, J.act_list, J.act_cap, b);
     }
   }
   // Mark the group-0 tiles of changed leaves: the first marker of a tile in
-  // this checkpoint appends it to the dirty list.
+  // this checkpoint appends it to the dirty list.  A baseline visits every
+  // tile and every leaf block densely, so it keeps no lists.
   uint32_t T = pending ? leaf_tile(J.D, J.C - 1 + k) : 0;
-  bool todo = pending;
+  bool todo = pending && !J.is_baseline;
   while (true) {
     const unsigned long long act = __ballot(todo);
     if (!act) break;
@@ -449,11 +450,15 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* jobs, const
     }
     return;
   }
-  load_list_prefix(J.act_cnt, pre);
-  const uint32_t nitems = pre[kPendSpread];
+  const bool dense = J.is_baseline != 0;
+  uint32_t nitems = (uint32_t)((J.C + 255) / 256);
+  if (!dense) {
+    load_list_prefix(J.act_cnt, pre);
+    nitems = pre[kPendSpread];
+  }
   const uint32_t K = J.ckpt_id;
   for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint32_t b = list_item(J.act_list, J.act_cap, pre, it);
+    const uint32_t b = dense ? it : list_item(J.act_list, J.act_cap, pre, it);
     const uint32_t n = J.blk_pend[b];
     if (threadIdx.x >= n) continue;
     const uint64_t leaf = J.C - 1 + J.pend_list[(uint64_t)b * 256 + threadIdx.x];
@@ -519,16 +524,20 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobDev* jobs, con
   __shared__ uint32_t pre[kPendSpread + 1];
   uint32_t claims = 0;
   if (!(J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
-    load_list_prefix(J.dirty_cnt, pre);
-    const uint32_t nd = pre[kPendSpread];
+    const bool dense = J.is_baseline != 0;
     const uint32_t ntiles0 = 1u << (J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0);
+    uint32_t nd = ntiles0;
+    if (!dense) {
+      load_list_prefix(J.dirty_cnt, pre);
+      nd = pre[kPendSpread];
+    }
     const uint32_t nup = J.ntiles - ntiles0;  // tiles of the groups above group 0
     for (uint32_t it = wg; it < nd + nup; it += nwg) {
       uint32_t T;
       int g = 0;
       uint64_t t;
       if (it < nd) {
-        T = list_item(J.dirty_list, J.dirty_cap, pre, it);
+        T = dense ? it : list_item(J.dirty_list, J.dirty_cap, pre, it);
         t = T;
       } else {
         T = ntiles0 + (it - nd);
@@ -573,13 +582,13 @@ __device__ __forceinline__ uint8_t resolve_leaf(const JobDev& J, uint64_t leaf) 
 // such a group-0 tile changed, the tile was not visited this checkpoint and
 // its stored labels are stale: every node of it is FIXED.
 __device__ __forceinline__ bool child_tile_clean(const JobDev& J, int g, uint64_t x) {
-  if (g != 1) return false;
+  if (g != 1 || J.is_baseline) return false;
   const uint32_t Lr0 = J.D - kTileLevels;  // group 1 exists only when D > kTileLevels
   return J.tile_epoch[x - ((1ull << Lr0) - 1)] != J.epoch;
 }
 
 __device__ __forceinline__ uint32_t group_items(const JobDev& J, int g, uint32_t* pre) {
-  if (g == 0) {
+  if (g == 0 && !J.is_baseline) {
     load_list_prefix(J.dirty_cnt, pre);
     return pre[kPendSpread];
   }
@@ -601,7 +610,7 @@ __global__ __launch_bounds__(256) void k_pass1(const JobDev* jobs, const uint32_
   const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
   for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint64_t t = g == 0 ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
+    const uint64_t t = (g == 0 && !J.is_baseline) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
     const TileRef tr = tile_ref(J, g, t);
     const int H = tr.H;
     const uint64_t r = tr.r;
@@ -665,7 +674,7 @@ __global__ __launch_bounds__(256) void k_pass2(const JobDev* jobs, const uint32_
   const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
   for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint64_t t = g == 0 ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
+    const uint64_t t = (g == 0 && !J.is_baseline) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
     const TileRef tr = tile_ref(J, g, t);
     const int H = tr.H;
     const uint64_t r = tr.r;
@@ -748,6 +757,7 @@ __device__ __forceinline__ void node_block_range(const JobDev& J, uint32_t wg, u
 // Nodes of group-0 tiles with no changed leaf were not visited this
 // checkpoint: all of them are FIXED whatever their stored label says.
 __device__ __forceinline__ bool node_in_clean_tile(const JobDev& J, uint64_t v) {
+  if (J.is_baseline) return false;
   const uint32_t Lr0 = J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0;
   const uint32_t lv = ilog2_64(v + 1);
   if (lv < Lr0) return false;

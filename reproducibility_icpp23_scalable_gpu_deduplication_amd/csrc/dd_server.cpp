@@ -1,0 +1,233 @@
+// dd_server.cpp -- request loop over many checkpoint streams (include/dedup_c.h,
+// dd_server_*).  Host threads run admitted requests concurrently; each request
+// runs on its handle's HIP stream, so work of different ranks overlaps on the
+// device.  Admission is in arrival order, bounded by execution slots, one
+// request in flight per handle, and an HBM budget.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "dd_engine.hpp"
+
+namespace {
+
+double steady_now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  uint64_t ticket = 0;
+  uint32_t kind = 0;  // 0 checkpoint, 1 restart
+  dd_dedup* h = nullptr;
+  // checkpoint
+  const void* data = nullptr;
+  uint64_t len = 0;
+  int make_baseline = 0;
+  void* out = nullptr;
+  uint64_t out_cap = 0;
+  // restart
+  std::vector<const void*> chain;
+  std::vector<uint64_t> lens;
+  uint32_t id = 0;
+  // accounting
+  uint64_t footprint = 0;
+  dd_request_result res{};
+  bool done = false;
+  bool waited = false;
+};
+
+// Device bytes a handle keeps for a buffer of `len` bytes: tree (16 B/node),
+// labels and slot index (5 B/node), the digest map (2N slots of 32 B) and the
+// per-chunk scratch arrays (about 40 B/chunk).
+uint64_t state_bytes(uint64_t len, uint32_t chunk) {
+  const uint64_t C = (len + chunk - 1) / chunk, N = 2 * C - 1;
+  uint64_t cap = 1;
+  while (cap < 2 * N) cap <<= 1;
+  return 21 * N + 32 * cap + 40 * C;
+}
+
+}  // namespace
+
+struct dd_server {
+  uint64_t budget = 0, in_use = 0, peak = 0;
+  uint32_t slots = 0;
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;  // arrival order
+  std::unordered_map<uint64_t, Request*> by_ticket;
+  std::unordered_set<dd_dedup*> busy;
+  std::unordered_map<dd_dedup*, uint64_t> charged;  // state bytes charged per handle
+  uint64_t next_ticket = 1;
+  bool stopping = false;
+  std::vector<std::thread> workers;
+
+  // First queued request that may start now, or nullptr.  Requests of a busy
+  // rank are skipped (they wait for their predecessor); the first request of
+  // an idle rank that does not fit the budget stops the scan, so large
+  // requests are not starved.  When nothing is in flight the first eligible
+  // request always starts, even if it (with the device state the handles
+  // keep charged) exceeds the budget, so the loop never stalls.
+  Request* pick() {
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+      Request* r = *it;
+      if (busy.count(r->h)) continue;
+      const bool new_state = r->kind == 0 && !charged.count(r->h);
+      const uint64_t state = new_state ? state_bytes(r->len, r->h->st.chunk_size) : 0;
+      const uint64_t need = r->footprint + state;
+      if (in_use + need > budget && !busy.empty()) return nullptr;
+      queue.erase(it);
+      if (new_state) charged[r->h] = state;
+      in_use += need;
+      if (in_use > peak) peak = in_use;
+      busy.insert(r->h);
+      return r;
+    }
+    return nullptr;
+  }
+
+  void worker() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+      Request* r = nullptr;
+      cv_work.wait(lk, [&] { return (r = pick()) != nullptr || (stopping && queue.empty()); });
+      if (!r) return;  // stopping and nothing left to start
+      lk.unlock();
+      r->res.t_start = steady_now();
+      dd_status st;
+      if (r->kind == 0) {
+        uint64_t n = 0;
+        st = dd_checkpoint_device(r->h, r->data, r->len, r->make_baseline, r->out, r->out_cap, &n);
+        r->res.out_len = n;
+      } else {
+        st = dd_restart_device(r->h, r->chain.data(), r->lens.data(), (uint32_t)r->chain.size(), r->id, r->out,
+                               r->len);
+        r->res.out_len = st == DD_OK ? r->len : 0;
+      }
+      r->res.t_end = steady_now();
+      r->res.status = st;
+      lk.lock();
+      busy.erase(r->h);
+      in_use -= r->footprint;  // the handle's state stays charged
+      r->done = true;
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+
+  dd_status submit(Request* r, uint64_t* ticket) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (stopping) return DD_ESTATE;
+    r->ticket = next_ticket++;
+    r->res.t_submit = steady_now();
+    r->res.kind = r->kind;
+    by_ticket[r->ticket] = r;
+    queue.push_back(r);
+    *ticket = r->ticket;
+    cv_work.notify_all();
+    return DD_OK;
+  }
+};
+
+extern "C" {
+
+double dd_now(void) { return steady_now(); }
+
+dd_status dd_server_create(uint64_t hbm_budget_bytes, uint32_t slots, dd_server** out) {
+  if (!out || slots == 0 || hbm_budget_bytes == 0) return DD_EINVAL;
+  *out = nullptr;
+  dd_server* s = new (std::nothrow) dd_server;
+  if (!s) return DD_ENOMEM;
+  s->budget = hbm_budget_bytes;
+  s->slots = slots;
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    return DD_EHIP;
+  }
+  for (uint32_t i = 0; i < slots; i++) s->workers.emplace_back([s] { s->worker(); });
+  *out = s;
+  return DD_OK;
+}
+
+dd_status dd_server_destroy(dd_server* s) {
+  if (!s) return DD_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    s->cv_work.notify_all();
+  }
+  for (auto& t : s->workers) t.join();
+  for (auto& kv : s->by_ticket) delete kv.second;
+  delete s;
+  return DD_OK;
+}
+
+dd_status dd_server_submit_checkpoint(dd_server* s, dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline,
+                                      void* dev_out, uint64_t out_cap, uint64_t* ticket) {
+  if (!s || !h || !dev_data || !dev_out || !ticket) return DD_EINVAL;
+  Request* r = new (std::nothrow) Request;
+  if (!r) return DD_ENOMEM;
+  r->kind = 0;
+  r->h = h;
+  r->data = dev_data;
+  r->len = len;
+  r->make_baseline = make_baseline;
+  r->out = dev_out;
+  r->out_cap = out_cap;
+  r->footprint = len + out_cap;
+  const dd_status st = s->submit(r, ticket);
+  if (st != DD_OK) delete r;
+  return st;
+}
+
+dd_status dd_server_submit_restart(dd_server* s, dd_dedup* h, const void* const* dev_chain, const uint64_t* lens,
+                                   uint32_t nchain, uint32_t chkpt_id, void* dev_out, uint64_t len, uint64_t* ticket) {
+  if (!s || !h || !dev_chain || !lens || !dev_out || !ticket) return DD_EINVAL;
+  Request* r = new (std::nothrow) Request;
+  if (!r) return DD_ENOMEM;
+  r->kind = 1;
+  r->h = h;
+  r->chain.assign(dev_chain, dev_chain + nchain);
+  r->lens.assign(lens, lens + nchain);
+  r->id = chkpt_id;
+  r->out = dev_out;
+  r->len = len;
+  // output buffer plus the restart scratch (chunk sources and cover table,
+  // 24 B per chunk at the smallest chunk size the chain may use)
+  r->footprint = len + 24 * (len / 16 + 1);
+  const dd_status st = s->submit(r, ticket);
+  if (st != DD_OK) delete r;
+  return st;
+}
+
+dd_status dd_server_wait(dd_server* s, uint64_t ticket, dd_request_result* out) {
+  if (!s || !out) return DD_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->by_ticket.find(ticket);
+  if (it == s->by_ticket.end()) return DD_EINVAL;
+  Request* r = it->second;
+  s->cv_done.wait(lk, [&] { return r->done; });
+  *out = r->res;
+  s->by_ticket.erase(it);
+  delete r;
+  return DD_OK;
+}
+
+dd_status dd_server_usage(dd_server* s, uint64_t* in_use, uint64_t* peak) {
+  if (!s) return DD_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (in_use) *in_use = s->in_use;
+  if (peak) *peak = s->peak;
+  return DD_OK;
+}
+
+}  // extern "C"

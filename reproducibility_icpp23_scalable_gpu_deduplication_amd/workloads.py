@@ -139,8 +139,10 @@ def chain_float_field(seed, n_values, nckpt, zero_frac=0.6, block_bytes=4096, pe
 
 
 def chain_uniform_changes_gpu(torch, seed, size, chunk, nckpt, fresh_frac=0.01, copy_frac=0.02,
-                              zero_bytes=1 << 20, device="cuda"):
-    """chain_uniform_changes generated in device memory (torch RNG)."""
+                              zero_bytes=1 << 20, device="cuda", zero_count=1):
+    """chain_uniform_changes generated in device memory (torch RNG);
+    `zero_count` aligned regions of `zero_bytes` are zero-filled per
+    checkpoint (one per 64 MiB keeps the 64 MiB recipe's density)."""
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
     C = size // chunk
@@ -160,8 +162,8 @@ def chain_uniform_changes_gpu(torch, seed, size, chunk, nckpt, fresh_frac=0.01, 
             src = torch.randint(0, C, (nc,), device=device, generator=gen)
             v[perm[nf:nf + nc]] = pv[src]
         if zero_bytes and size >= zero_bytes:
-            z = int(torch.randint(0, size // zero_bytes, (1,), generator=gen, device=device).item()) * zero_bytes
-            cur[z:z + zero_bytes] = 0
+            for z in torch.randint(0, size // zero_bytes, (zero_count,), generator=gen, device=device).tolist():
+                cur[z * zero_bytes:(z + 1) * zero_bytes] = 0
         out.append(cur)
     return out
 
