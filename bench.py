@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--cpu-size", type=int, default=256 << 20)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-buffer batched comparison")
     ap.add_argument("--batch-buffers", type=int, default=64)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (scales PMC byte counters) and exit")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop (serving) measurement")
     ap.add_argument("--serve-ranks", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals, requests/s")
@@ -343,6 +345,15 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        src = torch.randint(0, 256, (args.copy_calibration,), dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "copies": 5}), flush=True)
+        return
     kind = dd.HASH_MD5 if args.hash == "md5" else dd.HASH_MURMUR3
     r = run_config(torch, dd, W, args.size, args.chunk, args.frac, args.nckpt, args.steps, args.warmup, kind)
     workload = f"plain_{args.size >> 20}MiB_c{args.chunk}_{args.frac * 100:g}pct_{args.hash}"

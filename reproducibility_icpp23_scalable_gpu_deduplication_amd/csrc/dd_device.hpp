@@ -200,7 +200,7 @@ __device__ __forceinline__ uint32_t load_le32_bytes(const uint8_t* p, uint32_t n
 }
 
 // Murmur3 of an arbitrary byte range (any alignment).
-__device__ inline Digest murmur3_bytes(const uint8_t* p, uint64_t len) {
+__device__ __forceinline__ Digest murmur3_bytes(const uint8_t* p, uint64_t len) {
   Murmur m;
   uint64_t nb = len / 16;
   for (uint64_t i = 0; i < nb; i++) m.block(load_le64_bytes(p + 16 * i, 8), load_le64_bytes(p + 16 * i + 8, 8));
@@ -213,7 +213,7 @@ __device__ inline Digest murmur3_bytes(const uint8_t* p, uint64_t len) {
 }
 
 // Murmur3 of 16-byte aligned data.
-__device__ inline Digest murmur3_aligned(const uint8_t* p, uint64_t len) {
+__device__ __forceinline__ Digest murmur3_aligned(const uint8_t* p, uint64_t len) {
   Murmur m;
   const uint4* q = reinterpret_cast<const uint4*>(p);
   uint64_t nb = len / 16;
@@ -241,7 +241,7 @@ __device__ inline Digest murmur3_aligned(const uint8_t* p, uint64_t len) {
 
 // MD5 of an arbitrary byte range; `aligned` promises 4-byte alignment.
 template <bool Aligned>
-__device__ inline Digest md5_range(const uint8_t* p, uint64_t len) {
+__device__ __forceinline__ Digest md5_range(const uint8_t* p, uint64_t len) {
   uint32_t st[4];
   md5_init(st);
   uint64_t nfull = len / 64;

@@ -146,10 +146,13 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   DD_TRY(ptab.ensure(((uint64_t)K + 1) * 8));
   DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * 8, true, s));
   const uint64_t nlb = ceil_div(C, 256);
-  // blk_pend (one per leaf block) then blk_claim (one per insert_leaves
-  // workgroup: a baseline launches one per leaf block).
-  DD_TRY(leafblk.ensure((nlb + std::max<uint64_t>(nlb, kMaxStageWG)) * 4));
-  DD_TRY(pend_list.ensure(nlb * 256 * 4));
+  // blk_claim (one per insert_leaves workgroup) then int_claim (one per
+  // insert_interior workgroup); a baseline launches one of each per leaf block.
+  const uint64_t nclaim = std::max<uint64_t>(nlb, kMaxStageWG);
+  DD_TRY(leafblk.ensure(2 * nclaim * 4));
+  // Pending leaves: sub-list r takes leaf blocks b = r (mod 32), 256 each.
+  const uint64_t pend_cap = ceil_div(nlb, 32) * 256;
+  DD_TRY(pend_list.ensure(32 * pend_cap * 4));
   const TreeGeom gm = make_geom(C);
   uint64_t ntiles = 0;
   uint32_t tile_base[4] = {0, 0, 0, 0};
@@ -157,9 +160,10 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     tile_base[g] = (uint32_t)ntiles;
     ntiles += tiles_in_group(gm.D, g);
   }
-  // tile_fint (one per tile) then tile_claim (one per insert_interior
-  // workgroup: a baseline launches one per tile).
-  DD_TRY(tiles.ensure((ntiles + std::max<uint64_t>(ntiles, kMaxStageWG)) * 4));
+  // First-occurrence interior nodes: sub-list r takes tiles T = r (mod 32),
+  // at most 255 interior nodes each.
+  const uint64_t fint_cap = ceil_div(ntiles, 32) * 255;
+  DD_TRY(tiles.ensure(32 * fint_cap * 4));
   // Per group-0 tile stamp of the last checkpoint that changed one of its
   // leaves; zero-filled when (re)allocated, epochs start at 1.
   const uint64_t ntiles0 = tiles_in_group(gm.D, 0);
@@ -169,8 +173,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     epoch = 0;
   }
   epoch += 1;
-  const uint64_t dirty_cap = ceil_div(ntiles0, 32), act_cap = ceil_div(nlb, 32);
-  DD_TRY(lists.ensure((64 + 32 * dirty_cap + 32 * act_cap) * 4));
+  const uint64_t dirty_cap = ceil_div(ntiles0, 32);
+  DD_TRY(lists.ensure((32 + 32 * dirty_cap) * 4));
   const uint64_t need = max_diff_size(len, chunk_size, K);
   if (!out) {
     DD_TRY(diff.ensure(need));
@@ -199,22 +203,22 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.map_limit = map_limit_of(map_cap);
   job.ctr = ctr.as<JobCounters>();
   job.host_ctr = nullptr;  // set by the pipeline
-  job.blk_pend = leafblk.as<uint32_t>();
-  job.blk_claim = job.blk_pend + nlb;
-  job.pend_acc = reinterpret_cast<unsigned long long*>(ctr.as<uint8_t>() + sizeof(JobCounters));
+  job.blk_claim = leafblk.as<uint32_t>();
+  job.int_claim = job.blk_claim + nclaim;
+  // 32 pending-leaf counters and 32 interior-list counters follow the counters
+  job.pend_cnt = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
+  job.fint_cnt = job.pend_cnt + 32;
   job.pend_list = pend_list.as<uint32_t>();
+  job.pend_cap = (uint32_t)pend_cap;
+  job.fint_list = tiles.as<uint32_t>();
+  job.fint_cap = (uint32_t)fint_cap;
   for (int g = 0; g < 4; g++) job.tile_base[g] = tile_base[g];
   job.ntiles = (uint32_t)ntiles;
-  job.tile_fint = tiles.as<uint32_t>();
-  job.tile_claim = job.tile_fint + ntiles;
   job.epoch = epoch;
   job.tile_epoch = tile_epoch.as<uint32_t>();
   job.dirty_cnt = lists.as<uint32_t>();
-  job.act_cnt = job.dirty_cnt + 32;
   job.dirty_cap = (uint32_t)dirty_cap;
-  job.act_cap = (uint32_t)act_cap;
-  job.dirty_list = job.act_cnt + 32;
-  job.act_list = job.dirty_list + 32 * dirty_cap;
+  job.dirty_list = job.dirty_cnt + 32;
   job.blk_f = blk.as<uint32_t>();
   job.blk_fc = job.blk_f + kMaxStageWG;
   job.blk_s = job.blk_fc + kMaxStageWG;
@@ -311,15 +315,17 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
         case S_LEAF: n = ceil_div(J.C, 256); break;
         // A baseline visits every leaf block and tile (one workgroup each);
         // an incremental checkpoint strides over its work lists.
-        case S_INSL: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
-        case S_TILES: n = J.is_baseline ? J.ntiles : std::min<uint64_t>(J.ntiles, kMaxStageWG); break;
+        // The insert stages take one list entry per lane: a baseline lists
+        // (nearly) every leaf and interior node.
+        case S_INSL:
+        case S_TILES: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
         case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kMaxStageWG); break;
         case S_STILE: n = ceil_div(J.C, kSTile); break;
         case S_META: n = std::min<uint64_t>(ceil_div(J.C, 256), 256); break;
         case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
         default: {
           const uint64_t tg = tiles_in_group(J.D, (int)st - S_TILE0);
-          n = J.is_baseline ? tg : std::min<uint64_t>(tg, kMaxStageWG);
+          n = J.is_baseline ? tg : std::min<uint64_t>(tg, kMaxTileWG);
           break;
         }
       }

@@ -110,17 +110,17 @@ __device__ __forceinline__ T block_sum(T v, T* sh) {
 // begin: zero the counters of every job (or only the flags, for a re-run of
 // the map inserts after the map has grown).
 // ---------------------------------------------------------------------------
-__global__ void k_begin(const JobDev* jobs, uint32_t njobs, int rerun) {
+__global__ void k_begin(const JobDev* __restrict__ jobs, uint32_t njobs, int rerun) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= njobs) return;
   JobCounters* c = jobs[j].ctr;
   c->map_count = jobs[j].map_count_at_start;
   c->flags = 0;
-  if (rerun) return;  // keep the changed-leaf counts of leaf_hash when re-running the inserts
+  for (uint32_t i = 0; i < 32; i++) jobs[j].fint_cnt[i] = 0;  // pass 1 runs again on a re-run
+  if (rerun) return;  // keep the changed-leaf lists of leaf_hash when re-running the inserts
   for (uint32_t i = 0; i < 32; i++) {
-    jobs[j].pend_acc[i] = 0;
+    jobs[j].pend_cnt[i] = 0;
     jobs[j].dirty_cnt[i] = 0;
-    jobs[j].act_cnt[i] = 0;
   }
   c->pending = 0;
   c->nfirst = c->nshift = c->nprior = 0;
@@ -135,10 +135,9 @@ __global__ void k_begin(const JobDev* jobs, uint32_t njobs, int rerun) {
 // Reference: "Checkpoint K: Leaves", src/tree_approach.cpp:305-351 and
 // include/reference_impl.hpp:43-71.
 // ---------------------------------------------------------------------------
-// The changed leaves of a leaf block (256 chunks) go to its segment of
-// pend_list and their number to blk_pend[b].  The job's total is accumulated
-// in 32 counters picked by block index: one counter hit by every workgroup
-// would serialize (32K same-address atomics per GiB at 128-byte chunks).
+// Work lists are split into 32 sub-lists picked by the producer's block or
+// tile index: one counter hit by every workgroup would serialize (32K
+// same-address atomics per GiB at 128-byte chunks).
 constexpr uint32_t kPendSpread = 32;
 
 // Group-0 tile of a leaf (leaves sit on the deepest level D or on D-1).
@@ -154,17 +153,18 @@ __device__ __forceinline__ void list_append(uint32_t* cnt, uint32_t* list, uint3
   list[(uint64_t)r * cap + k] = item;
 }
 
+// The leaves of leaf block b (256 chunks) that need a map insert are appended
+// to sub-list b % 32 of pend_list: one atomic per block reserves the space.
 __device__ __forceinline__ void record_pending(const JobDev& J, uint32_t b, uint64_t k, bool pending) {
   __shared__ uint32_t sh[4];
+  __shared__ uint32_t sbase;
   uint32_t tot;
   const int pos = block_compact(pending, sh, tot);
-  if (pos >= 0) J.pend_list[(uint64_t)b * 256 + pos] = (uint32_t)k;
-  if (threadIdx.x == 0) {
-    J.blk_pend[b] = tot;
-    if (tot) {
-      atomicAdd(&J.pend_acc[b % kPendSpread], (unsigned long long)tot);
-      if (!J.is_baseline) list_append(J.act_cnt, J.act_list, J.act_cap, b);
-    }
+  if (tot) {  // uniform over the block
+    const uint32_t r = b % kPendSpread;
+    if (threadIdx.x == 0) sbase = atomicAdd(&J.pend_cnt[r], tot);
+    __syncthreads();
+    if (pos >= 0) J.pend_list[(uint64_t)r * J.pend_cap + sbase + pos] = (uint32_t)k;
   }
   // Mark the group-0 tiles of changed leaves: the first marker of a tile in
   // this checkpoint appends it to the dirty list.  A baseline visits every
@@ -206,12 +206,12 @@ __device__ __forceinline__ uint32_t list_item(const uint32_t* list, uint32_t cap
 __device__ __forceinline__ uint64_t pending_total(const JobDev& J) {
   uint64_t t = 0;
 #pragma unroll
-  for (uint32_t i = 0; i < kPendSpread; i++) t += J.pend_acc[i];
+  for (uint32_t i = 0; i < kPendSpread; i++) t += J.pend_cnt[i];
   return t;
 }
 
 template <int Kind, bool Aligned16>
-__global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   const uint64_t k = (uint64_t)(blockIdx.x - prefix[j]) * blockDim.x + threadIdx.x;
@@ -275,7 +275,7 @@ __device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const D
 // next SEG bytes of its own chunk out of LDS.  Needs chunk % SEG == 0 and a
 // 16-byte aligned buffer; the partial last chunk is hashed from global memory.
 template <int Kind, int SEG>
-__global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   constexpr int PITCH = SEG + 16;
   constexpr int VPC = SEG / 16;        // 16-byte vectors per chunk per round
   constexpr int LOADS = VPC;           // 64 chunks x VPC vectors spread over 64 lanes
@@ -376,14 +376,28 @@ struct InsertResult {
 // checkpoint was claimed by a leaf of the same kernel, so its key is read from
 // the tree; during the interior kernel, leaf entries already have their key
 // in the slot and only interior entries need the tree.
+// The job fields an insert needs, read once by the caller (the compiler would
+// reload them from the job table after every atomic).
+struct InsertCtx {
+  MapView m;
+  const Digest* tree;
+  uint64_t C;
+  uint32_t ckpt;
+  bool cas_first;  // a baseline starts from an empty map: CAS before any load
+};
+
+__device__ __forceinline__ InsertCtx insert_ctx(const JobDev& J) {
+  return InsertCtx{J.map, J.tree, J.C, J.ckpt_id, J.is_baseline != 0};
+}
+
 template <bool Leaf>
-__device__ inline InsertResult map_insert(const JobDev& J, const Digest& k, uint32_t node) {
-  const MapView& m = J.map;
-  const unsigned long long mine = pack_node(J.ckpt_id, node);
+__device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, uint32_t node) {
+  const MapView& m = J.m;
+  const unsigned long long mine = pack_node(J.ckpt, node);
   uint64_t s = map_start(m, k) & m.mask;
   // In a baseline the map starts empty, so the first probe goes straight to
   // the CAS (a failed CAS returns the slot like a load would).
-  const bool cas_first = J.is_baseline != 0;
+  const bool cas_first = J.cas_first;
   for (uint64_t it = 0; it <= m.mask; it++) {
     Slot* sl = &m.slots[s];
     unsigned long long v = (cas_first && it == 0)
@@ -399,7 +413,7 @@ __device__ inline InsertResult map_insert(const JobDev& J, const Digest& k, uint
       v = old;
     }
     const uint32_t vt = (uint32_t)(v >> 32), vn = (uint32_t)v;
-    const bool this_kernel = vt == J.ckpt_id && (Leaf || vn < J.C - 1);
+    const bool this_kernel = vt == J.ckpt && (Leaf || vn < J.C - 1);
     Digest e;
     if (this_kernel) {
       e = J.tree[vn];
@@ -409,7 +423,7 @@ __device__ inline InsertResult map_insert(const JobDev& J, const Digest& k, uint
     }
     if (digest_eq(e, k)) {
       unsigned long long prev = v;
-      if (vt == J.ckpt_id) prev = Leaf ? atomicMin(&sl->val, mine) : atomicMax(&sl->val, mine);
+      if (vt == J.ckpt) prev = Leaf ? atomicMin(&sl->val, mine) : atomicMax(&sl->val, mine);
       return InsertResult{true, false, s, prev};
     }
     s = (s + 1) & m.mask;
@@ -423,8 +437,8 @@ __device__ __forceinline__ bool map_guard_tripped(const JobDev& J) {
   return J.map_count_at_start + 2 * pending_total(J) > J.map_limit;
 }
 
-// Workgroups stride over the leaf blocks that hold changed leaves.  Labels
-// settle without a later map read: a changed leaf stays PENDING
+// One lane per entry of the pending-leaf list (grid-stride).  Labels settle
+// without a later map read: a changed leaf stays PENDING
 // (first-occurrence candidate) unless it meets an older entry or a lower leaf
 // of this checkpoint (then it labels itself SHIFT), and a leaf that lowers an
 // entry demotes the previous lowest leaf to SHIFT.  Every leaf that is ever
@@ -432,14 +446,16 @@ __device__ __forceinline__ bool map_guard_tripped(const JobDev& J) {
 // reference's first occurrence stays PENDING (include/reference_impl.hpp:49-67).
 // FIXED|PENDING leaves (baseline, zero digest) take part in the ordering but
 // keep their label.
-__global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   uint32_t claims = 0;
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
-  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) {
+  // Other workgroups of this kernel may raise MAP_FULL while it runs: the
+  // decision is made uniform over the workgroup before any barrier.
+  if (__syncthreads_or((J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0)) {
     if (threadIdx.x == 0) J.blk_claim[wg] = 0;
     return;
   }
@@ -450,25 +466,30 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* jobs, const
     }
     return;
   }
+  // A baseline inserts every leaf: lane k takes chunk k directly (the list
+  // would hold every chunk too).
   const bool dense = J.is_baseline != 0;
-  uint32_t nitems = (uint32_t)((J.C + 255) / 256);
+  uint32_t nitems = (uint32_t)J.C;
   if (!dense) {
-    load_list_prefix(J.act_cnt, pre);
+    load_list_prefix(J.pend_cnt, pre);
     nitems = pre[kPendSpread];
   }
   const uint32_t K = J.ckpt_id;
-  for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint32_t b = dense ? it : list_item(J.act_list, J.act_cap, pre, it);
-    const uint32_t n = J.blk_pend[b];
-    if (threadIdx.x >= n) continue;
-    const uint64_t leaf = J.C - 1 + J.pend_list[(uint64_t)b * 256 + threadIdx.x];
-    const uint8_t own = J.labels[leaf];
-    const InsertResult r = map_insert<true>(J, J.tree[leaf], (uint32_t)leaf);
+  const InsertCtx ic = insert_ctx(J);
+  uint8_t* const labels = J.labels;
+  uint32_t* const slot_of = J.slot_of;
+  const uint32_t* const pend_list = J.pend_list;
+  const uint32_t pend_cap = J.pend_cap;
+  uint32_t* const flags = &J.ctr->flags;
+  for (uint32_t it = wg * blockDim.x + threadIdx.x; it < nitems; it += nwg * blockDim.x) {
+    const uint64_t leaf = ic.C - 1 + (dense ? it : list_item(pend_list, pend_cap, pre, it));
+    const uint8_t own = labels[leaf];
+    const InsertResult r = map_insert<true>(ic, ic.tree[leaf], (uint32_t)leaf);
     if (!r.ok) {
-      atomicOr(&J.ctr->flags, FLAG_MAP_FULL);
+      atomicOr(flags, FLAG_MAP_FULL);
       continue;
     }
-    J.slot_of[leaf] = (uint32_t)r.slot;
+    slot_of[leaf] = (uint32_t)r.slot;
     if (r.claimed) {
       claims++;
       continue;
@@ -478,12 +499,12 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* jobs, const
     if (pt == K) {
       if (pn > leaf) {
         self_shift = false;  // this leaf is now the lowest; the previous lowest is not
-        if (J.labels[pn] == PENDING) J.labels[pn] = SHIFT_DUPL;
+        if (labels[pn] == PENDING) labels[pn] = SHIFT_DUPL;
       } else if (pn == leaf) {
         self_shift = false;  // re-run after a map growth: own entry
       }
     }
-    if (self_shift && own == PENDING) J.labels[leaf] = SHIFT_DUPL;
+    if (self_shift && own == PENDING) labels[leaf] = SHIFT_DUPL;
   }
   const uint32_t n = block_sum<uint32_t>(claims, sh);
   if (threadIdx.x == 0) J.blk_claim[wg] = n;
@@ -514,49 +535,39 @@ __device__ __forceinline__ uint64_t tile_node(uint64_t r, uint32_t i) {
   return ((r + 1) << q) - 1 + (i + 1 - (1u << q));
 }
 
-// Work items of the interior inserts and of the upper tile groups: the dirty
-// group-0 tiles, then every tile of the groups above.
-__global__ __launch_bounds__(256) void k_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+// Map inserts of the first-occurrence interior nodes that pass 1 listed: one
+// lane per list entry (grid-stride).
+__global__ __launch_bounds__(256) void k_insert_interior(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
   uint32_t claims = 0;
-  if (!(J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
+  if (!__syncthreads_or((J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0)) {
+    // A baseline visits every interior node 0 .. C-2 by label (most of them
+    // are first occurrences); an incremental checkpoint walks the list.
     const bool dense = J.is_baseline != 0;
-    const uint32_t ntiles0 = 1u << (J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0);
-    uint32_t nd = ntiles0;
+    uint32_t nitems = (uint32_t)(J.C - 1);
     if (!dense) {
-      load_list_prefix(J.dirty_cnt, pre);
-      nd = pre[kPendSpread];
+      load_list_prefix(J.fint_cnt, pre);
+      nitems = pre[kPendSpread];
     }
-    const uint32_t nup = J.ntiles - ntiles0;  // tiles of the groups above group 0
-    for (uint32_t it = wg; it < nd + nup; it += nwg) {
-      uint32_t T;
-      int g = 0;
-      uint64_t t;
-      if (it < nd) {
-        T = dense ? it : list_item(J.dirty_list, J.dirty_cap, pre, it);
-        t = T;
-      } else {
-        T = ntiles0 + (it - nd);
-        g = 1;
-        while (g + 1 < 4 && g + 1 < tile_groups(J.D) && J.tile_base[g + 1] <= T) g++;
-        t = T - J.tile_base[g];
-      }
-      if (J.tile_fint[T] == 0) continue;
-      const TileRef tr = tile_ref(J, g, t);
-      if (threadIdx.x + 1 >= (1u << tr.H)) continue;  // 2^H - 1 interior nodes
-      const uint64_t x = tile_node(tr.r, threadIdx.x);
-      if (x + 1 >= J.C || J.labels[x] != FIRST_OCUR) continue;
-      const InsertResult r = map_insert<false>(J, J.tree[x], (uint32_t)x);
-      if (!r.ok) atomicOr(&J.ctr->flags, FLAG_MAP_FULL);
+    const InsertCtx ic = insert_ctx(J);
+    const uint8_t* const labels = J.labels;
+    const uint32_t* const fint_list = J.fint_list;
+    const uint32_t fint_cap = J.fint_cap;
+    uint32_t* const flags = &J.ctr->flags;
+    for (uint32_t it = wg * blockDim.x + threadIdx.x; it < nitems; it += nwg * blockDim.x) {
+      const uint32_t x = dense ? it : list_item(fint_list, fint_cap, pre, it);
+      if (dense && labels[x] != FIRST_OCUR) continue;
+      const InsertResult r = map_insert<false>(ic, ic.tree[x], x);
+      if (!r.ok) atomicOr(flags, FLAG_MAP_FULL);
       claims += r.claimed ? 1 : 0;
     }
   }
   const uint32_t n = block_sum<uint32_t>(claims, sh);
-  if (threadIdx.x == 0) J.tile_claim[wg] = n;
+  if (threadIdx.x == 0) J.int_claim[wg] = n;
 }
 
 // ---------------------------------------------------------------------------
@@ -596,136 +607,180 @@ __device__ __forceinline__ uint32_t group_items(const JobDev& J, int g, uint32_t
   return 1u << (Lb > kTileLevels ? Lb - kTileLevels : 0);
 }
 
+// Both passes process a tile with one wavefront (a 64-lane workgroup) so that
+// many tiles are in flight: the per-tile work is a chain of dependent loads
+// and level steps, so the passes are bound by latency, not by bandwidth.
+// One LDS buffer holds the current level; every step is split into a read
+// phase (children) and a write phase (parents) by a barrier, so parents can
+// overwrite the children in place.  The loops over 64-node chunks have a
+// trip count that is uniform over the workgroup.
+constexpr uint32_t kTileThreads = 64;
+
 // Pass 1, the first-occurrence forest (include/reference_impl.hpp:73-82).
 // Group 0 visits the dirty tiles only; the groups above visit every tile.
-__global__ __launch_bounds__(256) void k_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, int g) {
+// In an incremental checkpoint the first-occurrence interior nodes of a tile
+// are collected in LDS and appended to the fint list for the interior map
+// inserts (a baseline's interior inserts visit every node instead).
+// Job fields are read once into locals: the compiler cannot tell the job
+// table apart from the arrays the kernel stores to, and would reload them.
+__global__ __launch_bounds__(kTileThreads) void k_pass1(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
+                                                        int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   if (g >= tile_groups(J.D)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
-  __shared__ uint8_t lab[2][256];
-  __shared__ Digest dig[2][256];
+  __shared__ uint8_t lab[256];
+  __shared__ Digest dig[256];
   __shared__ uint32_t pre[kPendSpread + 1];
+  __shared__ uint32_t fl[256];
+  __shared__ uint32_t fn, fbase;
   const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
+  const bool listing = !J.is_baseline;
+  const uint64_t N = J.N, C = J.C;
+  uint8_t* const labels = J.labels;
+  Digest* const tree = J.tree;
+  const int kind = J.hash_kind;
+  uint32_t* const fint_list = J.fint_list;
+  uint32_t* const fint_cnt = J.fint_cnt;
+  const uint64_t fint_cap = J.fint_cap;
+  const uint32_t tbase = J.tile_base[g];
   for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint64_t t = (g == 0 && !J.is_baseline) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
+    const uint64_t t = (g == 0 && listing) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
     const TileRef tr = tile_ref(J, g, t);
     const int H = tr.H;
     const uint64_t r = tr.r;
     const uint32_t width = 1u << H;
-    if (i < width) {
-      const uint64_t x = ((r + 1) << H) - 1 + i;
+    if (i == 0) fn = 0;
+    for (uint32_t idx = i; idx < width; idx += kTileThreads) {
+      const uint64_t x = ((r + 1) << H) - 1 + idx;
       uint8_t L = 0xff;
-      Digest d{0, 0};
-      if (x < J.N) {
-        if (g == 0) L = resolve_leaf(J, x);
-        else L = child_tile_clean(J, g, x) ? (uint8_t)DONE : J.labels[x];
-        if (L == FIRST_OCUR) d = J.tree[x];
-      }
-      lab[0][i] = L;
-      dig[0][i] = d;
+      if (x < N) L = g == 0 ? resolve_leaf(J, x) : child_tile_clean(J, g, x) ? (uint8_t)DONE : labels[x];
+      lab[idx] = L;
+      dig[idx] = L == FIRST_OCUR ? tree[x] : Digest{0, 0};
     }
-    bool any_first = false;
-    int cur = 0;
     for (int st = 1; st <= H; st++) {
-      __syncthreads();
       const uint32_t w = width >> st;
-      if (i < w) {
-        const uint64_t x = ((r + 1) << (H - st)) - 1 + i;
+      const uint64_t lvl = ((r + 1) << (H - st)) - 1;
+      for (uint32_t base = 0; base < w; base += kTileThreads) {
+        const uint32_t idx = base + i;
+        const bool act = idx < w;
+        const uint64_t x = lvl + idx;
+        const bool interior = act && x < C - 1;
+        __syncthreads();  // the previous writes are visible
+        const bool both = interior && lab[2 * idx] == FIRST_OCUR && lab[2 * idx + 1] == FIRST_OCUR;
+        const Digest a = both ? dig[2 * idx] : Digest{0, 0};
+        const Digest b = both ? dig[2 * idx + 1] : Digest{0, 0};
+        __syncthreads();  // every child of this chunk is read before any parent is written
+        if (!act) continue;
         uint8_t L;
         Digest d{0, 0};
-        if (x >= J.C - 1) {  // a leaf one level above the deepest
+        if (!interior) {  // a leaf one level above the deepest
           L = resolve_leaf(J, x);
-          if (L == FIRST_OCUR) d = J.tree[x];
+          if (L == FIRST_OCUR) d = tree[x];
+        } else if (both) {
+          L = FIRST_OCUR;
+          d = hash_pair(kind, a, b);
+          tree[x] = d;
+          labels[x] = L;
+          if (listing) fl[atomicAdd(&fn, 1u)] = (uint32_t)x;
         } else {
-          const uint8_t a = lab[cur][2 * i], b = lab[cur][2 * i + 1];
-          if (a == FIRST_OCUR && b == FIRST_OCUR) {
-            L = FIRST_OCUR;
-            d = hash_pair(J.hash_kind, dig[cur][2 * i], dig[cur][2 * i + 1]);
-            J.tree[x] = d;
-            any_first = true;
-          } else {
-            L = DONE;
-          }
-          J.labels[x] = L;
+          L = DONE;
+          labels[x] = L;
         }
-        lab[cur ^ 1][i] = L;
-        dig[cur ^ 1][i] = d;
+        lab[idx] = L;
+        dig[idx] = d;
       }
-      cur ^= 1;
     }
-    const bool tf = __syncthreads_or(any_first);
-    if (i == 0) J.tile_fint[J.tile_base[g] + t] = tf ? 1u : 0u;
+    __syncthreads();
+    const uint32_t cnt = fn;  // at most 255 interior nodes per tile
+    if (cnt) {
+      const uint32_t rl = (tbase + (uint32_t)t) % kPendSpread;
+      if (i == 0) fbase = atomicAdd(&fint_cnt[rl], cnt);
+      __syncthreads();
+      for (uint32_t q = i; q < cnt; q += kTileThreads) fint_list[rl * fint_cap + fbase + q] = fl[q];
+    }
+    __syncthreads();  // fl, fn and the level buffer are reused by the next tile
   }
 }
 
 // Pass 2, the duplicate forest (include/reference_impl.hpp:83-104).
-__global__ __launch_bounds__(256) void k_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, int g) {
+__global__ __launch_bounds__(kTileThreads) void k_pass2(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
+                                                        int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   if (g >= tile_groups(J.D)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
-  __shared__ uint8_t lab[2][256];
-  __shared__ Digest dig[2][256];
+  __shared__ uint8_t lab[256];
+  __shared__ Digest dig[256];
   __shared__ uint32_t pre[kPendSpread + 1];
   const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
+  const uint64_t N = J.N, C = J.C;
+  uint8_t* const labels = J.labels;
+  Digest* const tree = J.tree;
+  uint32_t* const slot_of = J.slot_of;
+  const int kind = J.hash_kind;
+  const MapView map = J.map;
   for (uint32_t it = wg; it < nitems; it += nwg) {
     const uint64_t t = (g == 0 && !J.is_baseline) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
     const TileRef tr = tile_ref(J, g, t);
     const int H = tr.H;
     const uint64_t r = tr.r;
     const uint32_t width = 1u << H;
-    if (i < width) {
-      const uint64_t x = ((r + 1) << H) - 1 + i;
+    for (uint32_t idx = i; idx < width; idx += kTileThreads) {
+      const uint64_t x = ((r + 1) << H) - 1 + idx;
       uint8_t L = 0xff;
-      Digest d{0, 0};
-      if (x < J.N) {
-        L = child_tile_clean(J, g, x) ? (uint8_t)FIXED_DUPL : J.labels[x];
-        if (L == SHIFT_DUPL) d = J.tree[x];
-      }
-      lab[0][i] = L;
-      dig[0][i] = d;
+      if (x < N) L = child_tile_clean(J, g, x) ? (uint8_t)FIXED_DUPL : labels[x];
+      lab[idx] = L;
+      dig[idx] = L == SHIFT_DUPL ? tree[x] : Digest{0, 0};
     }
-    int cur = 0;
     for (int st = 1; st <= H; st++) {
-      __syncthreads();
       const uint32_t w = width >> st;
-      if (i < w) {
-        const uint64_t x = ((r + 1) << (H - st)) - 1 + i;
+      const uint64_t lvl = ((r + 1) << (H - st)) - 1;
+      for (uint32_t base = 0; base < w; base += kTileThreads) {
+        const uint32_t idx = base + i;
+        const bool act = idx < w;
+        const uint64_t x = lvl + idx;
+        const bool interior = act && x < C - 1;
+        __syncthreads();  // the previous writes are visible
+        const uint8_t la = interior ? lab[2 * idx] : 0, lb = interior ? lab[2 * idx + 1] : 0;
+        const bool both = interior && la == SHIFT_DUPL && lb == SHIFT_DUPL;
+        const Digest a = both ? dig[2 * idx] : Digest{0, 0};
+        const Digest b = both ? dig[2 * idx + 1] : Digest{0, 0};
+        __syncthreads();  // every child of this chunk is read before any parent is written
+        if (!act) continue;
         uint8_t L;
         Digest d{0, 0};
-        if (x >= J.C - 1) {
-          L = J.labels[x];
-          if (L == SHIFT_DUPL) d = J.tree[x];
+        if (!interior) {
+          L = labels[x];
+          if (L == SHIFT_DUPL) d = tree[x];
         } else {
-          const uint8_t a = lab[cur][2 * i], b = lab[cur][2 * i + 1];
-          if (a != b) {
+          if (la != lb) {
             L = DONE;
-          } else if (a == SHIFT_DUPL) {
-            d = hash_pair(J.hash_kind, dig[cur][2 * i], dig[cur][2 * i + 1]);
-            const uint64_t s = map_find(J.map, d);
+          } else if (both) {
+            d = hash_pair(kind, a, b);
+            const uint64_t s = map_find(map, d);
             if (s != ~0ull) {
               L = SHIFT_DUPL;
-              J.tree[x] = d;
-              J.slot_of[x] = (uint32_t)s;
+              tree[x] = d;
+              slot_of[x] = (uint32_t)s;
             } else {
               L = DONE;
+              d = Digest{0, 0};
             }
           } else {
-            L = a;  // both FIXED, both FIRST or both DONE
+            L = la;  // both FIXED, both FIRST or both DONE
           }
-          J.labels[x] = L;
+          labels[x] = L;
         }
-        lab[cur ^ 1][i] = L;
-        dig[cur ^ 1][i] = d;
+        lab[idx] = L;
+        dig[idx] = d;
       }
-      cur ^= 1;
     }
-    __syncthreads();
+    __syncthreads();  // the level buffer is reused by the next tile
   }
 }
 
@@ -733,13 +788,6 @@ __global__ __launch_bounds__(256) void k_pass2(const JobDev* jobs, const uint32_
 // Roots.  Node v is a root when it is FIRST or SHIFT and its parent is DONE;
 // node 0 is a root when it is FIRST (include/reference_impl.hpp:85-106).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint8_t root_kind(const uint8_t* labels, uint64_t v) {
-  const uint8_t L = labels[v];
-  if (L != FIRST_OCUR && L != SHIFT_DUPL) return 0xff;
-  if (v == 0) return L == FIRST_OCUR ? L : 0xff;
-  return labels[(v - 1) >> 1] == DONE ? L : 0xff;
-}
-
 __device__ __forceinline__ uint64_t leaf_count(const TreeGeom& g, uint64_t v) {
   LeafRuns r = subtree_runs(g, v);
   return r.n0 + r.n1;
@@ -754,17 +802,6 @@ __device__ __forceinline__ void node_block_range(const JobDev& J, uint32_t wg, u
   b1 = min(J.nblk, b0 + per);
 }
 
-// Nodes of group-0 tiles with no changed leaf were not visited this
-// checkpoint: all of them are FIXED whatever their stored label says.
-__device__ __forceinline__ bool node_in_clean_tile(const JobDev& J, uint64_t v) {
-  if (J.is_baseline) return false;
-  const uint32_t Lr0 = J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0;
-  const uint32_t lv = ilog2_64(v + 1);
-  if (lv < Lr0) return false;
-  const uint64_t T0 = ((v + 1) >> (lv - Lr0)) - (1ull << Lr0);
-  return J.tile_epoch[T0] != J.epoch;
-}
-
 // True when some byte of the 16 labels is FIRST (0) or SHIFT (2).
 __device__ __forceinline__ bool any_root_candidate(const uint4 w) {
   auto z = [](uint32_t x) {
@@ -774,33 +811,58 @@ __device__ __forceinline__ bool any_root_candidate(const uint4 w) {
   return z(w.x) || z(w.y) || z(w.z) || z(w.w);
 }
 
-// Counts (first roots, chunks under them, shifted roots) of the 16 nodes
-// starting at v0; `kinds` receives each node's root kind (0xff: none).
-__device__ __forceinline__ void scan16(const JobDev& J, const TreeGeom& g, uint64_t v0, uint8_t kinds[16],
-                                       uint32_t& f, uint32_t& fc, uint32_t& s) {
-  f = fc = s = 0;
-#pragma unroll
-  for (int t = 0; t < 16; t++) kinds[t] = 0xff;
+__device__ __forceinline__ uint8_t byte_of(const uint4 w, int t) {
+  const uint32_t x = t < 4 ? w.x : t < 8 ? w.y : t < 12 ? w.z : w.w;
+  return (uint8_t)(x >> (8 * (t & 3)));
+}
+
+// Root kinds of the 16 nodes starting at v0 (a multiple of 16) as a bit mask
+// per kind (bit t: node v0 + t).  The nodes' labels are one 16-byte load; the
+// parents (v - 1) / 2 of v0 .. v0+15 are v0/2 - 1 .. v0/2 + 7, read as two
+// aligned 8-byte loads.  Labels of nodes the checkpoint did not visit are
+// FIXED (tile reset after every checkpoint), so most groups stop at the
+// first load.  The labels array is padded past N.
+__device__ __forceinline__ void scan16(const JobDev& J, uint64_t v0, uint32_t& fmask, uint32_t& smask) {
+  fmask = smask = 0;
   if (v0 >= J.N) return;
-  const uint4 w = *reinterpret_cast<const uint4*>(J.labels + v0);  // labels are padded past N
+  const uint4 w = *reinterpret_cast<const uint4*>(J.labels + v0);
   if (!any_root_candidate(w)) return;
+  const uint64_t h = v0 >> 1;
+  const uint2 lo = v0 >= 16 ? *reinterpret_cast<const uint2*>(J.labels + h - 8) : make_uint2(0, 0);
+  const uint2 hi = *reinterpret_cast<const uint2*>(J.labels + h);
+  const uint64_t pw = (uint64_t)hi.x | ((uint64_t)hi.y << 32);  // parent bytes v0/2 .. v0/2+7
+  const uint8_t p_first = (uint8_t)(lo.y >> 24);                 // parent byte v0/2 - 1
 #pragma unroll
   for (int t = 0; t < 16; t++) {
     const uint64_t v = v0 + t;
     if (v >= J.N) break;
-    uint8_t rk = root_kind(J.labels, v);
-    if (rk != 0xff && node_in_clean_tile(J, v)) rk = 0xff;  // stale label of an unvisited tile
-    kinds[t] = rk;
-    if (rk == FIRST_OCUR) {
-      f++;
-      fc += (uint32_t)leaf_count(g, v);
-    } else if (rk == SHIFT_DUPL) {
-      s++;
+    const uint8_t L = byte_of(w, t);
+    if (L != FIRST_OCUR && L != SHIFT_DUPL) continue;
+    bool root;
+    if (v == 0) {
+      root = L == FIRST_OCUR;  // node 0 is emitted only as a first occurrence
+    } else {
+      const uint8_t P = t == 0 ? p_first : (uint8_t)(pw >> (8 * ((t - 1) >> 1)));
+      root = P == DONE;
     }
+    if (!root) continue;
+    if (L == FIRST_OCUR) fmask |= 1u << t;
+    else smask |= 1u << t;
   }
 }
 
-__global__ __launch_bounds__(256) void k_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+// Chunks under the first-occurrence roots of a mask.
+__device__ __forceinline__ uint32_t first_chunks(const TreeGeom& g, uint64_t v0, uint32_t fmask) {
+  uint32_t fc = 0;
+  while (fmask) {
+    const int t = __builtin_ctz(fmask);
+    fmask &= fmask - 1;
+    fc += (uint32_t)leaf_count(g, v0 + t);
+  }
+  return fc;
+}
+
+__global__ __launch_bounds__(256) void k_roots_count(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
@@ -810,12 +872,12 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobDev* jobs, const u
   const TreeGeom g = make_geom(J.C);
   uint32_t f = 0, fc = 0, s = 0;
   for (uint32_t b = b0; b < b1; b++) {
-    uint8_t kinds[16];
-    uint32_t a, c, d;
-    scan16(J, g, (uint64_t)b * kNodesPerBlock + threadIdx.x * 16, kinds, a, c, d);
-    f += a;
-    fc += c;
-    s += d;
+    const uint64_t v0 = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
+    uint32_t fm, sm;
+    scan16(J, v0, fm, sm);
+    f += __popc(fm);
+    fc += first_chunks(g, v0, fm);
+    s += __popc(sm);
   }
   __shared__ uint32_t sh[4];
   f = block_sum(f, sh);
@@ -828,39 +890,53 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobDev* jobs, const u
   }
 }
 
-// One workgroup per job: exclusive scans of the per-workgroup counts.
-__global__ __launch_bounds__(1024) void k_roots_scan(const JobDev* jobs, const uint32_t* prefix) {
+// One workgroup per job: exclusive scans of the per-workgroup counts.  Each
+// lane owns up to kScanPer consecutive entries (the roots grid is at most
+// 1024 * kScanPer workgroups per job), so one block scan per array suffices.
+constexpr uint32_t kScanPer = 4;
+static_assert(kScanPer * 1024 >= kMaxStageWG, "roots grid larger than one scan workgroup covers");
+__global__ __launch_bounds__(1024) void k_roots_scan(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix) {
   const JobDev& J = jobs[blockIdx.x];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t nwg = prefix[blockIdx.x + 1] - prefix[blockIdx.x];
   __shared__ uint32_t sh[17];
-  uint32_t carry_f = 0, carry_fc = 0, carry_s = 0;
-  for (uint32_t base = 0; base < nwg; base += 1024) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t f = i < nwg ? J.blk_f[i] : 0;
-    const uint32_t fc = i < nwg ? J.blk_fc[i] : 0;
-    const uint32_t s = i < nwg ? J.blk_s[i] : 0;
-    uint32_t tf, tfc, ts;
-    const uint32_t ef = block_excl_scan<16>(f, sh, tf);
-    const uint32_t efc = block_excl_scan<16>(fc, sh, tfc);
-    const uint32_t es = block_excl_scan<16>(s, sh, ts);
+  const uint32_t e0 = threadIdx.x * kScanPer;
+  uint32_t f[kScanPer], fc[kScanPer], s[kScanPer];
+  uint32_t sf = 0, sfc = 0, ss = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kScanPer; q++) {
+    const uint32_t i = e0 + q;
+    f[q] = i < nwg ? J.blk_f[i] : 0;
+    fc[q] = i < nwg ? J.blk_fc[i] : 0;
+    s[q] = i < nwg ? J.blk_s[i] : 0;
+    sf += f[q];
+    sfc += fc[q];
+    ss += s[q];
+  }
+  uint32_t tf, tfc, ts;
+  uint32_t pf = block_excl_scan<16>(sf, sh, tf);
+  uint32_t pfc = block_excl_scan<16>(sfc, sh, tfc);
+  uint32_t ps = block_excl_scan<16>(ss, sh, ts);
+#pragma unroll
+  for (uint32_t q = 0; q < kScanPer; q++) {
+    const uint32_t i = e0 + q;
     if (i < nwg) {
-      J.blk_f[i] = carry_f + ef;
-      J.blk_fc[i] = carry_fc + efc;
-      J.blk_s[i] = carry_s + es;
+      J.blk_f[i] = pf;
+      J.blk_fc[i] = pfc;
+      J.blk_s[i] = ps;
     }
-    carry_f += tf;
-    carry_fc += tfc;
-    carry_s += ts;
+    pf += f[q];
+    pfc += fc[q];
+    ps += s[q];
   }
   if (threadIdx.x == 0) {
-    J.ctr->nfirst = carry_f;
-    J.ctr->dchunks = carry_fc;
-    J.ctr->nshift = carry_s;
+    J.ctr->nfirst = tf;
+    J.ctr->dchunks = tfc;
+    J.ctr->nshift = ts;
   }
 }
 
-__global__ __launch_bounds__(256) void k_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_roots_write(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
@@ -872,9 +948,10 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobDev* jobs, const u
   __shared__ uint32_t sh[5];
   for (uint32_t b = b0; b < b1; b++) {
     const uint64_t v0 = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
-    uint8_t kinds[16];
-    uint32_t f, fc, s;
-    scan16(J, g, v0, kinds, f, fc, s);
+    uint32_t fm, sm;
+    scan16(J, v0, fm, sm);
+    if (!__syncthreads_or((fm | sm) != 0)) continue;  // no root in this node block
+    const uint32_t f = __popc(fm), fc = first_chunks(g, v0, fm), s = __popc(sm);
     uint32_t tf, tfc, ts;
     uint32_t pf = block_excl_scan<4>(f, sh, tf) + run_f;
     uint32_t pfc = block_excl_scan<4>(fc, sh, tfc) + run_fc;
@@ -882,21 +959,33 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobDev* jobs, const u
     run_f += tf;
     run_fc += tfc;
     run_s += ts;
-    if (f + s == 0) continue;
-#pragma unroll
-    for (int t = 0; t < 16; t++) {
+    while (fm) {
+      const int t = __builtin_ctz(fm);
+      fm &= fm - 1;
       const uint64_t v = v0 + t;
-      if (kinds[t] == FIRST_OCUR) {
-        J.f_list[pf] = (uint32_t)v;
-        J.f_off[pf] = pfc;
-        pf++;
-        pfc += (uint32_t)leaf_count(g, v);
-      } else if (kinds[t] == SHIFT_DUPL) {
-        const unsigned long long val = J.map.slots[J.slot_of[v]].val;
-        J.s_node[ps] = (uint32_t)v;
-        J.s_prev[ps] = (uint32_t)val;
-        J.s_tree[ps] = (uint32_t)(val >> 32);
-        ps++;
+      J.f_list[pf] = (uint32_t)v;
+      J.f_off[pf] = pfc;
+      pf++;
+      pfc += (uint32_t)leaf_count(g, v);
+    }
+    // Shifted roots: every load is issued before any store (fully unrolled,
+    // so the arrays stay in registers): map slots, then the source
+    // (checkpoint, node) of each.
+    if (sm) {
+      uint32_t slots[16];
+      unsigned long long vals[16];
+#pragma unroll
+      for (int t = 0; t < 16; t++) slots[t] = ((sm >> t) & 1) ? J.slot_of[v0 + t] : 0u;
+#pragma unroll
+      for (int t = 0; t < 16; t++) vals[t] = ((sm >> t) & 1) ? J.map.slots[slots[t]].val : 0ull;
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        if ((sm >> t) & 1) {
+          J.s_node[ps] = (uint32_t)(v0 + t);
+          J.s_prev[ps] = (uint32_t)vals[t];
+          J.s_tree[ps] = (uint32_t)(vals[t] >> 32);
+          ps++;
+        }
       }
     }
   }
@@ -909,11 +998,37 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobDev* jobs, const u
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t stiles_of(const JobDev& J) { return (J.ctr->nshift + kSTile - 1) / kSTile; }
 
-__global__ __launch_bounds__(256) void k_sbin_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+// Label hygiene, run once the roots are written: the interior labels (tile
+// root included) of every group-0 tile visited by this checkpoint are reset to
+// FIXED, which is what pass 2 computes for a tile whose leaves are all fixed.
+// A later checkpoint that does not visit the tile then reads FIXED labels
+// there, so the root scan needs no per-tile check.  Leaf labels are rewritten
+// by every leaf kernel.
+__device__ __forceinline__ void reset_visited_tiles(const JobDev& J, uint32_t wg, uint32_t nwg) {
+  const uint32_t Lr0 = J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0;
+  const uint32_t ntiles0 = 1u << Lr0;
+  const uint32_t H = J.D - Lr0;
+  __shared__ uint32_t pre[kPendSpread + 1];
+  uint32_t nitems = ntiles0;
+  if (!J.is_baseline) {
+    load_list_prefix(J.dirty_cnt, pre);
+    nitems = pre[kPendSpread];
+  }
+  for (uint32_t it = wg; it < nitems; it += nwg) {
+    const uint32_t t = J.is_baseline ? it : list_item(J.dirty_list, J.dirty_cap, pre, it);
+    if (threadIdx.x + 1 < (1u << H)) {
+      const uint64_t x = tile_node(((uint64_t)1 << Lr0) - 1 + t, threadIdx.x);
+      if (x < J.N) J.labels[x] = FIXED_DUPL;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sbin_count(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t t = blockIdx.x - prefix[j];
+  reset_visited_tiles(J, t, prefix[j + 1] - prefix[j]);
   const uint32_t nt = stiles_of(J);
   if (t >= nt) return;
   const uint32_t nbins = J.ckpt_id + 1;
@@ -956,13 +1071,13 @@ __device__ __forceinline__ void export_counters(const JobDev& J) {
 // One workgroup per job: new map entries of the checkpoint, scan of the
 // shifted-root histogram, the prior table, the diff length; then exports the
 // counters to the host.
-__global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* jobs) {
+__global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* __restrict__ jobs) {
   const JobDev& J = jobs[blockIdx.x];
   __shared__ uint32_t sh[17];
   __shared__ unsigned long long shl[16];
   unsigned long long claims = 0;
   for (uint32_t i = threadIdx.x; i < J.nwg_ins_leaf; i += blockDim.x) claims += J.blk_claim[i];
-  for (uint32_t i = threadIdx.x; i < J.nwg_ins_int; i += blockDim.x) claims += J.tile_claim[i];
+  for (uint32_t i = threadIdx.x; i < J.nwg_ins_int; i += blockDim.x) claims += J.int_claim[i];
   claims = block_sum(claims, shl);
   const bool skip = (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0;
   __syncthreads();
@@ -1024,7 +1139,7 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* jobs) {
 // One wavefront per tile walks its entries in order; lanes with the same bin
 // are ranked with a ballot so equal bins keep node order.
 constexpr uint32_t kLdsBins = 8192;
-__global__ __launch_bounds__(64) void k_sbin_scatter(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+__global__ __launch_bounds__(64) void k_sbin_scatter(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
@@ -1070,7 +1185,7 @@ __global__ __launch_bounds__(64) void k_sbin_scatter(const JobDev* jobs, const u
 // Diff metadata: header_t, first-occurrence roots, prior table, shifted
 // entries (src/tree_approach.cpp:593,622-706).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_write_meta(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_write_meta(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
@@ -1151,7 +1266,7 @@ __device__ __forceinline__ void copy_piece(uint8_t* dst, const uint8_t* src, uin
 // workgroup copies the group with 16-byte pieces (1-byte pieces when the
 // chunk size is not a multiple of 16 or the input is not 16-byte aligned).
 template <bool Vec>
-__global__ __launch_bounds__(256) void k_gather(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_gather(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
@@ -1384,13 +1499,13 @@ void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t n
   if (grid) DD_LAUNCH(k_insert_leaves, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
-  if (grid) DD_LAUNCH(k_pass1, grid, 256, 0, s, jobs, prefix, njobs, g);
+  if (grid) DD_LAUNCH(k_pass1, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
 }
 void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   if (grid) DD_LAUNCH(k_insert_interior, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
-  if (grid) DD_LAUNCH(k_pass2, grid, 256, 0, s, jobs, prefix, njobs, g);
+  if (grid) DD_LAUNCH(k_pass2, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
 }
 void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   if (grid) DD_LAUNCH(k_roots_count, grid, 256, 0, s, jobs, prefix, njobs);

@@ -18,6 +18,7 @@ constexpr int kTileLevels = 8;          // levels per tree tile (256 bottom node
 constexpr int kNodesPerBlock = 4096;    // root scan: 256 threads x 16 nodes
 constexpr int kSTile = 1024;            // shifted-root sort tile
 constexpr uint32_t kMaxStageWG = 4096;  // workgroups per job of a grid-stride stage
+constexpr uint32_t kMaxTileWG = 8192;   // one-wavefront workgroups per job of a tile pass
 
 // Per-job counters written by the device and read back by the host.
 struct JobCounters {
@@ -53,26 +54,24 @@ struct JobDev {
   uint64_t map_limit;           // max entries the map may hold
   JobCounters* ctr;             // device counters
   JobCounters* host_ctr;        // host-mapped copy written at the end of the pipeline
-  // per leaf block (256 chunks): changed leaves and new map entries
-  uint32_t* blk_pend;            // ceil(C/256)
-  unsigned long long* pend_acc;  // 32 partial sums of the changed leaves
-  uint32_t* pend_list;           // 256 per leaf block: chunk indices of changed leaves
-  uint32_t* blk_claim;           // per insert_leaves workgroup: new map entries
-  // Work lists built by the leaf kernel (32 sub-lists each, picked by index
-  // modulo 32, so no single counter is hit by every workgroup).
+  // Work lists.  Each is split into 32 sub-lists (picked by the producing
+  // leaf block or tile index modulo 32) with one counter each, so no single
+  // counter is hit by every workgroup; consumers prefix-sum the 32 counts.
+  uint32_t* pend_cnt;    // 32 counters
+  uint32_t* pend_list;   // 32 x pend_cap: chunk indices of the leaves that need a map insert
+  uint32_t pend_cap;
+  uint32_t* fint_cnt;    // 32 counters
+  uint32_t* fint_list;   // 32 x fint_cap: first-occurrence interior nodes (pass 1)
+  uint32_t fint_cap;
   uint32_t epoch;        // stamp of this checkpoint (never 0)
   uint32_t* tile_epoch;  // per group-0 tile: epoch of the last checkpoint that changed one of its leaves
   uint32_t* dirty_cnt;   // 32 counters
   uint32_t* dirty_list;  // 32 x dirty_cap: group-0 tiles with a changed leaf
   uint32_t dirty_cap;
-  uint32_t* act_cnt;     // 32 counters
-  uint32_t* act_list;    // 32 x act_cap: leaf blocks with a changed leaf
-  uint32_t act_cap;
-  // per tree tile (all groups): whether it holds first-occurrence interior nodes
   uint32_t tile_base[4];  // first tile index of each group
   uint32_t ntiles;        // tiles over all groups
-  uint32_t* tile_fint;    // ntiles
-  uint32_t* tile_claim;   // per insert_interior workgroup: new map entries
+  uint32_t* blk_claim;    // per insert_leaves workgroup: new map entries
+  uint32_t* int_claim;    // per insert_interior workgroup: new map entries
   uint32_t nwg_ins_leaf, nwg_ins_int;  // workgroups of the two insert stages
   // root scan
   uint32_t* blk_f;    // per root-scan workgroup: first roots
