@@ -446,6 +446,12 @@ __device__ __forceinline__ bool map_guard_tripped(const JobDev& J) {
 // reference's first occurrence stays PENDING (include/reference_impl.hpp:49-67).
 // FIXED|PENDING leaves (baseline, zero digest) take part in the ordering but
 // keep their label.
+//
+// Runs of equal digests are compressed within a wavefront first: a lane whose
+// left neighbour holds the same digest at a lower leaf is not the first
+// occurrence, so only the run's first lane touches the map and the others
+// label themselves SHIFT with the first lane's slot.  Long runs of one digest
+// (zero-filled regions) would otherwise serialize on one slot's atomics.
 __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
@@ -481,30 +487,59 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
   const uint32_t* const pend_list = J.pend_list;
   const uint32_t pend_cap = J.pend_cap;
   uint32_t* const flags = &J.ctr->flags;
-  for (uint32_t it = wg * blockDim.x + threadIdx.x; it < nitems; it += nwg * blockDim.x) {
-    const uint64_t leaf = ic.C - 1 + (dense ? it : list_item(pend_list, pend_cap, pre, it));
-    const uint8_t own = labels[leaf];
-    const InsertResult r = map_insert<true>(ic, ic.tree[leaf], (uint32_t)leaf);
-    if (!r.ok) {
-      atomicOr(flags, FLAG_MAP_FULL);
-      continue;
+  const uint32_t lane = lane_id();
+  // The loop bound is uniform over the workgroup, so every lane of a
+  // wavefront takes part in its shuffles; lanes past the list are inactive.
+  for (uint32_t base = wg * blockDim.x; base < nitems; base += nwg * blockDim.x) {
+    const uint32_t it = base + threadIdx.x;
+    const bool act = it < nitems;
+    uint64_t leaf = 0;
+    uint8_t own = 0;
+    Digest k{0, 0};
+    if (act) {
+      leaf = ic.C - 1 + (dense ? it : list_item(pend_list, pend_cap, pre, it));
+      own = labels[leaf];
+      k = ic.tree[leaf];
     }
-    slot_of[leaf] = (uint32_t)r.slot;
-    if (r.claimed) {
-      claims++;
-      continue;
-    }
-    const uint32_t pt = (uint32_t)(r.prev >> 32), pn = (uint32_t)r.prev;
-    bool self_shift = true;
-    if (pt == K) {
-      if (pn > leaf) {
-        self_shift = false;  // this leaf is now the lowest; the previous lowest is not
-        if (labels[pn] == PENDING) labels[pn] = SHIFT_DUPL;
-      } else if (pn == leaf) {
-        self_shift = false;  // re-run after a map growth: own entry
+    const unsigned long long pk0 = __shfl_up(k.k0, 1, 64), pk1 = __shfl_up(k.k1, 1, 64);
+    const unsigned long long pleaf = __shfl_up((unsigned long long)leaf, 1, 64);
+    const int pact = __shfl_up((int)act, 1, 64);
+    const bool head = !(act && lane > 0 && pact && pk0 == k.k0 && pk1 == k.k1 && pleaf < leaf);
+    bool ok = true;
+    uint64_t slot = 0;
+    if (act && head) {
+      const InsertResult r = map_insert<true>(ic, k, (uint32_t)leaf);
+      ok = r.ok;
+      slot = r.slot;
+      if (!r.ok) {
+        atomicOr(flags, FLAG_MAP_FULL);
+      } else if (r.claimed) {
+        claims++;
+      } else {
+        const uint32_t pt = (uint32_t)(r.prev >> 32), pn = (uint32_t)r.prev;
+        bool self_shift = true;
+        if (pt == K) {
+          if (pn > leaf) {
+            self_shift = false;  // this leaf is now the lowest; the previous lowest is not
+            if (labels[pn] == PENDING) labels[pn] = SHIFT_DUPL;
+          } else if (pn == leaf) {
+            self_shift = false;  // re-run after a map growth: own entry
+          }
+        }
+        if (self_shift && own == PENDING) labels[leaf] = SHIFT_DUPL;
       }
     }
-    if (self_shift && own == PENDING) labels[leaf] = SHIFT_DUPL;
+    // Each lane's run head: the highest head lane at or below it.
+    const unsigned long long heads = __ballot(head);
+    const uint32_t hl = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+    const unsigned long long hslot = __shfl((unsigned long long)slot, hl, 64);
+    const int hok = __shfl((int)ok, hl, 64);
+    if (act && !head && hok) {
+      slot_of[leaf] = (uint32_t)hslot;
+      if (own == PENDING) labels[leaf] = SHIFT_DUPL;  // a lower leaf of this run holds the digest
+    } else if (act && head && ok) {
+      slot_of[leaf] = (uint32_t)slot;
+    }
   }
   const uint32_t n = block_sum<uint32_t>(claims, sh);
   if (threadIdx.x == 0) J.blk_claim[wg] = n;
@@ -751,6 +786,16 @@ __global__ __launch_bounds__(kTileThreads) void k_pass2(const JobDev* __restrict
         const Digest a = both ? dig[2 * idx] : Digest{0, 0};
         const Digest b = both ? dig[2 * idx + 1] : Digest{0, 0};
         __syncthreads();  // every child of this chunk is read before any parent is written
+        // Both-shifted parents hash their children; equal digests of
+        // neighbouring lanes share one lookup (runs of identical subtrees).
+        const Digest hd = both ? hash_pair(kind, a, b) : Digest{0, 0};
+        const unsigned long long p0 = __shfl_up(hd.k0, 1, 64), p1 = __shfl_up(hd.k1, 1, 64);
+        const int pboth = __shfl_up((int)both, 1, 64);
+        const bool head = !(both && i > 0 && pboth && p0 == hd.k0 && p1 == hd.k1);
+        unsigned long long found = ~0ull;
+        if (both && head) found = map_find(map, hd);
+        const unsigned long long heads = __ballot(head);
+        found = __shfl(found, 63 - __clzll(heads & ((2ull << i) - 1ull)), 64);
         if (!act) continue;
         uint8_t L;
         Digest d{0, 0};
@@ -761,8 +806,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass2(const JobDev* __restrict
           if (la != lb) {
             L = DONE;
           } else if (both) {
-            d = hash_pair(kind, a, b);
-            const uint64_t s = map_find(map, d);
+            d = hd;
+            const uint64_t s = found;
             if (s != ~0ull) {
               L = SHIFT_DUPL;
               tree[x] = d;

@@ -64,6 +64,32 @@ def test_md5_float_field(torch_gpu):
     check_chain(torch_gpu, 64, datas, dd.HASH_MD5)
 
 
+@pytest.mark.parametrize("C,cs,kind", [(1 << 13, 64, dd.HASH_MURMUR3), (6001, 32, dd.HASH_MURMUR3),
+                                       (3 << 10, 64, dd.HASH_MD5)])
+def test_long_runs_of_equal_chunks(torch_gpu, C, cs, kind):
+    # Mostly-zero buffers with a few repeated patterns: long runs of one
+    # digest at the baseline (run compression in the leaf inserts and in the
+    # duplicate-forest lookups), runs that toggle between checkpoints, and a
+    # repeated non-zero pattern written at several offsets.
+    rng = np.random.default_rng(C + cs)
+    n = C * cs
+    x = np.zeros(n, dtype=np.uint8)
+    pat = rng.integers(0, 256, 4 * cs, dtype=np.uint8)
+    for off in rng.integers(0, C - 8, 20):
+        x[off * cs:off * cs + pat.size] = pat
+    x[: n // 8] = rng.integers(0, 256, n // 8, dtype=np.uint8)
+    datas = [x.copy()]
+    for _ in range(4):
+        y = datas[-1].copy()
+        for st in rng.integers(0, C - 64, 6):
+            ln = int(rng.integers(1, 64))
+            y[st * cs:(st + ln) * cs] = 0 if rng.random() < 0.5 else rng.integers(0, 256, ln * cs, dtype=np.uint8)
+        for off in rng.integers(0, C - 8, 5):
+            y[off * cs:off * cs + pat.size] = pat
+        datas.append(y)
+    check_chain(torch_gpu, cs, datas, kind, baselines=(0, 3))
+
+
 def test_partial_last_chunk(torch_gpu):
     rng = np.random.default_rng(9)
     base = rng.integers(0, 256, 128 * 37 + 77, dtype=np.uint8)
