@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-size", type=int, default=256 << 20)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-buffer batched comparison")
     ap.add_argument("--batch-buffers", type=int, default=64)
+    ap.add_argument("--field", action="store_true",
+                    help="also run the MD5 float-field chain (2^30 float32 values, 64 B chunks, 20 checkpoints)")
+    ap.add_argument("--field-values", type=int, default=1 << 30)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (scales PMC byte counters) and exit")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop (serving) measurement")
@@ -176,6 +179,55 @@ def run_batched(torch, dd, W, nbuf, nckpt, chunk, kind, seed=1931):
     del chains, outs_p, outs_b, plain, hs, batch
     torch.cuda.empty_cache()
     return out
+
+
+def run_field(torch, dd, W, n_values, nckpt, chunk, seed=1931):
+    """MD5 on a float32 simulation-like field: 2^30 values (4 GiB), 64-byte
+    chunks, 20 checkpoints; 60 % of the 4 KiB blocks are exact zeros, each
+    version perturbs 5 % of the nonzero values by N(0,1e-3) and flips 1 % of
+    the blocks between zero and nonzero.  Each version is generated in HBM
+    from the previous one (untimed); each checkpoint call is timed on its own
+    (the call synchronizes), with HIP events around the leaf-hash kernel.
+    Checkpoints 0, nckpt/2 and nckpt-1 are restarted from the device-resident
+    chain and compared byte for byte with the saved versions."""
+    size = n_values * 4
+    g = dd.TreeDeduplicator(chunk, dd.HASH_MD5)
+    g.set_profiling(1)
+    cap = dd.max_diff_size(size, chunk, nckpt)
+    out = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    keep_ids = sorted({0, nckpt // 2, nckpt - 1})
+    saved, diffs, times, leaf, diff_bytes = {}, [], [], [], []
+    for k, v in enumerate(W.float_field_versions_gpu(torch, seed, n_values, nckpt)):
+        if k in keep_ids:
+            saved[k] = v.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = g.checkpoint_device(v.data_ptr(), size, out.data_ptr(), cap, k == 0)
+        times.append(time.perf_counter() - t0)
+        leaf.append(g.stage_times()["leaf_hash"])
+        diff_bytes.append(n)
+        diffs.append(out[:n].clone())
+        print(json.dumps({"field_ckpt": k, "ms": round(times[-1] * 1e3, 3), "diff_bytes": n,
+                          "first_shift": g.counts()}), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ok = True
+    for k in keep_ids:
+        o = torch.empty(size, dtype=torch.uint8, device="cuda")
+        g.restart_device([d.data_ptr() for d in diffs[:k + 1]], [d.numel() for d in diffs[:k + 1]], k,
+                         o.data_ptr(), size)
+        ok = ok and torch.equal(o, saved[k])
+    C = size // chunk
+    leaf_ms = sum(leaf) / len(leaf)
+    res = dict(workload=f"field_{size >> 20}MiB_c{chunk}_md5", checkpoints=nckpt, buffer_bytes=size,
+               chunk_size=chunk, gbs=round(nckpt * size / sum(times) / 1e9, 2),
+               ms_per_checkpoint=round(sum(times) * 1e3 / nckpt, 3),
+               ms_baseline=round(times[0] * 1e3, 3),
+               ms_incremental_avg=round(sum(times[1:]) * 1e3 / max(1, nckpt - 1), 3),
+               leaf_kernel_ms=round(leaf_ms, 4), leaf_kernel_gbs=round(C * (chunk + 33) / (leaf_ms * 1e-3) / 1e9, 1),
+               diff_bytes_total=sum(diff_bytes), restarts_verified=keep_ids, restart_bit_exact=ok)
+    del diffs, saved, out, g
+    torch.cuda.empty_cache()
+    return res
 
 
 def _pct(xs, q):
@@ -409,6 +461,9 @@ def main():
     if not args.no_batched:
         line["batched"] = run_batched(torch, dd, W, args.batch_buffers, args.nckpt, args.chunk, kind)
         print(json.dumps({"batched": line["batched"]}), file=sys.stderr, flush=True)
+    if args.field:
+        line["field_md5"] = run_field(torch, dd, W, args.field_values, 20, 64)
+        print(json.dumps({"field_md5": line["field_md5"]}), file=sys.stderr, flush=True)
     if not args.no_serve:
         line["serving"] = run_serving(torch, dd, W, args.serve_ranks, args.serve_rate, args.serve_seconds,
                                       int(args.serve_budget_gib * (1 << 30)), args.serve_slots, args.chunk, kind)

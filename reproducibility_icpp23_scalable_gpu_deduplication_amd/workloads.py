@@ -173,3 +173,32 @@ def batch_sizes(seed, n=64, lo=4 << 20, hi=512 << 20, align=4096):
     rng = np.random.default_rng(seed)
     x = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
     return [int(max(align, round(v / align) * align)) for v in x]
+
+
+def float_field_versions_gpu(torch, seed, n_values, nckpt, zero_frac=0.6, block_bytes=4096, perturb_frac=0.05,
+                             toggle_frac=0.01, device="cuda"):
+    """chain_float_field generated in device memory (torch RNG), one version
+    at a time: yields a uint8 view of the float32 field, then updates the
+    same buffer in place for the next version.  `zero_frac` of the
+    `block_bytes` blocks are exact zeros, the rest N(0,1); each later version
+    adds N(0,1e-3) to a Bernoulli(`perturb_frac`) sample of the nonzero values
+    and flips `toggle_frac` of the blocks between zero and fresh N(0,1)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    per = block_bytes // 4
+    nblk = n_values // per
+    zero = torch.rand(nblk, generator=gen, device=device) < zero_frac
+    x = torch.randn(n_values, generator=gen, device=device, dtype=torch.float32)
+    x.view(nblk, per)[zero] = 0
+    ntog = max(1, int(toggle_frac * nblk))
+    for k in range(nckpt):
+        if k:
+            pick = (torch.rand(n_values, generator=gen, device=device) < perturb_frac) & (x != 0)
+            noise = torch.randn(n_values, generator=gen, device=device, dtype=torch.float32) * 1e-3
+            x += torch.where(pick, noise, torch.zeros((), device=device))
+            del pick, noise
+            tog = torch.randperm(nblk, generator=gen, device=device)[:ntog]
+            fresh = torch.randn(ntog, per, generator=gen, device=device, dtype=torch.float32)
+            x.view(nblk, per)[tog] = torch.where(zero[tog].unsqueeze(1), fresh, torch.zeros((), device=device))
+            zero[tog] = ~zero[tog]
+        yield x.view(torch.uint8)
