@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--hash", choices=["murmur3", "md5"], default="murmur3")
     ap.add_argument("--sweep", action="store_true", help="also run chunk {128,512,4096} x change {0.1,1,10}%%")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-size", type=int, default=256 << 20)
+    ap.add_argument("--cpu-size", type=int, default=512 << 20)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-buffer batched comparison")
     ap.add_argument("--batch-buffers", type=int, default=64)
     ap.add_argument("--field", action="store_true",
@@ -368,19 +368,39 @@ def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, 
     return out
 
 
-def cpu_baseline(W, size, chunk, frac, nckpt):
+def cpu_baseline(W, size, chunk, frac, nckpt, serial_size=64 << 20):
+    """The OpenMP restatement of the checkpoint path (oracle/dedup_omp.c, the
+    serial oracle's bytes on all OpenMP threads) on a `size`-byte chain of the
+    benchmark's recipe; the single-thread serial oracle on a `serial_size`
+    chain beside it."""
+    import platform
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    chain = W.chain_runs(1931, size, chunk, nckpt, frac)
-    o = O.OracleDeduplicator(chunk)
-    t0 = time.perf_counter()
-    for i, x in enumerate(chain):
-        o.checkpoint_array(x, i == 0)
-    el = time.perf_counter() - t0
-    return dict(value=round(nckpt * size / el / 1e9, 4), unit="GB/s", cores=1, kind="port",
-                sample=f"serial CPU oracle, {nckpt}-checkpoint chain of {size >> 20} MiB, chunk {chunk} B, "
-                       f"{frac * 100:g}% changed per checkpoint ({el:.1f} s)")
+    def rate(cls, n):
+        chain = W.chain_runs(1931, n, chunk, nckpt, frac)
+        o = cls(chunk)
+        t0 = time.perf_counter()
+        for i, x in enumerate(chain):
+            o.checkpoint_array(x, i == 0)
+        return nckpt * n / (time.perf_counter() - t0) / 1e9
+
+    par = rate(O.ParallelOracleDeduplicator, size)
+    ser = rate(O.OracleDeduplicator, serial_size)
+    threads = O.ParallelOracleDeduplicator.threads()
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), cpu)
+    except OSError:
+        pass
+    return dict(value=round(par, 4), unit="GB/s", cores=threads, kind="port",
+                sample=f"OpenMP restatement of the checkpoint path, {threads} threads ({cpu}), "
+                       f"{nckpt}-checkpoint chain of {size >> 20} MiB, chunk {chunk} B, {frac * 100:g}% changed "
+                       f"per checkpoint (same recipe as the GPU workload)",
+                serial_oracle_gbs=round(ser, 4),
+                serial_sample=f"single-thread serial oracle, {nckpt}-checkpoint chain of {serial_size >> 20} MiB")
 
 
 def main():

@@ -79,6 +79,18 @@ uint64_t or_subtree_leaves(uint32_t node, uint32_t num_chunks, uint32_t* out);
 
 void or_free(void* p);
 
+/* Parallel (OpenMP) restatement of or_checkpoint (dedup_omp.c): the same diff
+ * bytes, computed on omp_get_max_threads() host threads.  CPU baseline only. */
+typedef struct om_dedup om_dedup;
+om_dedup* om_create(uint32_t chunk_size, int hash_kind);
+void om_destroy(om_dedup* d);
+int om_checkpoint(om_dedup* d, const uint8_t* data, uint64_t len, int make_baseline,
+                  uint8_t** out, uint64_t* out_len);
+void om_counts(const om_dedup* d, uint64_t* nfirst, uint64_t* nshift);
+uint32_t om_current_id(const om_dedup* d);
+/* Host threads om_checkpoint uses. */
+int om_max_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,18 @@ def lib():
         L.or_subtree_leaves.restype = ctypes.c_uint64
         L.or_free.argtypes = [ctypes.c_void_p]
         L.or_free.restype = None
+        L.om_create.argtypes = [ctypes.c_uint32, ctypes.c_int]
+        L.om_create.restype = ctypes.c_void_p
+        L.om_destroy.argtypes = [ctypes.c_void_p]
+        L.om_destroy.restype = None
+        L.om_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_uint64)]
+        L.om_checkpoint.restype = ctypes.c_int
+        L.om_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                ctypes.POINTER(ctypes.c_uint64)]
+        L.om_counts.restype = None
+        L.om_max_threads.argtypes = []
+        L.om_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -175,6 +187,51 @@ class OracleDeduplicator:
 
     def map_size(self):
         return lib().or_map_size(self._h)
+
+
+class ParallelOracleDeduplicator:
+    """The OpenMP restatement (dedup_omp.c): the serial oracle's diff bytes on
+    omp_get_max_threads() host threads (set OMP_NUM_THREADS before the first
+    call).  Used as the CPU baseline of the benchmark."""
+
+    def __init__(self, chunk_size, hash_kind=MURMUR3):
+        self._h = lib().om_create(chunk_size, hash_kind)
+        if not self._h:
+            raise ValueError("bad chunk size")
+        self.chunk_size = chunk_size
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().om_destroy(h)
+            self._h = None
+
+    def _run(self, ptr, n, make_baseline):
+        out = ctypes.POINTER(ctypes.c_uint8)()
+        m = ctypes.c_uint64()
+        rc = lib().om_checkpoint(self._h, ptr, n, 1 if make_baseline else 0, ctypes.byref(out), ctypes.byref(m))
+        if rc != 0:
+            raise RuntimeError(f"om_checkpoint failed: {rc}")
+        res = ctypes.string_at(out, m.value)
+        lib().or_free(out)
+        return res
+
+    def checkpoint(self, data, make_baseline=False):
+        b = bytes(data) if not isinstance(data, bytes) else data
+        return self._run(ctypes.c_char_p(b), len(b), make_baseline)
+
+    def checkpoint_array(self, arr, make_baseline=False):
+        return self._run(ctypes.c_void_p(arr.ctypes.data), arr.nbytes, make_baseline)
+
+    def counts(self):
+        f = ctypes.c_uint64()
+        s = ctypes.c_uint64()
+        lib().om_counts(self._h, ctypes.byref(f), ctypes.byref(s))
+        return f.value, s.value
+
+    @staticmethod
+    def threads():
+        return lib().om_max_threads()
 
 
 def restart(chain, ckpt_id, datalen):
