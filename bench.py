@@ -117,6 +117,7 @@ def run_config(torch, dd, W, size, chunk, frac, nckpt, steps, warmup, kind, seed
         gpu_ms_baseline=(sum(base) / len(base)) if base else None,
         gpu_ms_incremental=(sum(incr) / len(incr)) if incr else None,
         diff_bytes_avg=sum(diff_bytes) / len(diff_bytes),
+        leaf_mapping=g.leaf_mapping(),
     )
     del chain, out, g
     torch.cuda.empty_cache()
@@ -174,6 +175,7 @@ def run_batched(torch, dd, W, nbuf, nckpt, chunk, kind, seed=1931):
         buffers=nbuf, total_bytes=sum(sizes), checkpoints=nckpt, chunk_size=chunk,
         plain_gbs=round(total / res["plain"] / 1e9, 2), batched_gbs=round(total / res["batched"] / 1e9, 2),
         speedup=round(res["plain"] / res["batched"], 3), bit_exact=mismatches == 0, mismatches=mismatches,
+        leaf_mapping=batch.leaf_mapping(),
         ms_per_checkpoint_index=dict(plain=round(res["plain"] * 1e3 / nckpt, 3),
                                      batched=round(res["batched"] * 1e3 / nckpt, 3)))
     del chains, outs_p, outs_b, plain, hs, batch
@@ -452,6 +454,7 @@ def main():
             "change_fraction": args.frac,
             "change_pattern": "geometric runs (mean 16 chunks), half fresh, half copies",
             "parallelism": "single GPU",
+            "leaf_mapping": r["leaf_mapping"],
         },
         "roofline": {
             "bound": "hbm",
@@ -475,7 +478,8 @@ def main():
             for frac in (0.001, 0.01, 0.1):
                 s = run_config(torch, dd, W, args.size, chunk, frac, args.nckpt, args.nckpt, args.nckpt, kind)
                 sweep.append(dict(chunk=chunk, frac=frac, gbs=round(s["gbs"], 2), leaf_gbs=round(s["leaf_gbs"], 1),
-                                  ms_baseline=s["gpu_ms_baseline"], ms_incremental=s["gpu_ms_incremental"]))
+                                  leaf_mapping=s["leaf_mapping"], ms_baseline=round(s["gpu_ms_baseline"], 4),
+                                  ms_incremental=round(s["gpu_ms_incremental"], 4)))
                 print(json.dumps({"sweep_point": sweep[-1]}), file=sys.stderr, flush=True)
         line["sweep"] = sweep
     if not args.no_batched:

@@ -182,6 +182,14 @@ double dd_now(void); /* the steady clock of dd_request_result */
 #define DD_NUM_STAGES 7
 dd_status dd_set_profiling(dd_dedup* h, int level);
 dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]);
+/* Leaf kernel mapping (no counterpart in the reference) of the last
+ * checkpoint of a handle or batch: 0 one lane per chunk, 1 the same with
+ * deeper load pipelining, 2/3/4 LDS-staged in 64/128/256-byte segments.  The
+ * library picks it by timing the valid mappings once per process for each
+ * hash, alignment and chunk-size combination (DD_LEAF_KERNEL=direct|deep|
+ * lds64|lds128|lds256 forces one). */
+dd_status dd_get_leaf_mapping(dd_dedup* h, int* mapping);
+dd_status dd_batch_get_leaf_mapping(dd_batch* b, int* mapping);
 
 void dd_buf_free(dd_buf* b);
 const char* dd_strerror(int status);

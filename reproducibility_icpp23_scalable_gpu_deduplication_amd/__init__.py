@@ -6,6 +6,7 @@ include/tree_deduplicator.hpp.  This package only binds that library for
 Python callers (tests, bench).
 """
 from ._lib import (  # noqa: F401
-    DD_ECORRUPT, DD_EHIP, DD_EINVAL, DD_ENOMEM, DD_ESTATE, DD_OK, EXPORTED, HASH_MD5, HASH_MURMUR3, LIB_PATH,
+    DD_ECORRUPT, DD_EHIP, DD_EINVAL, DD_ENOMEM, DD_ESTATE, DD_OK, EXPORTED, HASH_MD5, HASH_MURMUR3, LEAF_MAPPINGS,
+    LIB_PATH,
     Batch, DedupError, Server, TreeDeduplicator, load, max_diff_size, strerror,
 )

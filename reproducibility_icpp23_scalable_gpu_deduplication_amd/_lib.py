@@ -21,8 +21,9 @@ EXPORTED = [
     "dd_restart", "dd_restart_device", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
     "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
     "dd_server_submit_restart", "dd_server_wait", "dd_server_usage", "dd_now", "dd_set_profiling",
-    "dd_get_stage_times", "dd_buf_free", "dd_strerror",
+    "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_buf_free", "dd_strerror",
 ]
+LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256"]
 STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
           "meta_gather"]
 
@@ -87,6 +88,8 @@ def load():
         "dd_now": ([], ctypes.c_double),
         "dd_set_profiling": ([vp, i32], i32),
         "dd_get_stage_times": ([vp, P(ctypes.c_double)], i32),
+        "dd_get_leaf_mapping": ([vp, P(i32)], i32),
+        "dd_batch_get_leaf_mapping": ([vp, P(i32)], i32),
         "dd_buf_free": ([P(dd_buf)], None),
         "dd_strerror": ([i32], ctypes.c_char_p),
     }
@@ -222,6 +225,12 @@ class TreeDeduplicator:
         _check(load().dd_get_stage_times(self._h, ms), "dd_get_stage_times")
         return dict(zip(STAGES, list(ms)))
 
+    def leaf_mapping(self):
+        """Name of the leaf kernel mapping of the last checkpoint."""
+        m = ctypes.c_int()
+        _check(load().dd_get_leaf_mapping(self._h, ctypes.byref(m)), "dd_get_leaf_mapping")
+        return LEAF_MAPPINGS[m.value]
+
     def timers(self):
         t = (ctypes.c_double * 4)()
         r = (ctypes.c_double * 2)()
@@ -249,6 +258,11 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+    def leaf_mapping(self):
+        m = ctypes.c_int()
+        _check(load().dd_batch_get_leaf_mapping(self._h, ctypes.byref(m)), "dd_batch_get_leaf_mapping")
+        return LEAF_MAPPINGS[m.value]
 
     def checkpoint_device(self, dev_ptrs, lens, make_baseline, out_ptrs, out_caps):
         n = len(self._dedups)

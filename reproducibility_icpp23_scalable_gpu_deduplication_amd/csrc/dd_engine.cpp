@@ -271,21 +271,7 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
     aligned16_ = aligned16_ && al;
     vec16_ = vec16_ && al;
   }
-  // Leaf kernel mapping.  The lane-strided kernel measured faster than the
-  // LDS-staged one on MI355X for 128-byte chunks (4.97 vs 4.50 TB/s), so it
-  // is the default; DD_LEAF_KERNEL=lds selects the LDS-staged kernel, which
-  // needs 16-byte aligned buffers and chunk sizes that are multiples of its
-  // segment.
-  leaf_seg_ = 0;
-  if (const char* e = std::getenv("DD_LEAF_KERNEL")) {
-    if (!strcmp(e, "lds") && aligned16_) {
-      int seg = 256;
-      for (const JobDev& J : jobs) {
-        while (seg >= 64 && (J.chunk_size % seg) != 0) seg >>= 1;
-      }
-      if (seg >= 64) leaf_seg_ = seg;
-    }
-  }
+  leaf_map_ = leaf_mapping_for(jobs, aligned16_, hash_kind_, s);
   const size_t nstages = S_TILE0 + max_groups_;
   const size_t need_prefix = nstages * (nj + 1);
   const size_t jobs_bytes = (nj * sizeof(JobDev) + 255) & ~size_t(255);
@@ -352,7 +338,7 @@ std::vector<uint64_t> Pipeline::signature() const {
   std::vector<uint64_t> sig;
   sig.push_back(njobs_);
   sig.push_back((uint64_t)max_groups_);
-  sig.push_back((uint64_t)leaf_seg_);
+  sig.push_back((uint64_t)leaf_map_);
   sig.push_back((uint64_t)aligned16_ | ((uint64_t)vec16_ << 1));
   sig.push_back((uint64_t)hash_kind_);
   sig.push_back(reinterpret_cast<uintptr_t>(dev_table_.p));
@@ -379,8 +365,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, hipEvent_t ev_dedup, h
   launch_begin(J, nullptr, nj, rerun, s);
   DD_TRY(mark(0));
   if (!rerun) {
-    if (leaf_seg_) launch_leaf_hash_lds(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_seg_, hash_kind_, s);
-    else launch_leaf_hash(J, P(S_LEAF), nj, grid_[S_LEAF], aligned16_, hash_kind_, s);
+    launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s);
   }
   DD_TRY(mark(1));
   launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], s);
@@ -967,6 +952,18 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
     out_lens[i] = c[i].diff_len;
     st.current_id += 1;
   }
+  return DD_OK;
+}
+
+dd_status dd_get_leaf_mapping(dd_dedup* h, int* mapping) {
+  if (!h || !mapping) return DD_EINVAL;
+  *mapping = h->pipe.leaf_mapping();
+  return DD_OK;
+}
+
+dd_status dd_batch_get_leaf_mapping(dd_batch* b, int* mapping) {
+  if (!b || !mapping) return DD_EINVAL;
+  *mapping = b->pipe.leaf_mapping();
   return DD_OK;
 }
 

@@ -97,10 +97,18 @@ class Pipeline {
   int max_groups_ = 0;
   bool aligned16_ = true, vec16_ = true;
   int hash_kind_ = 0;
-  int leaf_seg_ = 0;  // 0: lane-strided leaf kernel; 64/128/256: LDS-staged with that segment
+  int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
+
+ public:
+  int leaf_mapping() const { return leaf_map_; }
 };
 
 uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
+
+// Leaf kernel mapping for a job table (dd_tune.cpp): the fastest mapping
+// measured on this device for the table's hash, alignment and chunk sizes,
+// timed on first use and cached for the process.
+int leaf_mapping_for(const std::vector<JobDev>& jobs, bool aligned16, int hash_kind, hipStream_t s);
 
 }  // namespace dd
 

@@ -210,7 +210,9 @@ __device__ __forceinline__ uint64_t pending_total(const JobDev& J) {
   return t;
 }
 
-template <int Kind, bool Aligned16>
+// Deep: Murmur3 keeps two 128-byte groups of loads in flight per lane (long
+// chunks); MD5 ignores it.
+template <int Kind, bool Aligned16, bool Deep>
 __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jo
     if (Kind == HASH_MD5) {
       d = Aligned16 ? md5_range<true>(p, nb) : md5_range<false>(p, nb);
     } else {
-      d = Aligned16 ? murmur3_aligned(p, nb) : murmur3_bytes(p, nb);
+      d = !Aligned16 ? murmur3_bytes(p, nb) : Deep ? murmur3_aligned_deep(p, nb) : murmur3_aligned(p, nb);
     }
     const uint64_t leaf = J.C - 1 + k;
     Digest prev{0, 0};
@@ -1514,29 +1516,30 @@ void launch_begin(const JobDev* jobs, const uint32_t*, uint32_t njobs, bool reru
   DD_LAUNCH(k_begin, (njobs + 63) / 64, 64, 0, s, jobs, njobs, rerun ? 1 : 0);
 }
 
-void launch_leaf_hash_lds(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int seg, int kind,
-                          hipStream_t s) {
+void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping, bool al16,
+                 int kind, hipStream_t s) {
   if (grid == 0) return;
+  if (!al16) mapping = LEAF_DIRECT;  // the other mappings need 16-byte aligned buffers
   if (kind == HASH_MD5) {
-    if (seg == 64) DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>), grid, 256, 0, s, jobs, prefix, njobs);
-    else if (seg == 128) DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>), grid, 256, 0, s, jobs, prefix, njobs);
-    else DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>), grid, 256, 0, s, jobs, prefix, njobs);
-  } else {
-    if (seg == 64) DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>), grid, 256, 0, s, jobs, prefix, njobs);
-    else if (seg == 128) DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>), grid, 256, 0, s, jobs, prefix, njobs);
-    else DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>), grid, 256, 0, s, jobs, prefix, njobs);
+    switch (mapping) {
+      case LEAF_LDS64: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>), grid, 256, 0, s, jobs, prefix, njobs); return;
+      case LEAF_LDS128: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>), grid, 256, 0, s, jobs, prefix, njobs); return;
+      case LEAF_LDS256: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>), grid, 256, 0, s, jobs, prefix, njobs); return;
+      default:
+        if (al16) DD_LAUNCH((k_leaf_hash<HASH_MD5, true, false>), grid, 256, 0, s, jobs, prefix, njobs);
+        else DD_LAUNCH((k_leaf_hash<HASH_MD5, false, false>), grid, 256, 0, s, jobs, prefix, njobs);
+        return;
+    }
   }
-}
-
-void launch_leaf_hash(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool al16, int kind,
-                      hipStream_t s) {
-  if (grid == 0) return;
-  if (kind == HASH_MD5) {
-    if (al16) DD_LAUNCH((k_leaf_hash<HASH_MD5, true>), grid, 256, 0, s, jobs, prefix, njobs);
-    else DD_LAUNCH((k_leaf_hash<HASH_MD5, false>), grid, 256, 0, s, jobs, prefix, njobs);
-  } else {
-    if (al16) DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true>), grid, 256, 0, s, jobs, prefix, njobs);
-    else DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, false>), grid, 256, 0, s, jobs, prefix, njobs);
+  switch (mapping) {
+    case LEAF_DEEP: DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, true>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    case LEAF_LDS64: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    case LEAF_LDS128: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    case LEAF_LDS256: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    default:
+      if (al16) DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, false>), grid, 256, 0, s, jobs, prefix, njobs);
+      else DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, false>), grid, 256, 0, s, jobs, prefix, njobs);
+      return;
   }
 }
 

@@ -114,13 +114,15 @@ constexpr uint32_t kCoverFirst = 0xffffffffu;
 
 // Launch helpers (dd_kernels.hip) -------------------------------------------
 void launch_begin(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, bool rerun, hipStream_t s);
-void launch_leaf_hash(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool all_aligned16,
-                      int hash_kind, hipStream_t s);
-// LDS-staged variant: `seg` in {64, 128, 256} bytes per chunk per round;
-// every job needs chunk_size % seg == 0 and a 16-byte aligned buffer.  One
-// workgroup covers 256 chunks, like launch_leaf_hash.
-void launch_leaf_hash_lds(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int seg,
-                          int hash_kind, hipStream_t s);
+// Chunk-to-lane mappings of the leaf kernel.  DIRECT: one lane per chunk,
+// 16-byte loads, 64 bytes in flight per lane; DEEP: the same with 256 bytes
+// in flight (Murmur3); LDS<seg>: each wavefront loads `seg` bytes of 64
+// chunks with contiguous 16-byte loads into LDS per round, then each lane
+// hashes its chunk from LDS (needs chunk_size % seg == 0).  All but DIRECT
+// need 16-byte aligned buffers; one workgroup covers 256 chunks in every case.
+enum LeafMapping : int { LEAF_DIRECT = 0, LEAF_DEEP = 1, LEAF_LDS64 = 2, LEAF_LDS128 = 3, LEAF_LDS256 = 4, LEAF_NMAP = 5 };
+void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping,
+                 bool all_aligned16, int hash_kind, hipStream_t s);
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
 void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);

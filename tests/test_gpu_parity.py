@@ -192,3 +192,22 @@ def test_error_paths(torch_gpu):
     g.restart([d0, d1], 1, out.data_ptr(), 1024)
     torch.cuda.synchronize()
     assert out.cpu().numpy().tobytes() == bytes(range(256)) * 4
+
+
+@pytest.mark.parametrize("mapping", ["direct", "deep", "lds64", "lds128", "lds256", None])
+def test_leaf_mappings_match_oracle(torch_gpu, mapping, monkeypatch):
+    # Every leaf kernel mapping (forced through DD_LEAF_KERNEL) and the tuned
+    # choice (None) give the oracle's diffs; 256-byte chunks admit all five.
+    if mapping:
+        monkeypatch.setenv("DD_LEAF_KERNEL", mapping)
+    else:
+        monkeypatch.delenv("DD_LEAF_KERNEL", raising=False)
+    for cs, kind in ((256, dd.HASH_MURMUR3), (256, dd.HASH_MD5)):
+        datas = W.chain_uniform_changes(17 + cs, 3000 * cs + 7, cs, 4, fresh_frac=0.03, copy_frac=0.05,
+                                        zero_bytes=8 * cs)
+        g, _ = check_chain(torch_gpu, cs, datas, kind, restart_ids=[0, 3])
+        got = g.leaf_mapping()
+        if mapping and not (mapping == "deep" and kind == dd.HASH_MD5):
+            assert got == mapping
+        else:
+            assert got in dd.LEAF_MAPPINGS
