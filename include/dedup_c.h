@@ -101,7 +101,10 @@ dd_status dd_get_ids(dd_dedup* h, uint32_t* current_id, uint32_t* baseline_id);
 dd_status dd_set_current_id(dd_dedup* h, uint32_t current_id);
 
 /* BaseDeduplicator::timers[4] (setup, dedup, gather, copy-to-host, seconds) and
- * restart_timers[2] (copy, total), include/deduplicator_interface.hpp:22-23. */
+ * restart_timers[2] (copy, total), include/deduplicator_interface.hpp:22-23.
+ * The GPU time is split into dedup and gather only while profiling is on
+ * (dd_set_profiling >= 1: the split costs an event between the stages);
+ * otherwise timers[1] holds the whole GPU time and timers[2] is 0. */
 dd_status dd_get_timers(dd_dedup* h, double timers[4], double restart_timers[2]);
 
 /* TreeDeduplicator::restart(uint8_t* data_ptr, size_t len,

@@ -134,7 +134,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     last_map_ckpt = -1;
   }
   const uint64_t nblk = ceil_div(N, kNodesPerBlock);
-  DD_TRY(blk.ensure(3 * kMaxStageWG * 4));
+  DD_TRY(blk.ensure(3 * nblk * 4));  // per root-scan workgroup (at most one per node block): counts, prefixes
   DD_TRY(f_list.ensure((C + 1) * 4));
   DD_TRY(f_off.ensure((C + 1) * 4));
   DD_TRY(s_node.ensure(C * 4));
@@ -220,8 +220,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.dirty_cap = (uint32_t)dirty_cap;
   job.dirty_list = job.dirty_cnt + 32;
   job.blk_f = blk.as<uint32_t>();
-  job.blk_fc = job.blk_f + kMaxStageWG;
-  job.blk_s = job.blk_fc + kMaxStageWG;
+  job.blk_fc = job.blk_f + nblk;
+  job.blk_s = job.blk_fc + nblk;
   job.nblk = (uint32_t)nblk;
   job.f_list = f_list.as<uint32_t>();
   job.f_off = f_off.as<uint32_t>();
@@ -306,7 +306,7 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
         case S_INSL:
         case S_TILES: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
         case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kMaxStageWG); break;
-        case S_STILE: n = ceil_div(J.C, kSTile); break;
+        case S_STILE: n = std::min<uint64_t>(ceil_div(J.C, kSTile), 512); break;  // grid-stride over sort tiles
         case S_META: n = std::min<uint64_t>(ceil_div(J.C, 256), 256); break;
         case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
         default: {
@@ -347,14 +347,14 @@ std::vector<uint64_t> Pipeline::signature() const {
   return sig;
 }
 
-dd_status Pipeline::run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, hipEvent_t ev_dedup,
+dd_status Pipeline::run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup,
                         hipEvent_t* stage_ev) {
   if (jobs.empty()) return DD_EINVAL;
   DD_TRY(upload(jobs, s));
-  return launch_all(s, rerun, ev_dedup, stage_ev);
+  return launch_all(s, rerun, begin, ev_dedup, stage_ev);
 }
 
-dd_status Pipeline::launch_all(hipStream_t s, bool rerun, hipEvent_t ev_dedup, hipEvent_t* stage_ev) {
+dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup, hipEvent_t* stage_ev) {
   const uint32_t nj = njobs_;
   const JobDev* J = dev_jobs_;
   auto P = [&](int st) { return dev_prefix_ + off_[st]; };
@@ -362,7 +362,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, hipEvent_t ev_dedup, h
     if (stage_ev && (profile_level_ >= 2 || i <= 1)) DD_HIP(hipEventRecord(stage_ev[i], s));
     return DD_OK;
   };
-  launch_begin(J, nullptr, nj, rerun, s);
+  if (rerun || begin) launch_begin(J, nullptr, nj, rerun, s);
   DD_TRY(mark(0));
   if (!rerun) {
     launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s);
@@ -387,6 +387,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, hipEvent_t ev_dedup, h
   launch_write_meta(J, P(S_META), nj, grid_[S_META], s);
   launch_gather(J, P(S_GATHER), nj, grid_[S_GATHER], vec16_, s);
   DD_TRY(mark(7));
+  launch_finish(J, nj, s);
   DD_HIP(hipGetLastError());
   return DD_OK;
 }
@@ -421,9 +422,15 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   DD_TRY(st.prepare(reinterpret_cast<const uint8_t*>(dev_data), len, baseline, out, out_cap, job, h->stream));
   std::vector<JobDev> jobs(1, job);
   st.timers[0] = now_s() - t0;
+  // The counters are armed by the previous successful checkpoint (k_finish);
+  // a baseline resets the entry count, so it starts with k_begin.
+  const bool begin = baseline || !st.ctr_armed;
+  st.ctr_armed = false;
   DD_HIP(hipEventRecord(h->ev[0], h->stream));
   h->pipe.profile_level_ = h->profile;
-  DD_TRY(h->pipe.run(jobs, h->stream, false, h->ev[1], h->profile ? h->pev : nullptr));
+  // The event between the two halves costs a few microseconds of GPU idle
+  // time: it is recorded only while profiling.
+  DD_TRY(h->pipe.run(jobs, h->stream, false, begin, h->profile ? h->ev[1] : nullptr, h->profile ? h->pev : nullptr));
   DD_HIP(hipEventRecord(h->ev[2], h->stream));
   DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
   JobCounters c = h->pipe.host_ctr[0];
@@ -433,7 +440,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
     jobs[0].map = make_view(st.map.p, st.map_cap);
     jobs[0].map_count_at_start = c.map_count;
     jobs[0].map_limit = map_limit_of(st.map_cap);
-    DD_TRY(h->pipe.run(jobs, h->stream, true));
+    DD_TRY(h->pipe.run(jobs, h->stream, true, true));
     DD_HIP(hipEventRecord(h->ev[2], h->stream));
     DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
     c = h->pipe.host_ctr[0];
@@ -441,8 +448,12 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   if (c.flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return DD_ENOMEM;
   if (c.flags & FLAG_BAD_SOURCE) return DD_ECORRUPT;
   float ms1 = 0, ms2 = 0;
-  (void)hipEventElapsedTime(&ms1, h->ev[0], h->ev[1]);
-  (void)hipEventElapsedTime(&ms2, h->ev[1], h->ev[2]);
+  if (h->profile) {
+    (void)hipEventElapsedTime(&ms1, h->ev[0], h->ev[1]);
+    (void)hipEventElapsedTime(&ms2, h->ev[1], h->ev[2]);
+  } else {
+    (void)hipEventElapsedTime(&ms1, h->ev[0], h->ev[2]);
+  }
   st.timers[1] = ms1 * 1e-3;
   st.timers[2] = ms2 * 1e-3;
   if (h->profile) {
@@ -453,6 +464,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
     }
   }
   st.commit(c, baseline);
+  st.ctr_armed = true;  // k_finish re-armed the counters
   *result = c;
   return DD_OK;
 }
@@ -895,23 +907,27 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
   const size_t n = b->hs.size();
   std::vector<JobDev> jobs(n);
   std::vector<char> base(n);
+  bool begin = false;  // k_begin for every job unless all counters are armed
   for (size_t i = 0; i < n; i++) {
     if (!dev_data[i] || !dev_out[i]) return DD_EINVAL;
     StreamState& st = b->hs[i]->st;
     base[i] = make_baseline[i] || !st.have_tree || st.current_id == 0;
+    begin = begin || base[i] || !st.ctr_armed;
     DD_TRY(st.prepare(static_cast<const uint8_t*>(dev_data[i]), lens[i], base[i], static_cast<uint8_t*>(dev_out[i]),
                       out_caps[i], jobs[i], b->stream));
   }
+  for (size_t i = 0; i < n; i++) b->hs[i]->st.ctr_armed = false;
   DD_TRY(b->pipe.upload(jobs, b->stream));
   if (b->use_graph) {
     std::vector<uint64_t> sig = b->pipe.signature();
+    sig.push_back(begin ? 1 : 0);
     if (!b->exec || sig != b->sig) {
       if (b->exec) (void)hipGraphExecDestroy(b->exec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->exec = nullptr;
       b->graph = nullptr;
       DD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      const dd_status rc = b->pipe.launch_all(b->stream, false);
+      const dd_status rc = b->pipe.launch_all(b->stream, false, begin);
       const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
       if (rc != DD_OK) return rc;
       if (ec != hipSuccess) return DD_EHIP;
@@ -920,7 +936,7 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
     }
     DD_HIP(hipGraphLaunch(b->exec, b->stream));
   } else {
-    DD_TRY(b->pipe.launch_all(b->stream, false));
+    DD_TRY(b->pipe.launch_all(b->stream, false, begin));
   }
   DD_HIP(hipStreamSynchronize(b->stream));
   std::vector<JobCounters> c(b->pipe.host_ctr, b->pipe.host_ctr + n);
@@ -938,7 +954,7 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
       jobs[i].map_limit = map_limit_of(st.map_cap);
       sub.push_back(jobs[i]);
     }
-    DD_TRY(b->pipe_rerun.run(sub, b->stream, true));
+    DD_TRY(b->pipe_rerun.run(sub, b->stream, true, true));
     DD_HIP(hipStreamSynchronize(b->stream));
     for (size_t k = 0; k < redo.size(); k++) c[redo[k]] = b->pipe_rerun.host_ctr[k];
   }
@@ -949,6 +965,7 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
   for (size_t i = 0; i < n; i++) {
     StreamState& st = b->hs[i]->st;
     st.commit(c[i], base[i]);
+    st.ctr_armed = true;
     out_lens[i] = c[i].diff_len;
     st.current_id += 1;
   }

@@ -33,6 +33,7 @@ struct StreamState {
   int hash_kind = 0;
   uint32_t current_id = 0, baseline_id = 0;
   bool have_tree = false;
+  bool ctr_armed = false;  // the device counters are zero and hold the current map count
   uint64_t data_len = 0, C = 0, N = 0;
   int64_t last_map_ckpt = -1;  // checkpoint id of the last insert into the map
 
@@ -69,13 +70,16 @@ class Pipeline {
   // receives the counters (pinned, njobs entries) once `s` is synchronized.
   // `stage_ev`, when given, holds DD_NUM_STAGES + 1 events recorded around
   // the stages; `ev_dedup` is recorded after the duplicate forest.
-  dd_status run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, hipEvent_t ev_dedup = nullptr,
+  // `begin` launches k_begin first; without it the counters must be armed
+  // (k_finish of a previous successful run, or zero-filled allocation).
+  dd_status run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup = nullptr,
                 hipEvent_t* stage_ev = nullptr);
   // The two halves of run(): upload the job table (a copy on `s`), then
   // enqueue the kernels.  launch_all() enqueues nothing but kernels and event
   // records, so it can be captured into a hipGraph.
   dd_status upload(const std::vector<JobDev>& jobs, hipStream_t s);
-  dd_status launch_all(hipStream_t s, bool rerun, hipEvent_t ev_dedup = nullptr, hipEvent_t* stage_ev = nullptr);
+  dd_status launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup = nullptr,
+                       hipEvent_t* stage_ev = nullptr);
   // Identifies the launch sequence of the last upload (grids, kernel
   // variants, table address): a captured graph is valid while it is equal.
   std::vector<uint64_t> signature() const;

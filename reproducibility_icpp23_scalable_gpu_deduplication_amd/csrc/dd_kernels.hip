@@ -129,6 +129,28 @@ __global__ void k_begin(const JobDev* __restrict__ jobs, uint32_t njobs, int rer
   c->unresolved = 0;
 }
 
+// finish: after a checkpoint that needs no re-run, re-arm the counters for
+// the next one (the final entry count stays: it is the next checkpoint's
+// starting count).  Runs while the host wakes from its synchronization, so
+// the next checkpoint starts with the leaf kernel and no k_begin.
+__global__ void k_finish(const JobDev* __restrict__ jobs, uint32_t njobs) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  const JobDev& J = jobs[j];
+  JobCounters* c = J.ctr;
+  if (c->flags) return;  // a re-run or a failed checkpoint: the host re-arms with k_begin
+  for (uint32_t i = 0; i < 32; i++) {
+    J.pend_cnt[i] = 0;
+    J.dirty_cnt[i] = 0;
+    J.fint_cnt[i] = 0;
+  }
+  c->pending = 0;
+  c->nfirst = c->nshift = c->nprior = 0;
+  c->dchunks = 0;
+  c->diff_len = 0;
+  c->unresolved = 0;
+}
+
 // ---------------------------------------------------------------------------
 // leaf_hash: one lane per chunk.  Leaf digests are written in place over the
 // previous checkpoint's digests, only when they change.
@@ -869,11 +891,15 @@ __device__ __forceinline__ uint8_t byte_of(const uint4 w, int t) {
 // aligned 8-byte loads.  Labels of nodes the checkpoint did not visit are
 // FIXED (tile reset after every checkpoint), so most groups stop at the
 // first load.  The labels array is padded past N.
-__device__ __forceinline__ void scan16(const JobDev& J, uint64_t v0, uint32_t& fmask, uint32_t& smask) {
+// The 16 labels at v0, or FIXED bytes (no candidate) past the last node.
+__device__ __forceinline__ uint4 load16(const JobDev& J, uint64_t v0) {
+  return v0 < J.N ? *reinterpret_cast<const uint4*>(J.labels + v0)
+                  : make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
+}
+
+__device__ __forceinline__ void scan16(const JobDev& J, uint64_t v0, const uint4 w, uint32_t& fmask, uint32_t& smask) {
   fmask = smask = 0;
-  if (v0 >= J.N) return;
-  const uint4 w = *reinterpret_cast<const uint4*>(J.labels + v0);
-  if (!any_root_candidate(w)) return;
+  if (v0 >= J.N || !any_root_candidate(w)) return;
   const uint64_t h = v0 >> 1;
   const uint2 lo = v0 >= 16 ? *reinterpret_cast<const uint2*>(J.labels + h - 8) : make_uint2(0, 0);
   const uint2 hi = *reinterpret_cast<const uint2*>(J.labels + h);
@@ -921,7 +947,7 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobDev* __restrict__ 
   for (uint32_t b = b0; b < b1; b++) {
     const uint64_t v0 = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
     uint32_t fm, sm;
-    scan16(J, v0, fm, sm);
+    scan16(J, v0, load16(J, v0), fm, sm);
     f += __popc(fm);
     fc += first_chunks(g, v0, fm);
     s += __popc(sm);
@@ -996,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobDev* __restrict__ 
   for (uint32_t b = b0; b < b1; b++) {
     const uint64_t v0 = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
     uint32_t fm, sm;
-    scan16(J, v0, fm, sm);
+    scan16(J, v0, load16(J, v0), fm, sm);
     if (!__syncthreads_or((fm | sm) != 0)) continue;  // no root in this node block
     const uint32_t f = __popc(fm), fc = first_chunks(g, v0, fm), s = __popc(sm);
     uint32_t tf, tfc, ts;
@@ -1045,7 +1071,7 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobDev* __restrict__ 
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t stiles_of(const JobDev& J) { return (J.ctr->nshift + kSTile - 1) / kSTile; }
 
-// Label hygiene, run once the roots are written: the interior labels (tile
+// Label hygiene, run by k_gather once the roots are written: the interior labels (tile
 // root included) of every group-0 tile visited by this checkpoint are reset to
 // FIXED, which is what pass 2 computes for a tile whose leaves are all fixed.
 // A later checkpoint that does not visit the tile then reads FIXED labels
@@ -1070,18 +1096,19 @@ __device__ __forceinline__ void reset_visited_tiles(const JobDev& J, uint32_t wg
   }
 }
 
+// Workgroups stride over the sort tiles (kSTile shifted roots each).
 __global__ __launch_bounds__(256) void k_sbin_count(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  const uint32_t t = blockIdx.x - prefix[j];
-  reset_visited_tiles(J, t, prefix[j + 1] - prefix[j]);
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   const uint32_t nt = stiles_of(J);
-  if (t >= nt) return;
   const uint32_t nbins = J.ckpt_id + 1;
+  const uint32_t S = J.ctr->nshift;
+  for (uint32_t t = wg; t < nt; t += nwg) {
+  __syncthreads();  // the previous tile's atomics are done before its bins are reused
   for (uint32_t bn = threadIdx.x; bn < nbins; bn += blockDim.x) J.s_hist[(uint64_t)bn * nt + t] = 0;
   __syncthreads();
-  const uint32_t S = J.ctr->nshift;
   const uint32_t e1 = min(S, (t + 1) * kSTile);
   for (uint32_t base = t * kSTile; base < e1; base += blockDim.x) {
     const uint32_t e = base + threadIdx.x;
@@ -1105,6 +1132,7 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobDev* __restrict__ j
       if (lane_id() == leader) atomicAdd(&J.s_hist[(uint64_t)lb * nt + t], (uint32_t)__popcll(same));
       if (todo && bn == lb) todo = false;
     }
+  }
   }
 }
 
@@ -1184,47 +1212,66 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* __restrict__ j
 }
 
 // One wavefront per tile walks its entries in order; lanes with the same bin
-// are ranked with a ballot so equal bins keep node order.
+// are ranked with a ballot so equal bins keep node order.  The tile's entries
+// (kSTile / 64 per lane) are loaded into registers before the ranking loop,
+// which then only touches LDS and issues stores.
 constexpr uint32_t kLdsBins = 8192;
+constexpr uint32_t kSChunks = kSTile / 64;
 __global__ __launch_bounds__(64) void k_sbin_scatter(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  const uint32_t t = blockIdx.x - prefix[j];
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   const uint32_t nt = stiles_of(J);
-  if (t >= nt) return;
   const uint32_t nbins = J.ckpt_id + 1;
   const uint32_t S = J.ctr->nshift;
-  const uint32_t e0 = t * kSTile, e1 = min(S, (t + 1) * kSTile);
+  const uint32_t* const s_tree = J.s_tree;
+  const uint32_t* const s_node = J.s_node;
+  const uint32_t* const s_prev = J.s_prev;
+  uint32_t* const ss_node = J.ss_node;
+  uint32_t* const ss_prev = J.ss_prev;
   __shared__ uint32_t base[kLdsBins];
-  for (uint32_t lo = 0; lo < nbins; lo += kLdsBins) {
-    const uint32_t hi = min(nbins, lo + kLdsBins);
-    for (uint32_t bn = lo + threadIdx.x; bn < hi; bn += 64) base[bn - lo] = J.s_hist[(uint64_t)bn * nt + t];
-    __syncthreads();
-    for (uint32_t e = e0; e < e1; e += 64) {
-      const uint32_t idx = e + threadIdx.x;
-      uint32_t bn = idx < e1 ? J.s_tree[idx] : 0xffffffffu;
-      if (bn != 0xffffffffu && bn >= nbins) bn = nbins - 1;
-      bool todo = idx < e1 && bn >= lo && bn < hi;
-      while (true) {
-        unsigned long long act = __ballot(todo);
-        if (!act) break;
-        const uint32_t leader = __ffsll((long long)act) - 1;
-        const uint32_t lb = __shfl(bn, leader, 64);
-        unsigned long long same = __ballot(todo && bn == lb);
-        const uint32_t cur = base[lb - lo];
-        if (todo && bn == lb) {
-          const uint32_t pos = cur + __popcll(same & lanemask_lt());
-          J.ss_node[pos] = J.s_node[idx];
-          J.ss_prev[pos] = J.s_prev[idx];
-          todo = false;
-        }
-        __syncthreads();
-        if (threadIdx.x == leader) base[lb - lo] = cur + __popcll(same);
-        __syncthreads();
-      }
+  for (uint32_t t = wg; t < nt; t += nwg) {
+    const uint32_t e0 = t * kSTile, e1 = min(S, (t + 1) * kSTile);
+    uint32_t tb[kSChunks], tn[kSChunks], tp[kSChunks];
+#pragma unroll
+    for (uint32_t q = 0; q < kSChunks; q++) {
+      const uint32_t idx = e0 + q * 64 + threadIdx.x;
+      const bool in = idx < e1;
+      uint32_t bn = in ? s_tree[idx] : 0xffffffffu;
+      if (in && bn >= nbins) bn = nbins - 1;
+      tb[q] = bn;
+      tn[q] = in ? s_node[idx] : 0u;
+      tp[q] = in ? s_prev[idx] : 0u;
     }
-    __syncthreads();
+    for (uint32_t lo = 0; lo < nbins; lo += kLdsBins) {
+      const uint32_t hi = min(nbins, lo + kLdsBins);
+      for (uint32_t bn = lo + threadIdx.x; bn < hi; bn += 64) base[bn - lo] = J.s_hist[(uint64_t)bn * nt + t];
+      __syncthreads();
+#pragma unroll
+      for (uint32_t q = 0; q < kSChunks; q++) {
+        const uint32_t bn = tb[q];
+        bool todo = bn != 0xffffffffu && bn >= lo && bn < hi;
+        while (true) {
+          const unsigned long long act = __ballot(todo);
+          if (!act) break;
+          const uint32_t leader = __ffsll((long long)act) - 1;
+          const uint32_t lb = __shfl(bn, leader, 64);
+          const unsigned long long same = __ballot(todo && bn == lb);
+          const uint32_t cur = base[lb - lo];
+          if (todo && bn == lb) {
+            const uint32_t pos = cur + __popcll(same & lanemask_lt());
+            ss_node[pos] = tn[q];
+            ss_prev[pos] = tp[q];
+            todo = false;
+          }
+          __syncthreads();
+          if (threadIdx.x == leader) base[lb - lo] = cur + __popcll(same);
+          __syncthreads();
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -1317,6 +1364,9 @@ __global__ __launch_bounds__(256) void k_gather(const JobDev* __restrict__ jobs,
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
+  // The roots are written: reset the labels of the tiles this checkpoint
+  // visited (gather runs a large grid whatever the diff size).
+  reset_visited_tiles(J, blockIdx.x - prefix[j], prefix[j + 1] - prefix[j]);
   const uint32_t F = J.ctr->nfirst, P = J.ctr->nprior, S = J.ctr->nshift;
   const uint64_t Dq = J.ctr->dchunks;
   if (Dq == 0) return;
@@ -1541,6 +1591,10 @@ void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uin
       else DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, false>), grid, 256, 0, s, jobs, prefix, njobs);
       return;
   }
+}
+
+void launch_finish(const JobDev* jobs, uint32_t njobs, hipStream_t s) {
+  DD_LAUNCH(k_finish, (njobs + 63) / 64, 64, 0, s, jobs, njobs);
 }
 
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {

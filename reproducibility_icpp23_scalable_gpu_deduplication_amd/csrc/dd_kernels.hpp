@@ -74,9 +74,9 @@ struct JobDev {
   uint32_t* int_claim;    // per insert_interior workgroup: new map entries
   uint32_t nwg_ins_leaf, nwg_ins_int;  // workgroups of the two insert stages
   // root scan
-  uint32_t* blk_f;    // per root-scan workgroup: first roots
-  uint32_t* blk_fc;   // per root-scan workgroup: chunks under first roots
-  uint32_t* blk_s;    // per root-scan workgroup: shifted roots
+  uint32_t* blk_f;    // per root-scan workgroup: first roots (then their exclusive prefix)
+  uint32_t* blk_fc;   // per root-scan workgroup: chunks under first roots (prefix)
+  uint32_t* blk_s;    // per root-scan workgroup: shifted roots (prefix)
   uint32_t nblk;      // node blocks = ceil(N / kNodesPerBlock)
   uint32_t* f_list;   // first roots, ascending
   uint32_t* f_off;    // chunk offset of each first region in the data section
@@ -114,6 +114,8 @@ constexpr uint32_t kCoverFirst = 0xffffffffu;
 
 // Launch helpers (dd_kernels.hip) -------------------------------------------
 void launch_begin(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, bool rerun, hipStream_t s);
+// Re-arms the counters of every job that finished without a flag.
+void launch_finish(const JobDev* jobs, uint32_t njobs, hipStream_t s);
 // Chunk-to-lane mappings of the leaf kernel.  DIRECT: one lane per chunk,
 // 16-byte loads, 64 bytes in flight per lane; DEEP: the same with 256 bytes
 // in flight (Murmur3); LDS<seg>: each wavefront loads `seg` bytes of 64
