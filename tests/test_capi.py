@@ -64,3 +64,16 @@ def test_no_fallback_when_library_missing(tmp_path, monkeypatch):
         raise AssertionError("load() must raise without the HIP library")
     except ImportError:
         pass
+
+
+def test_file_drivers_built_and_reject_other_methods(tmp_path):
+    # The drivers exit with status 2 on a non-tree method before using a GPU.
+    import os
+    import subprocess
+
+    bindir = os.path.join(os.path.dirname(dd.LIB_PATH), "bin")
+    for name in ("dedup_chkpt_files", "restart_chkpt_files"):
+        assert os.access(os.path.join(bindir, name), os.X_OK)
+    r = subprocess.run([os.path.join(bindir, "dedup_chkpt_files"), "128", "1", "--run-list-chkpt", "x"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unsupported" in r.stderr

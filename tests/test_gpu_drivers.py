@@ -1,0 +1,53 @@
+"""File drivers (bin/dedup_chkpt_files, bin/restart_chkpt_files): the
+.hashtree.incr_chkpt files are the oracle's diffs byte for byte, and the
+restart driver prints the oracle's Murmur3 digest of the original data."""
+import os
+import subprocess
+
+import pytest
+
+import oracle as O
+import reproducibility_icpp23_scalable_gpu_deduplication_amd as dd
+from reproducibility_icpp23_scalable_gpu_deduplication_amd import workloads as W
+from gpu_util import first_mismatch, oracle_chain
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(dd.LIB_PATH), "bin")
+
+
+def digest_line(data):
+    d = O.hash_bytes(data)
+    hexs = "".join(f"{b:02X}" + (" " if i % 4 == 3 else "") for i, b in enumerate(d))
+    return "Hashtree digest:       " + hexs
+
+
+def run(args, cwd):
+    r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, f"{args[0]} failed: {r.stderr}"
+    return r.stdout
+
+
+@pytest.mark.parametrize("size,chunk", [((1 << 20) + 77, 128), (3000 * 64, 64)])
+def test_file_drivers_round_trip(torch_gpu, tmp_path, size, chunk):
+    chain = W.chain_uniform_changes(5 + chunk, size, chunk, 5, fresh_frac=0.02, copy_frac=0.04, zero_bytes=4096)
+    files = []
+    for k, x in enumerate(chain):
+        p = tmp_path / f"ckpt{k}.dat"
+        x.tofile(p)
+        files.append(str(p))
+    run([os.path.join(BIN, "dedup_chkpt_files"), str(chunk), str(len(files)), "--run-tree-chkpt", *files], tmp_path)
+    od, _ = oracle_chain(chunk, chain)
+    for k, f in enumerate(files):
+        got = open(f + ".hashtree.incr_chkpt", "rb").read()
+        assert got == od[k], f"ckpt {k}: {first_mismatch(got, od[k])}"
+    assert (tmp_path / f"ckpt0.dat.chunk_size.{chunk}.csv").exists()
+    for k in (0, 2, len(files) - 1):
+        out = run([os.path.join(BIN, "restart_chkpt_files"), str(k), str(len(files)), "2", str(chunk),
+                   "--run-tree-low-offset-chkpt", *files], tmp_path)
+        assert out.count(digest_line(chain[k].tobytes())) == 2, out
+    # Without the original data file the restored length comes from the diff header.
+    os.remove(files[3])
+    out = run([os.path.join(BIN, "restart_chkpt_files"), "3", str(len(files)), "1", str(chunk), "--run-tree-chkpt",
+               *files], tmp_path)
+    assert digest_line(chain[3].tobytes()) in out
