@@ -37,6 +37,17 @@ typedef enum dd_hash_kind {
   DD_HASH_MD5 = 1      /* include/kokkos_md5.hpp:287-323 */
 } dd_hash_kind;
 
+/* Deduplication method of a handle (the reference's DedupMode,
+ * include/utils.hpp:40-49, restricted to the approaches built here). */
+typedef enum dd_method {
+  DD_METHOD_TREE = 0, /* TreeDeduplicator, src/tree_approach.cpp; also
+                         TreeLowRootDeduplicator, whose checkpoint runs the same
+                         labeling (src/tree_low_root_approach.cpp:393-399) */
+  DD_METHOD_LIST = 1  /* ListDeduplicator, src/list_approach.cpp:23-256: one
+                         digest per chunk, no tree; diff node ids are chunk
+                         indices */
+} dd_method;
+
 /* header_t, include/utils.hpp:29-37 (32 bytes, little-endian). */
 typedef struct dd_header {
   uint32_t ref_id;
@@ -62,6 +73,15 @@ typedef struct dd_dedup dd_dedup;
  * (include/hash_functions.hpp:16-20).  `hip_stream` may be NULL: the handle
  * then creates its own stream on the current device. */
 dd_status dd_create(uint32_t chunk_size, int hash_kind, void* hip_stream, dd_dedup** out);
+/* dd_create with a method: DD_METHOD_TREE is dd_create; DD_METHOD_LIST is
+ * ListDeduplicator(uint32_t bytes_per_chunk), src/list_approach.cpp:5-9.
+ * Every other entry point takes handles of either method; a list handle's
+ * diffs use the list layout (src/list_approach.cpp:64-256) and its restarts
+ * read list diffs.  The chunk count of a list chain is fixed between
+ * baselines (DD_ESTATE otherwise), as for the tree. */
+dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void* hip_stream, dd_dedup** out);
+/* The method a handle was created with. */
+dd_status dd_get_method(dd_dedup* h, int* method);
 dd_status dd_destroy(dd_dedup* h);
 
 /* TreeDeduplicator::checkpoint(uint8_t* data_ptr, size_t len, HostMirror& diff_h,
@@ -128,7 +148,8 @@ dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uin
  * dd_checkpoint_device on each), running each stage as a single launch over
  * all buffers.  With graphs enabled the launch sequence is captured once per
  * buffer-shape set and replayed.  The handles must not be used elsewhere
- * while a batch call runs; the batch has its own HIP stream. */
+ * while a batch call runs; the batch has its own HIP stream.  All handles of
+ * a batch share the device, the hash and the method (DD_EINVAL otherwise). */
 typedef struct dd_batch dd_batch;
 dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out);
 dd_status dd_batch_destroy(dd_batch* b);

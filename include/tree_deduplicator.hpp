@@ -1,6 +1,8 @@
 // tree_deduplicator.hpp -- C++ host interface mirroring the reference's
 // TreeDeduplicator (include/tree_approach.hpp:16-205, include/
-// deduplicator_interface.hpp:14-184), implemented over the C ABI in
+// deduplicator_interface.hpp:14-184), TreeLowRootDeduplicator
+// (include/tree_approach.hpp:207-218) and ListDeduplicator
+// (include/list_approach.hpp:21-200), implemented over the C ABI in
 // dedup_c.h.  Kokkos views are replaced by raw device pointers plus lengths
 // (device data) and std::vector<uint8_t> (host diffs).  Errors raise
 // std::runtime_error where the reference would print or abort.
@@ -41,7 +43,8 @@ class TreeDeduplicator {
   // src/tree_approach.cpp:1809 -- rebuilds checkpoint `chkpt_id` into the
   // device buffer data_ptr (len must equal the header's datalen).
   void restart(uint8_t* data_ptr, size_t len, std::vector<HostBuf>& chkpts, std::string& logname, uint32_t chkpt_id);
-  // src/tree_approach.cpp:1827 -- reads "<name>.hashtree.incr_chkpt" files.
+  // src/tree_approach.cpp:1827 -- reads "<name>.hashtree.incr_chkpt" files
+  // (list handles: "<name>.hashlist.incr_chkpt").
   void restart(uint8_t* data_ptr, size_t len, std::vector<std::string>& chkpt_filenames, std::string& logname,
                uint32_t chkpt_id);
 
@@ -58,11 +61,40 @@ class TreeDeduplicator {
   void set_current_id(uint32_t id);
   dd_dedup* handle() const { return h_; }
 
+ protected:
+  // A handle of method `method` (dd_method); `approach` names it in the logs
+  // and `suffix` is appended to checkpoint file names.
+  TreeDeduplicator(uint32_t chunk_size, HashKind kind, void* hip_stream, int method, const char* approach,
+                   const char* suffix);
+
  private:
   dd_dedup* h_ = nullptr;
   uint32_t chunk_size_;
+  int method_ = DD_METHOD_TREE;
+  std::string approach_ = "Tree", suffix_ = ".hashtree.incr_chkpt";
   uint64_t data_len_ = 0;
-  std::pair<uint64_t, uint64_t> datasizes_{0, 0};  // (data incl. header, metadata), src/tree_approach.cpp:706-709
+  // Tree: (data incl. header, metadata), src/tree_approach.cpp:706-709;
+  // list: (data, header + metadata), src/list_approach.cpp:249-254.
+  std::pair<uint64_t, uint64_t> datasizes_{0, 0};
+};
+
+// TreeLowRootDeduplicator(uint32_t), src/tree_low_root_approach.cpp:8-12.  Its
+// checkpoint runs the same labeling as TreeDeduplicator
+// (src/tree_low_root_approach.cpp:393-399; dedup_data_low_root is never
+// called), so it is the tree method with the tree's files and logs.
+class TreeLowRootDeduplicator : public TreeDeduplicator {
+ public:
+  explicit TreeLowRootDeduplicator(uint32_t chunk_size, HashKind kind = HashKind::Murmur3, void* hip_stream = nullptr)
+      : TreeDeduplicator(chunk_size, kind, hip_stream) {}
+};
+
+// ListDeduplicator(uint32_t bytes_per_chunk), src/list_approach.cpp:5-9: the
+// same surface; diffs in the list layout, "<name>.hashlist.incr_chkpt" files
+// (src/list_approach.cpp:996-1010) and "List" log rows.
+class ListDeduplicator : public TreeDeduplicator {
+ public:
+  explicit ListDeduplicator(uint32_t chunk_size, HashKind kind = HashKind::Murmur3, void* hip_stream = nullptr)
+      : TreeDeduplicator(chunk_size, kind, hip_stream, DD_METHOD_LIST, "List", ".hashlist.incr_chkpt") {}
 };
 
 // src/utils.cpp:38-124: header, first-occurrence metadata and per-source-

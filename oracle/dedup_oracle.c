@@ -662,3 +662,246 @@ done:
   free(node_list); free(dist); free(rep); free(la); free(lb);
   return rc;
 }
+
+/* ------------------------------------------------------------------------- */
+/* List approach, src/list_approach.cpp.                                       */
+/* ------------------------------------------------------------------------- */
+
+struct or_list {
+  uint32_t chunk_size;
+  int hash_kind;
+  uint32_t current_id, baseline_id;
+  int have_list;
+  uint64_t data_len;
+  uint64_t C;
+  uint8_t* list; /* C * 16: HashList, include/kokkos_hash_list.hpp:11-40 */
+  dmap map;
+  uint64_t nfirst, nshift;
+};
+
+or_list* or_list_create(uint32_t chunk_size, int hash_kind) {
+  if (chunk_size == 0) return NULL;
+  or_list* d = (or_list*)calloc(1, sizeof *d);
+  if (!d) return NULL;
+  d->chunk_size = chunk_size;
+  d->hash_kind = hash_kind;
+  return d;
+}
+
+void or_list_destroy(or_list* d) {
+  if (!d) return;
+  free(d->list);
+  dmap_free(&d->map);
+  free(d);
+}
+
+void or_list_counts(const or_list* d, uint64_t* nf, uint64_t* ns) { *nf = d->nfirst; *ns = d->nshift; }
+uint32_t or_list_current_id(const or_list* d) { return d->current_id; }
+
+int or_list_checkpoint(or_list* d, const uint8_t* data, uint64_t len, int make_baseline,
+                       uint8_t** out, uint64_t* out_len) {
+  *out = NULL;
+  *out_len = 0;
+  const uint64_t cs = d->chunk_size;
+  if (len == 0) return OR_EINVAL;
+  const uint64_t C = (len + cs - 1) / cs;
+  if (C > 0x7fffffffULL) return OR_EINVAL;
+  const uint32_t K = d->current_id;
+  /* checkpoint(), :860-893: a baseline starts a zero list and an empty map. */
+  int baseline = make_baseline || !d->have_list || K == 0;
+  if (!baseline && len != d->data_len) return OR_EINVAL;
+  if (baseline) {
+    free(d->list);
+    d->list = (uint8_t*)calloc(C, 16);
+    dmap_free(&d->map);
+    if (!d->list || dmap_init(&d->map, 2 * C) != OR_OK) return OR_ENOMEM;
+    d->have_list = 1;
+    d->data_len = len;
+    d->C = C;
+    d->baseline_id = K;
+  }
+
+  /* dedup_data, :23-62, in chunk order. */
+  uint32_t* flist = (uint32_t*)malloc((C ? C : 1) * sizeof(uint32_t));
+  shift_rec* slist = (shift_rec*)malloc((C ? C : 1) * sizeof(shift_rec));
+  uint64_t* prior = (uint64_t*)calloc((uint64_t)K + 1, sizeof(uint64_t));
+  if (!flist || !slist || !prior) { free(flist); free(slist); free(prior); return OR_ENOMEM; }
+  uint64_t nf = 0, ns = 0;
+  for (uint64_t i = 0; i < C; i++) {
+    const uint64_t off = i * cs;
+    const uint64_t nb = (i == C - 1) ? len - off : cs;
+    uint8_t dg[16];
+    or_hash(d->hash_kind, data + off, nb, dg);
+    if (memcmp(d->list + 16 * i, dg, 16) == 0) continue; /* unchanged chunk */
+    uint64_t slot;
+    node_id nv = {(uint32_t)i, K};
+    if (dmap_insert(&d->map, dg, nv, &slot)) {
+      flist[nf++] = (uint32_t)i;
+    } else {
+      node_id pv = d->map.vals[slot];
+      slist[ns].node = (uint32_t)i;
+      slist[ns].prev_node = pv.node;
+      slist[ns].prev_tree = pv.tree;
+      prior[pv.tree] += 1;
+      ns++;
+    }
+    memcpy(d->list + 16 * i, dg, 16);
+  }
+  /* collect_diff, :64-256: shifted entries sorted by source checkpoint
+   * (BinSort, :165-183), chunk order within a checkpoint. */
+  qsort(slist, ns, sizeof(shift_rec), cmp_shift);
+  uint64_t np = 0;
+  for (uint64_t t = 0; t <= K; t++) np += prior[t] ? 1 : 0;
+  const uint64_t blen = 32 + 4 * nf + 8 * np + 8 * ns + nf * cs;
+  uint8_t* buf = (uint8_t*)calloc(blen, 1);
+  if (!buf) { free(flist); free(slist); free(prior); return OR_ENOMEM; }
+  put32(buf + 0, d->baseline_id); /* header, :228-234 */
+  put32(buf + 4, K);
+  put64(buf + 8, len);
+  put32(buf + 16, d->chunk_size);
+  put32(buf + 20, (uint32_t)nf);
+  put32(buf + 24, (uint32_t)np);
+  put32(buf + 28, (uint32_t)ns);
+  uint8_t* p = buf + 32;
+  for (uint64_t i = 0; i < nf; i++, p += 4) put32(p, flist[i]);
+  for (uint64_t t = 0; t <= K; t++) {
+    if (!prior[t]) continue;
+    put32(p, (uint32_t)t);
+    put32(p + 4, (uint32_t)prior[t]);
+    p += 8;
+  }
+  for (uint64_t i = 0; i < ns; i++, p += 8) {
+    put32(p, slist[i].node);
+    put32(p + 4, slist[i].prev_node);
+  }
+  for (uint64_t i = 0; i < nf; i++, p += cs) { /* "Copy data", :207-222 */
+    const uint64_t src = (uint64_t)flist[i] * cs;
+    const uint64_t w = (flist[i] == C - 1) ? len - src : cs;
+    memcpy(p, data + src, w);
+  }
+  d->nfirst = nf;
+  d->nshift = ns;
+  d->current_id = K + 1;
+  free(flist); free(slist); free(prior);
+  *out = buf;
+  *out_len = blen;
+  return OR_OK;
+}
+
+/* Per checkpoint of the walk: data offset of each first-occurrence chunk and
+ * the (prev, source ckpt) of each shifted chunk, indexed by chunk. */
+static int list_tables(const parsed_diff* h, uint64_t C, int64_t* dist, node_id* rep) {
+  const uint64_t cs = h->chunk_size;
+  for (uint64_t c = 0; c < C; c++) { dist[c] = -1; rep[c].node = rep[c].tree = 0xffffffffu; }
+  if ((uint64_t)h->nfirst * cs > h->data_bytes) return OR_ECORRUPT;
+  for (uint32_t i = 0; i < h->nfirst; i++) {
+    const uint32_t node = get32(h->first + 4ULL * i);
+    if (node >= C) return OR_ECORRUPT;
+    dist[node] = (int64_t)((uint64_t)i * cs);
+  }
+  /* Source checkpoint of shifted entry i: the first t with i < cum[t]
+   * (:334-354). */
+  uint64_t* cum = (uint64_t*)calloc((uint64_t)h->chkpt_id + 2, sizeof(uint64_t));
+  if (!cum) return OR_ENOMEM;
+  for (uint32_t i = 0; i < h->nprior; i++) {
+    const uint32_t ck = get32(h->table + 8ULL * i), cnt = get32(h->table + 8ULL * i + 4);
+    if (ck > h->chkpt_id) { free(cum); return OR_ECORRUPT; }
+    cum[ck] = cnt;
+  }
+  for (uint64_t t = 1; t <= h->chkpt_id; t++) cum[t] += cum[t - 1];
+  for (uint32_t i = 0; i < h->nshift; i++) {
+    const uint32_t node = get32(h->shift + 8ULL * i), prev = get32(h->shift + 8ULL * i + 4);
+    if (node >= C || prev >= C) { free(cum); return OR_ECORRUPT; }
+    uint32_t tree = 0;
+    for (uint64_t t = (uint64_t)h->chkpt_id + 1; t-- > 0;) if (i < cum[t]) tree = (uint32_t)t;
+    rep[node].node = prev;
+    rep[node].tree = tree;
+  }
+  free(cum);
+  return OR_OK;
+}
+
+int or_list_restart(const uint8_t* const* chain, const uint64_t* lens, uint32_t nchain,
+                    uint32_t id, uint8_t* out, uint64_t out_len) {
+  if (id >= nchain) return OR_EINVAL;
+  parsed_diff h;
+  int rc = parse_diff(chain[id], lens[id], &h);
+  if (rc) return rc;
+  if (h.datalen != out_len) return OR_EINVAL;
+  const uint64_t cs = h.chunk_size;
+  const uint64_t C = (h.datalen + cs - 1) / cs;
+  node_id* node_list = (node_id*)malloc(C * sizeof(node_id));
+  int64_t* dist = (int64_t*)malloc(C * sizeof(int64_t));
+  node_id* rep = (node_id*)malloc(C * sizeof(node_id));
+  uint8_t* written = (uint8_t*)calloc(C ? C : 1, 1);
+  if (!node_list || !dist || !rep || !written) { rc = OR_ENOMEM; goto done; }
+  for (uint64_t c = 0; c < C; c++) node_list[c].node = node_list[c].tree = 0xffffffffu;
+
+  int main_ckpt = 1;
+  int64_t idx = id;
+  uint32_t ref_id = h.ref_id;
+  while (idx >= (int64_t)ref_id) {
+    if (!main_ckpt) {
+      rc = parse_diff(chain[idx], lens[idx], &h);
+      if (rc) goto done;
+      if (h.datalen != out_len || h.chunk_size != cs) { rc = OR_ECORRUPT; goto done; }
+    }
+    const uint32_t cur = h.chkpt_id;
+    rc = list_tables(&h, C, dist, rep);
+    if (rc) goto done;
+    for (uint64_t c = 0; c < C; c++) {
+      const uint64_t dst = c * cs;
+      const uint64_t w = dst + cs > h.datalen ? h.datalen - dst : cs;
+      if (main_ckpt) {
+        /* Newest checkpoint, :303-363: its own chunks, shifted chunks of its
+         * own data, and the rest default to (chunk, cur-1). */
+        if (dist[c] >= 0) {
+          memcpy(out + dst, h.data + dist[c], w);
+          written[c] = 1;
+          node_list[c].node = (uint32_t)c;
+          node_list[c].tree = cur;
+        } else if (rep[c].node != 0xffffffffu) {
+          node_list[c] = rep[c];
+          if (rep[c].tree == cur) {
+            if (dist[rep[c].node] < 0) { rc = OR_ECORRUPT; goto done; }
+            memcpy(out + dst, h.data + dist[rep[c].node], w);
+            written[c] = 1;
+          }
+        } else {
+          node_list[c].node = (uint32_t)c;
+          node_list[c].tree = cur - 1;
+        }
+      } else {
+        /* Older checkpoints, :366-468. */
+        if (node_list[c].tree != cur) continue;
+        const uint32_t x = node_list[c].node;
+        if (dist[x] >= 0) {
+          memcpy(out + dst, h.data + dist[x], w);
+          written[c] = 1;
+        } else if (rep[x].node != 0xffffffffu) {
+          const node_id pv = rep[x];
+          if (pv.tree == cur) {
+            if (dist[pv.node] < 0) { rc = OR_ECORRUPT; goto done; }
+            memcpy(out + dst, h.data + dist[pv.node], w);
+            written[c] = 1;
+          } else {
+            node_list[c] = pv;
+          }
+        } else {
+          node_list[c].tree = cur - 1;
+        }
+      }
+    }
+    ref_id = h.ref_id;
+    main_ckpt = 0;
+    idx--;
+  }
+  /* Every chunk must have been written by some checkpoint of the chain. */
+  for (uint64_t c = 0; c < C; c++) {
+    if (!written[c]) { rc = OR_ECORRUPT; goto done; }
+  }
+  rc = OR_OK;
+done:
+  free(node_list); free(dist); free(rep); free(written);
+  return rc;
+}

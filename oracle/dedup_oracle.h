@@ -79,6 +79,27 @@ uint64_t or_subtree_leaves(uint32_t node, uint32_t num_chunks, uint32_t* out);
 
 void or_free(void* p);
 
+/* List approach (ListDeduplicator, src/list_approach.cpp:23-256), serial:
+ * one digest per chunk compared with the previous checkpoint's list; a
+ * changed chunk is inserted into the digest map as (chunk, ckpt) and is a
+ * first occurrence when the insert creates the entry, a shifted duplicate
+ * otherwise.  The diff lists chunk indices: first occurrences ascending, the
+ * prior table ascending, shifted entries grouped by source checkpoint then
+ * chunk (the order a one-thread run of the reference produces).  An
+ * incremental checkpoint keeps the chain's length (OR_EINVAL otherwise). */
+typedef struct or_list or_list;
+or_list* or_list_create(uint32_t chunk_size, int hash_kind);
+void or_list_destroy(or_list* d);
+int or_list_checkpoint(or_list* d, const uint8_t* data, uint64_t len, int make_baseline,
+                       uint8_t** out, uint64_t* out_len);
+void or_list_counts(const or_list* d, uint64_t* nfirst, uint64_t* nshift);
+uint32_t or_list_current_id(const or_list* d);
+/* ListDeduplicator::restart_chkpt (src/list_approach.cpp:258-512): the
+ * per-chunk (node, ckpt) walk from checkpoint `id` down to its baseline.
+ * Chunks no checkpoint of the chain names are an error (OR_ECORRUPT). */
+int or_list_restart(const uint8_t* const* chain, const uint64_t* lens, uint32_t nchain,
+                    uint32_t id, uint8_t* out, uint64_t out_len);
+
 /* Parallel (OpenMP) restatement of or_checkpoint (dedup_omp.c): the same diff
  * bytes, computed on omp_get_max_threads() host threads.  CPU baseline only. */
 typedef struct om_dedup om_dedup;

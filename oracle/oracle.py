@@ -6,8 +6,8 @@ cpu_baseline leg use this module, and only as the checker.  The product path
 
 The oracle restates the reference's serial low-offset labeler
 (include/reference_impl.hpp:15-132), its diff serializer
-(src/tree_approach.cpp:511-710) and restart (src/tree_approach.cpp:712-1128);
-see oracle/dedup_oracle.h.
+(src/tree_approach.cpp:511-710) and restart (src/tree_approach.cpp:712-1128),
+and the list approach (src/list_approach.cpp:23-512); see oracle/dedup_oracle.h.
 """
 import ctypes
 import os
@@ -78,6 +78,20 @@ def lib():
         L.om_counts.restype = None
         L.om_max_threads.argtypes = []
         L.om_max_threads.restype = ctypes.c_int
+        L.or_list_create.argtypes = [ctypes.c_uint32, ctypes.c_int]
+        L.or_list_create.restype = ctypes.c_void_p
+        L.or_list_destroy.argtypes = [ctypes.c_void_p]
+        L.or_list_destroy.restype = None
+        L.or_list_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                         ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_uint64)]
+        L.or_list_checkpoint.restype = ctypes.c_int
+        L.or_list_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.POINTER(ctypes.c_uint64)]
+        L.or_list_counts.restype = None
+        L.or_list_current_id.argtypes = [ctypes.c_void_p]
+        L.or_list_current_id.restype = ctypes.c_uint32
+        L.or_list_restart.argtypes = L.or_restart.argtypes
+        L.or_list_restart.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -232,6 +246,71 @@ class ParallelOracleDeduplicator:
     @staticmethod
     def threads():
         return lib().om_max_threads()
+
+
+class OracleListDeduplicator:
+    """Serial CPU oracle of the list approach (ListDeduplicator,
+    src/list_approach.cpp) with the same checkpoint surface."""
+
+    def __init__(self, chunk_size, hash_kind=MURMUR3):
+        self._h = lib().or_list_create(chunk_size, hash_kind)
+        if not self._h:
+            raise ValueError("bad chunk size")
+        self.chunk_size = chunk_size
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().or_list_destroy(h)
+            self._h = None
+
+    def _run(self, ptr, n, make_baseline):
+        out = ctypes.POINTER(ctypes.c_uint8)()
+        m = ctypes.c_uint64()
+        rc = lib().or_list_checkpoint(self._h, ptr, n, 1 if make_baseline else 0, ctypes.byref(out),
+                                      ctypes.byref(m))
+        if rc != 0:
+            raise RuntimeError(f"or_list_checkpoint failed: {rc}")
+        res = ctypes.string_at(out, m.value)
+        lib().or_free(out)
+        return res
+
+    def checkpoint(self, data, make_baseline=False):
+        b = bytes(data) if not isinstance(data, bytes) else data
+        return self._run(ctypes.c_char_p(b), len(b), make_baseline)
+
+    def checkpoint_array(self, arr, make_baseline=False):
+        return self._run(ctypes.c_void_p(arr.ctypes.data), arr.nbytes, make_baseline)
+
+    @property
+    def current_id(self):
+        return lib().or_list_current_id(self._h)
+
+    def counts(self):
+        f = ctypes.c_uint64()
+        s = ctypes.c_uint64()
+        lib().or_list_counts(self._h, ctypes.byref(f), ctypes.byref(s))
+        return f.value, s.value
+
+
+def _restart_call(fn, chain, ckpt_id, datalen):
+    keep = [bytes(c) for c in chain]
+    ptrs = (ctypes.c_void_p * len(keep))(*[ctypes.cast(ctypes.c_char_p(b), ctypes.c_void_p) for b in keep])
+    lens = (ctypes.c_uint64 * len(keep))(*[len(b) for b in keep])
+    out = ctypes.create_string_buffer(datalen)
+    return fn(ptrs, lens, len(keep), ckpt_id, out, datalen), out.raw
+
+
+def restart_list(chain, ckpt_id, datalen):
+    """Rebuild checkpoint `ckpt_id` from a chain of list-approach diffs."""
+    rc, out = _restart_call(lib().or_list_restart, chain, ckpt_id, datalen)
+    if rc != 0:
+        raise RuntimeError(f"or_list_restart failed: {rc}")
+    return out
+
+
+def restart_list_rc(chain, ckpt_id, datalen):
+    return _restart_call(lib().or_list_restart, chain, ckpt_id, datalen)[0]
 
 
 def restart(chain, ckpt_id, datalen):

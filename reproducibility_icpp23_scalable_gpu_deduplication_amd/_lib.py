@@ -13,10 +13,12 @@ LIB_PATH = os.path.join(_HERE, "libdedup_hip.so")
 DD_OK, DD_EINVAL, DD_ENOMEM, DD_ECORRUPT, DD_EHIP, DD_ESTATE = range(6)
 HASH_MURMUR3 = 0
 HASH_MD5 = 1
+METHOD_TREE = 0
+METHOD_LIST = 1
 
 # Every symbol include/dedup_c.h declares.
 EXPORTED = [
-    "dd_create", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
+    "dd_create", "dd_create_method", "dd_get_method", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
     "dd_max_diff_size", "dd_counts", "dd_get_ids", "dd_set_current_id", "dd_get_timers",
     "dd_restart", "dd_restart_device", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
     "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
@@ -64,6 +66,8 @@ def load():
     P = ctypes.POINTER
     sig = {
         "dd_create": ([u32, i32, vp, P(vp)], i32),
+        "dd_create_method": ([u32, i32, i32, vp, P(vp)], i32),
+        "dd_get_method": ([vp, P(i32)], i32),
         "dd_destroy": ([vp], i32),
         "dd_checkpoint": ([vp, vp, u64, i32, P(dd_buf)], i32),
         "dd_checkpoint_header": ([vp, P(dd_header), vp, u64, i32, P(dd_buf)], i32),
@@ -119,13 +123,21 @@ class TreeDeduplicator:
     (include/tree_approach.hpp:16-205).  Data arguments are device addresses
     (int) plus lengths; diffs come back as ``bytes``."""
 
+    METHOD = METHOD_TREE
+
     def __init__(self, chunk_size, hash_kind=HASH_MURMUR3, stream=None):
         self._h = None
         L = load()
         h = ctypes.c_void_p()
-        _check(L.dd_create(chunk_size, hash_kind, stream, ctypes.byref(h)), "dd_create")
+        _check(L.dd_create_method(chunk_size, hash_kind, self.METHOD, stream, ctypes.byref(h)), "dd_create_method")
         self._h = h
         self.chunk_size = chunk_size
+
+    @property
+    def method(self):
+        m = ctypes.c_int()
+        _check(load().dd_get_method(self._h, ctypes.byref(m)), "dd_get_method")
+        return m.value
 
     def close(self):
         if getattr(self, "_h", None):
@@ -236,6 +248,20 @@ class TreeDeduplicator:
         r = (ctypes.c_double * 2)()
         _check(load().dd_get_timers(self._h, t, r), "dd_get_timers")
         return list(t), list(r)
+
+
+class TreeLowRootDeduplicator(TreeDeduplicator):
+    """TreeLowRootDeduplicator (include/tree_approach.hpp:207-218): its
+    checkpoint runs the tree's low-offset labeling
+    (src/tree_low_root_approach.cpp:393-399), so it is the tree method."""
+
+
+class ListDeduplicator(TreeDeduplicator):
+    """ListDeduplicator (include/list_approach.hpp:21-200,
+    src/list_approach.cpp): one digest per chunk, first occurrences and
+    shifted duplicates named by chunk index."""
+
+    METHOD = METHOD_LIST
 
 
 class Batch:

@@ -195,6 +195,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.ckpt_id = K;
   job.baseline_id = baseline ? K : baseline_id;
   job.is_baseline = baseline ? 1 : 0;
+  job.list_mode = method == DD_METHOD_LIST ? 1 : 0;
+  job.node_bias = job.list_mode ? (uint32_t)(C - 1) : 0;
   job.tree = tree.as<Digest>();
   job.labels = labels.as<uint8_t>();
   job.slot_of = slot_of.as<uint32_t>();
@@ -264,8 +266,9 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   max_groups_ = 0;
   aligned16_ = vec16_ = true;
   hash_kind_ = jobs[0].hash_kind;
+  list_mode_ = jobs[0].list_mode != 0;
   for (const JobDev& J : jobs) {
-    if (J.hash_kind != hash_kind_) return DD_EINVAL;
+    if (J.hash_kind != hash_kind_ || (J.list_mode != 0) != list_mode_) return DD_EINVAL;
     max_groups_ = std::max(max_groups_, groups_of(J.D));
     const bool al = (reinterpret_cast<uintptr_t>(J.data) & 15) == 0 && (J.chunk_size & 15) == 0;
     aligned16_ = aligned16_ && al;
@@ -325,7 +328,9 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
     hjobs[j] = jobs[j];
     hjobs[j].host_ctr = &host_ctr[j];
     hjobs[j].nwg_ins_leaf = hprefix[off_[S_INSL] + j + 1] - hprefix[off_[S_INSL] + j];
-    hjobs[j].nwg_ins_int = hprefix[off_[S_TILES] + j + 1] - hprefix[off_[S_TILES] + j];
+    // A list job launches no interior inserts, so it has no interior claims
+    // for sbin_scan to add to its entry count.
+    hjobs[j].nwg_ins_int = jobs[j].list_mode ? 0 : hprefix[off_[S_TILES] + j + 1] - hprefix[off_[S_TILES] + j];
   }
   DD_TRY(dev_table_.ensure(table_bytes));
   dev_jobs_ = dev_table_.as<JobDev>();
@@ -340,7 +345,7 @@ std::vector<uint64_t> Pipeline::signature() const {
   sig.push_back((uint64_t)max_groups_);
   sig.push_back((uint64_t)leaf_map_);
   sig.push_back((uint64_t)aligned16_ | ((uint64_t)vec16_ << 1));
-  sig.push_back((uint64_t)hash_kind_);
+  sig.push_back((uint64_t)hash_kind_ | ((uint64_t)list_mode_ << 8));
   sig.push_back(reinterpret_cast<uintptr_t>(dev_table_.p));
   for (uint32_t o : off_) sig.push_back(o);
   for (uint32_t g : grid_) sig.push_back(g);
@@ -370,11 +375,18 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   DD_TRY(mark(1));
   launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], s);
   DD_TRY(mark(2));
-  for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
-  DD_TRY(mark(3));
-  launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], s);
-  DD_TRY(mark(4));
-  for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
+  if (list_mode_) {
+    // The list method has no tree: its chunks are classified by the inserts.
+    launch_list_resolve(J, P(S_INSL), nj, grid_[S_INSL], s);
+    DD_TRY(mark(3));
+    DD_TRY(mark(4));
+  } else {
+    for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
+    DD_TRY(mark(3));
+    launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], s);
+    DD_TRY(mark(4));
+    for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
+  }
   DD_TRY(mark(5));
   if (ev_dedup) DD_HIP(hipEventRecord(ev_dedup, s));
   launch_roots_count(J, P(S_NODEBLK), nj, grid_[S_NODEBLK], s);
@@ -521,7 +533,7 @@ struct HostCkpt {
 // Validates one diff of the chain and builds its region prefixes.  Every node
 // id, leaf count and data range the kernels will touch is checked here.
 dd_status parse_ckpt(const uint8_t* hdr32, const uint8_t* meta, uint64_t total, uint32_t idx, uint64_t len,
-                     uint32_t cs, HostCkpt& hc) {
+                     uint32_t cs, uint32_t bias, HostCkpt& hc) {
   if (total < 32) return DD_ECORRUPT;
   memcpy(&hc.hd, hdr32, 32);
   const dd_header& hd = hc.hd;
@@ -538,7 +550,7 @@ dd_status parse_ckpt(const uint8_t* hdr32, const uint8_t* meta, uint64_t total, 
   hc.f_off.resize((size_t)hd.num_first_ocur + 1);
   uint64_t acc = 0;
   for (uint32_t i = 0; i < hd.num_first_ocur; i++) {
-    const uint32_t node = rd32(fl + 4ull * i);
+    const uint64_t node = (uint64_t)rd32(fl + 4ull * i) + bias;
     if (node >= g.N) return DD_ECORRUPT;
     hc.f_off[i] = (uint32_t)acc;
     const LeafRuns r = subtree_runs(g, node);
@@ -568,7 +580,7 @@ dd_status parse_ckpt(const uint8_t* hdr32, const uint8_t* meta, uint64_t total, 
   }
   acc = 0;
   for (uint32_t i = 0; i < hd.num_shift_dupl; i++) {
-    const uint32_t node = rd32(sl + 8ull * i), prev = rd32(sl + 8ull * i + 4);
+    const uint64_t node = (uint64_t)rd32(sl + 8ull * i) + bias, prev = (uint64_t)rd32(sl + 8ull * i + 4) + bias;
     if (node >= g.N || prev >= g.N) return DD_ECORRUPT;
     const LeafRuns a = subtree_runs(g, node), b = subtree_runs(g, prev);
     if (a.n0 + a.n1 != b.n0 + b.n1) return DD_ECORRUPT;
@@ -642,6 +654,8 @@ dd_status restart_walk(dd_dedup* h, std::vector<HostCkpt>& ckpts, const std::vec
     rc.first_off = 32;
     rc.shift_off = 32 + 4ull * hc.hd.num_first_ocur + 8ull * hc.hd.num_prior_chkpts;
     rc.data_off = 32 + hc.meta_bytes;
+    rc.node_bias = h->st.method == DD_METHOD_LIST ? (uint32_t)(g.C - 1) : 0;
+    rc.pad = 0;
   }
   DD_HIP(hipMemcpyAsync(h->rs_desc.p, desc.data(), desc.size() * sizeof(RestartCkpt), hipMemcpyHostToDevice, s));
   DD_HIP(hipStreamSynchronize(s));
@@ -667,9 +681,10 @@ dd_status restart_walk(dd_dedup* h, std::vector<HostCkpt>& ckpts, const std::vec
 
 // Collects the checkpoints the walk visits: chkpt_id down to the reference id
 // named by each visited header (src/tree_approach.cpp:943).
+// `list`: the diffs are list-method diffs, whose node ids are chunk indices.
 template <typename GetHdr>
-dd_status collect_chain(uint32_t nchain, uint32_t chkpt_id, uint64_t len, GetHdr get, std::vector<HostCkpt>& out,
-                        std::vector<uint32_t>& ids) {
+dd_status collect_chain(uint32_t nchain, uint32_t chkpt_id, uint64_t len, bool list, GetHdr get,
+                        std::vector<HostCkpt>& out, std::vector<uint32_t>& ids) {
   if (chkpt_id >= nchain) return DD_EINVAL;
   uint8_t hdr[32];
   uint64_t total = 0;
@@ -681,13 +696,16 @@ dd_status collect_chain(uint32_t nchain, uint32_t chkpt_id, uint64_t len, GetHdr
   if (h0.datalen != len) return DD_EINVAL;
   if (h0.chunk_size == 0) return DD_ECORRUPT;
   const uint32_t cs = h0.chunk_size;
+  const uint64_t Cn = ceil_div(len, cs);
+  if (Cn > 0x7fffffffull) return DD_ECORRUPT;
+  const uint32_t bias = list ? (uint32_t)(Cn - 1) : 0;
   int64_t idx = chkpt_id;
   int64_t ref = h0.ref_id;
   bool first = true;
   while (idx >= ref) {
     if (!first) DD_TRY(get((uint32_t)idx, hdr, meta, total));
     HostCkpt hc;
-    DD_TRY(parse_ckpt(hdr, meta.data(), total, (uint32_t)idx, len, cs, hc));
+    DD_TRY(parse_ckpt(hdr, meta.data(), total, (uint32_t)idx, len, cs, bias, hc));
     ref = hc.hd.ref_id;
     out.push_back(std::move(hc));
     ids.push_back((uint32_t)idx);
@@ -702,11 +720,17 @@ dd_status collect_chain(uint32_t nchain, uint32_t chkpt_id, uint64_t len, GetHdr
 extern "C" {
 
 dd_status dd_create(uint32_t chunk_size, int hash_kind, void* hip_stream, dd_dedup** out) {
+  return dd_create_method(chunk_size, hash_kind, DD_METHOD_TREE, hip_stream, out);
+}
+
+dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void* hip_stream, dd_dedup** out) {
   if (!out) return DD_EINVAL;
   *out = nullptr;
   if (chunk_size == 0 || (hash_kind != DD_HASH_MURMUR3 && hash_kind != DD_HASH_MD5)) return DD_EINVAL;
+  if (method != DD_METHOD_TREE && method != DD_METHOD_LIST) return DD_EINVAL;
   dd_dedup* h = new (std::nothrow) dd_dedup;
   if (!h) return DD_ENOMEM;
+  h->st.method = method;
   if (hipGetDevice(&h->device) != hipSuccess) {
     delete h;
     return DD_EHIP;
@@ -735,6 +759,12 @@ dd_status dd_create(uint32_t chunk_size, int hash_kind, void* hip_stream, dd_ded
     }
   }
   *out = h;
+  return DD_OK;
+}
+
+dd_status dd_get_method(dd_dedup* h, int* method) {
+  if (!h || !method) return DD_EINVAL;
+  *method = h->st.method;
   return DD_OK;
 }
 
@@ -823,7 +853,7 @@ dd_status dd_restart(dd_dedup* h, const dd_buf* chain, uint32_t nchain, uint32_t
   };
   std::vector<HostCkpt> ck;
   std::vector<uint32_t> ids;
-  DD_TRY(collect_chain(nchain, chkpt_id, len, get, ck, ids));
+  DD_TRY(collect_chain(nchain, chkpt_id, len, h->st.method == DD_METHOD_LIST, get, ck, ids));
   std::vector<const uint8_t*> hd(ck.size()), dv(ck.size(), nullptr);
   for (size_t k = 0; k < ck.size(); k++) hd[k] = chain[ids[k]].data;
   return restart_walk(h, ck, hd, dv, chkpt_id, dev_out, len);
@@ -854,7 +884,7 @@ dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uin
   };
   std::vector<HostCkpt> ck;
   std::vector<uint32_t> ids;
-  DD_TRY(collect_chain(nchain, chkpt_id, len, get, ck, ids));
+  DD_TRY(collect_chain(nchain, chkpt_id, len, h->st.method == DD_METHOD_LIST, get, ck, ids));
   std::vector<const uint8_t*> hd(ck.size(), nullptr), dv(ck.size());
   for (size_t k = 0; k < ck.size(); k++) dv[k] = static_cast<const uint8_t*>(dev_chain[ids[k]]);
   return restart_walk(h, ck, hd, dv, chkpt_id, dev_out, len);
@@ -865,7 +895,8 @@ dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out) 
   *out = nullptr;
   for (uint32_t i = 0; i < n; i++) {
     if (!handles[i]) return DD_EINVAL;
-    if (handles[i]->device != handles[0]->device || handles[i]->st.hash_kind != handles[0]->st.hash_kind)
+    if (handles[i]->device != handles[0]->device || handles[i]->st.hash_kind != handles[0]->st.hash_kind ||
+        handles[i]->st.method != handles[0]->st.method)
       return DD_EINVAL;
     for (uint32_t k = 0; k < i; k++)
       if (handles[k] == handles[i]) return DD_EINVAL;

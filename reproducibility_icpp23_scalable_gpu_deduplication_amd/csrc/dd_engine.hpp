@@ -31,6 +31,7 @@ struct DevBuf {
 struct StreamState {
   uint32_t chunk_size = 0;
   int hash_kind = 0;
+  int method = DD_METHOD_TREE;  // DD_METHOD_TREE or DD_METHOD_LIST
   uint32_t current_id = 0, baseline_id = 0;
   bool have_tree = false;
   bool ctr_armed = false;  // the device counters are zero and hold the current map count
@@ -101,6 +102,7 @@ class Pipeline {
   int max_groups_ = 0;
   bool aligned16_ = true, vec16_ = true;
   int hash_kind_ = 0;
+  bool list_mode_ = false;  // list-method jobs: no tree passes
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
 
  public:

@@ -190,9 +190,10 @@ __device__ __forceinline__ void record_pending(const JobDev& J, uint32_t b, uint
   }
   // Mark the group-0 tiles of changed leaves: the first marker of a tile in
   // this checkpoint appends it to the dirty list.  A baseline visits every
-  // tile and every leaf block densely, so it keeps no lists.
+  // tile and every leaf block densely, so it keeps no lists; the list method
+  // has no tiles.
   uint32_t T = pending ? leaf_tile(J.D, J.C - 1 + k) : 0;
-  bool todo = pending && !J.is_baseline;
+  bool todo = pending && !J.is_baseline && !J.list_mode;
   while (true) {
     const unsigned long long act = __ballot(todo);
     if (!act) break;
@@ -258,10 +259,13 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jo
     if (digest_eq(d, prev)) {
       // A fixed leaf of an incremental checkpoint: its digest is already in
       // the map (every leaf digest of the previous checkpoint was inserted),
-      // so the insert the serial reference performs is a no-op.  In a
-      // baseline the previous digest is all-zero and the insert still happens.
-      lab = J.is_baseline ? (uint8_t)(FIXED_DUPL | PENDING) : (uint8_t)FIXED_DUPL;
-      pending = J.is_baseline;
+      // so the insert the serial reference performs is a no-op.  In a tree
+      // baseline the previous digest is all-zero and the insert still
+      // happens; the list method inserts only chunks whose digest differs
+      // from the list's (src/list_approach.cpp:50-60).
+      const bool ins = J.is_baseline && !J.list_mode;
+      lab = ins ? (uint8_t)(FIXED_DUPL | PENDING) : (uint8_t)FIXED_DUPL;
+      pending = ins;
     } else {
       lab = PENDING;
       pending = true;
@@ -281,8 +285,9 @@ __device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const D
   bool pending;
   uint8_t lab;
   if (digest_eq(d, prev)) {
-    lab = J.is_baseline ? (uint8_t)(FIXED_DUPL | PENDING) : (uint8_t)FIXED_DUPL;
-    pending = J.is_baseline;
+    const bool ins = J.is_baseline && !J.list_mode;
+    lab = ins ? (uint8_t)(FIXED_DUPL | PENDING) : (uint8_t)FIXED_DUPL;
+    pending = ins;
   } else {
     lab = PENDING;
     pending = true;
@@ -516,14 +521,17 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
   // wavefront takes part in its shuffles; lanes past the list are inactive.
   for (uint32_t base = wg * blockDim.x; base < nitems; base += nwg * blockDim.x) {
     const uint32_t it = base + threadIdx.x;
-    const bool act = it < nitems;
+    bool act = it < nitems;
     uint64_t leaf = 0;
     uint8_t own = 0;
     Digest k{0, 0};
     if (act) {
       leaf = ic.C - 1 + (dense ? it : list_item(pend_list, pend_cap, pre, it));
       own = labels[leaf];
-      k = ic.tree[leaf];
+      // A list baseline leaves all-zero digests out (they equal the fresh
+      // list's entry); every other leaf of a dense pass needs its insert.
+      if (own == FIXED_DUPL) act = false;
+      else k = ic.tree[leaf];
     }
     const unsigned long long pk0 = __shfl_up(k.k0, 1, 64), pk1 = __shfl_up(k.k1, 1, 64);
     const unsigned long long pleaf = __shfl_up((unsigned long long)leaf, 1, 64);
@@ -567,6 +575,37 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
   }
   const uint32_t n = block_sum<uint32_t>(claims, sh);
   if (threadIdx.x == 0) J.blk_claim[wg] = n;
+}
+
+// ---------------------------------------------------------------------------
+// List method (src/list_approach.cpp:23-62): a changed chunk is a first
+// occurrence when its insert created the entry or lowered it to this chunk
+// (the serial loop's first chunk of a digest), and a shifted duplicate
+// otherwise.  insert_leaves leaves exactly those first occurrences PENDING.
+// Interior labels are DONE from the baseline on, so the root scan emits
+// every FIRST or SHIFT leaf.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_list_resolve(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobDev& J = jobs[j];
+  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;  // written before this kernel: uniform
+  const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
+  const uint64_t C = J.C;
+  uint8_t* const labels = J.labels;
+  if (J.is_baseline) {
+    for (uint64_t i = (uint64_t)wg * blockDim.x + threadIdx.x; i < C; i += (uint64_t)nwg * blockDim.x) {
+      if (labels[C - 1 + i] == PENDING) labels[C - 1 + i] = FIRST_OCUR;
+      if (i + 1 < C) labels[i] = DONE;
+    }
+    return;
+  }
+  __shared__ uint32_t pre[kPendSpread + 1];
+  load_list_prefix(J.pend_cnt, pre);
+  const uint32_t n = pre[kPendSpread];
+  for (uint32_t it = wg * blockDim.x + threadIdx.x; it < n; it += nwg * blockDim.x) {
+    const uint64_t leaf = C - 1 + list_item(J.pend_list, J.pend_cap, pre, it);
+    if (labels[leaf] == PENDING) labels[leaf] = FIRST_OCUR;
+  }
 }
 
 __device__ __forceinline__ int tile_groups(uint32_t D) { return D == 0 ? 1 : (int)((D + kTileLevels - 1) / kTileLevels); }
@@ -913,7 +952,9 @@ __device__ __forceinline__ void scan16(const JobDev& J, uint64_t v0, const uint4
     if (L != FIRST_OCUR && L != SHIFT_DUPL) continue;
     bool root;
     if (v == 0) {
-      root = L == FIRST_OCUR;  // node 0 is emitted only as a first occurrence
+      // A tree emits node 0 only as a first occurrence; in a list job node 0
+      // is a candidate only when it is the single chunk.
+      root = L == FIRST_OCUR || J.list_mode;
     } else {
       const uint8_t P = t == 0 ? p_first : (uint8_t)(pw >> (8 * ((t - 1) >> 1)));
       root = P == DONE;
@@ -1300,14 +1341,15 @@ __global__ __launch_bounds__(256) void k_write_meta(const JobDev* __restrict__ j
   uint8_t* pf = out + 32;
   uint8_t* pp = pf + 4ull * F;
   uint8_t* ps = pp + 8ull * P;
-  for (uint32_t i = tid; i < F; i += nth) store_u32(pf + 4ull * i, J.f_list[i]);
+  const uint32_t bias = J.node_bias;  // list diffs name chunks, not leaves
+  for (uint32_t i = tid; i < F; i += nth) store_u32(pf + 4ull * i, J.f_list[i] - bias);
   for (uint32_t i = tid; i < P; i += nth) {
     store_u32(pp + 8ull * i, J.ptab[2 * i]);
     store_u32(pp + 8ull * i + 4, J.ptab[2 * i + 1]);
   }
   for (uint32_t i = tid; i < S; i += nth) {
-    store_u32(ps + 8ull * i, J.ss_node[i]);
-    store_u32(ps + 8ull * i + 4, J.ss_prev[i]);
+    store_u32(ps + 8ull * i, J.ss_node[i] - bias);
+    store_u32(ps + 8ull * i + 4, J.ss_prev[i] - bias);
   }
 }
 
@@ -1366,7 +1408,7 @@ __global__ __launch_bounds__(256) void k_gather(const JobDev* __restrict__ jobs,
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   // The roots are written: reset the labels of the tiles this checkpoint
   // visited (gather runs a large grid whatever the diff size).
-  reset_visited_tiles(J, blockIdx.x - prefix[j], prefix[j + 1] - prefix[j]);
+  if (!J.list_mode) reset_visited_tiles(J, blockIdx.x - prefix[j], prefix[j + 1] - prefix[j]);
   const uint32_t F = J.ctr->nfirst, P = J.ctr->nprior, S = J.ctr->nshift;
   const uint64_t Dq = J.ctr->dchunks;
   if (Dq == 0) return;
@@ -1465,7 +1507,7 @@ __global__ __launch_bounds__(256) void k_restart_cover(const RestartCkpt* rcp, T
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nfl + nsl; e += stride) {
     if (e < nfl) {
       const uint32_t r = region_of(rc.f_off, rc.nfirst, e);
-      const uint32_t root = load_u32(rc.diff + rc.first_off + 4ull * r);
+      const uint32_t root = load_u32(rc.diff + rc.first_off + 4ull * r) + rc.node_bias;
       const uint64_t leaf = run_leaf(subtree_runs(g, root), e - rc.f_off[r]);
       CoverEntry ce;
       ce.stamp = stamp;
@@ -1475,8 +1517,8 @@ __global__ __launch_bounds__(256) void k_restart_cover(const RestartCkpt* rcp, T
     } else {
       const uint64_t q = e - nfl;
       const uint32_t r = region_of(rc.s_off, rc.nshift, q);
-      const uint32_t node = load_u32(rc.diff + rc.shift_off + 8ull * r);
-      const uint32_t prev = load_u32(rc.diff + rc.shift_off + 8ull * r + 4);
+      const uint32_t node = load_u32(rc.diff + rc.shift_off + 8ull * r) + rc.node_bias;
+      const uint32_t prev = load_u32(rc.diff + rc.shift_off + 8ull * r + 4) + rc.node_bias;
       const uint64_t idx = q - rc.s_off[r];
       const uint64_t leaf = run_leaf(subtree_runs(g, node), idx);
       const uint64_t src = run_leaf(subtree_runs(g, prev), idx);
@@ -1599,6 +1641,9 @@ void launch_finish(const JobDev* jobs, uint32_t njobs, hipStream_t s) {
 
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   if (grid) DD_LAUNCH(k_insert_leaves, grid, 256, 0, s, jobs, prefix, njobs);
+}
+void launch_list_resolve(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  if (grid) DD_LAUNCH(k_list_resolve, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
   if (grid) DD_LAUNCH(k_pass1, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);

@@ -45,6 +45,11 @@ struct JobDev {
   uint32_t ckpt_id;
   uint32_t baseline_id;
   uint32_t is_baseline;
+  // Method: 0 = tree, 1 = list (ListDeduplicator, src/list_approach.cpp:23-256):
+  // leaves only, every interior label DONE, so each changed chunk is its own
+  // root.  Diff node ids of a list job are chunk indices (leaf - node_bias).
+  uint32_t list_mode;
+  uint32_t node_bias;
   // persistent state
   Digest* tree;        // N digests, heap order
   uint8_t* labels;     // N labels (padded for 16-byte reads)
@@ -103,6 +108,8 @@ struct RestartCkpt {
   uint64_t first_off;       // byte offset of the first-occurrence list
   uint64_t shift_off;       // byte offset of the shifted list
   uint64_t data_off;        // byte offset of the data section
+  uint32_t node_bias;       // added to every node id of the diff (C-1 for list diffs: chunk -> leaf)
+  uint32_t pad;
 };
 
 struct CoverEntry {
@@ -126,6 +133,9 @@ enum LeafMapping : int { LEAF_DIRECT = 0, LEAF_DEEP = 1, LEAF_LDS64 = 2, LEAF_LD
 void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping,
                  bool all_aligned16, int hash_kind, hipStream_t s);
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+// List method: the leaves left PENDING by insert_leaves are first
+// occurrences; a baseline also sets every interior label to DONE.
+void launch_list_resolve(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
 void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);

@@ -23,15 +23,24 @@ HostBuf take(dd_buf& b) {
   return v;
 }
 
-std::pair<uint64_t, uint64_t> sizes_of(const header_t& hd, uint64_t total) {
+std::pair<uint64_t, uint64_t> sizes_of(int method, const header_t& hd, uint64_t total) {
+  if (method == DD_METHOD_LIST) {
+    const uint64_t data = (uint64_t)hd.chunk_size * hd.num_first_ocur;
+    return {data, total - data};
+  }
   const uint64_t meta = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
   return {total - meta, meta};
 }
 
 }  // namespace
 
-TreeDeduplicator::TreeDeduplicator(uint32_t chunk_size, HashKind kind, void* hip_stream) : chunk_size_(chunk_size) {
-  check(dd_create(chunk_size, static_cast<int>(kind), hip_stream, &h_), "dd_create");
+TreeDeduplicator::TreeDeduplicator(uint32_t chunk_size, HashKind kind, void* hip_stream)
+    : TreeDeduplicator(chunk_size, kind, hip_stream, DD_METHOD_TREE, "Tree", ".hashtree.incr_chkpt") {}
+
+TreeDeduplicator::TreeDeduplicator(uint32_t chunk_size, HashKind kind, void* hip_stream, int method,
+                                   const char* approach, const char* suffix)
+    : chunk_size_(chunk_size), method_(method), approach_(approach), suffix_(suffix) {
+  check(dd_create_method(chunk_size, static_cast<int>(kind), method, hip_stream, &h_), "dd_create_method");
 }
 
 TreeDeduplicator::~TreeDeduplicator() {
@@ -58,7 +67,7 @@ void TreeDeduplicator::checkpoint(header_t& header, uint8_t* data_ptr, size_t da
   check(dd_checkpoint_header(h_, &header, data_ptr, data_len, make_baseline ? 1 : 0, &b), "checkpoint");
   diff_h = take(b);
   data_len_ = data_len;
-  datasizes_ = sizes_of(header, diff_h.size());
+  datasizes_ = sizes_of(method_, header, diff_h.size());
 }
 
 void TreeDeduplicator::checkpoint(uint8_t* data_ptr, size_t len, std::string& filename, std::string& logname,
@@ -104,7 +113,7 @@ void TreeDeduplicator::restart(uint8_t* data_ptr, size_t len, std::vector<std::s
   for (size_t i = 0; i < chkpt_filenames.size(); i++) {
     std::ifstream f;
     f.exceptions(std::ifstream::failbit | std::ifstream::badbit);
-    f.open(chkpt_filenames[i] + ".hashtree.incr_chkpt", std::ifstream::in | std::ifstream::binary);
+    f.open(chkpt_filenames[i] + suffix_, std::ifstream::in | std::ifstream::binary);
     f.seekg(0, std::ios::end);
     const std::streamoff n = f.tellg();
     f.seekg(0, std::ios::beg);
@@ -164,11 +173,11 @@ void TreeDeduplicator::write_chkpt_log(header_t& header, HostBuf& diff_h, std::s
     result << "Approach,Chkpt ID,Chunk Size,Uncompressed Size,Compressed Size,Data Size,Metadata Size,"
               "Setup Time,Comparison Time,Gather Time,Write Time\n";
   const uint32_t id = current_id();
-  result << "Tree," << id << "," << chunk_size_ << "," << data_len_ << "," << datasizes_.first + datasizes_.second
+  result << approach_ << "," << id << "," << chunk_size_ << "," << data_len_ << "," << datasizes_.first + datasizes_.second
          << "," << datasizes_.first << "," << datasizes_.second << "," << t[0] << "," << t[1] << "," << t[2] << ","
          << t[3] << "\n";
-  timing << "Tree," << id << "," << chunk_size_ << "," << t[1] << "," << t[2] << "," << t[3] << "\n";
-  size_file << "Tree," << id << "," << chunk_size_ << "," << datasizes_.first << "," << datasizes_.second << ","
+  timing << approach_ << "," << id << "," << chunk_size_ << "," << t[1] << "," << t[2] << "," << t[3] << "\n";
+  size_file << approach_ << "," << id << "," << chunk_size_ << "," << datasizes_.first << "," << datasizes_.second << ","
             << metadata_breakdown(header, diff_h, id);
 }
 
@@ -176,7 +185,7 @@ void TreeDeduplicator::write_restart_log(uint32_t select_chkpt, std::string& log
   double t[4], rt[2];
   dd_get_timers(h_, t, rt);
   std::fstream f(logname, std::fstream::out | std::fstream::app);
-  f << "Tree," << select_chkpt << "," << chunk_size_ << "," << rt[0] << "," << rt[1] << "\n";
+  f << approach_ << "," << select_chkpt << "," << chunk_size_ << "," << rt[0] << "," << rt[1] << "\n";
 }
 
 }  // namespace dd

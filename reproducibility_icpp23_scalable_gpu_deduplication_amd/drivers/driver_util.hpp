@@ -8,25 +8,30 @@
 #include <string.h>
 
 #include <fstream>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "tree_deduplicator.hpp"
+
 namespace dd_drivers {
 
 // The reference's mode flags (src/utils.cpp:15-37).  Its drivers hand every
-// tree mode to TreeDeduplicator (src/dedup_files.cpp:61-63), and so do these;
-// the full, basic and list methods are not part of this library.
-enum class Mode { Unknown, Tree, Other };
+// tree mode to TreeDeduplicator (src/dedup_files.cpp:61-63) and the list
+// mode to ListDeduplicator (:59-60), and so do these; the full and basic
+// methods are not part of this library.
+enum class Mode { Unknown, Tree, List, Other };
 
 inline Mode get_mode(int argc, char** argv) {
   static const char* tree_flags[] = {"--run-tree-chkpt", "--run-tree-low-offset-ref-chkpt",
                                      "--run-tree-low-offset-chkpt", "--run-tree-low-root-ref-chkpt",
                                      "--run-tree-low-root-chkpt"};
-  static const char* other_flags[] = {"--run-full-chkpt", "--run-basic-chkpt", "--run-list-chkpt"};
+  static const char* other_flags[] = {"--run-full-chkpt", "--run-basic-chkpt"};
   for (int i = 0; i < argc; i++) {
     for (const char* f : tree_flags)
       if (!strcmp(argv[i], f)) return Mode::Tree;
+    if (!strcmp(argv[i], "--run-list-chkpt")) return Mode::List;
     for (const char* f : other_flags)
       if (!strcmp(argv[i], f)) return Mode::Other;
   }
@@ -35,12 +40,23 @@ inline Mode get_mode(int argc, char** argv) {
 
 inline void print_mode_help() {
   printf("Modes (every tree mode runs the tree approach with lowest-offset first occurrence):\n");
+  printf("List checkpoint:                                   --run-list-chkpt\n");
   printf("Tree checkpoint:                                   --run-tree-chkpt\n");
   printf("Tree checkpoint lowest offset reference algorithm: --run-tree-low-offset-ref-chkpt\n");
   printf("Tree checkpoint lowest offset algorithm:           --run-tree-low-offset-chkpt\n");
   printf("Tree checkpoint lowest root reference algorithm:   --run-tree-low-root-ref-chkpt\n");
   printf("Tree checkpoint lowest root algorithm:             --run-tree-low-root-chkpt\n");
 }
+
+// The deduplicator of a mode, the suffix of its checkpoint files
+// (src/dedup_files.cpp:94-99) and the label of its restart digest
+// (src/restart_chkpt_files.cpp:165,194).
+inline std::unique_ptr<dd::TreeDeduplicator> make_dedup(Mode m, uint32_t chunk_size) {
+  if (m == Mode::List) return std::unique_ptr<dd::TreeDeduplicator>(new dd::ListDeduplicator(chunk_size));
+  return std::unique_ptr<dd::TreeDeduplicator>(new dd::TreeDeduplicator(chunk_size));
+}
+inline const char* file_suffix(Mode m) { return m == Mode::List ? ".hashlist.incr_chkpt" : ".hashtree.incr_chkpt"; }
+inline const char* digest_label(Mode m) { return m == Mode::List ? "Hashlist digest:       " : "Hashtree digest:       "; }
 
 inline std::vector<uint8_t> read_file(const std::string& name) {
   std::ifstream f;

@@ -13,8 +13,8 @@ def to_dev(torch, data):
     return t
 
 
-def gpu_chain(torch, chunk, datas, kind=dd.HASH_MURMUR3, baselines=(0,)):
-    g = dd.TreeDeduplicator(chunk, kind)
+def gpu_chain(torch, chunk, datas, kind=dd.HASH_MURMUR3, baselines=(0,), cls=dd.TreeDeduplicator):
+    g = cls(chunk, kind)
     diffs, counts = [], []
     for i, x in enumerate(datas):
         t = to_dev(torch, x)
@@ -24,8 +24,8 @@ def gpu_chain(torch, chunk, datas, kind=dd.HASH_MURMUR3, baselines=(0,)):
     return g, diffs, counts
 
 
-def oracle_chain(chunk, datas, kind=O.MURMUR3, baselines=(0,)):
-    o = O.OracleDeduplicator(chunk, kind)
+def oracle_chain(chunk, datas, kind=O.MURMUR3, baselines=(0,), cls=O.OracleDeduplicator):
+    o = cls(chunk, kind)
     diffs, counts = [], []
     for i, x in enumerate(datas):
         if isinstance(x, np.ndarray):

@@ -67,13 +67,24 @@ def test_no_fallback_when_library_missing(tmp_path, monkeypatch):
 
 
 def test_file_drivers_built_and_reject_other_methods(tmp_path):
-    # The drivers exit with status 2 on a non-tree method before using a GPU.
+    # The drivers exit with status 2 on the full and basic methods (not built
+    # here) and on an unknown mode, before using a GPU.
     import os
     import subprocess
 
     bindir = os.path.join(os.path.dirname(dd.LIB_PATH), "bin")
     for name in ("dedup_chkpt_files", "restart_chkpt_files"):
         assert os.access(os.path.join(bindir, name), os.X_OK)
-    r = subprocess.run([os.path.join(bindir, "dedup_chkpt_files"), "128", "1", "--run-list-chkpt", "x"],
-                       cwd=tmp_path, capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "unsupported" in r.stderr
+    for mode, msg in (("--run-full-chkpt", "unsupported"), ("--run-basic-chkpt", "unsupported"),
+                      ("--run-nothing", "incorrect")):
+        r = subprocess.run([os.path.join(bindir, "dedup_chkpt_files"), "128", "1", mode, "x"],
+                           cwd=tmp_path, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and msg in r.stderr, (mode, r.stderr)
+
+
+def test_method_constants_match_header():
+    import re
+
+    hdr = open(os.path.join(ROOT, "include", "dedup_c.h")).read()
+    assert int(re.search(r"DD_METHOD_TREE = (\d+)", hdr).group(1)) == dd.METHOD_TREE
+    assert int(re.search(r"DD_METHOD_LIST = (\d+)", hdr).group(1)) == dd.METHOD_LIST

@@ -10,9 +10,9 @@ from gpu_util import first_mismatch, gpu_restart, oracle_chain
 pytestmark = pytest.mark.gpu
 
 
-def run_batch(torch, chains, chunk, graph, baselines=(0,)):
+def run_batch(torch, chains, chunk, graph, baselines=(0,), cls=dd.TreeDeduplicator):
     n = len(chains)
-    handles = [dd.TreeDeduplicator(chunk) for _ in range(n)]
+    handles = [cls(chunk) for _ in range(n)]
     b = dd.Batch(handles, graph=graph)
     nck = len(chains[0])
     diffs = [[] for _ in range(n)]
