@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-size", type=int, default=512 << 20)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-buffer batched comparison")
+    ap.add_argument("--no-list", action="store_true", help="skip the list-approach measurement")
+    ap.add_argument("--method", choices=["tree", "list"], default="tree",
+                    help="method of the main measurement (list: for profiling the list kernels alone)")
     ap.add_argument("--batch-buffers", type=int, default=64)
     ap.add_argument("--field", action="store_true",
                     help="also run the MD5 float-field chain (2^30 float32 values, 64 B chunks, 20 checkpoints)")
@@ -66,10 +69,10 @@ def traffic_from_profiles(kernel, workload):
         return json.load(f).get("bytes_per_launch")
 
 
-def run_config(torch, dd, W, size, chunk, frac, nckpt, steps, warmup, kind, seed=1931):
+def run_config(torch, dd, W, size, chunk, frac, nckpt, steps, warmup, kind, seed=1931, cls=None):
     chain = W.chain_runs_gpu(torch, seed, size, chunk, nckpt, frac)
     torch.cuda.synchronize()
-    g = dd.TreeDeduplicator(chunk, kind)
+    g = (cls or dd.TreeDeduplicator)(chunk, kind)
     cap = dd.max_diff_size(size, chunk, warmup + steps + 2 * nckpt + 1)
     out = torch.empty(cap, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
@@ -429,11 +432,13 @@ def main():
         print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "copies": 5}), flush=True)
         return
     kind = dd.HASH_MD5 if args.hash == "md5" else dd.HASH_MURMUR3
-    r = run_config(torch, dd, W, args.size, args.chunk, args.frac, args.nckpt, args.steps, args.warmup, kind)
+    main_cls = dd.ListDeduplicator if args.method == "list" else dd.TreeDeduplicator
+    r = run_config(torch, dd, W, args.size, args.chunk, args.frac, args.nckpt, args.steps, args.warmup, kind,
+                   cls=main_cls)
     workload = f"plain_{args.size >> 20}MiB_c{args.chunk}_{args.frac * 100:g}pct_{args.hash}"
     traffic = traffic_from_profiles("leaf_hash", workload)
     line = {
-        "metric": "input GB/s deduped (plain tree approach), bit-exact",
+        "metric": f"input GB/s deduped (plain {args.method} approach), bit-exact",
         "value": round(r["gbs"], 3),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -482,6 +487,18 @@ def main():
                                   ms_incremental=round(s["gpu_ms_incremental"], 4)))
                 print(json.dumps({"sweep_point": sweep[-1]}), file=sys.stderr, flush=True)
         line["sweep"] = sweep
+    if not args.no_list and args.method == "tree":
+        # The list approach (ListDeduplicator) on the same chain: one map
+        # entry and one metadata record per changed chunk, no tree passes.
+        lr = run_config(torch, dd, W, args.size, args.chunk, args.frac, args.nckpt, args.steps, args.warmup, kind,
+                        cls=dd.ListDeduplicator)
+        line["list"] = dict(gbs=round(lr["gbs"], 3), ms_per_step=round(lr["ms_per_step"], 4),
+                            leaf_gbs=round(lr["leaf_gbs"], 1), leaf_mapping=lr["leaf_mapping"],
+                            gpu_ms_baseline_ckpt=round(lr["gpu_ms_baseline"], 4),
+                            gpu_ms_incremental_ckpt=round(lr["gpu_ms_incremental"], 4),
+                            diff_bytes_avg=int(lr["diff_bytes_avg"]),
+                            diff_bytes_vs_tree=round(lr["diff_bytes_avg"] / r["diff_bytes_avg"], 3))
+        print(json.dumps({"list": line["list"]}), file=sys.stderr, flush=True)
     if not args.no_batched:
         line["batched"] = run_batched(torch, dd, W, args.batch_buffers, args.nckpt, args.chunk, kind)
         print(json.dumps({"batched": line["batched"]}), file=sys.stderr, flush=True)
