@@ -48,6 +48,17 @@ class TreeDeduplicator {
   void restart(uint8_t* data_ptr, size_t len, std::vector<std::string>& chkpt_filenames, std::string& logname,
                uint32_t chkpt_id);
 
+  // Checkpoints one buffer per deduplicator in one call (dd_batch_*: one
+  // launch per stage over all buffers) and writes each diff to filenames[i]
+  // and its logs under lognames[i], as checkpoint(ptr, len, filename,
+  // logname, make_baseline) does for one; advances every current_id.  The
+  // deduplicators share the device, hash and method.  No counterpart in the
+  // reference, whose multi-rank driver (src/dedup_files_mpi.cpp:54-120) runs
+  // one deduplicator per MPI rank.
+  static void checkpoint_batch(const std::vector<TreeDeduplicator*>& dedups, const std::vector<uint8_t*>& data,
+                               const std::vector<size_t>& lens, std::vector<std::string>& filenames,
+                               std::vector<std::string>& lognames, bool make_baseline);
+
   // include/tree_approach.hpp:53-59
   size_t num_first_ocur();
   size_t num_shift_dupl();

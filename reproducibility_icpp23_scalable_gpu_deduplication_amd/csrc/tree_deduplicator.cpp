@@ -3,6 +3,8 @@
 
 #include <string.h>
 
+#include <hip/hip_runtime.h>
+
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -126,6 +128,57 @@ void TreeDeduplicator::restart(uint8_t* data_ptr, size_t len, std::vector<std::s
   write_restart_log(chkpt_id, logname);
 }
 
+void TreeDeduplicator::checkpoint_batch(const std::vector<TreeDeduplicator*>& dedups,
+                                        const std::vector<uint8_t*>& data, const std::vector<size_t>& lens,
+                                        std::vector<std::string>& filenames, std::vector<std::string>& lognames,
+                                        bool make_baseline) {
+  const size_t n = dedups.size();
+  if (n == 0) return;
+  if (data.size() != n || lens.size() != n || filenames.size() != n || lognames.size() != n)
+    throw std::runtime_error("checkpoint_batch: argument sizes differ");
+  std::vector<dd_dedup*> hs(n);
+  for (size_t i = 0; i < n; i++) hs[i] = dedups[i]->h_;
+  dd_batch* b = nullptr;
+  check(dd_batch_create(hs.data(), (uint32_t)n, &b), "dd_batch_create");
+  // Buffer shapes change from call to call here: no graph capture.
+  dd_batch_use_graph(b, 0);
+  std::vector<void*> outs(n, nullptr);
+  std::vector<uint64_t> caps(n), out_lens(n), ulens(n);
+  std::vector<int> base(n, make_baseline ? 1 : 0);
+  std::vector<const void*> ins(n);
+  dd_status st = DD_OK;
+  for (size_t i = 0; i < n && st == DD_OK; i++) {
+    caps[i] = dd_max_diff_size(lens[i], dedups[i]->chunk_size_, dedups[i]->current_id());
+    ulens[i] = lens[i];
+    ins[i] = data[i];
+    if (hipMalloc(&outs[i], caps[i]) != hipSuccess) st = DD_ENOMEM;
+  }
+  if (st == DD_OK)
+    st = dd_batch_checkpoint_device(b, ins.data(), ulens.data(), base.data(), outs.data(), caps.data(), out_lens.data());
+  std::vector<HostBuf> diffs(n);
+  for (size_t i = 0; i < n && st == DD_OK; i++) {
+    diffs[i].resize(out_lens[i]);
+    if (hipMemcpy(diffs[i].data(), outs[i], out_lens[i], hipMemcpyDeviceToHost) != hipSuccess) st = DD_EHIP;
+  }
+  for (void* p : outs)
+    if (p) (void)hipFree(p);
+  dd_batch_destroy(b);
+  check(st, "checkpoint_batch");
+  for (size_t i = 0; i < n; i++) {
+    TreeDeduplicator& d = *dedups[i];
+    header_t header;
+    memcpy(&header, diffs[i].data(), sizeof header);
+    d.data_len_ = lens[i];
+    d.datasizes_ = sizes_of(d.method_, header, diffs[i].size());
+    d.write_chkpt_log(header, diffs[i], lognames[i]);
+    std::ofstream file;
+    file.exceptions(std::ofstream::failbit | std::ofstream::badbit);
+    file.open(filenames[i], std::ofstream::out | std::ofstream::binary);
+    file.write(reinterpret_cast<const char*>(diffs[i].data()), (std::streamsize)diffs[i].size());
+    file.close();
+  }
+}
+
 size_t TreeDeduplicator::num_first_ocur() {
   uint64_t f = 0;
   dd_counts(h_, &f, nullptr);
@@ -172,7 +225,7 @@ void TreeDeduplicator::write_chkpt_log(header_t& header, HostBuf& diff_h, std::s
   if (result.tellp() == 0)
     result << "Approach,Chkpt ID,Chunk Size,Uncompressed Size,Compressed Size,Data Size,Metadata Size,"
               "Setup Time,Comparison Time,Gather Time,Write Time\n";
-  const uint32_t id = current_id();
+  const uint32_t id = header.chkpt_id;  // the id of the checkpoint being logged
   result << approach_ << "," << id << "," << chunk_size_ << "," << data_len_ << "," << datasizes_.first + datasizes_.second
          << "," << datasizes_.first << "," << datasizes_.second << "," << t[0] << "," << t[1] << "," << t[2] << ","
          << t[3] << "\n";

@@ -73,7 +73,7 @@ def test_file_drivers_built_and_reject_other_methods(tmp_path):
     import subprocess
 
     bindir = os.path.join(os.path.dirname(dd.LIB_PATH), "bin")
-    for name in ("dedup_chkpt_files", "restart_chkpt_files"):
+    for name in ("dedup_chkpt_files", "dedup_chkpt_files_ranks", "restart_chkpt_files", "data_generation"):
         assert os.access(os.path.join(bindir, name), os.X_OK)
     for mode, msg in (("--run-full-chkpt", "unsupported"), ("--run-basic-chkpt", "unsupported"),
                       ("--run-nothing", "incorrect")):
