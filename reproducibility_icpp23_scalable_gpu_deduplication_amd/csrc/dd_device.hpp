@@ -41,6 +41,22 @@ struct alignas(32) Slot {
 
 constexpr unsigned long long SLOT_EMPTY = ~0ull;
 
+// Streaming reads through the global address space.  Pointers read from a
+// job table are generic, and generic (flat) loads also count against the
+// scalar-memory counter: a wait for a scalar load of a job field would then
+// wait for every chunk load in flight.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4_t global_u32x4;
+// 16 bytes at p (16-byte aligned) through a global load.
+__device__ __forceinline__ uint4 gload16(const void* p) {
+  const u32x4_t v = *(const global_u32x4*)(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ Digest load_digest(const Digest* p) {
+  const uint4 v = gload16(p);
+  return Digest{v.x | ((uint64_t)v.y << 32), v.z | ((uint64_t)v.w << 32)};
+}
+
 __host__ __device__ inline unsigned long long pack_node(uint32_t tree, uint32_t node) {
   return ((unsigned long long)tree << 32) | node;
 }
@@ -215,18 +231,19 @@ __device__ __forceinline__ Digest murmur3_bytes(const uint8_t* p, uint64_t len) 
 // Murmur3 of 16-byte aligned data.
 __device__ __forceinline__ Digest murmur3_aligned(const uint8_t* p, uint64_t len) {
   Murmur m;
-  const uint4* q = reinterpret_cast<const uint4*>(p);
+  const uint8_t* q = p;  // 16-byte vectors through global loads
   uint64_t nb = len / 16;
   uint64_t i = 0;
   for (; i + 4 <= nb; i += 4) {
-    uint4 a = q[i], b = q[i + 1], c = q[i + 2], d = q[i + 3];
+    uint4 a = gload16(q + 16 * i), b = gload16(q + 16 * (i + 1)), c = gload16(q + 16 * (i + 2)),
+          d = gload16(q + 16 * (i + 3));
     m.block(a.x | ((uint64_t)a.y << 32), a.z | ((uint64_t)a.w << 32));
     m.block(b.x | ((uint64_t)b.y << 32), b.z | ((uint64_t)b.w << 32));
     m.block(c.x | ((uint64_t)c.y << 32), c.z | ((uint64_t)c.w << 32));
     m.block(d.x | ((uint64_t)d.y << 32), d.z | ((uint64_t)d.w << 32));
   }
   for (; i < nb; i++) {
-    uint4 a = q[i];
+    uint4 a = gload16(q + 16 * i);
     m.block(a.x | ((uint64_t)a.y << 32), a.z | ((uint64_t)a.w << 32));
   }
   uint32_t rem = (uint32_t)(len & 15);
@@ -244,17 +261,17 @@ __device__ __forceinline__ Digest murmur3_aligned(const uint8_t* p, uint64_t len
 // one lane then keeps 256 bytes in flight instead of 64.
 __device__ __forceinline__ Digest murmur3_aligned_deep(const uint8_t* p, uint64_t len) {
   Murmur m;
-  const uint4* q = reinterpret_cast<const uint4*>(p);
+  const uint8_t* q = p;  // 16-byte vectors through global loads
   const uint64_t nb = len / 16;
   uint64_t i = 0;
   if (nb >= 8) {
     uint4 cur[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) cur[k] = q[k];
+    for (int k = 0; k < 8; k++) cur[k] = gload16(q + 16 * k);
     for (; i + 16 <= nb; i += 8) {
       uint4 nxt[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) nxt[k] = q[i + 8 + k];
+      for (int k = 0; k < 8; k++) nxt[k] = gload16(q + 16 * (i + 8 + k));
 #pragma unroll
       for (int k = 0; k < 8; k++) m.block(cur[k].x | ((uint64_t)cur[k].y << 32), cur[k].z | ((uint64_t)cur[k].w << 32));
 #pragma unroll
@@ -265,7 +282,7 @@ __device__ __forceinline__ Digest murmur3_aligned_deep(const uint8_t* p, uint64_
     i += 8;
   }
   for (; i < nb; i++) {
-    const uint4 a = q[i];
+    const uint4 a = gload16(q + 16 * i);
     m.block(a.x | ((uint64_t)a.y << 32), a.z | ((uint64_t)a.w << 32));
   }
   const uint32_t rem = (uint32_t)(len & 15);
@@ -278,7 +295,7 @@ __device__ __forceinline__ Digest murmur3_aligned_deep(const uint8_t* p, uint64_
   return m.finish(len);
 }
 
-// MD5 of an arbitrary byte range; `aligned` promises 4-byte alignment.
+// MD5 of an arbitrary byte range; `Aligned` promises 16-byte alignment.
 template <bool Aligned>
 __device__ __forceinline__ Digest md5_range(const uint8_t* p, uint64_t len) {
   uint32_t st[4];
@@ -288,10 +305,9 @@ __device__ __forceinline__ Digest md5_range(const uint8_t* p, uint64_t len) {
   for (uint64_t b = 0; b < nfull; b++) {
     const uint8_t* q = p + 64 * b;
     if (Aligned) {
-      const uint4* v = reinterpret_cast<const uint4*>(q);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        uint4 w = v[i];
+        uint4 w = gload16(q + 16 * i);
         X[4 * i] = w.x; X[4 * i + 1] = w.y; X[4 * i + 2] = w.z; X[4 * i + 3] = w.w;
       }
     } else {

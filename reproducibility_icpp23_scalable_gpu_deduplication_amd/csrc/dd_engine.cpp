@@ -150,8 +150,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   // insert_interior workgroup); a baseline launches one of each per leaf block.
   const uint64_t nclaim = std::max<uint64_t>(nlb, kMaxStageWG);
   DD_TRY(leafblk.ensure(2 * nclaim * 4));
-  // Pending leaves: sub-list r takes leaf blocks b = r (mod 32), 256 each.
-  const uint64_t pend_cap = ceil_div(nlb, 32) * 256;
+  // Pending leaves: sub-list r takes wavefronts w = r (mod 32), 64 leaves each.
+  const uint64_t pend_cap = ceil_div(ceil_div(C, 64), 32) * 64;
   DD_TRY(pend_list.ensure(32 * pend_cap * 4));
   const TreeGeom gm = make_geom(C);
   uint64_t ntiles = 0;

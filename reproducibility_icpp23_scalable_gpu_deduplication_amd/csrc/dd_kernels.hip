@@ -75,24 +75,6 @@ __device__ __forceinline__ T block_excl_scan(T v, T* sh, T& total) {
 
 __device__ __forceinline__ void store_u32(uint8_t* p, uint32_t v) { *reinterpret_cast<uint32_t*>(p) = v; }
 
-// Rank of this lane among the lanes of the block whose `pred` holds (-1 if
-// it does not hold); `total` gets their number.  Every lane must call it.
-__device__ __forceinline__ int block_compact(bool pred, uint32_t* sh, uint32_t& total) {
-  const unsigned long long b = __ballot(pred);
-  const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if (lane_id() == 0) sh[w] = (uint32_t)__popcll(b);
-  __syncthreads();
-  uint32_t base = 0, tot = 0;
-  for (uint32_t i = 0; i < nw; i++) {
-    const uint32_t c = sh[i];
-    base += i < w ? c : 0;
-    tot += c;
-  }
-  __syncthreads();
-  total = tot;
-  return pred ? (int)(base + __popcll(b & lanemask_lt())) : -1;
-}
-
 // Sum over the block (every lane must call it).
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T* sh) {
@@ -175,25 +157,26 @@ __device__ __forceinline__ void list_append(uint32_t* cnt, uint32_t* list, uint3
   list[(uint64_t)r * cap + k] = item;
 }
 
-// The leaves of leaf block b (256 chunks) that need a map insert are appended
-// to sub-list b % 32 of pend_list: one atomic per block reserves the space.
-__device__ __forceinline__ void record_pending(const JobDev& J, uint32_t b, uint64_t k, bool pending) {
-  __shared__ uint32_t sh[4];
-  __shared__ uint32_t sbase;
-  uint32_t tot;
-  const int pos = block_compact(pending, sh, tot);
-  if (tot) {  // uniform over the block
-    const uint32_t r = b % kPendSpread;
-    if (threadIdx.x == 0) sbase = atomicAdd(&J.pend_cnt[r], tot);
-    __syncthreads();
-    if (pos >= 0) J.pend_list[(uint64_t)r * J.pend_cap + sbase + pos] = (uint32_t)k;
-  }
+// The leaves of a wavefront (64 consecutive chunks k, k >> 6 = w) that need a
+// map insert are appended to sub-list w % 32 of pend_list: one atomic per
+// wavefront with a changed leaf, no block barrier, so a block's waves retire
+// independently.  A baseline visits every leaf and tile densely and keeps no
+// lists.  Every lane of the wavefront calls this.
+__device__ __forceinline__ void record_pending(const JobDev& J, uint64_t k, bool pending) {
+  if (J.is_baseline) return;  // uniform: a workgroup belongs to one job
+  const unsigned long long b = __ballot(pending);
+  if (!b) return;
+  const uint32_t r = (uint32_t)(k >> 6) % kPendSpread;
+  const uint32_t leader = __ffsll((long long)b) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(&J.pend_cnt[r], (uint32_t)__popcll(b));
+  base = __shfl(base, leader, 64);
+  if (pending) J.pend_list[(uint64_t)r * J.pend_cap + base + __popcll(b & lanemask_lt())] = (uint32_t)k;
   // Mark the group-0 tiles of changed leaves: the first marker of a tile in
-  // this checkpoint appends it to the dirty list.  A baseline visits every
-  // tile and every leaf block densely, so it keeps no lists; the list method
-  // has no tiles.
+  // this checkpoint appends it to the dirty list.  The list method has no
+  // tiles.
   uint32_t T = pending ? leaf_tile(J.D, J.C - 1 + k) : 0;
-  bool todo = pending && !J.is_baseline && !J.list_mode;
+  bool todo = pending && !J.list_mode;
   while (true) {
     const unsigned long long act = __ballot(todo);
     if (!act) break;
@@ -246,15 +229,17 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jo
     const uint64_t off = k * cs;
     const uint64_t nb = (k == J.C - 1) ? J.len - off : cs;
     const uint8_t* p = J.data + off;
+    const uint64_t leaf = J.C - 1 + k;
+    // The previous digest is loaded before the chunk: its latency hides
+    // behind the chunk loads instead of following the hash.
+    Digest prev{0, 0};
+    if (!J.is_baseline) prev = load_digest(J.tree + leaf);
     Digest d;
     if (Kind == HASH_MD5) {
       d = Aligned16 ? md5_range<true>(p, nb) : md5_range<false>(p, nb);
     } else {
       d = !Aligned16 ? murmur3_bytes(p, nb) : Deep ? murmur3_aligned_deep(p, nb) : murmur3_aligned(p, nb);
     }
-    const uint64_t leaf = J.C - 1 + k;
-    Digest prev{0, 0};
-    if (!J.is_baseline) prev = J.tree[leaf];
     uint8_t lab;
     if (digest_eq(d, prev)) {
       // A fixed leaf of an incremental checkpoint: its digest is already in
@@ -273,15 +258,13 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jo
     if (pending) J.tree[leaf] = d;
     J.labels[leaf] = lab;
   }
-  record_pending(J, blockIdx.x - prefix[j], k, pending);
+  record_pending(J, k, pending);
 }
 
 // Shared tail of both leaf kernels: compare with the previous digest, record
 // the label, write the new digest when it changed.
-__device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const Digest& d) {
+__device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const Digest& d, const Digest& prev) {
   const uint64_t leaf = J.C - 1 + k;
-  Digest prev{0, 0};
-  if (!J.is_baseline) prev = J.tree[leaf];
   bool pending;
   uint8_t lab;
   if (digest_eq(d, prev)) {
@@ -319,6 +302,8 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict_
   uint8_t* L = lds[wave];
   const uint8_t* base = J.data + k0 * cs;
   const uint32_t rounds = (uint32_t)(cs / SEG);
+  Digest prev{0, 0};  // loaded first: its latency hides behind the chunk loads
+  if (k < J.C && !J.is_baseline) prev = load_digest(J.tree + (J.C - 1 + k));
   Murmur m;
   uint32_t st[4];
   md5_init(st);
@@ -329,7 +314,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict_
       const uint32_t v = i * 64 + lane;
       const uint32_t c = v / VPC, piece = v % VPC;
       if (k0 + c < full) {
-        const uint4 x = *reinterpret_cast<const uint4*>(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16);
+        const uint4 x = gload16(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16);
         *reinterpret_cast<uint4*>(L + c * PITCH + piece * 16) = x;
       }
     }
@@ -370,15 +355,15 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict_
     } else {
       d = m.finish(cs);
     }
-    pending = leaf_finish(J, k, d);
+    pending = leaf_finish(J, k, d, prev);
   } else if (k < J.C) {  // the partial last chunk
     const uint64_t off = k * cs;
     const uint8_t* p = J.data + off;
     const uint64_t nb = J.len - off;
     const Digest d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
-    pending = leaf_finish(J, k, d);
+    pending = leaf_finish(J, k, d, prev);
   }
-  record_pending(J, blockIdx.x - prefix[j], k, pending);
+  record_pending(J, k, pending);
 }
 
 // ---------------------------------------------------------------------------
@@ -1200,7 +1185,7 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* __restrict__ j
   if (skip) {
     if (threadIdx.x == 0) {
       J.ctr->map_count = J.map_count_at_start + claims;
-      J.ctr->pending = pending_total(J);
+      J.ctr->pending = J.is_baseline ? J.C : pending_total(J);
       export_counters(J);
     }
     return;
@@ -1247,7 +1232,7 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* __restrict__ j
     const uint64_t meta = 4ull * J.ctr->nfirst + 8ull * nprior + 8ull * J.ctr->nshift;
     J.ctr->diff_len = 32 + meta + J.ctr->dchunks * (uint64_t)J.chunk_size;
     J.ctr->map_count = J.map_count_at_start + claims;
-    J.ctr->pending = pending_total(J);
+    J.ctr->pending = J.is_baseline ? J.C : pending_total(J);
     export_counters(J);
   }
 }

@@ -93,7 +93,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   uint64_t tl = std::min<uint64_t>(R->len, 256ull << 20);
   if (tl >= cs) tl = tl / cs * cs;
   const uint64_t C = (tl + cs - 1) / cs, N = 2 * C - 1, nlb = (C + 255) / 256;
-  const uint64_t pend_cap = (nlb + 31) / 32 * 256;
+  const uint64_t pend_cap = ((C + 63) / 64 + 31) / 32 * 64;  // unused: a baseline keeps no pending list
   DevBuf tree, labels, ctr, plist, table;
   int best = LEAF_DIRECT;
   if (tree.ensure(16 * N) != DD_OK || labels.ensure(N + 64) != DD_OK ||
