@@ -25,7 +25,7 @@ EXPORTED = [
     "dd_server_submit_restart", "dd_server_wait", "dd_server_usage", "dd_now", "dd_set_profiling",
     "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_buf_free", "dd_strerror",
 ]
-LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256"]
+LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma"]
 STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
           "meta_gather"]
 

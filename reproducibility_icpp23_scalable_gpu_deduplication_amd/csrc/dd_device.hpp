@@ -295,6 +295,36 @@ __device__ __forceinline__ Digest murmur3_aligned_deep(const uint8_t* p, uint64_
   return m.finish(len);
 }
 
+// Murmur3 of a whole chunk of 16·n bytes (n <= 8, 16-byte aligned): every
+// load is issued before any hashing, as in the deep variant, but without its
+// second buffer, so the kernel needs fewer registers and keeps more waves per
+// SIMD (chunks of at most 128 bytes).
+// Straight-line code per length: a load behind a per-vector condition would
+// be issued after the previous one had arrived.
+template <int N>
+__device__ __forceinline__ Digest murmur3_fixed(const uint8_t* p) {
+  uint4 v[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) v[k] = gload16(p + 16 * k);
+  Murmur m;
+#pragma unroll
+  for (int k = 0; k < N; k++) m.block(v[k].x | ((uint64_t)v[k].y << 32), v[k].z | ((uint64_t)v[k].w << 32));
+  return m.finish(16ull * N);
+}
+
+__device__ __forceinline__ Digest murmur3_short(const uint8_t* p, uint32_t n) {
+  switch (n) {
+    case 8: return murmur3_fixed<8>(p);
+    case 7: return murmur3_fixed<7>(p);
+    case 6: return murmur3_fixed<6>(p);
+    case 5: return murmur3_fixed<5>(p);
+    case 4: return murmur3_fixed<4>(p);
+    case 3: return murmur3_fixed<3>(p);
+    case 2: return murmur3_fixed<2>(p);
+    default: return murmur3_fixed<1>(p);
+  }
+}
+
 // MD5 of an arbitrary byte range; `Aligned` promises 16-byte alignment.
 template <bool Aligned>
 __device__ __forceinline__ Digest md5_range(const uint8_t* p, uint64_t len) {

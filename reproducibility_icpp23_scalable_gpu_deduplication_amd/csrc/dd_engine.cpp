@@ -191,6 +191,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.hash_kind = hash_kind;
   job.C = C;
   job.N = N;
+  job.nfull = len / chunk_size;
   job.D = make_geom(C).D;
   job.ckpt_id = K;
   job.baseline_id = baseline ? K : baseline_id;
@@ -301,7 +302,7 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
       hprefix[off_[st] + j] = (uint32_t)acc;
       uint64_t n = 0;
       switch (st) {
-        case S_LEAF: n = ceil_div(J.C, 256); break;
+        case S_LEAF: n = leaf_workgroups(leaf_map_, J.C); break;
         // A baseline visits every leaf block and tile (one workgroup each);
         // an incremental checkpoint strides over its work lists.
         // The insert stages take one list entry per lane: a baseline lists

@@ -216,9 +216,11 @@ __device__ __forceinline__ uint64_t pending_total(const JobDev& J) {
   return t;
 }
 
-// Deep: Murmur3 keeps two 128-byte groups of loads in flight per lane (long
-// chunks); MD5 ignores it.
-template <int Kind, bool Aligned16, bool Deep>
+// Pipe (Murmur3 with 16-byte aligned data; MD5 ignores it): LEAF_DIRECT
+// hashes 64 bytes per group of loads, LEAF_DEEP keeps two 128-byte groups in
+// flight per lane (long chunks), LEAF_SHORT loads a whole chunk of at most
+// 128 bytes before hashing it.
+template <int Kind, bool Aligned16, int Pipe>
 __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobDev& J = jobs[j];
@@ -238,7 +240,10 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jo
     if (Kind == HASH_MD5) {
       d = Aligned16 ? md5_range<true>(p, nb) : md5_range<false>(p, nb);
     } else {
-      d = !Aligned16 ? murmur3_bytes(p, nb) : Deep ? murmur3_aligned_deep(p, nb) : murmur3_aligned(p, nb);
+      if (!Aligned16) d = murmur3_bytes(p, nb);
+      else if (Pipe == LEAF_DEEP) d = murmur3_aligned_deep(p, nb);
+      else if (Pipe == LEAF_SHORT && nb == cs) d = murmur3_short(p, (uint32_t)(cs / 16));
+      else d = murmur3_aligned(p, nb);  // direct, or the partial last chunk
     }
     uint8_t lab;
     if (digest_eq(d, prev)) {
@@ -298,7 +303,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict_
   const uint64_t k0 = ((uint64_t)(blockIdx.x - prefix[j]) * 4 + wave) * 64;  // first chunk of this wave
   const uint64_t k = k0 + lane;
   const uint64_t cs = J.chunk_size;
-  const uint64_t full = J.len / cs;  // chunks that are complete
+  const uint64_t full = J.nfull;  // chunks that are complete
   uint8_t* L = lds[wave];
   const uint8_t* base = J.data + k0 * cs;
   const uint32_t rounds = (uint32_t)(cs / SEG);
@@ -341,6 +346,118 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict_
       }
     }
     __syncthreads();
+  }
+  bool pending = false;
+  if (k < full) {
+    Digest d;
+    if (Kind == HASH_MD5) {
+      uint32_t X[16] = {0x80u, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      const uint64_t bits = cs * 8;
+      X[14] = (uint32_t)bits;
+      X[15] = (uint32_t)(bits >> 32);
+      md5_compress(st, X);
+      d = md5_digest(st);
+    } else {
+      d = m.finish(cs);
+    }
+    pending = leaf_finish(J, k, d, prev);
+  } else if (k < J.C) {  // the partial last chunk
+    const uint64_t off = k * cs;
+    const uint8_t* p = J.data + off;
+    const uint64_t nb = J.len - off;
+    const Digest d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
+    pending = leaf_finish(J, k, d, prev);
+  }
+  record_pending(J, k, pending);
+}
+
+// leaf_hash, direct-to-LDS: each wavefront owns 64 consecutive chunks and an
+// 8 KiB LDS region.  Per 128-byte segment of its chunks it issues eight
+// global_load_lds_dwordx4 (each one contiguous 1 KiB, so every cache line is
+// requested once; the lane-per-chunk loads above request each line once per
+// 16-byte piece) that write LDS without passing through registers, waits for
+// them, and each lane reads its own segment back and hashes it.  The source
+// address of each lane is swizzled so that the eight 16-byte reads of a lane
+// go to distinct LDS banks within every group of 16 lanes.  The loads of the
+// next segment are issued before the current one is hashed.  No block
+// barrier: the region is private to the wavefront.  Needs chunk % 128 == 0
+// and a 16-byte aligned buffer; the partial last chunk is hashed from global
+// memory.
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void global_void_t;
+constexpr int kWaitVm0 = 0x0F70;    // s_waitcnt vmcnt(0)
+constexpr int kWaitLgkm0 = 0xC07F;  // s_waitcnt lgkmcnt(0)
+
+// LDS slot (16-byte unit) of piece p of chunk j (0..7) in 1 KiB block i of a
+// wavefront's region: each block's slots are a permutation of 0..63, and for
+// every p the chunks of an even and an odd block fill 16 distinct slots mod 16.
+__device__ __forceinline__ uint32_t dma_slot(uint32_t i, uint32_t j, uint32_t p) {
+  return ((i & 1) ? 8 * (p ^ 1) : 8 * p) + ((j + p) & 7);
+}
+
+template <int Kind>
+__global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  __shared__ __attribute__((aligned(16))) uint8_t lds[4][8192];
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobDev& J = jobs[j];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t k0 = ((uint64_t)(blockIdx.x - prefix[j]) * 4 + wave) * 64;  // first chunk of this wave
+  const uint64_t k = k0 + lane;
+  const uint64_t cs = J.chunk_size;
+  const uint64_t full = J.nfull;  // chunks that are complete
+  Digest prev{0, 0};  // loaded first: its latency hides behind the chunk loads
+  if (k < J.C && !J.is_baseline) prev = load_digest(J.tree + (J.C - 1 + k));
+  uint8_t* L = lds[wave];
+  // This lane's load of block i fills slot `lane` of that block: piece p of
+  // chunk jj with dma_slot(i, jj, p) == lane.  Chunks past the complete ones
+  // load from the wave's first chunk instead (their slots are never read).
+  const uint32_t pe = lane >> 3, po = pe ^ 1;
+  const uint32_t je = ((lane & 7) - pe) & 7, jo = ((lane & 7) - po) & 7;
+  const uint8_t* src[8];
+#pragma unroll
+  for (uint32_t i = 0; i < 8; i++) {
+    const uint32_t p = (i & 1) ? po : pe, jj = (i & 1) ? jo : je;
+    const uint64_t c = k0 + i * 8 + jj;
+    src[i] = J.data + (c < full ? c : k0) * cs + p * 16;
+  }
+  const bool have = k0 < full;  // uniform: some chunk of this wave is complete
+  const uint32_t rounds = (uint32_t)(cs >> 7);  // chunk_size is a multiple of 128
+  const uint32_t bi = lane >> 3, bj = lane & 7;  // this lane's chunk: block bi, position bj
+  Murmur m;
+  uint32_t st[4];
+  md5_init(st);
+  if (have) {
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++)
+      __builtin_amdgcn_global_load_lds((global_void_t*)src[i], (lds_void_t*)(L + i * 1024), 16, 0, 0);
+  }
+  for (uint32_t r = 0; have && r < rounds; r++) {
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    uint4 v[8];
+#pragma unroll
+    for (uint32_t p = 0; p < 8; p++) v[p] = *reinterpret_cast<const uint4*>(L + bi * 1024 + dma_slot(bi, bj, p) * 16);
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // the reads are done before the region is refilled
+    if (r + 1 < rounds) {
+#pragma unroll
+      for (uint32_t i = 0; i < 8; i++)
+        __builtin_amdgcn_global_load_lds((global_void_t*)(src[i] + (uint64_t)(r + 1) * 128), (lds_void_t*)(L + i * 1024),
+                                         16, 0, 0);
+    }
+    if (k < full) {
+      if (Kind == HASH_MD5) {
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          const uint32_t X[16] = {v[4 * b].x,     v[4 * b].y,     v[4 * b].z,     v[4 * b].w,
+                                  v[4 * b + 1].x, v[4 * b + 1].y, v[4 * b + 1].z, v[4 * b + 1].w,
+                                  v[4 * b + 2].x, v[4 * b + 2].y, v[4 * b + 2].z, v[4 * b + 2].w,
+                                  v[4 * b + 3].x, v[4 * b + 3].y, v[4 * b + 3].z, v[4 * b + 3].w};
+          md5_compress(st, X);
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < 8; p++) m.block(v[p].x | ((uint64_t)v[p].y << 32), v[p].z | ((uint64_t)v[p].w << 32));
+      }
+    }
   }
   bool pending = false;
   if (k < full) {
@@ -1602,20 +1719,23 @@ void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uin
       case LEAF_LDS64: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>), grid, 256, 0, s, jobs, prefix, njobs); return;
       case LEAF_LDS128: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>), grid, 256, 0, s, jobs, prefix, njobs); return;
       case LEAF_LDS256: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>), grid, 256, 0, s, jobs, prefix, njobs); return;
+      case LEAF_DMA: DD_LAUNCH((k_leaf_hash_dma<HASH_MD5>), grid, 256, 0, s, jobs, prefix, njobs); return;
       default:
-        if (al16) DD_LAUNCH((k_leaf_hash<HASH_MD5, true, false>), grid, 256, 0, s, jobs, prefix, njobs);
-        else DD_LAUNCH((k_leaf_hash<HASH_MD5, false, false>), grid, 256, 0, s, jobs, prefix, njobs);
+        if (al16) DD_LAUNCH((k_leaf_hash<HASH_MD5, true, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
+        else DD_LAUNCH((k_leaf_hash<HASH_MD5, false, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
         return;
     }
   }
   switch (mapping) {
-    case LEAF_DEEP: DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, true>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    case LEAF_DEEP: DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DEEP>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    case LEAF_SHORT: DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_SHORT>), grid, 256, 0, s, jobs, prefix, njobs); return;
     case LEAF_LDS64: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>), grid, 256, 0, s, jobs, prefix, njobs); return;
     case LEAF_LDS128: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>), grid, 256, 0, s, jobs, prefix, njobs); return;
     case LEAF_LDS256: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    case LEAF_DMA: DD_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3>), grid, 256, 0, s, jobs, prefix, njobs); return;
     default:
-      if (al16) DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, false>), grid, 256, 0, s, jobs, prefix, njobs);
-      else DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, false>), grid, 256, 0, s, jobs, prefix, njobs);
+      if (al16) DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
+      else DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
       return;
   }
 }

@@ -41,6 +41,7 @@ struct JobDev {
   uint32_t chunk_size;
   int32_t hash_kind;
   uint64_t C, N;
+  uint64_t nfull;  // complete chunks: len / chunk_size (C or C-1)
   uint32_t D;
   uint32_t ckpt_id;
   uint32_t baseline_id;
@@ -127,11 +128,27 @@ void launch_finish(const JobDev* jobs, uint32_t njobs, hipStream_t s);
 // 16-byte loads, 64 bytes in flight per lane; DEEP: the same with 256 bytes
 // in flight (Murmur3); LDS<seg>: each wavefront loads `seg` bytes of 64
 // chunks with contiguous 16-byte loads into LDS per round, then each lane
-// hashes its chunk from LDS (needs chunk_size % seg == 0).  All but DIRECT
-// need 16-byte aligned buffers; one workgroup covers 256 chunks in every case.
-enum LeafMapping : int { LEAF_DIRECT = 0, LEAF_DEEP = 1, LEAF_LDS64 = 2, LEAF_LDS128 = 3, LEAF_LDS256 = 4, LEAF_NMAP = 5 };
+// hashes its chunk from LDS (needs chunk_size % seg == 0); SHORT: one lane
+// per chunk of at most 128 bytes, the whole chunk in flight (Murmur3); DMA:
+// each wavefront loads 128-byte segments of its 64 chunks straight into LDS
+// with contiguous loads (global_load_lds) and each lane hashes its chunk
+// from LDS (needs chunk_size % 128 == 0).  All but DIRECT need 16-byte
+// aligned buffers; one workgroup covers 256 chunks in every case.
+enum LeafMapping : int {
+  LEAF_DIRECT = 0,
+  LEAF_DEEP = 1,
+  LEAF_LDS64 = 2,
+  LEAF_LDS128 = 3,
+  LEAF_LDS256 = 4,
+  LEAF_SHORT = 5,
+  LEAF_DMA = 6,
+  LEAF_NMAP = 7
+};
 void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping,
                  bool all_aligned16, int hash_kind, hipStream_t s);
+// Workgroups of the leaf stage for a job of C chunks: one per 256 chunks in
+// every mapping.
+inline uint64_t leaf_workgroups(int, uint64_t C) { return (C + 255) / 256; }
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 // List method: the leaves left PENDING by insert_leaves are first
 // occurrences; a baseline also sets every interior label to DONE.

@@ -10,7 +10,7 @@
 // on the largest job's first 256 MiB and keeps the fastest for the process.
 // The timed launches run as baselines on scratch tree, label and list arrays,
 // so the handles' state is not touched.  DD_LEAF_KERNEL=direct|deep|lds64|
-// lds128|lds256 forces a mapping (when it is valid for the table).
+// lds128|lds256|short|dma forces a mapping (when it is valid for the table).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,7 +35,13 @@ bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) 
   if (m == LEAF_DIRECT) return true;
   if (!al16) return false;
   if (m == LEAF_DEEP) return kind == HASH_MURMUR3;
-  const uint32_t seg = m == LEAF_LDS64 ? 64 : m == LEAF_LDS128 ? 128 : 256;
+  if (m == LEAF_SHORT) {
+    if (kind != HASH_MURMUR3) return false;
+    for (const JobDev& J : jobs)
+      if (J.chunk_size > 128) return false;
+    return true;
+  }
+  const uint32_t seg = m == LEAF_LDS64 ? 64 : (m == LEAF_LDS128 || m == LEAF_DMA) ? 128 : 256;
   for (const JobDev& J : jobs)
     if (J.chunk_size % seg) return false;
   return true;
@@ -44,7 +50,7 @@ bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) 
 int forced_mapping() {
   const char* e = std::getenv("DD_LEAF_KERNEL");
   if (!e) return -1;
-  static const char* names[LEAF_NMAP] = {"direct", "deep", "lds64", "lds128", "lds256"};
+  static const char* names[LEAF_NMAP] = {"direct", "deep", "lds64", "lds128", "lds256", "short", "dma"};
   for (int m = 0; m < LEAF_NMAP; m++)
     if (!strcmp(e, names[m])) return m;
   return -1;
@@ -92,13 +98,13 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   const uint64_t cs = R->chunk_size;
   uint64_t tl = std::min<uint64_t>(R->len, 256ull << 20);
   if (tl >= cs) tl = tl / cs * cs;
-  const uint64_t C = (tl + cs - 1) / cs, N = 2 * C - 1, nlb = (C + 255) / 256;
+  const uint64_t C = (tl + cs - 1) / cs, N = 2 * C - 1;
   const uint64_t pend_cap = ((C + 63) / 64 + 31) / 32 * 64;  // unused: a baseline keeps no pending list
   DevBuf tree, labels, ctr, plist, table;
   int best = LEAF_DIRECT;
   if (tree.ensure(16 * N) != DD_OK || labels.ensure(N + 64) != DD_OK ||
       ctr.ensure(sizeof(JobCounters) + 3 * 32 * 4) != DD_OK || plist.ensure(32 * pend_cap * 4) != DD_OK ||
-      table.ensure(sizeof(JobDev) + 2 * 4) != DD_OK)
+      table.ensure(sizeof(JobDev) + LEAF_NMAP * 2 * 4) != DD_OK)
     return best;  // no memory to tune: keep the default, do not cache
   JobDev t;
   memset(&t, 0, sizeof t);
@@ -108,6 +114,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   t.hash_kind = kind;
   t.C = C;
   t.N = N;
+  t.nfull = tl / cs;
   t.D = make_geom(C).D;
   t.is_baseline = 1;
   t.epoch = 1;
@@ -119,11 +126,16 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   t.dirty_cnt = t.fint_cnt + 32;
   t.pend_list = plist.as<uint32_t>();
   t.pend_cap = (uint32_t)pend_cap;
-  const uint32_t grid = (uint32_t)nlb;
-  uint8_t host[sizeof(JobDev) + 8];
+  // One workgroup prefix {0, grid} per mapping: the grid-stride mappings
+  // read their stride from it.
+  uint8_t host[sizeof(JobDev) + LEAF_NMAP * 8];
   memcpy(host, &t, sizeof t);
-  const uint32_t pre[2] = {0, grid};
-  memcpy(host + sizeof(JobDev), pre, sizeof pre);
+  uint32_t grids[LEAF_NMAP];
+  for (int m = 0; m < LEAF_NMAP; m++) {
+    grids[m] = (uint32_t)leaf_workgroups(m, C);
+    const uint32_t pre[2] = {0, grids[m]};
+    memcpy(host + sizeof(JobDev) + 8 * m, pre, sizeof pre);
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipMemcpyAsync(table.p, host, sizeof host, hipMemcpyHostToDevice, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
@@ -136,7 +148,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   double best_ms = -1;
   for (int m = 0; m < LEAF_NMAP; m++) {
     if (!mapping_valid(m, al16, kind, jobs)) continue;
-    const double ms = time_mapping(dj, dprefix, grid, m, al16, kind, s, e0, e1, 3);
+    const double ms = time_mapping(dj, dprefix + 2 * m, grids[m], m, al16, kind, s, e0, e1, 3);
     if (ms >= 0 && (best_ms < 0 || ms < best_ms)) {
       best_ms = ms;
       best = m;
