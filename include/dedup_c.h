@@ -122,8 +122,8 @@ dd_status dd_set_current_id(dd_dedup* h, uint32_t current_id);
 
 /* BaseDeduplicator::timers[4] (setup, dedup, gather, copy-to-host, seconds) and
  * restart_timers[2] (copy, total), include/deduplicator_interface.hpp:22-23.
- * The GPU time is split into dedup and gather only while profiling is on
- * (dd_set_profiling >= 1: the split costs an event between the stages);
+ * The GPU time is split into dedup and gather only at profiling level 2
+ * (dd_set_profiling(h, 2): the split costs an event between the stages);
  * otherwise timers[1] holds the whole GPU time and timers[2] is 0. */
 dd_status dd_get_timers(dd_dedup* h, double timers[4], double restart_timers[2]);
 
@@ -196,10 +196,11 @@ dd_status dd_server_usage(dd_server* s, uint64_t* in_use, uint64_t* peak);
 double dd_now(void); /* the steady clock of dd_request_result */
 
 /* Diagnostics (no counterpart in the reference; its equivalent is the
- * Kokkos::Profiling regions of src/tree_approach.cpp).  Level 1 records HIP
- * events on the handle's stream around the leaf-hash kernel only; level 2
- * around every stage (each event adds a few microseconds).  Level 0 turns
- * them off.  dd_get_stage_times returns the last checkpoint's GPU
+ * Kokkos::Profiling regions of src/tree_approach.cpp).  Level 1 times the
+ * leaf-hash kernel only, with HIP events attached to its own dispatch (no
+ * extra work on the stream); level 2 records events between every two stages
+ * (events in the middle of the stream cost tens of microseconds of host time
+ * per checkpoint).  Level 0 turns them off.  dd_get_stage_times returns the last checkpoint's GPU
  * milliseconds for {leaf hash, leaf inserts, first-occurrence forest,
  * interior inserts, duplicate forest, roots + grouping, metadata + gather};
  * stages not recorded read 0. */

@@ -142,8 +142,12 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   DD_TRY(s_tree.ensure(C * 4));
   DD_TRY(ss_node.ensure(C * 4));
   DD_TRY(ss_prev.ensure(C * 4));
-  DD_TRY(s_hist.ensure(((uint64_t)K + 1) * ceil_div(C, kSTile) * 4));
-  DD_TRY(ptab.ensure(((uint64_t)K + 1) * 8));
+  // Buffers that grow with the checkpoint id are sized for ids in steps of 64:
+  // an allocation per checkpoint (a free and a malloc of megabytes) would
+  // cost more host time than the whole incremental checkpoint's kernels.
+  const uint64_t Kcap = ((uint64_t)K | 63) + 1;  // ids 0 .. Kcap-1
+  DD_TRY(s_hist.ensure(Kcap * ceil_div(C, kSTile) * 4));
+  DD_TRY(ptab.ensure(Kcap * 8));
   DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * 8, true, s));
   const uint64_t nlb = ceil_div(C, 256);
   // blk_claim (one per insert_leaves workgroup) then int_claim (one per
@@ -177,7 +181,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   DD_TRY(lists.ensure((32 + 32 * dirty_cap) * 4));
   const uint64_t need = max_diff_size(len, chunk_size, K);
   if (!out) {
-    DD_TRY(diff.ensure(need));
+    DD_TRY(diff.ensure(max_diff_size(len, chunk_size, (uint32_t)(Kcap - 1))));
     out = diff.as<uint8_t>();
     out_cap = diff.bytes;
   } else if (out_cap < need) {
@@ -364,14 +368,18 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   const uint32_t nj = njobs_;
   const JobDev* J = dev_jobs_;
   auto P = [&](int st) { return dev_prefix_ + off_[st]; };
+  // Level 2 records an event between every two stages; level 1 times only
+  // the leaf kernel, with events attached to its dispatch.
   auto mark = [&](int i) -> dd_status {
-    if (stage_ev && (profile_level_ >= 2 || i <= 1)) DD_HIP(hipEventRecord(stage_ev[i], s));
+    if (stage_ev && profile_level_ >= 2) DD_HIP(hipEventRecord(stage_ev[i], s));
     return DD_OK;
   };
+  const bool leaf_events = stage_ev && profile_level_ == 1;
   if (rerun || begin) launch_begin(J, nullptr, nj, rerun, s);
   DD_TRY(mark(0));
   if (!rerun) {
-    launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s);
+    launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s,
+                leaf_events ? stage_ev[0] : nullptr, leaf_events ? stage_ev[1] : nullptr);
   }
   DD_TRY(mark(1));
   launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], s);
@@ -441,9 +449,11 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   st.ctr_armed = false;
   DD_HIP(hipEventRecord(h->ev[0], h->stream));
   h->pipe.profile_level_ = h->profile;
-  // The event between the two halves costs a few microseconds of GPU idle
-  // time: it is recorded only while profiling.
-  DD_TRY(h->pipe.run(jobs, h->stream, false, begin, h->profile ? h->ev[1] : nullptr, h->profile ? h->pev : nullptr));
+  // Events in the middle of the stream cost the host tens of microseconds per
+  // checkpoint: the dedup/gather split is recorded only at profiling level 2,
+  // and level 1 attaches its two events to the leaf kernel's dispatch.
+  DD_TRY(h->pipe.run(jobs, h->stream, false, begin, h->profile >= 2 ? h->ev[1] : nullptr,
+                     h->profile ? h->pev : nullptr));
   DD_HIP(hipEventRecord(h->ev[2], h->stream));
   DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
   JobCounters c = h->pipe.host_ctr[0];
@@ -461,7 +471,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   if (c.flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return DD_ENOMEM;
   if (c.flags & FLAG_BAD_SOURCE) return DD_ECORRUPT;
   float ms1 = 0, ms2 = 0;
-  if (h->profile) {
+  if (h->profile >= 2) {
     (void)hipEventElapsedTime(&ms1, h->ev[0], h->ev[1]);
     (void)hipEventElapsedTime(&ms2, h->ev[1], h->ev[2]);
   } else {
@@ -611,7 +621,9 @@ dd_status restart_walk(dd_dedup* h, std::vector<HostCkpt>& ckpts, const std::vec
     arena += align256(ckpts[k].f_off.size() * 4) + align256(ckpts[k].s_off.size() * 4) +
              align256(ckpts[k].s_tree.size() * 4 + 4);
   }
-  DD_TRY(h->rs_arena.ensure(arena));
+  // Grows by at least half its size: restarts of chains that grow by one
+  // diff at a time would otherwise reallocate on every call.
+  if (arena > h->rs_arena.bytes) DD_TRY(h->rs_arena.ensure(std::max<uint64_t>(arena, h->rs_arena.bytes * 3 / 2)));
   DD_TRY(h->rs_desc.ensure(ckpts.size() * sizeof(RestartCkpt)));
   DD_TRY(h->ref.ensure(g.C * 8));
   if (h->cover.bytes < g.C * sizeof(CoverEntry)) {
@@ -747,14 +759,18 @@ dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void*
     }
     h->own_stream = true;
   }
+  // Timing-only events: without the system-scope fence of a default event,
+  // a record in the middle of the pipeline does not write back and
+  // invalidate the caches (which cost ~65 us per checkpoint at 1 GiB).  They
+  // are read after the stream has been synchronized.
   for (auto& e : h->ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
       dd_destroy(h);
       return DD_EHIP;
     }
   }
   for (auto& e : h->pev) {
-    if (hipEventCreate(&e) != hipSuccess) {
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
       dd_destroy(h);
       return DD_EHIP;
     }

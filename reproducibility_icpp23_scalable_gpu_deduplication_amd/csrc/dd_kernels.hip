@@ -14,6 +14,7 @@
 //   sbin_*         shifted roots grouped by source checkpoint (stable)
 //   write_meta     header, first-occurrence list, prior table, shifted list
 //   gather         first-occurrence chunks into the data section
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "dd_kernels.hpp"
@@ -1710,35 +1711,45 @@ void launch_begin(const JobDev* jobs, const uint32_t*, uint32_t njobs, bool reru
   DD_LAUNCH(k_begin, (njobs + 63) / 64, 64, 0, s, jobs, njobs, rerun ? 1 : 0);
 }
 
+// With timing events the leaf kernel is launched with them attached to its
+// own dispatch (hipExtLaunchKernelGGL) instead of separate event records
+// around it, which cost the host tens of microseconds per checkpoint.
+#define DD_LEAF_LAUNCH(kern)                                                                           \
+  do {                                                                                                 \
+    if (ev_start) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0, jobs, \
+                                        prefix, njobs);                                                \
+    else DD_LAUNCH(kern, grid, 256, 0, s, jobs, prefix, njobs);                                        \
+    return;                                                                                            \
+  } while (0)
+
 void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping, bool al16,
-                 int kind, hipStream_t s) {
+                 int kind, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (grid == 0) return;
   if (!al16) mapping = LEAF_DIRECT;  // the other mappings need 16-byte aligned buffers
   if (kind == HASH_MD5) {
     switch (mapping) {
-      case LEAF_LDS64: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>), grid, 256, 0, s, jobs, prefix, njobs); return;
-      case LEAF_LDS128: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>), grid, 256, 0, s, jobs, prefix, njobs); return;
-      case LEAF_LDS256: DD_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>), grid, 256, 0, s, jobs, prefix, njobs); return;
-      case LEAF_DMA: DD_LAUNCH((k_leaf_hash_dma<HASH_MD5>), grid, 256, 0, s, jobs, prefix, njobs); return;
+      case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>));
+      case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>));
+      case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>));
+      case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5>));
       default:
-        if (al16) DD_LAUNCH((k_leaf_hash<HASH_MD5, true, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
-        else DD_LAUNCH((k_leaf_hash<HASH_MD5, false, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
-        return;
+        if (al16) DD_LEAF_LAUNCH((k_leaf_hash<HASH_MD5, true, LEAF_DIRECT>));
+        DD_LEAF_LAUNCH((k_leaf_hash<HASH_MD5, false, LEAF_DIRECT>));
     }
   }
   switch (mapping) {
-    case LEAF_DEEP: DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DEEP>), grid, 256, 0, s, jobs, prefix, njobs); return;
-    case LEAF_SHORT: DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_SHORT>), grid, 256, 0, s, jobs, prefix, njobs); return;
-    case LEAF_LDS64: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>), grid, 256, 0, s, jobs, prefix, njobs); return;
-    case LEAF_LDS128: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>), grid, 256, 0, s, jobs, prefix, njobs); return;
-    case LEAF_LDS256: DD_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>), grid, 256, 0, s, jobs, prefix, njobs); return;
-    case LEAF_DMA: DD_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3>), grid, 256, 0, s, jobs, prefix, njobs); return;
+    case LEAF_DEEP: DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DEEP>));
+    case LEAF_SHORT: DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_SHORT>));
+    case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>));
+    case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>));
+    case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>));
+    case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3>));
     default:
-      if (al16) DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
-      else DD_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, LEAF_DIRECT>), grid, 256, 0, s, jobs, prefix, njobs);
-      return;
+      if (al16) DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DIRECT>));
+      DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, LEAF_DIRECT>));
   }
 }
+#undef DD_LEAF_LAUNCH
 
 void launch_finish(const JobDev* jobs, uint32_t njobs, hipStream_t s) {
   DD_LAUNCH(k_finish, (njobs + 63) / 64, 64, 0, s, jobs, njobs);

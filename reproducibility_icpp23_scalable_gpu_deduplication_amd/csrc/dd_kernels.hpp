@@ -144,8 +144,11 @@ enum LeafMapping : int {
   LEAF_DMA = 6,
   LEAF_NMAP = 7
 };
+// ev_start/ev_stop (optional) are recorded at the start and the end of the
+// leaf kernel's own dispatch.
 void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping,
-                 bool all_aligned16, int hash_kind, hipStream_t s);
+                 bool all_aligned16, int hash_kind, hipStream_t s, hipEvent_t ev_start = nullptr,
+                 hipEvent_t ev_stop = nullptr);
 // Workgroups of the leaf stage for a job of C chunks: one per 256 chunks in
 // every mapping.
 inline uint64_t leaf_workgroups(int, uint64_t C) { return (C + 255) / 256; }

@@ -138,8 +138,8 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipMemcpyAsync(table.p, host, sizeof host, hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-      hipEventCreate(&e1) != hipSuccess) {
+      hipStreamSynchronize(s) != hipSuccess || hipEventCreateWithFlags(&e0, hipEventDisableSystemFence) != hipSuccess ||
+      hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
     return best;
   }
