@@ -192,21 +192,34 @@ __device__ __forceinline__ void record_pending(const JobDev& J, uint64_t k, bool
 
 // Item i of a 32-way split work list: sub-list counts are prefix-summed in
 // LDS by the caller (pre[r] = items before sub-list r, pre[32] = total).
+// Lane r of the first wavefront loads count r and the wavefront scans them
+// with shuffles: one load round trip per workgroup.  (A loop of one thread
+// over the counts waits for each load in turn, since the LDS stores between
+// them may alias the counts as far as the compiler can tell.)  Every caller
+// runs at least 64 threads.
 __device__ __forceinline__ void load_list_prefix(const uint32_t* cnt, uint32_t* pre) {
-  if (threadIdx.x == 0) {
-    uint32_t a = 0;
-    for (uint32_t r = 0; r < kPendSpread; r++) {
-      pre[r] = a;
-      a += cnt[r];
+  static_assert(kPendSpread <= 64, "one count per lane of a wavefront");
+  if (threadIdx.x < 64) {
+    const uint32_t l = threadIdx.x;
+    uint32_t v = l < kPendSpread ? cnt[l] : 0;
+#pragma unroll
+    for (uint32_t o = 1; o < kPendSpread; o <<= 1) {
+      const uint32_t u = __shfl_up(v, o, 64);
+      if (l >= o) v += u;
     }
-    pre[kPendSpread] = a;
+    if (l < kPendSpread) pre[l + 1] = v;
+    if (l == 0) pre[0] = 0;
   }
   __syncthreads();
 }
 
+// Sub-list of item i: the last r in [0, 32) with pre[r] <= i (pre is
+// non-decreasing and pre[0] = 0), by binary search.
 __device__ __forceinline__ uint32_t list_item(const uint32_t* list, uint32_t cap, const uint32_t* pre, uint32_t i) {
   uint32_t r = 0;
-  while (r + 1 < kPendSpread && pre[r + 1] <= i) r++;
+#pragma unroll
+  for (uint32_t step = kPendSpread / 2; step > 0; step >>= 1)
+    if (pre[r + step] <= i) r += step;
   return list[(uint64_t)r * cap + (i - pre[r])];
 }
 
