@@ -41,10 +41,40 @@ struct alignas(32) Slot {
 
 constexpr unsigned long long SLOT_EMPTY = ~0ull;
 
-// Streaming reads through the global address space.  Pointers read from a
-// job table are generic, and generic (flat) loads also count against the
-// scalar-memory counter: a wait for a scalar load of a job field would then
-// wait for every chunk load in flight.
+// Pointer kinds.  The host fills job tables and map views with plain
+// pointers; the kernels read the same bytes with every pointer typed as a
+// global-address-space pointer, so that loads, stores and atomics through
+// them compile to global_* instructions.  A pointer read from memory is
+// otherwise generic, and generic (flat) instructions also count against the
+// LDS/scalar-memory counter: every wait for an LDS access, a shuffle, a job
+// field or a barrier would then also wait for the loads and stores in flight.
+template <class T>
+using HostPtr = T*;
+template <class T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+
+// Atomics through global pointers, with the semantics of HIP's atomicAdd and
+// friends (relaxed order, device scope).
+#define DD_GLOBAL_ATOMIC(name, builtin, T)                                       \
+  __device__ __forceinline__ T name(GlobalPtr<T> p, T v) {                       \
+    return builtin(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);            \
+  }
+DD_GLOBAL_ATOMIC(g_add, __hip_atomic_fetch_add, uint32_t)
+DD_GLOBAL_ATOMIC(g_add, __hip_atomic_fetch_add, unsigned long long)
+DD_GLOBAL_ATOMIC(g_or, __hip_atomic_fetch_or, uint32_t)
+DD_GLOBAL_ATOMIC(g_min, __hip_atomic_fetch_min, unsigned long long)
+DD_GLOBAL_ATOMIC(g_max, __hip_atomic_fetch_max, unsigned long long)
+DD_GLOBAL_ATOMIC(g_exch, __hip_atomic_exchange, uint32_t)
+#undef DD_GLOBAL_ATOMIC
+// Compare-and-swap; returns the old value.
+__device__ __forceinline__ unsigned long long g_cas(GlobalPtr<unsigned long long> p, unsigned long long cmp,
+                                                    unsigned long long val) {
+  __hip_atomic_compare_exchange_strong(p, &cmp, val, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return cmp;
+}
+
+// Streaming reads through the global address space (chunk data pointers are
+// generic where they come from byte-range helpers).
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4_t global_u32x4;
 // 16 bytes at p (16-byte aligned) through a global load.
@@ -52,9 +82,17 @@ __device__ __forceinline__ uint4 gload16(const void* p) {
   const u32x4_t v = *(const global_u32x4*)(p);
   return make_uint4(v.x, v.y, v.z, v.w);
 }
-__device__ __forceinline__ Digest load_digest(const Digest* p) {
-  const uint4 v = gload16(p);
+__device__ __forceinline__ uint4 gload16(GlobalPtr<const void> p) {
+  const u32x4_t v = *(const global_u32x4*)(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ Digest load_digest(GlobalPtr<const Digest> p) {
+  const uint4 v = gload16((GlobalPtr<const void>)p);
   return Digest{v.x | ((uint64_t)v.y << 32), v.z | ((uint64_t)v.w << 32)};
+}
+// One 16-byte store (digest arrays are 16-byte aligned).
+__device__ __forceinline__ void store_digest(GlobalPtr<Digest> p, const Digest& d) {
+  *(GlobalPtr<u32x4_t>)p = u32x4_t{(uint32_t)d.k0, (uint32_t)(d.k0 >> 32), (uint32_t)d.k1, (uint32_t)(d.k1 >> 32)};
 }
 
 __host__ __device__ inline unsigned long long pack_node(uint32_t tree, uint32_t node) {
@@ -463,22 +501,27 @@ __host__ __device__ inline uint64_t run_leaf(const LeafRuns& r, uint64_t idx) {
 // ---------------------------------------------------------------------------
 // Digest map probing.
 // ---------------------------------------------------------------------------
-struct MapView {
-  Slot* slots;
+template <template <class> class P>
+struct MapT {
+  P<Slot> slots;
   uint64_t mask;
   uint32_t shift;  // 64 - log2(capacity)
 };
+using MapView = MapT<HostPtr>;  // as the host fills it
+using MapG = MapT<GlobalPtr>;   // as the kernels read it
+static_assert(sizeof(MapView) == sizeof(MapG), "one layout for both pointer kinds");
 
-__host__ __device__ inline uint64_t map_start(const MapView& m, const Digest& k) {
+template <class M>
+__host__ __device__ inline uint64_t map_start(const M& m, const Digest& k) {
   return ((k.k0 ^ (k.k1 * 0x9e3779b97f4a7c15ull)) * 0xff51afd7ed558ccdull) >> m.shift;
 }
 
 // Read-only lookup (every key of the map is published).  Returns the slot
 // index or ~0 when absent.
-__device__ inline uint64_t map_find(const MapView& m, const Digest& k) {
+__device__ inline uint64_t map_find(const MapG& m, const Digest& k) {
   uint64_t s = map_start(m, k) & m.mask;
   for (uint64_t it = 0; it <= m.mask; it++) {
-    const Slot* sl = &m.slots[s];
+    GlobalPtr<const Slot> sl = &m.slots[s];
     unsigned long long v = sl->val;
     if (v == SLOT_EMPTY) return ~0ull;
     if (sl->k0 == k.k0 && sl->k1 == k.k1) return s;

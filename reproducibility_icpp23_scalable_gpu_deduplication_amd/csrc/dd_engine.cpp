@@ -164,10 +164,9 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     tile_base[g] = (uint32_t)ntiles;
     ntiles += tiles_in_group(gm.D, g);
   }
-  // First-occurrence interior nodes: sub-list r takes tiles T = r (mod 32),
-  // at most 255 interior nodes each.
-  const uint64_t fint_cap = ceil_div(ntiles, 32) * 255;
-  DD_TRY(tiles.ensure(32 * fint_cap * 4));
+  // First-occurrence interior nodes: a region of kTileInterior entries per
+  // tile, then one count per tile.
+  DD_TRY(tiles.ensure(ntiles * (kTileInterior + 1) * 4));
   // Per group-0 tile stamp of the last checkpoint that changed one of its
   // leaves; zero-filled when (re)allocated, epochs start at 1.
   const uint64_t ntiles0 = tiles_in_group(gm.D, 0);
@@ -212,13 +211,12 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.host_ctr = nullptr;  // set by the pipeline
   job.blk_claim = leafblk.as<uint32_t>();
   job.int_claim = job.blk_claim + nclaim;
-  // 32 pending-leaf counters and 32 interior-list counters follow the counters
+  // 32 pending-leaf counters follow the counters
   job.pend_cnt = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
-  job.fint_cnt = job.pend_cnt + 32;
   job.pend_list = pend_list.as<uint32_t>();
   job.pend_cap = (uint32_t)pend_cap;
   job.fint_list = tiles.as<uint32_t>();
-  job.fint_cap = (uint32_t)fint_cap;
+  job.fint_n = job.fint_list + ntiles * kTileInterior;
   for (int g = 0; g < 4; g++) job.tile_base[g] = tile_base[g];
   job.ntiles = (uint32_t)ntiles;
   job.epoch = epoch;

@@ -74,7 +74,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T* sh, T& total) {
   return res;
 }
 
-__device__ __forceinline__ void store_u32(uint8_t* p, uint32_t v) { *reinterpret_cast<uint32_t*>(p) = v; }
+__device__ __forceinline__ void store_u32(GlobalPtr<uint8_t> p, uint32_t v) { *(GlobalPtr<uint32_t>)p = v; }
 
 // Sum over the block (every lane must call it).
 template <typename T>
@@ -93,13 +93,12 @@ __device__ __forceinline__ T block_sum(T v, T* sh) {
 // begin: zero the counters of every job (or only the flags, for a re-run of
 // the map inserts after the map has grown).
 // ---------------------------------------------------------------------------
-__global__ void k_begin(const JobDev* __restrict__ jobs, uint32_t njobs, int rerun) {
+__global__ void k_begin(const JobG* __restrict__ jobs, uint32_t njobs, int rerun) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= njobs) return;
-  JobCounters* c = jobs[j].ctr;
+  const GlobalPtr<JobCounters> c = jobs[j].ctr;
   c->map_count = jobs[j].map_count_at_start;
   c->flags = 0;
-  for (uint32_t i = 0; i < 32; i++) jobs[j].fint_cnt[i] = 0;  // pass 1 runs again on a re-run
   if (rerun) return;  // keep the changed-leaf lists of leaf_hash when re-running the inserts
   for (uint32_t i = 0; i < 32; i++) {
     jobs[j].pend_cnt[i] = 0;
@@ -116,16 +115,15 @@ __global__ void k_begin(const JobDev* __restrict__ jobs, uint32_t njobs, int rer
 // the next one (the final entry count stays: it is the next checkpoint's
 // starting count).  Runs while the host wakes from its synchronization, so
 // the next checkpoint starts with the leaf kernel and no k_begin.
-__global__ void k_finish(const JobDev* __restrict__ jobs, uint32_t njobs) {
+__global__ void k_finish(const JobG* __restrict__ jobs, uint32_t njobs) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= njobs) return;
-  const JobDev& J = jobs[j];
-  JobCounters* c = J.ctr;
+  const JobG& J = jobs[j];
+  const GlobalPtr<JobCounters> c = J.ctr;
   if (c->flags) return;  // a re-run or a failed checkpoint: the host re-arms with k_begin
   for (uint32_t i = 0; i < 32; i++) {
     J.pend_cnt[i] = 0;
     J.dirty_cnt[i] = 0;
-    J.fint_cnt[i] = 0;
   }
   c->pending = 0;
   c->nfirst = c->nshift = c->nprior = 0;
@@ -152,9 +150,10 @@ __device__ __forceinline__ uint32_t leaf_tile(uint32_t D, uint64_t leaf) {
   return (uint32_t)(((leaf + 1) >> (lv - Lr0)) - (1ull << Lr0));
 }
 
-__device__ __forceinline__ void list_append(uint32_t* cnt, uint32_t* list, uint32_t cap, uint32_t item) {
+__device__ __forceinline__ void list_append(GlobalPtr<uint32_t> cnt, GlobalPtr<uint32_t> list, uint32_t cap,
+                                            uint32_t item) {
   const uint32_t r = item % kPendSpread;
-  const uint32_t k = atomicAdd(&cnt[r], 1u);
+  const uint32_t k = g_add(&cnt[r], 1u);
   list[(uint64_t)r * cap + k] = item;
 }
 
@@ -163,14 +162,14 @@ __device__ __forceinline__ void list_append(uint32_t* cnt, uint32_t* list, uint3
 // wavefront with a changed leaf, no block barrier, so a block's waves retire
 // independently.  A baseline visits every leaf and tile densely and keeps no
 // lists.  Every lane of the wavefront calls this.
-__device__ __forceinline__ void record_pending(const JobDev& J, uint64_t k, bool pending) {
+__device__ __forceinline__ void record_pending(const JobG& J, uint64_t k, bool pending) {
   if (J.is_baseline) return;  // uniform: a workgroup belongs to one job
   const unsigned long long b = __ballot(pending);
   if (!b) return;
   const uint32_t r = (uint32_t)(k >> 6) % kPendSpread;
   const uint32_t leader = __ffsll((long long)b) - 1;
   uint32_t base = 0;
-  if (lane_id() == leader) base = atomicAdd(&J.pend_cnt[r], (uint32_t)__popcll(b));
+  if (lane_id() == leader) base = g_add(&J.pend_cnt[r], (uint32_t)__popcll(b));
   base = __shfl(base, leader, 64);
   if (pending) J.pend_list[(uint64_t)r * J.pend_cap + base + __popcll(b & lanemask_lt())] = (uint32_t)k;
   // Mark the group-0 tiles of changed leaves: the first marker of a tile in
@@ -184,7 +183,7 @@ __device__ __forceinline__ void record_pending(const JobDev& J, uint64_t k, bool
     const uint32_t leader = __ffsll((long long)act) - 1;
     const uint32_t lt = __shfl(T, leader, 64);
     if (lane_id() == leader) {
-      if (atomicExch(&J.tile_epoch[lt], J.epoch) != J.epoch) list_append(J.dirty_cnt, J.dirty_list, J.dirty_cap, lt);
+      if (g_exch(&J.tile_epoch[lt], J.epoch) != J.epoch) list_append(J.dirty_cnt, J.dirty_list, J.dirty_cap, lt);
     }
     if (todo && T == lt) todo = false;
   }
@@ -193,11 +192,10 @@ __device__ __forceinline__ void record_pending(const JobDev& J, uint64_t k, bool
 // Item i of a 32-way split work list: sub-list counts are prefix-summed in
 // LDS by the caller (pre[r] = items before sub-list r, pre[32] = total).
 // Lane r of the first wavefront loads count r and the wavefront scans them
-// with shuffles: one load round trip per workgroup.  (A loop of one thread
-// over the counts waits for each load in turn, since the LDS stores between
-// them may alias the counts as far as the compiler can tell.)  Every caller
-// runs at least 64 threads.
-__device__ __forceinline__ void load_list_prefix(const uint32_t* cnt, uint32_t* pre) {
+// with shuffles: one load round trip per workgroup (a loop of one thread over
+// the counts would wait for each load in turn).  Every caller runs at least
+// 64 threads.
+__device__ __forceinline__ void load_list_prefix(GlobalPtr<const uint32_t> cnt, uint32_t* pre) {
   static_assert(kPendSpread <= 64, "one count per lane of a wavefront");
   if (threadIdx.x < 64) {
     const uint32_t l = threadIdx.x;
@@ -215,7 +213,8 @@ __device__ __forceinline__ void load_list_prefix(const uint32_t* cnt, uint32_t* 
 
 // Sub-list of item i: the last r in [0, 32) with pre[r] <= i (pre is
 // non-decreasing and pre[0] = 0), by binary search.
-__device__ __forceinline__ uint32_t list_item(const uint32_t* list, uint32_t cap, const uint32_t* pre, uint32_t i) {
+__device__ __forceinline__ uint32_t list_item(GlobalPtr<const uint32_t> list, uint32_t cap, const uint32_t* pre,
+                                              uint32_t i) {
   uint32_t r = 0;
 #pragma unroll
   for (uint32_t step = kPendSpread / 2; step > 0; step >>= 1)
@@ -223,7 +222,7 @@ __device__ __forceinline__ uint32_t list_item(const uint32_t* list, uint32_t cap
   return list[(uint64_t)r * cap + (i - pre[r])];
 }
 
-__device__ __forceinline__ uint64_t pending_total(const JobDev& J) {
+__device__ __forceinline__ uint64_t pending_total(const JobG& J) {
   uint64_t t = 0;
 #pragma unroll
   for (uint32_t i = 0; i < kPendSpread; i++) t += J.pend_cnt[i];
@@ -235,16 +234,16 @@ __device__ __forceinline__ uint64_t pending_total(const JobDev& J) {
 // flight per lane (long chunks), LEAF_SHORT loads a whole chunk of at most
 // 128 bytes before hashing it.
 template <int Kind, bool Aligned16, int Pipe>
-__global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_leaf_hash(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   const uint64_t k = (uint64_t)(blockIdx.x - prefix[j]) * blockDim.x + threadIdx.x;
   bool pending = false;
   if (k < J.C) {
     const uint64_t cs = J.chunk_size;
     const uint64_t off = k * cs;
     const uint64_t nb = (k == J.C - 1) ? J.len - off : cs;
-    const uint8_t* p = J.data + off;
+    const uint8_t* p = (const uint8_t*)(J.data + off);
     const uint64_t leaf = J.C - 1 + k;
     // The previous digest is loaded before the chunk: its latency hides
     // behind the chunk loads instead of following the hash.
@@ -274,7 +273,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jo
       lab = PENDING;
       pending = true;
     }
-    if (pending) J.tree[leaf] = d;
+    if (pending) store_digest(J.tree + leaf, d);
     J.labels[leaf] = lab;
   }
   record_pending(J, k, pending);
@@ -282,7 +281,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobDev* __restrict__ jo
 
 // Shared tail of both leaf kernels: compare with the previous digest, record
 // the label, write the new digest when it changed.
-__device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const Digest& d, const Digest& prev) {
+__device__ __forceinline__ bool leaf_finish(const JobG& J, uint64_t k, const Digest& d, const Digest& prev) {
   const uint64_t leaf = J.C - 1 + k;
   bool pending;
   uint8_t lab;
@@ -294,7 +293,7 @@ __device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const D
     lab = PENDING;
     pending = true;
   }
-  if (pending) J.tree[leaf] = d;
+  if (pending) store_digest(J.tree + leaf, d);
   J.labels[leaf] = lab;
   return pending;
 }
@@ -306,20 +305,20 @@ __device__ __forceinline__ bool leaf_finish(const JobDev& J, uint64_t k, const D
 // next SEG bytes of its own chunk out of LDS.  Needs chunk % SEG == 0 and a
 // 16-byte aligned buffer; the partial last chunk is hashed from global memory.
 template <int Kind, int SEG>
-__global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   constexpr int PITCH = SEG + 16;
   constexpr int VPC = SEG / 16;        // 16-byte vectors per chunk per round
   constexpr int LOADS = VPC;           // 64 chunks x VPC vectors spread over 64 lanes
   __shared__ __attribute__((aligned(16))) uint8_t lds[4][64 * PITCH];
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint64_t k0 = ((uint64_t)(blockIdx.x - prefix[j]) * 4 + wave) * 64;  // first chunk of this wave
   const uint64_t k = k0 + lane;
   const uint64_t cs = J.chunk_size;
   const uint64_t full = J.nfull;  // chunks that are complete
   uint8_t* L = lds[wave];
-  const uint8_t* base = J.data + k0 * cs;
+  const GlobalPtr<const uint8_t> base = J.data + k0 * cs;
   const uint32_t rounds = (uint32_t)(cs / SEG);
   Digest prev{0, 0};  // loaded first: its latency hides behind the chunk loads
   if (k < J.C && !J.is_baseline) prev = load_digest(J.tree + (J.C - 1 + k));
@@ -377,7 +376,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobDev* __restrict_
     pending = leaf_finish(J, k, d, prev);
   } else if (k < J.C) {  // the partial last chunk
     const uint64_t off = k * cs;
-    const uint8_t* p = J.data + off;
+    const uint8_t* p = (const uint8_t*)(J.data + off);
     const uint64_t nb = J.len - off;
     const Digest d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
     pending = leaf_finish(J, k, d, prev);
@@ -410,10 +409,10 @@ __device__ __forceinline__ uint32_t dma_slot(uint32_t i, uint32_t j, uint32_t p)
 }
 
 template <int Kind>
-__global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   __shared__ __attribute__((aligned(16))) uint8_t lds[4][8192];
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint64_t k0 = ((uint64_t)(blockIdx.x - prefix[j]) * 4 + wave) * 64;  // first chunk of this wave
   const uint64_t k = k0 + lane;
@@ -427,7 +426,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobDev* __restrict_
   // load from the wave's first chunk instead (their slots are never read).
   const uint32_t pe = lane >> 3, po = pe ^ 1;
   const uint32_t je = ((lane & 7) - pe) & 7, jo = ((lane & 7) - po) & 7;
-  const uint8_t* src[8];
+  GlobalPtr<const uint8_t> src[8];
 #pragma unroll
   for (uint32_t i = 0; i < 8; i++) {
     const uint32_t p = (i & 1) ? po : pe, jj = (i & 1) ? jo : je;
@@ -489,7 +488,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobDev* __restrict_
     pending = leaf_finish(J, k, d, prev);
   } else if (k < J.C) {  // the partial last chunk
     const uint64_t off = k * cs;
-    const uint8_t* p = J.data + off;
+    const uint8_t* p = (const uint8_t*)(J.data + off);
     const uint64_t nb = J.len - off;
     const Digest d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
     pending = leaf_finish(J, k, d, prev);
@@ -524,32 +523,32 @@ struct InsertResult {
 // The job fields an insert needs, read once by the caller (the compiler would
 // reload them from the job table after every atomic).
 struct InsertCtx {
-  MapView m;
-  const Digest* tree;
+  MapG m;
+  GlobalPtr<const Digest> tree;
   uint64_t C;
   uint32_t ckpt;
   bool cas_first;  // a baseline starts from an empty map: CAS before any load
 };
 
-__device__ __forceinline__ InsertCtx insert_ctx(const JobDev& J) {
+__device__ __forceinline__ InsertCtx insert_ctx(const JobG& J) {
   return InsertCtx{J.map, J.tree, J.C, J.ckpt_id, J.is_baseline != 0};
 }
 
 template <bool Leaf>
 __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, uint32_t node) {
-  const MapView& m = J.m;
+  const MapG& m = J.m;
   const unsigned long long mine = pack_node(J.ckpt, node);
   uint64_t s = map_start(m, k) & m.mask;
   // In a baseline the map starts empty, so the first probe goes straight to
   // the CAS (a failed CAS returns the slot like a load would).
   const bool cas_first = J.cas_first;
   for (uint64_t it = 0; it <= m.mask; it++) {
-    Slot* sl = &m.slots[s];
+    const GlobalPtr<Slot> sl = &m.slots[s];
     unsigned long long v = (cas_first && it == 0)
                                ? SLOT_EMPTY
                                : __hip_atomic_load(&sl->val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (v == SLOT_EMPTY) {
-      const unsigned long long old = atomicCAS(&sl->val, SLOT_EMPTY, mine);
+      const unsigned long long old = g_cas(&sl->val, SLOT_EMPTY, mine);
       if (old == SLOT_EMPTY) {
         sl->k0 = k.k0;
         sl->k1 = k.k1;
@@ -561,14 +560,14 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
     const bool this_kernel = vt == J.ckpt && (Leaf || vn < J.C - 1);
     Digest e;
     if (this_kernel) {
-      e = J.tree[vn];
+      e = load_digest(J.tree + vn);
     } else {
       e.k0 = sl->k0;
       e.k1 = sl->k1;
     }
     if (digest_eq(e, k)) {
       unsigned long long prev = v;
-      if (vt == J.ckpt) prev = Leaf ? atomicMin(&sl->val, mine) : atomicMax(&sl->val, mine);
+      if (vt == J.ckpt) prev = Leaf ? g_min(&sl->val, mine) : g_max(&sl->val, mine);
       return InsertResult{true, false, s, prev};
     }
     s = (s + 1) & m.mask;
@@ -576,7 +575,7 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
   return InsertResult{false, false, 0, 0};
 }
 
-__device__ __forceinline__ bool map_guard_tripped(const JobDev& J) {
+__device__ __forceinline__ bool map_guard_tripped(const JobG& J) {
   // Worst case of this checkpoint: one entry per changed leaf plus one per
   // first-occurrence interior node (fewer than the changed leaves).
   return J.map_count_at_start + 2 * pending_total(J) > J.map_limit;
@@ -597,9 +596,9 @@ __device__ __forceinline__ bool map_guard_tripped(const JobDev& J) {
 // occurrence, so only the run's first lane touches the map and the others
 // label themselves SHIFT with the first lane's slot.  Long runs of one digest
 // (zero-filled regions) would otherwise serialize on one slot's atomics.
-__global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   uint32_t claims = 0;
   __shared__ uint32_t sh[4];
@@ -612,7 +611,7 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
   }
   if (map_guard_tripped(J)) {
     if (threadIdx.x == 0) {
-      atomicOr(&J.ctr->flags, FLAG_NEED_GROW);
+      g_or(&J.ctr->flags, FLAG_NEED_GROW);
       J.blk_claim[wg] = 0;
     }
     return;
@@ -627,11 +626,11 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
   }
   const uint32_t K = J.ckpt_id;
   const InsertCtx ic = insert_ctx(J);
-  uint8_t* const labels = J.labels;
-  uint32_t* const slot_of = J.slot_of;
-  const uint32_t* const pend_list = J.pend_list;
+  const GlobalPtr<uint8_t> labels = J.labels;
+  const GlobalPtr<uint32_t> slot_of = J.slot_of;
+  const GlobalPtr<const uint32_t> pend_list = J.pend_list;
   const uint32_t pend_cap = J.pend_cap;
-  uint32_t* const flags = &J.ctr->flags;
+  const GlobalPtr<uint32_t> flags = &J.ctr->flags;
   const uint32_t lane = lane_id();
   // The loop bound is uniform over the workgroup, so every lane of a
   // wavefront takes part in its shuffles; lanes past the list are inactive.
@@ -647,7 +646,7 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
       // A list baseline leaves all-zero digests out (they equal the fresh
       // list's entry); every other leaf of a dense pass needs its insert.
       if (own == FIXED_DUPL) act = false;
-      else k = ic.tree[leaf];
+      else k = load_digest(ic.tree + leaf);
     }
     const unsigned long long pk0 = __shfl_up(k.k0, 1, 64), pk1 = __shfl_up(k.k1, 1, 64);
     const unsigned long long pleaf = __shfl_up((unsigned long long)leaf, 1, 64);
@@ -660,7 +659,7 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
       ok = r.ok;
       slot = r.slot;
       if (!r.ok) {
-        atomicOr(flags, FLAG_MAP_FULL);
+        g_or(flags, FLAG_MAP_FULL);
       } else if (r.claimed) {
         claims++;
       } else {
@@ -701,13 +700,13 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobDev* __restrict_
 // Interior labels are DONE from the baseline on, so the root scan emits
 // every FIRST or SHIFT leaf.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_list_resolve(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_list_resolve(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;  // written before this kernel: uniform
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   const uint64_t C = J.C;
-  uint8_t* const labels = J.labels;
+  const GlobalPtr<uint8_t> labels = J.labels;
   if (J.is_baseline) {
     for (uint64_t i = (uint64_t)wg * blockDim.x + threadIdx.x; i < C; i += (uint64_t)nwg * blockDim.x) {
       if (labels[C - 1 + i] == PENDING) labels[C - 1 + i] = FIRST_OCUR;
@@ -733,7 +732,7 @@ struct TileRef {
   uint64_t r;
 };
 
-__device__ __forceinline__ TileRef tile_ref(const JobDev& J, int g, uint64_t t) {
+__device__ __forceinline__ TileRef tile_ref(const JobG& J, int g, uint64_t t) {
   TileRef tr;
   tr.g = g;
   tr.Lb = (int)J.D - kTileLevels * g;
@@ -751,33 +750,45 @@ __device__ __forceinline__ uint64_t tile_node(uint64_t r, uint32_t i) {
 
 // Map inserts of the first-occurrence interior nodes that pass 1 listed: one
 // lane per list entry (grid-stride).
-__global__ __launch_bounds__(256) void k_insert_interior(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
   uint32_t claims = 0;
   if (!__syncthreads_or((J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0)) {
-    // A baseline visits every interior node 0 .. C-2 by label (most of them
-    // are first occurrences); an incremental checkpoint walks the list.
-    const bool dense = J.is_baseline != 0;
-    uint32_t nitems = (uint32_t)(J.C - 1);
-    if (!dense) {
-      load_list_prefix(J.fint_cnt, pre);
-      nitems = pre[kPendSpread];
-    }
     const InsertCtx ic = insert_ctx(J);
-    const uint8_t* const labels = J.labels;
-    const uint32_t* const fint_list = J.fint_list;
-    const uint32_t fint_cap = J.fint_cap;
-    uint32_t* const flags = &J.ctr->flags;
-    for (uint32_t it = wg * blockDim.x + threadIdx.x; it < nitems; it += nwg * blockDim.x) {
-      const uint32_t x = dense ? it : list_item(fint_list, fint_cap, pre, it);
-      if (dense && labels[x] != FIRST_OCUR) continue;
-      const InsertResult r = map_insert<false>(ic, ic.tree[x], x);
-      if (!r.ok) atomicOr(flags, FLAG_MAP_FULL);
-      claims += r.claimed ? 1 : 0;
+    const GlobalPtr<uint32_t> flags = &J.ctr->flags;
+    if (J.is_baseline) {
+      // A baseline visits every interior node 0 .. C-2 by label (most of
+      // them are first occurrences).
+      const GlobalPtr<const uint8_t> labels = J.labels;
+      for (uint64_t x = (uint64_t)wg * blockDim.x + threadIdx.x; x + 1 < J.C; x += (uint64_t)nwg * blockDim.x) {
+        if (labels[x] != FIRST_OCUR) continue;
+        const InsertResult r = map_insert<false>(ic, load_digest(ic.tree + x), (uint32_t)x);
+        if (!r.ok) g_or(flags, FLAG_MAP_FULL);
+        claims += r.claimed ? 1 : 0;
+      }
+    } else {
+      // An incremental checkpoint walks the tiles pass 1 visited (the dirty
+      // group-0 tiles, then every tile of the groups above), one wavefront
+      // per tile, one lane per entry of the tile's fint region.
+      load_list_prefix(J.dirty_cnt, pre);
+      const uint32_t ndirty = pre[kPendSpread];
+      const uint32_t ntiles0 = J.tile_base[1];  // group-0 tiles (tile_base[1] = ntiles without a group 1)
+      const uint32_t nitems = ndirty + (J.ntiles - ntiles0);
+      const uint32_t lane = lane_id(), nw = blockDim.x >> 6;
+      for (uint32_t it = wg * nw + (threadIdx.x >> 6); it < nitems; it += nwg * nw) {
+        const uint32_t T = it < ndirty ? list_item(J.dirty_list, J.dirty_cap, pre, it) : ntiles0 + (it - ndirty);
+        const uint32_t cnt = J.fint_n[T];
+        for (uint32_t q = lane; q < cnt; q += 64) {
+          const uint32_t x = J.fint_list[(uint64_t)T * kTileInterior + q];
+          const InsertResult r = map_insert<false>(ic, load_digest(ic.tree + x), x);
+          if (!r.ok) g_or(flags, FLAG_MAP_FULL);
+          claims += r.claimed ? 1 : 0;
+        }
+      }
     }
   }
   const uint32_t n = block_sum<uint32_t>(claims, sh);
@@ -794,25 +805,18 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobDev* __restric
 
 // Label of a leaf after the map inserts: a leaf still PENDING is the
 // checkpoint's first occurrence of its digest; FIXED|PENDING is FIXED.
-// Written back.
-__device__ __forceinline__ uint8_t resolve_leaf(const JobDev& J, uint64_t leaf) {
-  const uint8_t raw = J.labels[leaf];
-  if (!(raw & PENDING)) return raw;
-  const uint8_t lab = raw == PENDING ? FIRST_OCUR : FIXED_DUPL;
-  J.labels[leaf] = lab;
-  return lab;
+__device__ __forceinline__ uint8_t resolved_label(uint8_t raw) {
+  return (raw & PENDING) ? (raw == PENDING ? (uint8_t)FIRST_OCUR : (uint8_t)FIXED_DUPL) : raw;
 }
 
 // A group-1 tile's bottom nodes are group-0 tile roots.  When no leaf of
 // such a group-0 tile changed, the tile was not visited this checkpoint and
-// its stored labels are stale: every node of it is FIXED.
-__device__ __forceinline__ bool child_tile_clean(const JobDev& J, int g, uint64_t x) {
-  if (g != 1 || J.is_baseline) return false;
-  const uint32_t Lr0 = J.D - kTileLevels;  // group 1 exists only when D > kTileLevels
-  return J.tile_epoch[x - ((1ull << Lr0) - 1)] != J.epoch;
-}
+// its stored labels are stale: every node of it is FIXED.  Group-0 tile
+// roots start at heap node 2^(D-8) - 1.
+__device__ __forceinline__ bool check_child_tiles(const JobG& J, int g) { return g == 1 && !J.is_baseline; }
+__device__ __forceinline__ uint64_t child_tile_base(const JobG& J) { return (1ull << (J.D - kTileLevels)) - 1; }
 
-__device__ __forceinline__ uint32_t group_items(const JobDev& J, int g, uint32_t* pre) {
+__device__ __forceinline__ uint32_t group_items(const JobG& J, int g, uint32_t* pre) {
   if (g == 0 && !J.is_baseline) {
     load_list_prefix(J.dirty_cnt, pre);
     return pre[kPendSpread];
@@ -822,189 +826,412 @@ __device__ __forceinline__ uint32_t group_items(const JobDev& J, int g, uint32_t
 }
 
 // Both passes process a tile with one wavefront (a 64-lane workgroup) so that
-// many tiles are in flight: the per-tile work is a chain of dependent loads
-// and level steps, so the passes are bound by latency, not by bandwidth.
-// One LDS buffer holds the current level; every step is split into a read
-// phase (children) and a write phase (parents) by a barrier, so parents can
-// overwrite the children in place.  The loops over 64-node chunks have a
-// trip count that is uniform over the workgroup.
+// many tiles are in flight.  Lane i owns bottom positions 4i .. 4i+3 of the
+// tile, the two level-1 nodes above them and the level-2 node above those,
+// so the first two levels are computed in registers; the levels above (at
+// most 63 nodes) go through LDS, one barrier per level.  Every global load
+// of a tile (labels, then the digests they select) is issued before the
+// level steps and every global store after them: loads and stores share one
+// in-order completion counter, so a load issued after a store waits for the
+// store to be acknowledged, which would cost a memory round trip per level.
+// Leaves one level above the deepest (chunk counts that are not a power of
+// two) sit on level 1 of group-0 tiles and are loaded with the bottom nodes.
+// Job fields are read once into locals: the compiler cannot tell the job
+// table apart from the arrays the kernel stores to, and would reload them.
 constexpr uint32_t kTileThreads = 64;
+constexpr uint32_t kTopNodes = 63;  // tile nodes on levels 3 .. 8 above the bottom
+
+// One lane's share of a tile of height H (H <= 8) with root r: bottom nodes
+// xb .. xb+3 (positions 4i .. 4i+3), level-1 nodes x1, x1+1 (positions
+// 2i, 2i+1), level-2 node x2 (position i); positions past the level's width
+// do not exist.  LDS position q < 2^(H-2) - 1 holds tile node q in
+// level-major order from the tile root (levels 3 .. H above the bottom).
+struct TileLane {
+  int H;
+  uint64_t r;
+  uint32_t width;       // bottom positions, 2^H
+  uint64_t xb, x1, x2;  // first heap node of this lane on levels 0, 1, 2
+  bool vb[4];           // bottom node exists
+  bool v1[2], v2;       // level-1 / level-2 position exists
+  uint32_t nq;          // LDS positions (levels >= 3)
+};
+
+__device__ __forceinline__ TileLane tile_lane(const TileRef& tr, uint64_t N, uint32_t i) {
+  TileLane L;
+  L.H = tr.H;
+  L.r = tr.r;
+  L.width = 1u << tr.H;
+  L.xb = ((tr.r + 1) << tr.H) - 1 + 4 * i;
+  L.x1 = tr.H >= 1 ? ((tr.r + 1) << (tr.H - 1)) - 1 + 2 * i : 0;
+  L.x2 = tr.H >= 2 ? ((tr.r + 1) << (tr.H - 2)) - 1 + i : 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) L.vb[k] = 4 * i + k < L.width && L.xb + k < N;
+#pragma unroll
+  for (int k = 0; k < 2; k++) L.v1[k] = tr.H >= 1 && 2 * i + k < (L.width >> 1);
+  L.v2 = tr.H >= 2 && i < (L.width >> 2);
+  L.nq = tr.H >= 3 ? (1u << (tr.H - 2)) - 1 : 0;
+  return L;
+}
+
+// Labels of a lane's bottom nodes (0xff where absent; a clean child tile's
+// root reads as the pass's clean label: DONE in pass 1, FIXED in pass 2,
+// chosen by the caller through `clean`) and of its level-1 leaves (vu).  The
+// loads are straight-line, from clamped addresses, with the results selected
+// afterwards, so that all of them are in flight together.
+__device__ __forceinline__ void tile_labels(const JobG& J, int g, const TileLane& T, uint64_t C, uint8_t raw[4],
+                                            uint8_t rawu[2], bool vu[2], uint8_t clean) {
+  const GlobalPtr<const uint8_t> labels = J.labels;
+  uint8_t b[4], u[2];
+#pragma unroll
+  for (int k = 0; k < 4; k++) b[k] = labels[T.vb[k] ? T.xb + k : 0];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    vu[k] = T.v1[k] && T.x1 + k >= C - 1;
+    u[k] = labels[vu[k] ? T.x1 + k : 0];
+  }
+  bool cl[4] = {false, false, false, false};
+  if (check_child_tiles(J, g)) {  // uniform
+    const uint64_t base = child_tile_base(J);
+    uint32_t e[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) e[k] = J.tile_epoch[T.vb[k] ? T.xb + k - base : 0];
+#pragma unroll
+    for (int k = 0; k < 4; k++) cl[k] = e[k] != J.epoch;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) raw[k] = !T.vb[k] ? (uint8_t)0xff : cl[k] ? clean : b[k];
+#pragma unroll
+  for (int k = 0; k < 2; k++) rawu[k] = vu[k] ? u[k] : (uint8_t)0xff;
+}
+
+// Digests of the nodes labelled `want` among a lane's bottom nodes and
+// level-1 leaves (zero elsewhere), loaded together from clamped addresses.
+__device__ __forceinline__ void tile_digests(GlobalPtr<const Digest> tree, const TileLane& T, const uint8_t bl[4],
+                                             const uint8_t ul[2], uint8_t want, Digest bd[4], Digest ud[2]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) bd[k] = load_digest(tree + (bl[k] == want ? T.xb + k : 0));
+#pragma unroll
+  for (int k = 0; k < 2; k++) ud[k] = load_digest(tree + (ul[k] == want ? T.x1 + k : 0));
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (bl[k] != want) bd[k] = Digest{0, 0};
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+    if (ul[k] != want) ud[k] = Digest{0, 0};
+}
 
 // Pass 1, the first-occurrence forest (include/reference_impl.hpp:73-82).
 // Group 0 visits the dirty tiles only; the groups above visit every tile.
 // In an incremental checkpoint the first-occurrence interior nodes of a tile
-// are collected in LDS and appended to the fint list for the interior map
-// inserts (a baseline's interior inserts visit every node instead).
-// Job fields are read once into locals: the compiler cannot tell the job
-// table apart from the arrays the kernel stores to, and would reload them.
-__global__ __launch_bounds__(kTileThreads) void k_pass1(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
+// are appended to the fint list for the interior map inserts (a baseline's
+// interior inserts visit every node instead).  Group-0 leaves left PENDING
+// by the inserts are resolved and written back here.
+__global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
                                                         int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   if (g >= tile_groups(J.D)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
-  __shared__ uint8_t lab[256];
-  __shared__ Digest dig[256];
   __shared__ uint32_t pre[kPendSpread + 1];
-  __shared__ uint32_t fl[256];
-  __shared__ uint32_t fn, fbase;
+  __shared__ uint8_t tlab[kTopNodes + 64];
+  __shared__ Digest tdig[kTopNodes + 64];
   const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
   const bool listing = !J.is_baseline;
   const uint64_t N = J.N, C = J.C;
-  uint8_t* const labels = J.labels;
-  Digest* const tree = J.tree;
+  const GlobalPtr<uint8_t> labels = J.labels;
+  const GlobalPtr<Digest> tree = J.tree;
   const int kind = J.hash_kind;
-  uint32_t* const fint_list = J.fint_list;
-  uint32_t* const fint_cnt = J.fint_cnt;
-  const uint64_t fint_cap = J.fint_cap;
+  const GlobalPtr<uint32_t> fint_list = J.fint_list;
+  const GlobalPtr<uint32_t> fint_n = J.fint_n;
   const uint32_t tbase = J.tile_base[g];
   for (uint32_t it = wg; it < nitems; it += nwg) {
     const uint64_t t = (g == 0 && listing) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
-    const TileRef tr = tile_ref(J, g, t);
-    const int H = tr.H;
-    const uint64_t r = tr.r;
-    const uint32_t width = 1u << H;
-    if (i == 0) fn = 0;
-    for (uint32_t idx = i; idx < width; idx += kTileThreads) {
-      const uint64_t x = ((r + 1) << H) - 1 + idx;
-      uint8_t L = 0xff;
-      if (x < N) L = g == 0 ? resolve_leaf(J, x) : child_tile_clean(J, g, x) ? (uint8_t)DONE : labels[x];
-      lab[idx] = L;
-      dig[idx] = L == FIRST_OCUR ? tree[x] : Digest{0, 0};
-    }
-    for (int st = 1; st <= H; st++) {
-      const uint32_t w = width >> st;
-      const uint64_t lvl = ((r + 1) << (H - st)) - 1;
-      for (uint32_t base = 0; base < w; base += kTileThreads) {
-        const uint32_t idx = base + i;
-        const bool act = idx < w;
-        const uint64_t x = lvl + idx;
-        const bool interior = act && x < C - 1;
-        __syncthreads();  // the previous writes are visible
-        const bool both = interior && lab[2 * idx] == FIRST_OCUR && lab[2 * idx + 1] == FIRST_OCUR;
-        const Digest a = both ? dig[2 * idx] : Digest{0, 0};
-        const Digest b = both ? dig[2 * idx + 1] : Digest{0, 0};
-        __syncthreads();  // every child of this chunk is read before any parent is written
-        if (!act) continue;
-        uint8_t L;
-        Digest d{0, 0};
-        if (!interior) {  // a leaf one level above the deepest
-          L = resolve_leaf(J, x);
-          if (L == FIRST_OCUR) d = tree[x];
-        } else if (both) {
-          L = FIRST_OCUR;
-          d = hash_pair(kind, a, b);
-          tree[x] = d;
-          labels[x] = L;
-          if (listing) fl[atomicAdd(&fn, 1u)] = (uint32_t)x;
-        } else {
-          L = DONE;
-          labels[x] = L;
-        }
-        lab[idx] = L;
-        dig[idx] = d;
+    const TileLane T = tile_lane(tile_ref(J, g, t), N, i);
+    // Loads: labels of the bottom nodes and of the level-1 leaves ...
+    uint8_t raw[4], rawu[2], bl[4], ul[2];
+    bool vu[2];
+    tile_labels(J, g, T, C, raw, rawu, vu, DONE);
+#pragma unroll
+    for (int k = 0; k < 4; k++) bl[k] = g == 0 && T.vb[k] ? resolved_label(raw[k]) : raw[k];
+#pragma unroll
+    for (int k = 0; k < 2; k++) ul[k] = vu[k] ? resolved_label(rawu[k]) : (uint8_t)0xff;
+    // ... then the digests of the first occurrences among them.
+    Digest bd[4], ud[2];
+    tile_digests(tree, T, bl, ul, FIRST_OCUR, bd, ud);
+    // Levels 1 and 2 in registers.
+    uint8_t l1[2];
+    Digest d1[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      l1[k] = 0xff;
+      d1[k] = Digest{0, 0};
+      if (vu[k]) {
+        l1[k] = ul[k];
+        d1[k] = ud[k];
+      } else if (T.v1[k]) {
+        const bool both = bl[2 * k] == FIRST_OCUR && bl[2 * k + 1] == FIRST_OCUR;
+        l1[k] = both ? (uint8_t)FIRST_OCUR : (uint8_t)DONE;
+        if (both) d1[k] = hash_pair(kind, bd[2 * k], bd[2 * k + 1]);
       }
     }
-    __syncthreads();
-    const uint32_t cnt = fn;  // at most 255 interior nodes per tile
-    if (cnt) {
-      const uint32_t rl = (tbase + (uint32_t)t) % kPendSpread;
-      if (i == 0) fbase = atomicAdd(&fint_cnt[rl], cnt);
-      __syncthreads();
-      for (uint32_t q = i; q < cnt; q += kTileThreads) fint_list[rl * fint_cap + fbase + q] = fl[q];
+    uint8_t l2 = 0xff;
+    Digest d2{0, 0};
+    if (T.v2) {
+      const bool both = l1[0] == FIRST_OCUR && l1[1] == FIRST_OCUR;
+      l2 = both ? (uint8_t)FIRST_OCUR : (uint8_t)DONE;
+      if (both) d2 = hash_pair(kind, d1[0], d1[1]);
     }
-    __syncthreads();  // fl, fn and the level buffer are reused by the next tile
+    // Levels 3 .. H in LDS (level-2 results at positions nq .. 2 nq).
+    if (T.nq) {
+      if (T.v2) {
+        tlab[T.nq + i] = l2;
+        tdig[T.nq + i] = d2;
+      }
+      for (int st = 3; st <= T.H; st++) {
+        __syncthreads();
+        const uint32_t q0 = (1u << (T.H - st)) - 1, qc0 = (1u << (T.H - st + 1)) - 1;
+        if (i < (T.width >> st)) {
+          const uint32_t qc = qc0 + 2 * i;
+          const bool both = tlab[qc] == FIRST_OCUR && tlab[qc + 1] == FIRST_OCUR;
+          tlab[q0 + i] = both ? (uint8_t)FIRST_OCUR : (uint8_t)DONE;
+          tdig[q0 + i] = both ? hash_pair(kind, tdig[qc], tdig[qc + 1]) : Digest{0, 0};
+        }
+      }
+      __syncthreads();
+    }
+    uint8_t lq = 0xff;
+    Digest dq{0, 0};
+    uint64_t xq = 0;
+    if (i < T.nq) {
+      lq = tlab[i];
+      dq = tdig[i];
+      xq = tile_node(T.r, i);
+    }
+    // Positions of the first-occurrence interior nodes in the tile's fint
+    // region (global tile id tbase + t).
+    const bool f1a = T.v1[0] && !vu[0] && l1[0] == FIRST_OCUR, f1b = T.v1[1] && !vu[1] && l1[1] == FIRST_OCUR;
+    const bool f2 = T.v2 && l2 == FIRST_OCUR, fq = i < T.nq && lq == FIRST_OCUR;
+    uint64_t fpos = 0;
+    if (listing) {
+      const uint32_t cnt = (uint32_t)f1a + (uint32_t)f1b + (uint32_t)f2 + (uint32_t)fq;
+      const uint32_t incl = wave_incl_scan(cnt);
+      const uint64_t tid = tbase + t;
+      fpos = tid * kTileInterior + incl - cnt;
+      if (i == 63) fint_n[tid] = incl;  // the tile's total
+    }
+    // Stores.
+    if (g == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (T.vb[k] && (raw[k] & PENDING)) labels[T.xb + k] = bl[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const uint64_t x = T.x1 + k;
+      if (vu[k]) {
+        if (rawu[k] & PENDING) labels[x] = ul[k];
+      } else if (T.v1[k]) {
+        labels[x] = l1[k];
+        if (l1[k] == FIRST_OCUR) store_digest(tree + x, d1[k]);
+      }
+    }
+    if (T.v2) {
+      labels[T.x2] = l2;
+      if (f2) store_digest(tree + T.x2, d2);
+    }
+    if (i < T.nq) {
+      labels[xq] = lq;
+      if (fq) store_digest(tree + xq, dq);
+    }
+    if (listing) {
+      if (f1a) fint_list[fpos++] = (uint32_t)T.x1;
+      if (f1b) fint_list[fpos++] = (uint32_t)(T.x1 + 1);
+      if (f2) fint_list[fpos++] = (uint32_t)T.x2;
+      if (fq) fint_list[fpos] = (uint32_t)xq;
+    }
+    __syncthreads();  // the LDS levels are reused by the next tile
   }
 }
 
-// Pass 2, the duplicate forest (include/reference_impl.hpp:83-104).
-__global__ __launch_bounds__(kTileThreads) void k_pass2(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
+// Pass 2, the duplicate forest (include/reference_impl.hpp:83-104): a parent
+// of two shifted children is shifted when its digest is in the map; two
+// children with different labels make a DONE parent, equal labels carry up.
+// Lookups go level by level (a parent is looked up only when both children
+// turned out shifted); a lane's two level-1 lookups are in flight together.
+__device__ __forceinline__ uint8_t dup_label(uint8_t la, uint8_t lb, bool found) {
+  if (la != lb) return DONE;
+  if (la != SHIFT_DUPL) return la;  // both FIXED, both FIRST or both DONE
+  return found ? (uint8_t)SHIFT_DUPL : (uint8_t)DONE;
+}
+
+// Slot s of the map as {k0, k1} and {val, pad}.
+__device__ __forceinline__ void load_slot(const MapG& m, uint64_t s, uint4& key, uint4& val) {
+  const GlobalPtr<const uint8_t> p = (GlobalPtr<const uint8_t>)&m.slots[s];
+  key = gload16((GlobalPtr<const void>)p);
+  val = gload16((GlobalPtr<const void>)(p + 16));
+}
+
+// Lookups of up to NK keys (need[n] false: skipped, out[n] = ~0).  Each
+// probe round loads two consecutive slots of every key still searching
+// before comparing any of them, so a lookup that ends within two slots of
+// its home slot (most of them at the map's load factor, found or not) costs
+// one memory round trip.
+template <int NK>
+__device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, const bool* need, uint64_t* out) {
+  uint64_t s[NK];
+  bool todo[NK];
+#pragma unroll
+  for (int n = 0; n < NK; n++) {
+    out[n] = ~0ull;
+    todo[n] = need[n];
+    s[n] = map_start(m, key[n]) & m.mask;
+  }
+  for (uint64_t it = 0; it <= m.mask; it += 2) {
+    uint4 k[NK][2], v[NK][2];
+#pragma unroll
+    for (int n = 0; n < NK; n++) {
+      if (!todo[n]) continue;
+      load_slot(m, s[n], k[n][0], v[n][0]);
+      load_slot(m, (s[n] + 1) & m.mask, k[n][1], v[n][1]);
+    }
+    bool any = false;
+#pragma unroll
+    for (int n = 0; n < NK; n++) {
+      if (!todo[n]) continue;
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        if (!todo[n]) break;
+        const unsigned long long val = v[n][e].x | ((unsigned long long)v[n][e].y << 32);
+        const uint64_t k0 = k[n][e].x | ((uint64_t)k[n][e].y << 32), k1 = k[n][e].z | ((uint64_t)k[n][e].w << 32);
+        if (val == SLOT_EMPTY) {
+          todo[n] = false;
+        } else if (k0 == key[n].k0 && k1 == key[n].k1) {
+          out[n] = (s[n] + e) & m.mask;
+          todo[n] = false;
+        }
+      }
+      s[n] = (s[n] + 2) & m.mask;
+      any |= todo[n];
+    }
+    if (!any) break;
+  }
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_pass2(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
                                                         int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   if (g >= tile_groups(J.D)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
-  __shared__ uint8_t lab[256];
-  __shared__ Digest dig[256];
   __shared__ uint32_t pre[kPendSpread + 1];
+  __shared__ uint8_t tlab[kTopNodes + 64];
+  __shared__ Digest tdig[kTopNodes + 64];
+  __shared__ uint32_t tslot[kTopNodes];
   const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
   const uint64_t N = J.N, C = J.C;
-  uint8_t* const labels = J.labels;
-  Digest* const tree = J.tree;
-  uint32_t* const slot_of = J.slot_of;
+  const GlobalPtr<uint8_t> labels = J.labels;
+  const GlobalPtr<Digest> tree = J.tree;
+  const GlobalPtr<uint32_t> slot_of = J.slot_of;
   const int kind = J.hash_kind;
-  const MapView map = J.map;
+  const MapG map = J.map;
   for (uint32_t it = wg; it < nitems; it += nwg) {
     const uint64_t t = (g == 0 && !J.is_baseline) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
-    const TileRef tr = tile_ref(J, g, t);
-    const int H = tr.H;
-    const uint64_t r = tr.r;
-    const uint32_t width = 1u << H;
-    for (uint32_t idx = i; idx < width; idx += kTileThreads) {
-      const uint64_t x = ((r + 1) << H) - 1 + idx;
-      uint8_t L = 0xff;
-      if (x < N) L = child_tile_clean(J, g, x) ? (uint8_t)FIXED_DUPL : labels[x];
-      lab[idx] = L;
-      dig[idx] = L == SHIFT_DUPL ? tree[x] : Digest{0, 0};
+    const TileLane T = tile_lane(tile_ref(J, g, t), N, i);
+    // Loads: labels (leaves were resolved by pass 1), then the digests of
+    // the shifted ones.
+    uint8_t bl[4], ul[2];
+    bool vu[2];
+    tile_labels(J, g, T, C, bl, ul, vu, FIXED_DUPL);
+    Digest bd[4], ud[2];
+    tile_digests(tree, T, bl, ul, SHIFT_DUPL, bd, ud);
+    // Level 1: both lookups of the lane at once.
+    bool look[2];
+    Digest d1[2];
+    uint64_t f1[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      look[k] = !vu[k] && T.v1[k] && bl[2 * k] == SHIFT_DUPL && bl[2 * k + 1] == SHIFT_DUPL;
+      d1[k] = vu[k] ? ud[k] : look[k] ? hash_pair(kind, bd[2 * k], bd[2 * k + 1]) : Digest{0, 0};
     }
-    for (int st = 1; st <= H; st++) {
-      const uint32_t w = width >> st;
-      const uint64_t lvl = ((r + 1) << (H - st)) - 1;
-      for (uint32_t base = 0; base < w; base += kTileThreads) {
-        const uint32_t idx = base + i;
-        const bool act = idx < w;
-        const uint64_t x = lvl + idx;
-        const bool interior = act && x < C - 1;
-        __syncthreads();  // the previous writes are visible
-        const uint8_t la = interior ? lab[2 * idx] : 0, lb = interior ? lab[2 * idx + 1] : 0;
-        const bool both = interior && la == SHIFT_DUPL && lb == SHIFT_DUPL;
-        const Digest a = both ? dig[2 * idx] : Digest{0, 0};
-        const Digest b = both ? dig[2 * idx + 1] : Digest{0, 0};
-        __syncthreads();  // every child of this chunk is read before any parent is written
-        // Both-shifted parents hash their children; equal digests of
-        // neighbouring lanes share one lookup (runs of identical subtrees).
-        const Digest hd = both ? hash_pair(kind, a, b) : Digest{0, 0};
-        const unsigned long long p0 = __shfl_up(hd.k0, 1, 64), p1 = __shfl_up(hd.k1, 1, 64);
-        const int pboth = __shfl_up((int)both, 1, 64);
-        const bool head = !(both && i > 0 && pboth && p0 == hd.k0 && p1 == hd.k1);
-        unsigned long long found = ~0ull;
-        if (both && head) found = map_find(map, hd);
-        const unsigned long long heads = __ballot(head);
-        found = __shfl(found, 63 - __clzll(heads & ((2ull << i) - 1ull)), 64);
-        if (!act) continue;
-        uint8_t L;
-        Digest d{0, 0};
-        if (!interior) {
-          L = labels[x];
-          if (L == SHIFT_DUPL) d = tree[x];
-        } else {
-          if (la != lb) {
-            L = DONE;
-          } else if (both) {
-            d = hd;
-            const uint64_t s = found;
-            if (s != ~0ull) {
-              L = SHIFT_DUPL;
-              tree[x] = d;
-              slot_of[x] = (uint32_t)s;
-            } else {
-              L = DONE;
-              d = Digest{0, 0};
-            }
-          } else {
-            L = la;  // both FIXED, both FIRST or both DONE
+    map_find_n<2>(map, d1, look, f1);
+    uint8_t l1[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      l1[k] = vu[k] ? ul[k] : T.v1[k] ? dup_label(bl[2 * k], bl[2 * k + 1], f1[k] != ~0ull) : (uint8_t)0xff;
+      if (!vu[k] && l1[k] != SHIFT_DUPL) d1[k] = Digest{0, 0};
+    }
+    // Level 2.
+    uint8_t l2 = 0xff;
+    Digest d2{0, 0};
+    uint64_t f2 = ~0ull;
+    if (T.v2) {
+      const bool lk = l1[0] == SHIFT_DUPL && l1[1] == SHIFT_DUPL;
+      if (lk) d2 = hash_pair(kind, d1[0], d1[1]);
+      map_find_n<1>(map, &d2, &lk, &f2);
+      l2 = dup_label(l1[0], l1[1], f2 != ~0ull);
+      if (l2 != SHIFT_DUPL) d2 = Digest{0, 0};
+    }
+    // Levels 3 .. H in LDS (level-2 results at positions nq .. 2 nq).
+    if (T.nq) {
+      if (T.v2) {
+        tlab[T.nq + i] = l2;
+        tdig[T.nq + i] = d2;
+      }
+      for (int st = 3; st <= T.H; st++) {
+        __syncthreads();
+        const uint32_t q0 = (1u << (T.H - st)) - 1, qc0 = (1u << (T.H - st + 1)) - 1;
+        if (i < (T.width >> st)) {
+          const uint32_t qc = qc0 + 2 * i;
+          const uint8_t la = tlab[qc], lb = tlab[qc + 1];
+          const bool lk = la == SHIFT_DUPL && lb == SHIFT_DUPL;
+          Digest d{0, 0};
+          uint64_t f = ~0ull;
+          if (lk) {
+            d = hash_pair(kind, tdig[qc], tdig[qc + 1]);
+            map_find_n<1>(map, &d, &lk, &f);
           }
-          labels[x] = L;
+          const uint8_t L = dup_label(la, lb, f != ~0ull);
+          tlab[q0 + i] = L;
+          tdig[q0 + i] = L == SHIFT_DUPL ? d : Digest{0, 0};
+          tslot[q0 + i] = (uint32_t)f;
         }
-        lab[idx] = L;
-        dig[idx] = d;
+      }
+      __syncthreads();
+    }
+    // Stores: labels of the interior nodes, digest and map slot of the
+    // shifted ones.
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      if (T.v1[k] && !vu[k]) {
+        const uint64_t x = T.x1 + k;
+        labels[x] = l1[k];
+        if (l1[k] == SHIFT_DUPL) {
+          store_digest(tree + x, d1[k]);
+          slot_of[x] = (uint32_t)f1[k];
+        }
       }
     }
-    __syncthreads();  // the level buffer is reused by the next tile
+    if (T.v2) {
+      labels[T.x2] = l2;
+      if (l2 == SHIFT_DUPL) {
+        store_digest(tree + T.x2, d2);
+        slot_of[T.x2] = (uint32_t)f2;
+      }
+    }
+    if (i < T.nq) {
+      const uint64_t x = tile_node(T.r, i);
+      const uint8_t L = tlab[i];
+      labels[x] = L;
+      if (L == SHIFT_DUPL) {
+        store_digest(tree + x, tdig[i]);
+        slot_of[x] = tslot[i];
+      }
+    }
+    __syncthreads();  // the LDS levels are reused by the next tile
   }
 }
 
@@ -1019,7 +1246,7 @@ __device__ __forceinline__ uint64_t leaf_count(const TreeGeom& g, uint64_t v) {
 
 // Node blocks of kNodesPerBlock ids; a workgroup owns a contiguous run of
 // them so that its counts are contiguous in node order.
-__device__ __forceinline__ void node_block_range(const JobDev& J, uint32_t wg, uint32_t nwg, uint32_t& b0,
+__device__ __forceinline__ void node_block_range(const JobG& J, uint32_t wg, uint32_t nwg, uint32_t& b0,
                                                  uint32_t& b1) {
   const uint32_t per = (J.nblk + nwg - 1) / nwg;
   b0 = min(J.nblk, wg * per);
@@ -1047,19 +1274,18 @@ __device__ __forceinline__ uint8_t byte_of(const uint4 w, int t) {
 // FIXED (tile reset after every checkpoint), so most groups stop at the
 // first load.  The labels array is padded past N.
 // The 16 labels at v0, or FIXED bytes (no candidate) past the last node.
-__device__ __forceinline__ uint4 load16(const JobDev& J, uint64_t v0) {
-  return v0 < J.N ? *reinterpret_cast<const uint4*>(J.labels + v0)
+__device__ __forceinline__ uint4 load16(const JobG& J, uint64_t v0) {
+  return v0 < J.N ? gload16((GlobalPtr<const void>)(J.labels + v0))
                   : make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
 }
 
-__device__ __forceinline__ void scan16(const JobDev& J, uint64_t v0, const uint4 w, uint32_t& fmask, uint32_t& smask) {
+__device__ __forceinline__ void scan16(const JobG& J, uint64_t v0, const uint4 w, uint32_t& fmask, uint32_t& smask) {
   fmask = smask = 0;
   if (v0 >= J.N || !any_root_candidate(w)) return;
   const uint64_t h = v0 >> 1;
-  const uint2 lo = v0 >= 16 ? *reinterpret_cast<const uint2*>(J.labels + h - 8) : make_uint2(0, 0);
-  const uint2 hi = *reinterpret_cast<const uint2*>(J.labels + h);
-  const uint64_t pw = (uint64_t)hi.x | ((uint64_t)hi.y << 32);  // parent bytes v0/2 .. v0/2+7
-  const uint8_t p_first = (uint8_t)(lo.y >> 24);                 // parent byte v0/2 - 1
+  const uint64_t lo = v0 >= 16 ? *(GlobalPtr<const uint64_t>)(J.labels + h - 8) : 0;
+  const uint64_t pw = *(GlobalPtr<const uint64_t>)(J.labels + h);  // parent bytes v0/2 .. v0/2+7
+  const uint8_t p_first = (uint8_t)(lo >> 56);                      // parent byte v0/2 - 1
 #pragma unroll
   for (int t = 0; t < 16; t++) {
     const uint64_t v = v0 + t;
@@ -1092,9 +1318,9 @@ __device__ __forceinline__ uint32_t first_chunks(const TreeGeom& g, uint64_t v0,
   return fc;
 }
 
-__global__ __launch_bounds__(256) void k_roots_count(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_roots_count(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   uint32_t b0, b1;
@@ -1125,8 +1351,8 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobDev* __restrict__ 
 // 1024 * kScanPer workgroups per job), so one block scan per array suffices.
 constexpr uint32_t kScanPer = 4;
 static_assert(kScanPer * 1024 >= kMaxStageWG, "roots grid larger than one scan workgroup covers");
-__global__ __launch_bounds__(1024) void k_roots_scan(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix) {
-  const JobDev& J = jobs[blockIdx.x];
+__global__ __launch_bounds__(1024) void k_roots_scan(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix) {
+  const JobG& J = jobs[blockIdx.x];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t nwg = prefix[blockIdx.x + 1] - prefix[blockIdx.x];
   __shared__ uint32_t sh[17];
@@ -1166,9 +1392,9 @@ __global__ __launch_bounds__(1024) void k_roots_scan(const JobDev* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(256) void k_roots_write(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_roots_write(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   uint32_t b0, b1;
@@ -1226,7 +1452,7 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobDev* __restrict__ 
 // reference uses Kokkos::BinSort, src/tree_approach.cpp:670-687).
 // s_hist is bin-major: entry (bin, tile) at bin * ntiles + tile.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t stiles_of(const JobDev& J) { return (J.ctr->nshift + kSTile - 1) / kSTile; }
+__device__ __forceinline__ uint32_t stiles_of(const JobG& J) { return (J.ctr->nshift + kSTile - 1) / kSTile; }
 
 // Label hygiene, run by k_gather once the roots are written: the interior labels (tile
 // root included) of every group-0 tile visited by this checkpoint are reset to
@@ -1234,7 +1460,7 @@ __device__ __forceinline__ uint32_t stiles_of(const JobDev& J) { return (J.ctr->
 // A later checkpoint that does not visit the tile then reads FIXED labels
 // there, so the root scan needs no per-tile check.  Leaf labels are rewritten
 // by every leaf kernel.
-__device__ __forceinline__ void reset_visited_tiles(const JobDev& J, uint32_t wg, uint32_t nwg) {
+__device__ __forceinline__ void reset_visited_tiles(const JobG& J, uint32_t wg, uint32_t nwg) {
   const uint32_t Lr0 = J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0;
   const uint32_t ntiles0 = 1u << Lr0;
   const uint32_t H = J.D - Lr0;
@@ -1254,9 +1480,9 @@ __device__ __forceinline__ void reset_visited_tiles(const JobDev& J, uint32_t wg
 }
 
 // Workgroups stride over the sort tiles (kSTile shifted roots each).
-__global__ __launch_bounds__(256) void k_sbin_count(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_sbin_count(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   const uint32_t nt = stiles_of(J);
@@ -1274,7 +1500,7 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobDev* __restrict__ j
     if (todo) {
       bn = J.s_tree[e];
       if (bn >= nbins) {
-        atomicOr(&J.ctr->flags, FLAG_BAD_SOURCE);
+        g_or(&J.ctr->flags, FLAG_BAD_SOURCE);
         bn = nbins - 1;
       }
     }
@@ -1286,7 +1512,7 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobDev* __restrict__ j
       const uint32_t leader = __ffsll((long long)act) - 1;
       const uint32_t lb = __shfl(bn, leader, 64);
       const unsigned long long same = __ballot(todo && bn == lb);
-      if (lane_id() == leader) atomicAdd(&J.s_hist[(uint64_t)lb * nt + t], (uint32_t)__popcll(same));
+      if (lane_id() == leader) g_add(&J.s_hist[(uint64_t)lb * nt + t], (uint32_t)__popcll(same));
       if (todo && bn == lb) todo = false;
     }
   }
@@ -1295,16 +1521,22 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobDev* __restrict__ j
 
 // Copies the final counters into host-mapped memory (read by the host once
 // the stream is synchronized; no copy command needed).
-__device__ __forceinline__ void export_counters(const JobDev& J) {
-  const JobCounters c = *J.ctr;
-  *J.host_ctr = c;
+__device__ __forceinline__ void export_counters(const JobG& J) {
+  static_assert(sizeof(JobCounters) % 8 == 0, "counters copied as 8-byte words");
+  const GlobalPtr<const uint64_t> src = (GlobalPtr<const uint64_t>)J.ctr;
+  const GlobalPtr<uint64_t> dst = (GlobalPtr<uint64_t>)J.host_ctr;
+  uint64_t w[sizeof(JobCounters) / 8];
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(JobCounters) / 8; i++) w[i] = src[i];
+#pragma unroll
+  for (uint32_t i = 0; i < sizeof(JobCounters) / 8; i++) dst[i] = w[i];
 }
 
 // One workgroup per job: new map entries of the checkpoint, scan of the
 // shifted-root histogram, the prior table, the diff length; then exports the
 // counters to the host.
-__global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* __restrict__ jobs) {
-  const JobDev& J = jobs[blockIdx.x];
+__global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ jobs) {
+  const JobG& J = jobs[blockIdx.x];
   __shared__ uint32_t sh[17];
   __shared__ unsigned long long shl[16];
   unsigned long long claims = 0;
@@ -1374,19 +1606,19 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobDev* __restrict__ j
 // which then only touches LDS and issues stores.
 constexpr uint32_t kLdsBins = 8192;
 constexpr uint32_t kSChunks = kSTile / 64;
-__global__ __launch_bounds__(64) void k_sbin_scatter(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(64) void k_sbin_scatter(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   const uint32_t nt = stiles_of(J);
   const uint32_t nbins = J.ckpt_id + 1;
   const uint32_t S = J.ctr->nshift;
-  const uint32_t* const s_tree = J.s_tree;
-  const uint32_t* const s_node = J.s_node;
-  const uint32_t* const s_prev = J.s_prev;
-  uint32_t* const ss_node = J.ss_node;
-  uint32_t* const ss_prev = J.ss_prev;
+  const GlobalPtr<const uint32_t> s_tree = J.s_tree;
+  const GlobalPtr<const uint32_t> s_node = J.s_node;
+  const GlobalPtr<const uint32_t> s_prev = J.s_prev;
+  const GlobalPtr<uint32_t> ss_node = J.ss_node;
+  const GlobalPtr<uint32_t> ss_prev = J.ss_prev;
   __shared__ uint32_t base[kLdsBins];
   for (uint32_t t = wg; t < nt; t += nwg) {
     const uint32_t e0 = t * kSTile, e1 = min(S, (t + 1) * kSTile);
@@ -1436,12 +1668,12 @@ __global__ __launch_bounds__(64) void k_sbin_scatter(const JobDev* __restrict__ 
 // Diff metadata: header_t, first-occurrence roots, prior table, shifted
 // entries (src/tree_approach.cpp:593,622-706).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_write_meta(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_write_meta(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   const uint32_t F = J.ctr->nfirst, P = J.ctr->nprior, S = J.ctr->nshift;
-  uint8_t* out = J.diff;
+  GlobalPtr<uint8_t> out = J.diff;
   const uint32_t tid = (blockIdx.x - prefix[j]) * blockDim.x + threadIdx.x;
   const uint32_t nth = (prefix[j + 1] - prefix[j]) * blockDim.x;
   if (tid == 0) {
@@ -1454,9 +1686,9 @@ __global__ __launch_bounds__(256) void k_write_meta(const JobDev* __restrict__ j
     store_u32(out + 24, P);
     store_u32(out + 28, S);
   }
-  uint8_t* pf = out + 32;
-  uint8_t* pp = pf + 4ull * F;
-  uint8_t* ps = pp + 8ull * P;
+  const GlobalPtr<uint8_t> pf = out + 32;
+  const GlobalPtr<uint8_t> pp = pf + 4ull * F;
+  const GlobalPtr<uint8_t> ps = pp + 8ull * P;
   const uint32_t bias = J.node_bias;  // list diffs name chunks, not leaves
   for (uint32_t i = tid; i < F; i += nth) store_u32(pf + 4ull * i, J.f_list[i] - bias);
   for (uint32_t i = tid; i < P; i += nth) {
@@ -1475,7 +1707,8 @@ __global__ __launch_bounds__(256) void k_write_meta(const JobDev* __restrict__ j
 // left-to-right order.  Copies 4-byte words (the data section is only 4-byte
 // aligned: it follows 4-byte first-occurrence ids).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t region_of(const uint32_t* f_off, uint32_t F, uint64_t q) {
+template <class Ptr>  // a global pointer (gather) or a generic one (restart)
+__device__ __forceinline__ uint32_t region_of(Ptr f_off, uint32_t F, uint64_t q) {
   uint32_t lo = 0, hi = F;  // f_off[lo] <= q < f_off[hi] (f_off[F] = total)
   while (hi - lo > 1) {
     uint32_t mid = (lo + hi) >> 1;
@@ -1490,11 +1723,11 @@ typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 // Copies `n` bytes of chunk data from `src` (16-byte aligned when Vec) to the
 // 4-byte aligned `dst`, zero-filling the bytes at and past `avail`.
 template <bool Vec>
-__device__ __forceinline__ void copy_piece(uint8_t* dst, const uint8_t* src, uint64_t avail) {
+__device__ __forceinline__ void copy_piece(GlobalPtr<uint8_t> dst, GlobalPtr<const uint8_t> src, uint64_t avail) {
   if (Vec) {
     if (avail >= 16) {
-      const uint4 v = *reinterpret_cast<const uint4*>(src);
-      *reinterpret_cast<u32x4a4*>(dst) = u32x4a4{v.x, v.y, v.z, v.w};
+      const uint4 v = gload16((GlobalPtr<const void>)src);
+      *(GlobalPtr<u32x4a4>)dst = u32x4a4{v.x, v.y, v.z, v.w};
       return;
     }
     uint32_t w[4];
@@ -1508,7 +1741,7 @@ __device__ __forceinline__ void copy_piece(uint8_t* dst, const uint8_t* src, uin
       }
       w[k] = x;
     }
-    *reinterpret_cast<u32x4a4*>(dst) = u32x4a4{w[0], w[1], w[2], w[3]};
+    *(GlobalPtr<u32x4a4>)dst = u32x4a4{w[0], w[1], w[2], w[3]};
   } else {
     *dst = avail > 0 ? *src : 0;
   }
@@ -1518,9 +1751,9 @@ __device__ __forceinline__ void copy_piece(uint8_t* dst, const uint8_t* src, uin
 // workgroup copies the group with 16-byte pieces (1-byte pieces when the
 // chunk size is not a multiple of 16 or the input is not 16-byte aligned).
 template <bool Vec>
-__global__ __launch_bounds__(256) void k_gather(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+__global__ __launch_bounds__(256) void k_gather(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobDev& J = jobs[j];
+  const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   // The roots are written: reset the labels of the tiles this checkpoint
   // visited (gather runs a large grid whatever the diff size).
@@ -1530,7 +1763,7 @@ __global__ __launch_bounds__(256) void k_gather(const JobDev* __restrict__ jobs,
   if (Dq == 0) return;
   const TreeGeom g = make_geom(J.C);
   const uint64_t cs = J.chunk_size;
-  uint8_t* dst_base = J.diff + 32 + 4ull * F + 8ull * P + 8ull * S;
+  GlobalPtr<uint8_t> dst_base = J.diff + 32 + 4ull * F + 8ull * P + 8ull * S;
   __shared__ uint64_t src[256];
   const uint32_t nblk = prefix[j + 1] - prefix[j];
   const uint64_t ngroups = (Dq + 255) / 256;
@@ -1720,7 +1953,12 @@ __global__ void k_restart_check(const uint64_t* ref, uint64_t C, JobCounters* ct
 // ---------------------------------------------------------------------------
 #define DD_LAUNCH(kern, grid, block, lds, s, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, s, __VA_ARGS__)
 
-void launch_begin(const JobDev* jobs, const uint32_t*, uint32_t njobs, bool rerun, hipStream_t s) {
+// The kernels read the host's job table through global-address-space
+// pointers (same layout).
+static const JobG* gview(const JobDev* j) { return reinterpret_cast<const JobG*>(j); }
+
+void launch_begin(const JobDev* jobs_h, const uint32_t*, uint32_t njobs, bool rerun, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   DD_LAUNCH(k_begin, (njobs + 63) / 64, 64, 0, s, jobs, njobs, rerun ? 1 : 0);
 }
 
@@ -1735,8 +1973,9 @@ void launch_begin(const JobDev* jobs, const uint32_t*, uint32_t njobs, bool reru
     return;                                                                                            \
   } while (0)
 
-void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping, bool al16,
+void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping, bool al16,
                  int kind, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+  const JobG* jobs = gview(jobs_h);
   if (grid == 0) return;
   if (!al16) mapping = LEAF_DIRECT;  // the other mappings need 16-byte aligned buffers
   if (kind == HASH_MD5) {
@@ -1764,46 +2003,61 @@ void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uin
 }
 #undef DD_LEAF_LAUNCH
 
-void launch_finish(const JobDev* jobs, uint32_t njobs, hipStream_t s) {
+void launch_finish(const JobDev* jobs_h, uint32_t njobs, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   DD_LAUNCH(k_finish, (njobs + 63) / 64, 64, 0, s, jobs, njobs);
 }
 
-void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_insert_leaves, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_list_resolve(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_list_resolve(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_list_resolve, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
+void launch_pass1(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_pass1, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
 }
-void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_insert_interior(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_insert_interior, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
+void launch_pass2(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_pass2, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
 }
-void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_roots_count(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_roots_count, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_roots_scan(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, hipStream_t s) {
+void launch_roots_scan(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   DD_LAUNCH(k_roots_scan, njobs, 1024, 0, s, jobs, prefix);
 }
-void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_roots_write(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_roots_write, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_sbin_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_sbin_count(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_sbin_count, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_sbin_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s) { DD_LAUNCH(k_sbin_scan, njobs, 1024, 0, s, jobs); }
-void launch_sbin_scatter(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_sbin_scan(const JobDev* jobs_h, uint32_t njobs, hipStream_t s) {
+  DD_LAUNCH(k_sbin_scan, njobs, 1024, 0, s, gview(jobs_h));
+}
+void launch_sbin_scatter(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_sbin_scatter, grid, 64, 0, s, jobs, prefix, njobs);
 }
-void launch_write_meta(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_write_meta(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_write_meta, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_gather(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool vec16,
+void launch_gather(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool vec16,
                    hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
   if (!grid) return;
   if (vec16) DD_LAUNCH(k_gather<true>, grid, 256, 0, s, jobs, prefix, njobs);
   else DD_LAUNCH(k_gather<false>, grid, 256, 0, s, jobs, prefix, njobs);

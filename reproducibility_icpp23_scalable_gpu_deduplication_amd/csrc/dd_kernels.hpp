@@ -19,6 +19,7 @@ constexpr int kNodesPerBlock = 4096;    // root scan: 256 threads x 16 nodes
 constexpr int kSTile = 1024;            // shifted-root sort tile
 constexpr uint32_t kMaxStageWG = 4096;  // workgroups per job of a grid-stride stage
 constexpr uint32_t kMaxTileWG = 8192;   // one-wavefront workgroups per job of a tile pass
+constexpr uint32_t kTileInterior = 255;  // interior nodes of a tile of 8 levels
 
 // Per-job counters written by the device and read back by the host.
 struct JobCounters {
@@ -34,9 +35,12 @@ struct JobCounters {
   uint32_t pad;
 };
 
-struct JobDev {
+// One job.  The host fills it as JobDev (plain pointers); kernels read the
+// same bytes as JobG (global-address-space pointers, see dd_device.hpp).
+template <template <class> class P>
+struct JobT {
   // input
-  const uint8_t* data;
+  P<const uint8_t> data;
   uint64_t len;
   uint32_t chunk_size;
   int32_t hash_kind;
@@ -52,51 +56,56 @@ struct JobDev {
   uint32_t list_mode;
   uint32_t node_bias;
   // persistent state
-  Digest* tree;        // N digests, heap order
-  uint8_t* labels;     // N labels (padded for 16-byte reads)
-  uint32_t* slot_of;   // N: map slot of pending leaves and shifted interior nodes
-  MapView map;
+  P<Digest> tree;        // N digests, heap order
+  P<uint8_t> labels;     // N labels (padded for 16-byte reads)
+  P<uint32_t> slot_of;   // N: map slot of pending leaves and shifted interior nodes
+  MapT<P> map;
   uint64_t map_count_at_start;  // entries before this checkpoint
   uint64_t map_limit;           // max entries the map may hold
-  JobCounters* ctr;             // device counters
-  JobCounters* host_ctr;        // host-mapped copy written at the end of the pipeline
+  P<JobCounters> ctr;       // device counters
+  P<JobCounters> host_ctr;  // host-mapped copy written at the end of the pipeline
   // Work lists.  Each is split into 32 sub-lists (picked by the producing
   // leaf block or tile index modulo 32) with one counter each, so no single
   // counter is hit by every workgroup; consumers prefix-sum the 32 counts.
-  uint32_t* pend_cnt;    // 32 counters
-  uint32_t* pend_list;   // 32 x pend_cap: chunk indices of the leaves that need a map insert
+  P<uint32_t> pend_cnt;    // 32 counters
+  P<uint32_t> pend_list;   // 32 x pend_cap: chunk indices of the leaves that need a map insert
   uint32_t pend_cap;
-  uint32_t* fint_cnt;    // 32 counters
-  uint32_t* fint_list;   // 32 x fint_cap: first-occurrence interior nodes (pass 1)
-  uint32_t fint_cap;
+  // First-occurrence interior nodes found by pass 1, per tile (global tile
+  // id T over all groups): fint_n[T] entries at fint_list[T * kTileInterior].
+  // Pass 1 writes fint_n for every tile it visits, so no clearing is needed.
+  P<uint32_t> fint_list;
+  P<uint32_t> fint_n;
   uint32_t epoch;        // stamp of this checkpoint (never 0)
-  uint32_t* tile_epoch;  // per group-0 tile: epoch of the last checkpoint that changed one of its leaves
-  uint32_t* dirty_cnt;   // 32 counters
-  uint32_t* dirty_list;  // 32 x dirty_cap: group-0 tiles with a changed leaf
+  P<uint32_t> tile_epoch;  // per group-0 tile: epoch of the last checkpoint that changed one of its leaves
+  P<uint32_t> dirty_cnt;   // 32 counters
+  P<uint32_t> dirty_list;  // 32 x dirty_cap: group-0 tiles with a changed leaf
   uint32_t dirty_cap;
   uint32_t tile_base[4];  // first tile index of each group
   uint32_t ntiles;        // tiles over all groups
-  uint32_t* blk_claim;    // per insert_leaves workgroup: new map entries
-  uint32_t* int_claim;    // per insert_interior workgroup: new map entries
+  P<uint32_t> blk_claim;    // per insert_leaves workgroup: new map entries
+  P<uint32_t> int_claim;    // per insert_interior workgroup: new map entries
   uint32_t nwg_ins_leaf, nwg_ins_int;  // workgroups of the two insert stages
   // root scan
-  uint32_t* blk_f;    // per root-scan workgroup: first roots (then their exclusive prefix)
-  uint32_t* blk_fc;   // per root-scan workgroup: chunks under first roots (prefix)
-  uint32_t* blk_s;    // per root-scan workgroup: shifted roots (prefix)
+  P<uint32_t> blk_f;    // per root-scan workgroup: first roots (then their exclusive prefix)
+  P<uint32_t> blk_fc;   // per root-scan workgroup: chunks under first roots (prefix)
+  P<uint32_t> blk_s;    // per root-scan workgroup: shifted roots (prefix)
   uint32_t nblk;      // node blocks = ceil(N / kNodesPerBlock)
-  uint32_t* f_list;   // first roots, ascending
-  uint32_t* f_off;    // chunk offset of each first region in the data section
-  uint32_t* s_node;   // shifted roots, ascending
-  uint32_t* s_prev;   // source node of each shifted root
-  uint32_t* s_tree;   // source checkpoint of each shifted root
-  uint32_t* ss_node;  // shifted roots grouped by source checkpoint
-  uint32_t* ss_prev;
-  uint32_t* s_hist;   // (ckpt_id+1) x ceil(C / kSTile) histogram / bases
-  uint32_t* ptab;     // prior table {ckpt, count} x (ckpt_id+1)
+  P<uint32_t> f_list;   // first roots, ascending
+  P<uint32_t> f_off;    // chunk offset of each first region in the data section
+  P<uint32_t> s_node;   // shifted roots, ascending
+  P<uint32_t> s_prev;   // source node of each shifted root
+  P<uint32_t> s_tree;   // source checkpoint of each shifted root
+  P<uint32_t> ss_node;  // shifted roots grouped by source checkpoint
+  P<uint32_t> ss_prev;
+  P<uint32_t> s_hist;   // (ckpt_id+1) x ceil(C / kSTile) histogram / bases
+  P<uint32_t> ptab;     // prior table {ckpt, count} x (ckpt_id+1)
   // output
-  uint8_t* diff;
+  P<uint8_t> diff;
   uint64_t diff_cap;
 };
+using JobDev = JobT<HostPtr>;
+using JobG = JobT<GlobalPtr>;
+static_assert(sizeof(JobDev) == sizeof(JobG), "one layout for both pointer kinds");
 
 // Restart descriptors -------------------------------------------------------
 struct RestartCkpt {

@@ -122,8 +122,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   t.labels = labels.as<uint8_t>();
   t.ctr = ctr.as<JobCounters>();
   t.pend_cnt = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
-  t.fint_cnt = t.pend_cnt + 32;
-  t.dirty_cnt = t.fint_cnt + 32;
+  t.dirty_cnt = t.pend_cnt + 64;
   t.pend_list = plist.as<uint32_t>();
   t.pend_cap = (uint32_t)pend_cap;
   // One workgroup prefix {0, grid} per mapping: the grid-stride mappings
