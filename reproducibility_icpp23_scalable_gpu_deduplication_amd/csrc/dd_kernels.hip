@@ -534,51 +534,93 @@ __device__ __forceinline__ InsertCtx insert_ctx(const JobG& J) {
   return InsertCtx{J.map, J.tree, J.C, J.ckpt_id, J.is_baseline != 0};
 }
 
+// Slot s of the map as {k0, k1} and {val, pad}.
+__device__ __forceinline__ void load_slot(const MapG& m, uint64_t s, uint4& key, uint4& val) {
+  const GlobalPtr<const uint8_t> p = (GlobalPtr<const uint8_t>)&m.slots[s];
+  key = gload16((GlobalPtr<const void>)p);
+  val = gload16((GlobalPtr<const void>)(p + 16));
+}
+
+// Settles an insert at slot s whose value is v (not EMPTY): the entry's key
+// comes from the tree when a lane of this kernel may have claimed it (its
+// slot key may not be published yet), else from the slot (`slot_key` when it
+// was loaded with the value, `have_key`, else loaded here).  Equal keys:
+// lowers (leaves) or raises (interior nodes) an entry of this checkpoint and
+// reports it found.
+template <bool Leaf>
+__device__ __forceinline__ bool map_settle(const InsertCtx& J, const Digest& k, uint64_t s, unsigned long long v,
+                                           const Digest& slot_key, bool have_key, unsigned long long mine,
+                                           InsertResult& r) {
+  const uint32_t vt = (uint32_t)(v >> 32), vn = (uint32_t)v;
+  const bool this_kernel = vt == J.ckpt && (Leaf || vn < J.C - 1);
+  Digest e;
+  if (this_kernel) {
+    e = load_digest(J.tree + vn);
+  } else if (have_key) {
+    e = slot_key;
+  } else {
+    e.k0 = J.m.slots[s].k0;
+    e.k1 = J.m.slots[s].k1;
+  }
+  if (!digest_eq(e, k)) return false;
+  unsigned long long prev = v;
+  if (vt == J.ckpt) prev = Leaf ? g_min(&J.m.slots[s].val, mine) : g_max(&J.m.slots[s].val, mine);
+  r = InsertResult{true, false, s, prev};
+  return true;
+}
+
+// Claims the EMPTY slot s for `mine`; on success publishes the key.  Returns
+// the slot's value before the CAS.
+__device__ __forceinline__ unsigned long long map_claim(const MapG& m, uint64_t s, const Digest& k,
+                                                        unsigned long long mine) {
+  const GlobalPtr<Slot> sl = &m.slots[s];
+  const unsigned long long old = g_cas(&sl->val, SLOT_EMPTY, mine);
+  if (old == SLOT_EMPTY) {
+    sl->k0 = k.k0;
+    sl->k1 = k.k1;
+  }
+  return old;
+}
+
+// Linear probing; every round loads two consecutive slots (keys and values)
+// before settling either, so an insert that ends within two slots of where
+// the round starts costs one load round trip plus its atomic.  A slot that
+// was EMPTY when loaded is claimed by CAS; a failed CAS returns the value of
+// the lane that won, settled like a loaded one.  In a baseline the map
+// starts empty, so the home slot is claimed by CAS without a load first.
 template <bool Leaf>
 __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, uint32_t node) {
   const MapG& m = J.m;
   const unsigned long long mine = pack_node(J.ckpt, node);
   uint64_t s = map_start(m, k) & m.mask;
-  // In a baseline the map starts empty, so the first probe goes straight to
-  // the CAS (a failed CAS returns the slot like a load would).
-  const bool cas_first = J.cas_first;
-  for (uint64_t it = 0; it <= m.mask; it++) {
-    const GlobalPtr<Slot> sl = &m.slots[s];
-    unsigned long long v = (cas_first && it == 0)
-                               ? SLOT_EMPTY
-                               : __hip_atomic_load(&sl->val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (v == SLOT_EMPTY) {
-      const unsigned long long old = g_cas(&sl->val, SLOT_EMPTY, mine);
-      if (old == SLOT_EMPTY) {
-        sl->k0 = k.k0;
-        sl->k1 = k.k1;
-        return InsertResult{true, true, s, SLOT_EMPTY};
-      }
-      v = old;
-    }
-    const uint32_t vt = (uint32_t)(v >> 32), vn = (uint32_t)v;
-    const bool this_kernel = vt == J.ckpt && (Leaf || vn < J.C - 1);
-    Digest e;
-    if (this_kernel) {
-      e = load_digest(J.tree + vn);
-    } else {
-      e.k0 = sl->k0;
-      e.k1 = sl->k1;
-    }
-    if (digest_eq(e, k)) {
-      unsigned long long prev = v;
-      if (vt == J.ckpt) prev = Leaf ? g_min(&sl->val, mine) : g_max(&sl->val, mine);
-      return InsertResult{true, false, s, prev};
-    }
+  InsertResult r;
+  const Digest none{0, 0};
+  if (J.cas_first) {
+    const unsigned long long old = map_claim(m, s, k, mine);
+    if (old == SLOT_EMPTY) return InsertResult{true, true, s, SLOT_EMPTY};
+    if (map_settle<Leaf>(J, k, s, old, none, false, mine, r)) return r;
     s = (s + 1) & m.mask;
   }
+  for (uint64_t it = 0; it <= m.mask; it += 2) {
+    uint4 kk[2], vv[2];
+    load_slot(m, s, kk[0], vv[0]);
+    load_slot(m, (s + 1) & m.mask, kk[1], vv[1]);
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const uint64_t se = (s + e) & m.mask;
+      unsigned long long v = vv[e].x | ((unsigned long long)vv[e].y << 32);
+      const Digest key{kk[e].x | ((uint64_t)kk[e].y << 32), kk[e].z | ((uint64_t)kk[e].w << 32)};
+      bool have_key = true;
+      if (v == SLOT_EMPTY) {
+        v = map_claim(m, se, k, mine);
+        if (v == SLOT_EMPTY) return InsertResult{true, true, se, SLOT_EMPTY};
+        have_key = false;  // the loaded key predates the claim
+      }
+      if (map_settle<Leaf>(J, k, se, v, key, have_key, mine, r)) return r;
+    }
+    s = (s + 2) & m.mask;
+  }
   return InsertResult{false, false, 0, 0};
-}
-
-__device__ __forceinline__ bool map_guard_tripped(const JobG& J) {
-  // Worst case of this checkpoint: one entry per changed leaf plus one per
-  // first-occurrence interior node (fewer than the changed leaves).
-  return J.map_count_at_start + 2 * pending_total(J) > J.map_limit;
 }
 
 // One lane per entry of the pending-leaf list (grid-stride).  Labels settle
@@ -603,26 +645,30 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
   uint32_t claims = 0;
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
-  // Other workgroups of this kernel may raise MAP_FULL while it runs: the
-  // decision is made uniform over the workgroup before any barrier.
-  if (__syncthreads_or((J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0)) {
+  __shared__ uint32_t sflags;
+  // A baseline inserts every leaf: lane k takes chunk k directly (the list
+  // would hold every chunk too).
+  const bool dense = J.is_baseline != 0;
+  // The flags and the pending-list counts in one round trip.  Other
+  // workgroups of this kernel may raise MAP_FULL while it runs: the decision
+  // is made once, uniform over the workgroup.
+  if (threadIdx.x == 0) sflags = J.ctr->flags;
+  if (!dense) load_list_prefix(J.pend_cnt, pre);
+  else __syncthreads();
+  if (sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) {
     if (threadIdx.x == 0) J.blk_claim[wg] = 0;
     return;
   }
-  if (map_guard_tripped(J)) {
+  // Worst case of this checkpoint: one entry per changed leaf plus one per
+  // first-occurrence interior node (fewer than the changed leaves).  A
+  // baseline keeps no list and starts from an empty map.
+  const uint32_t nitems = dense ? (uint32_t)J.C : pre[kPendSpread];
+  if (J.map_count_at_start + 2ull * (dense ? 0u : nitems) > J.map_limit) {
     if (threadIdx.x == 0) {
       g_or(&J.ctr->flags, FLAG_NEED_GROW);
       J.blk_claim[wg] = 0;
     }
     return;
-  }
-  // A baseline inserts every leaf: lane k takes chunk k directly (the list
-  // would hold every chunk too).
-  const bool dense = J.is_baseline != 0;
-  uint32_t nitems = (uint32_t)J.C;
-  if (!dense) {
-    load_list_prefix(J.pend_cnt, pre);
-    nitems = pre[kPendSpread];
   }
   const uint32_t K = J.ckpt_id;
   const InsertCtx ic = insert_ctx(J);
@@ -668,7 +714,12 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
         if (pt == K) {
           if (pn > leaf) {
             self_shift = false;  // this leaf is now the lowest; the previous lowest is not
-            if (labels[pn] == PENDING) labels[pn] = SHIFT_DUPL;
+            // Outside a baseline the previous lowest leaf (a changed leaf of
+            // this checkpoint) is PENDING or already SHIFT, so it is demoted
+            // without reading it; a baseline's may be FIXED|PENDING, which
+            // stays.
+            if (!dense) labels[pn] = SHIFT_DUPL;
+            else if (labels[pn] == PENDING) labels[pn] = SHIFT_DUPL;
           } else if (pn == leaf) {
             self_shift = false;  // re-run after a map growth: own entry
           }
@@ -756,8 +807,13 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
+  __shared__ uint32_t sflags;
   uint32_t claims = 0;
-  if (!__syncthreads_or((J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0)) {
+  // The flags and (incremental) the dirty-list counts in one round trip.
+  if (threadIdx.x == 0) sflags = J.ctr->flags;
+  if (!J.is_baseline) load_list_prefix(J.dirty_cnt, pre);
+  else __syncthreads();
+  if (!(sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
     const InsertCtx ic = insert_ctx(J);
     const GlobalPtr<uint32_t> flags = &J.ctr->flags;
     if (J.is_baseline) {
@@ -773,17 +829,20 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
     } else {
       // An incremental checkpoint walks the tiles pass 1 visited (the dirty
       // group-0 tiles, then every tile of the groups above), one wavefront
-      // per tile, one lane per entry of the tile's fint region.
-      load_list_prefix(J.dirty_cnt, pre);
+      // per tile, one lane per entry of the tile's fint region.  The count
+      // and the region's first 64 entries are loaded together (the region
+      // holds kTileInterior >= 64 entries, so the loads stay in it).
       const uint32_t ndirty = pre[kPendSpread];
       const uint32_t ntiles0 = J.tile_base[1];  // group-0 tiles (tile_base[1] = ntiles without a group 1)
       const uint32_t nitems = ndirty + (J.ntiles - ntiles0);
       const uint32_t lane = lane_id(), nw = blockDim.x >> 6;
       for (uint32_t it = wg * nw + (threadIdx.x >> 6); it < nitems; it += nwg * nw) {
         const uint32_t T = it < ndirty ? list_item(J.dirty_list, J.dirty_cap, pre, it) : ntiles0 + (it - ndirty);
+        const GlobalPtr<const uint32_t> region = J.fint_list + (uint64_t)T * kTileInterior;
         const uint32_t cnt = J.fint_n[T];
+        uint32_t x0 = region[lane];
         for (uint32_t q = lane; q < cnt; q += 64) {
-          const uint32_t x = J.fint_list[(uint64_t)T * kTileInterior + q];
+          const uint32_t x = q < 64 ? x0 : region[q];
           const InsertResult r = map_insert<false>(ic, load_digest(ic.tree + x), x);
           if (!r.ok) g_or(flags, FLAG_MAP_FULL);
           claims += r.claimed ? 1 : 0;
@@ -1064,13 +1123,6 @@ __device__ __forceinline__ uint8_t dup_label(uint8_t la, uint8_t lb, bool found)
   if (la != lb) return DONE;
   if (la != SHIFT_DUPL) return la;  // both FIXED, both FIRST or both DONE
   return found ? (uint8_t)SHIFT_DUPL : (uint8_t)DONE;
-}
-
-// Slot s of the map as {k0, k1} and {val, pad}.
-__device__ __forceinline__ void load_slot(const MapG& m, uint64_t s, uint4& key, uint4& val) {
-  const GlobalPtr<const uint8_t> p = (GlobalPtr<const uint8_t>)&m.slots[s];
-  key = gload16((GlobalPtr<const void>)p);
-  val = gload16((GlobalPtr<const void>)(p + 16));
 }
 
 // Lookups of up to NK keys (need[n] false: skipped, out[n] = ~0).  Each
