@@ -134,7 +134,9 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     last_map_ckpt = -1;
   }
   const uint64_t nblk = ceil_div(N, kNodesPerBlock);
-  DD_TRY(blk.ensure(3 * nblk * 4));  // per root-scan workgroup (at most one per node block): counts, prefixes
+  // Per root-scan workgroup (at most one per node block): three counts; then
+  // the root masks of every group of 16 nodes.
+  DD_TRY(blk.ensure((3 + kNodesPerBlock / 16) * nblk * 4));
   DD_TRY(f_list.ensure((C + 1) * 4));
   DD_TRY(f_off.ensure((C + 1) * 4));
   DD_TRY(s_node.ensure(C * 4));
@@ -228,6 +230,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.blk_fc = job.blk_f + nblk;
   job.blk_s = job.blk_fc + nblk;
   job.nblk = (uint32_t)nblk;
+  job.rmask = job.blk_s + nblk;
   job.f_list = f_list.as<uint32_t>();
   job.f_off = f_off.as<uint32_t>();
   job.s_node = s_node.as<uint32_t>();
@@ -311,7 +314,7 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
         // (nearly) every leaf and interior node.
         case S_INSL:
         case S_TILES: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
-        case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kMaxStageWG); break;
+        case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kRootWG); break;
         case S_STILE: n = std::min<uint64_t>(ceil_div(J.C, kSTile), 512); break;  // grid-stride over sort tiles
         case S_META: n = std::min<uint64_t>(ceil_div(J.C, 256), 256); break;
         case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
@@ -397,7 +400,6 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   DD_TRY(mark(5));
   if (ev_dedup) DD_HIP(hipEventRecord(ev_dedup, s));
   launch_roots_count(J, P(S_NODEBLK), nj, grid_[S_NODEBLK], s);
-  launch_roots_scan(J, P(S_NODEBLK), nj, s);
   launch_roots_write(J, P(S_NODEBLK), nj, grid_[S_NODEBLK], s);
   launch_sbin_count(J, P(S_STILE), nj, grid_[S_STILE], s);
   launch_sbin_scan(J, nj, s);

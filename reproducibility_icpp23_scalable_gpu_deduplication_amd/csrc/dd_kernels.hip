@@ -1320,24 +1320,16 @@ __device__ __forceinline__ uint8_t byte_of(const uint4 w, int t) {
 }
 
 // Root kinds of the 16 nodes starting at v0 (a multiple of 16) as a bit mask
-// per kind (bit t: node v0 + t).  The nodes' labels are one 16-byte load; the
-// parents (v - 1) / 2 of v0 .. v0+15 are v0/2 - 1 .. v0/2 + 7, read as two
-// aligned 8-byte loads.  Labels of nodes the checkpoint did not visit are
-// FIXED (tile reset after every checkpoint), so most groups stop at the
-// first load.  The labels array is padded past N.
-// The 16 labels at v0, or FIXED bytes (no candidate) past the last node.
-__device__ __forceinline__ uint4 load16(const JobG& J, uint64_t v0) {
-  return v0 < J.N ? gload16((GlobalPtr<const void>)(J.labels + v0))
-                  : make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
-}
-
-__device__ __forceinline__ void scan16(const JobG& J, uint64_t v0, const uint4 w, uint32_t& fmask, uint32_t& smask) {
+// per kind (bit t: node v0 + t), from their labels `w` and the labels of
+// their parents (v - 1) / 2, which are v0/2 - 1 .. v0/2 + 7: the last byte of
+// `lo` and the bytes of `pw`.  Labels of nodes the checkpoint did not visit
+// are FIXED (tile reset after every checkpoint), so most groups have no
+// candidate.
+__device__ __forceinline__ void scan16(const JobG& J, uint64_t v0, const uint4 w, uint64_t lo, uint64_t pw,
+                                       uint32_t& fmask, uint32_t& smask) {
   fmask = smask = 0;
   if (v0 >= J.N || !any_root_candidate(w)) return;
-  const uint64_t h = v0 >> 1;
-  const uint64_t lo = v0 >= 16 ? *(GlobalPtr<const uint64_t>)(J.labels + h - 8) : 0;
-  const uint64_t pw = *(GlobalPtr<const uint64_t>)(J.labels + h);  // parent bytes v0/2 .. v0/2+7
-  const uint8_t p_first = (uint8_t)(lo >> 56);                      // parent byte v0/2 - 1
+  const uint8_t p_first = (uint8_t)(lo >> 56);  // parent byte v0/2 - 1
 #pragma unroll
   for (int t = 0; t < 16; t++) {
     const uint64_t v = v0 + t;
@@ -1370,6 +1362,57 @@ __device__ __forceinline__ uint32_t first_chunks(const TreeGeom& g, uint64_t v0,
   return fc;
 }
 
+// The root scan runs at most kRootWG workgroups per job; each owns a
+// contiguous run of node blocks and loads the labels of kRootBatch of them at
+// once.
+constexpr uint32_t kRootBatch = 4;
+
+// Labels of node blocks b .. b+kRootBatch-1 and the parent labels of each
+// group of 16 (FIXED bytes past the run or past the last node; the labels
+// array is padded to whole node blocks).  The parent words are loaded for
+// every group so that all loads of the batch are in flight together (they
+// are contiguous: 2 KiB per node block).
+struct LabelGroup {
+  uint4 w;      // labels v0 .. v0+15
+  uint64_t lo;  // labels v0/2 - 8 .. v0/2 - 1 (zero for v0 = 0)
+  uint64_t pw;  // labels v0/2 .. v0/2 + 7
+};
+
+__device__ __forceinline__ void load_root_batch(const JobG& J, uint32_t b, uint32_t b1, LabelGroup g[kRootBatch]) {
+  const GlobalPtr<const uint8_t> labels = J.labels;
+#pragma unroll
+  for (uint32_t q = 0; q < kRootBatch; q++) {
+    const bool in = b + q < b1;
+    const uint64_t v0 = (uint64_t)(in ? b + q : b) * kNodesPerBlock + threadIdx.x * 16, h = v0 >> 1;
+    g[q].w = gload16((GlobalPtr<const void>)(labels + v0));
+    g[q].lo = *(GlobalPtr<const uint64_t>)(labels + (v0 >= 16 ? h - 8 : 0));
+    g[q].pw = *(GlobalPtr<const uint64_t>)(labels + h);
+    if (v0 < 16) g[q].lo = 0;
+    if (!in || v0 >= J.N) g[q].w = make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
+  }
+}
+
+// Sums of three counts over the block (256 threads); sh holds 12 entries.
+__device__ __forceinline__ void block_sum3(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t* sh) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane_id() == 0) {
+    sh[w] = a;
+    sh[4 + w] = b;
+    sh[8 + w] = c;
+  }
+  __syncthreads();
+  a = b = c = 0;
+  for (uint32_t i = 0; i < nw; i++) {
+    a += sh[i];
+    b += sh[4 + i];
+    c += sh[8 + i];
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_roots_count(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
@@ -1379,18 +1422,22 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobG* __restrict__ jo
   node_block_range(J, wg, nwg, b0, b1);
   const TreeGeom g = make_geom(J.C);
   uint32_t f = 0, fc = 0, s = 0;
-  for (uint32_t b = b0; b < b1; b++) {
-    const uint64_t v0 = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
-    uint32_t fm, sm;
-    scan16(J, v0, load16(J, v0), fm, sm);
-    f += __popc(fm);
-    fc += first_chunks(g, v0, fm);
-    s += __popc(sm);
+  for (uint32_t b = b0; b < b1; b += kRootBatch) {
+    LabelGroup lg[kRootBatch];
+    load_root_batch(J, b, b1, lg);
+#pragma unroll
+    for (uint32_t q = 0; q < kRootBatch; q++) {
+      const uint64_t v0 = (uint64_t)(b + q) * kNodesPerBlock + threadIdx.x * 16;
+      uint32_t fm, sm;
+      scan16(J, v0, lg[q].w, lg[q].lo, lg[q].pw, fm, sm);
+      f += __popc(fm);
+      fc += first_chunks(g, v0, fm);
+      s += __popc(sm);
+      if (b + q < b1) J.rmask[(uint64_t)(b + q) * (kNodesPerBlock / 16) + threadIdx.x] = fm | (sm << 16);
+    }
   }
-  __shared__ uint32_t sh[4];
-  f = block_sum(f, sh);
-  fc = block_sum(fc, sh);
-  s = block_sum(s, sh);
+  __shared__ uint32_t sh[12];
+  block_sum3(f, fc, s, sh);
   if (threadIdx.x == 0) {
     J.blk_f[wg] = f;
     J.blk_fc[wg] = fc;
@@ -1398,52 +1445,14 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobG* __restrict__ jo
   }
 }
 
-// One workgroup per job: exclusive scans of the per-workgroup counts.  Each
-// lane owns up to kScanPer consecutive entries (the roots grid is at most
-// 1024 * kScanPer workgroups per job), so one block scan per array suffices.
-constexpr uint32_t kScanPer = 4;
-static_assert(kScanPer * 1024 >= kMaxStageWG, "roots grid larger than one scan workgroup covers");
-__global__ __launch_bounds__(1024) void k_roots_scan(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix) {
-  const JobG& J = jobs[blockIdx.x];
-  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  const uint32_t nwg = prefix[blockIdx.x + 1] - prefix[blockIdx.x];
-  __shared__ uint32_t sh[17];
-  const uint32_t e0 = threadIdx.x * kScanPer;
-  uint32_t f[kScanPer], fc[kScanPer], s[kScanPer];
-  uint32_t sf = 0, sfc = 0, ss = 0;
-#pragma unroll
-  for (uint32_t q = 0; q < kScanPer; q++) {
-    const uint32_t i = e0 + q;
-    f[q] = i < nwg ? J.blk_f[i] : 0;
-    fc[q] = i < nwg ? J.blk_fc[i] : 0;
-    s[q] = i < nwg ? J.blk_s[i] : 0;
-    sf += f[q];
-    sfc += fc[q];
-    ss += s[q];
-  }
-  uint32_t tf, tfc, ts;
-  uint32_t pf = block_excl_scan<16>(sf, sh, tf);
-  uint32_t pfc = block_excl_scan<16>(sfc, sh, tfc);
-  uint32_t ps = block_excl_scan<16>(ss, sh, ts);
-#pragma unroll
-  for (uint32_t q = 0; q < kScanPer; q++) {
-    const uint32_t i = e0 + q;
-    if (i < nwg) {
-      J.blk_f[i] = pf;
-      J.blk_fc[i] = pfc;
-      J.blk_s[i] = ps;
-    }
-    pf += f[q];
-    pfc += fc[q];
-    ps += s[q];
-  }
-  if (threadIdx.x == 0) {
-    J.ctr->nfirst = tf;
-    J.ctr->dchunks = tfc;
-    J.ctr->nshift = ts;
-  }
-}
-
+// Writes the roots in node order, from the root masks k_roots_count stored:
+// first roots (node, chunk offset of its region) and the nodes of the
+// shifted roots (their sources are looked up by k_sbin_count).  Each
+// workgroup starts from the sums of the counts of the workgroups before it;
+// the last one also writes the totals.  Per node block the three counts are
+// scanned together, packed (a block holds at most 4096 roots of each kind,
+// and its first regions at most 2^32 - 1 chunks).  The masks are the only
+// loads after the prefix, so the stores never hold up a load.
 __global__ __launch_bounds__(256) void k_roots_write(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
@@ -1452,48 +1461,53 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobG* __restrict__ jo
   uint32_t b0, b1;
   node_block_range(J, wg, nwg, b0, b1);
   const TreeGeom g = make_geom(J.C);
-  uint32_t run_f = J.blk_f[wg], run_fc = J.blk_fc[wg], run_s = J.blk_s[wg];
-  __shared__ uint32_t sh[5];
-  for (uint32_t b = b0; b < b1; b++) {
-    const uint64_t v0 = (uint64_t)b * kNodesPerBlock + threadIdx.x * 16;
-    uint32_t fm, sm;
-    scan16(J, v0, load16(J, v0), fm, sm);
-    if (!__syncthreads_or((fm | sm) != 0)) continue;  // no root in this node block
-    const uint32_t f = __popc(fm), fc = first_chunks(g, v0, fm), s = __popc(sm);
-    uint32_t tf, tfc, ts;
-    uint32_t pf = block_excl_scan<4>(f, sh, tf) + run_f;
-    uint32_t pfc = block_excl_scan<4>(fc, sh, tfc) + run_fc;
-    uint32_t ps = block_excl_scan<4>(s, sh, ts) + run_s;
-    run_f += tf;
-    run_fc += tfc;
-    run_s += ts;
-    while (fm) {
-      const int t = __builtin_ctz(fm);
-      fm &= fm - 1;
-      const uint64_t v = v0 + t;
-      J.f_list[pf] = (uint32_t)v;
-      J.f_off[pf] = pfc;
-      pf++;
-      pfc += (uint32_t)leaf_count(g, v);
-    }
-    // Shifted roots: every load is issued before any store (fully unrolled,
-    // so the arrays stay in registers): map slots, then the source
-    // (checkpoint, node) of each.
-    if (sm) {
-      uint32_t slots[16];
-      unsigned long long vals[16];
+  __shared__ uint32_t sh[12];
+  __shared__ uint64_t sh64[5];
+  uint32_t run_f = 0, run_fc = 0, run_s = 0;
+  for (uint32_t i = threadIdx.x; i < wg; i += blockDim.x) {
+    run_f += J.blk_f[i];
+    run_fc += J.blk_fc[i];
+    run_s += J.blk_s[i];
+  }
+  block_sum3(run_f, run_fc, run_s, sh);
+  if (wg == nwg - 1 && threadIdx.x == 0) {
+    J.ctr->nfirst = run_f + J.blk_f[wg];
+    J.ctr->dchunks = run_fc + J.blk_fc[wg];
+    J.ctr->nshift = run_s + J.blk_s[wg];
+  }
+  for (uint32_t b = b0; b < b1; b += kRootBatch) {
+    uint32_t rm[kRootBatch];
 #pragma unroll
-      for (int t = 0; t < 16; t++) slots[t] = ((sm >> t) & 1) ? J.slot_of[v0 + t] : 0u;
+    for (uint32_t q = 0; q < kRootBatch; q++)
+      rm[q] = J.rmask[(uint64_t)(b + q < b1 ? b + q : b) * (kNodesPerBlock / 16) + threadIdx.x];
 #pragma unroll
-      for (int t = 0; t < 16; t++) vals[t] = ((sm >> t) & 1) ? J.map.slots[slots[t]].val : 0ull;
-#pragma unroll
-      for (int t = 0; t < 16; t++) {
-        if ((sm >> t) & 1) {
-          J.s_node[ps] = (uint32_t)(v0 + t);
-          J.s_prev[ps] = (uint32_t)vals[t];
-          J.s_tree[ps] = (uint32_t)(vals[t] >> 32);
-          ps++;
-        }
+    for (uint32_t q = 0; q < kRootBatch; q++) {
+      if (b + q >= b1) break;  // uniform
+      const uint64_t v0 = (uint64_t)(b + q) * kNodesPerBlock + threadIdx.x * 16;
+      uint32_t fm = rm[q] & 0xffffu, sm = rm[q] >> 16;
+      if (!__syncthreads_or((fm | sm) != 0)) continue;  // no root in this node block (uniform)
+      const uint32_t f = __popc(fm), fc = first_chunks(g, v0, fm), s = __popc(sm);
+      uint64_t tot;
+      const uint64_t ex = block_excl_scan<4>((uint64_t)f | ((uint64_t)s << 16) | ((uint64_t)fc << 32), sh64, tot);
+      uint32_t pf = run_f + (uint32_t)(ex & 0xffff), ps = run_s + (uint32_t)((ex >> 16) & 0xffff);
+      uint32_t pfc = run_fc + (uint32_t)(ex >> 32);
+      run_f += (uint32_t)(tot & 0xffff);
+      run_s += (uint32_t)((tot >> 16) & 0xffff);
+      run_fc += (uint32_t)(tot >> 32);
+      // Stores.
+      while (fm) {
+        const int t = __builtin_ctz(fm);
+        fm &= fm - 1;
+        const uint64_t v = v0 + t;
+        J.f_list[pf] = (uint32_t)v;
+        J.f_off[pf] = pfc;
+        pf++;
+        pfc += (uint32_t)leaf_count(g, v);
+      }
+      while (sm) {
+        const int t = __builtin_ctz(sm);
+        sm &= sm - 1;
+        J.s_node[ps++] = (uint32_t)(v0 + t);
       }
     }
   }
@@ -1550,7 +1564,12 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobG* __restrict__ job
     uint32_t bn = 0;
     bool todo = e < e1;
     if (todo) {
-      bn = J.s_tree[e];
+      // The root's source: the map entry its digest matched (slot recorded
+      // by pass 2, or by the leaf inserts for a shifted leaf).
+      const unsigned long long val = J.map.slots[J.slot_of[J.s_node[e]]].val;
+      J.s_prev[e] = (uint32_t)val;
+      J.s_tree[e] = (uint32_t)(val >> 32);
+      bn = (uint32_t)(val >> 32);
       if (bn >= nbins) {
         g_or(&J.ctr->flags, FLAG_BAD_SOURCE);
         bn = nbins - 1;
@@ -1585,8 +1604,7 @@ __device__ __forceinline__ void export_counters(const JobG& J) {
 }
 
 // One workgroup per job: new map entries of the checkpoint, scan of the
-// shifted-root histogram, the prior table, the diff length; then exports the
-// counters to the host.
+// shifted-root histogram, the prior table, the diff length.
 __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ jobs) {
   const JobG& J = jobs[blockIdx.x];
   __shared__ uint32_t sh[17];
@@ -1601,7 +1619,6 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ job
     if (threadIdx.x == 0) {
       J.ctr->map_count = J.map_count_at_start + claims;
       J.ctr->pending = J.is_baseline ? J.C : pending_total(J);
-      export_counters(J);
     }
     return;
   }
@@ -1648,7 +1665,6 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ job
     J.ctr->diff_len = 32 + meta + J.ctr->dchunks * (uint64_t)J.chunk_size;
     J.ctr->map_count = J.map_count_at_start + claims;
     J.ctr->pending = J.is_baseline ? J.C : pending_total(J);
-    export_counters(J);
   }
 }
 
@@ -1806,6 +1822,10 @@ template <bool Vec>
 __global__ __launch_bounds__(256) void k_gather(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
+  // The counters are final: the job's first workgroup copies them to the
+  // host-mapped block (its slow stores overlap the copy below), also for a
+  // checkpoint that must re-run.
+  if (blockIdx.x == prefix[j] && threadIdx.x == 0) export_counters(J);
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
   // The roots are written: reset the labels of the tiles this checkpoint
   // visited (gather runs a large grid whatever the diff size).
@@ -2083,10 +2103,6 @@ void launch_pass2(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, 
 void launch_roots_count(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_roots_count, grid, 256, 0, s, jobs, prefix, njobs);
-}
-void launch_roots_scan(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, hipStream_t s) {
-  const JobG* jobs = gview(jobs_h);
-  DD_LAUNCH(k_roots_scan, njobs, 1024, 0, s, jobs, prefix);
 }
 void launch_roots_write(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);

@@ -16,6 +16,7 @@ namespace dd {
 
 constexpr int kTileLevels = 8;          // levels per tree tile (256 bottom nodes)
 constexpr int kNodesPerBlock = 4096;    // root scan: 256 threads x 16 nodes
+constexpr uint32_t kRootWG = 1024;      // root-scan workgroups per job
 constexpr int kSTile = 1024;            // shifted-root sort tile
 constexpr uint32_t kMaxStageWG = 4096;  // workgroups per job of a grid-stride stage
 constexpr uint32_t kMaxTileWG = 8192;   // one-wavefront workgroups per job of a tile pass
@@ -90,6 +91,7 @@ struct JobT {
   P<uint32_t> blk_fc;   // per root-scan workgroup: chunks under first roots (prefix)
   P<uint32_t> blk_s;    // per root-scan workgroup: shifted roots (prefix)
   uint32_t nblk;      // node blocks = ceil(N / kNodesPerBlock)
+  P<uint32_t> rmask;  // per group of 16 nodes: first-root bits | shifted-root bits << 16
   P<uint32_t> f_list;   // first roots, ascending
   P<uint32_t> f_off;    // chunk offset of each first region in the data section
   P<uint32_t> s_node;   // shifted roots, ascending
@@ -169,8 +171,6 @@ void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, ui
 void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
 void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
-// `prefix` is the workgroup prefix of the roots_count/roots_write stage.
-void launch_roots_scan(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, hipStream_t s);
 void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_sbin_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_sbin_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s);
