@@ -150,7 +150,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   const uint64_t Kcap = ((uint64_t)K | 63) + 1;  // ids 0 .. Kcap-1
   DD_TRY(s_hist.ensure(Kcap * ceil_div(C, kSTile) * 4));
   DD_TRY(ptab.ensure(Kcap * 8));
-  DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * 8, true, s));
+  DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * kCntStride * 4, true, s));
   const uint64_t nlb = ceil_div(C, 256);
   // blk_claim (one per insert_leaves workgroup) then int_claim (one per
   // insert_interior workgroup); a baseline launches one of each per leaf block.
@@ -179,7 +179,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   }
   epoch += 1;
   const uint64_t dirty_cap = ceil_div(ntiles0, 32);
-  DD_TRY(lists.ensure((32 + 32 * dirty_cap) * 4));
+  DD_TRY(lists.ensure((32 * kCntStride + 32 * dirty_cap) * 4));
   const uint64_t need = max_diff_size(len, chunk_size, K);
   if (!out) {
     DD_TRY(diff.ensure(max_diff_size(len, chunk_size, (uint32_t)(Kcap - 1))));
@@ -225,7 +225,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.tile_epoch = tile_epoch.as<uint32_t>();
   job.dirty_cnt = lists.as<uint32_t>();
   job.dirty_cap = (uint32_t)dirty_cap;
-  job.dirty_list = job.dirty_cnt + 32;
+  job.dirty_list = job.dirty_cnt + 32 * kCntStride;
   job.blk_f = blk.as<uint32_t>();
   job.blk_fc = job.blk_f + nblk;
   job.blk_s = job.blk_fc + nblk;

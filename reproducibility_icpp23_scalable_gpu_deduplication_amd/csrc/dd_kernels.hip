@@ -101,8 +101,8 @@ __global__ void k_begin(const JobG* __restrict__ jobs, uint32_t njobs, int rerun
   c->flags = 0;
   if (rerun) return;  // keep the changed-leaf lists of leaf_hash when re-running the inserts
   for (uint32_t i = 0; i < 32; i++) {
-    jobs[j].pend_cnt[i] = 0;
-    jobs[j].dirty_cnt[i] = 0;
+    jobs[j].pend_cnt[i * kCntStride] = 0;
+    jobs[j].dirty_cnt[i * kCntStride] = 0;
   }
   c->pending = 0;
   c->nfirst = c->nshift = c->nprior = 0;
@@ -122,8 +122,8 @@ __global__ void k_finish(const JobG* __restrict__ jobs, uint32_t njobs) {
   const GlobalPtr<JobCounters> c = J.ctr;
   if (c->flags) return;  // a re-run or a failed checkpoint: the host re-arms with k_begin
   for (uint32_t i = 0; i < 32; i++) {
-    J.pend_cnt[i] = 0;
-    J.dirty_cnt[i] = 0;
+    J.pend_cnt[i * kCntStride] = 0;
+    J.dirty_cnt[i * kCntStride] = 0;
   }
   c->pending = 0;
   c->nfirst = c->nshift = c->nprior = 0;
@@ -153,7 +153,7 @@ __device__ __forceinline__ uint32_t leaf_tile(uint32_t D, uint64_t leaf) {
 __device__ __forceinline__ void list_append(GlobalPtr<uint32_t> cnt, GlobalPtr<uint32_t> list, uint32_t cap,
                                             uint32_t item) {
   const uint32_t r = item % kPendSpread;
-  const uint32_t k = g_add(&cnt[r], 1u);
+  const uint32_t k = g_add(&cnt[r * kCntStride], 1u);
   list[(uint64_t)r * cap + k] = item;
 }
 
@@ -169,7 +169,7 @@ __device__ __forceinline__ void record_pending(const JobG& J, uint64_t k, bool p
   const uint32_t r = (uint32_t)(k >> 6) % kPendSpread;
   const uint32_t leader = __ffsll((long long)b) - 1;
   uint32_t base = 0;
-  if (lane_id() == leader) base = g_add(&J.pend_cnt[r], (uint32_t)__popcll(b));
+  if (lane_id() == leader) base = g_add(&J.pend_cnt[r * kCntStride], (uint32_t)__popcll(b));
   base = __shfl(base, leader, 64);
   if (pending) J.pend_list[(uint64_t)r * J.pend_cap + base + __popcll(b & lanemask_lt())] = (uint32_t)k;
   // Mark the group-0 tiles of changed leaves: the first marker of a tile in
@@ -199,7 +199,7 @@ __device__ __forceinline__ void load_list_prefix(GlobalPtr<const uint32_t> cnt, 
   static_assert(kPendSpread <= 64, "one count per lane of a wavefront");
   if (threadIdx.x < 64) {
     const uint32_t l = threadIdx.x;
-    uint32_t v = l < kPendSpread ? cnt[l] : 0;
+    uint32_t v = l < kPendSpread ? cnt[l * kCntStride] : 0;
 #pragma unroll
     for (uint32_t o = 1; o < kPendSpread; o <<= 1) {
       const uint32_t u = __shfl_up(v, o, 64);
@@ -225,7 +225,7 @@ __device__ __forceinline__ uint32_t list_item(GlobalPtr<const uint32_t> list, ui
 __device__ __forceinline__ uint64_t pending_total(const JobG& J) {
   uint64_t t = 0;
 #pragma unroll
-  for (uint32_t i = 0; i < kPendSpread; i++) t += J.pend_cnt[i];
+  for (uint32_t i = 0; i < kPendSpread; i++) t += J.pend_cnt[i * kCntStride];
   return t;
 }
 

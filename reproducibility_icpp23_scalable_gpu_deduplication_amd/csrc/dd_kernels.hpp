@@ -21,6 +21,10 @@ constexpr int kSTile = 1024;            // shifted-root sort tile
 constexpr uint32_t kMaxStageWG = 4096;  // workgroups per job of a grid-stride stage
 constexpr uint32_t kMaxTileWG = 8192;   // one-wavefront workgroups per job of a tile pass
 constexpr uint32_t kTileInterior = 255;  // interior nodes of a tile of 8 levels
+// Work-list counters (32 per list) sit kCntStride words apart, one per
+// 256-byte block: counters that share a cache line serialize their atomics
+// in one L2 channel (one atomic per wavefront with a changed chunk).
+constexpr uint32_t kCntStride = 64;
 
 // Per-job counters written by the device and read back by the host.
 struct JobCounters {
@@ -68,7 +72,7 @@ struct JobT {
   // Work lists.  Each is split into 32 sub-lists (picked by the producing
   // leaf block or tile index modulo 32) with one counter each, so no single
   // counter is hit by every workgroup; consumers prefix-sum the 32 counts.
-  P<uint32_t> pend_cnt;    // 32 counters
+  P<uint32_t> pend_cnt;    // 32 counters, kCntStride words apart
   P<uint32_t> pend_list;   // 32 x pend_cap: chunk indices of the leaves that need a map insert
   uint32_t pend_cap;
   // First-occurrence interior nodes found by pass 1, per tile (global tile
@@ -78,7 +82,7 @@ struct JobT {
   P<uint32_t> fint_n;
   uint32_t epoch;        // stamp of this checkpoint (never 0)
   P<uint32_t> tile_epoch;  // per group-0 tile: epoch of the last checkpoint that changed one of its leaves
-  P<uint32_t> dirty_cnt;   // 32 counters
+  P<uint32_t> dirty_cnt;   // 32 counters, kCntStride words apart
   P<uint32_t> dirty_list;  // 32 x dirty_cap: group-0 tiles with a changed leaf
   uint32_t dirty_cap;
   uint32_t tile_base[4];  // first tile index of each group

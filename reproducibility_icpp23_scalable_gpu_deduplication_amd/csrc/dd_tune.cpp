@@ -103,7 +103,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   DevBuf tree, labels, ctr, plist, table;
   int best = LEAF_DIRECT;
   if (tree.ensure(16 * N) != DD_OK || labels.ensure(N + 64) != DD_OK ||
-      ctr.ensure(sizeof(JobCounters) + 3 * 32 * 4) != DD_OK || plist.ensure(32 * pend_cap * 4) != DD_OK ||
+      ctr.ensure(sizeof(JobCounters) + 2 * 32 * kCntStride * 4) != DD_OK || plist.ensure(32 * pend_cap * 4) != DD_OK ||
       table.ensure(sizeof(JobDev) + LEAF_NMAP * 2 * 4) != DD_OK)
     return best;  // no memory to tune: keep the default, do not cache
   JobDev t;
@@ -122,7 +122,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   t.labels = labels.as<uint8_t>();
   t.ctr = ctr.as<JobCounters>();
   t.pend_cnt = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
-  t.dirty_cnt = t.pend_cnt + 64;
+  t.dirty_cnt = t.pend_cnt + 32 * kCntStride;
   t.pend_list = plist.as<uint32_t>();
   t.pend_cap = (uint32_t)pend_cap;
   // One workgroup prefix {0, grid} per mapping: the grid-stride mappings
