@@ -257,7 +257,7 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
 // ---------------------------------------------------------------------------
 // Pipeline
 // ---------------------------------------------------------------------------
-enum Stage { S_LEAF = 0, S_INSL, S_TILES, S_NODEBLK, S_STILE, S_META, S_GATHER, S_TILE0 };
+enum Stage { S_LEAF = 0, S_INSL, S_TILES, S_NODEBLK, S_STILE, S_GATHER, S_TILE0 };
 
 
 Pipeline::~Pipeline() {
@@ -316,7 +316,6 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
         case S_TILES: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
         case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kRootWG); break;
         case S_STILE: n = std::min<uint64_t>(ceil_div(J.C, kSTile), 512); break;  // grid-stride over sort tiles
-        case S_META: n = std::min<uint64_t>(ceil_div(J.C, 256), 256); break;
         case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
         default: {
           const uint64_t tg = tiles_in_group(J.D, (int)st - S_TILE0);
@@ -405,7 +404,6 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   launch_sbin_scan(J, nj, s);
   launch_sbin_scatter(J, P(S_STILE), nj, grid_[S_STILE], s);
   DD_TRY(mark(6));
-  launch_write_meta(J, P(S_META), nj, grid_[S_META], s);
   launch_gather(J, P(S_GATHER), nj, grid_[S_GATHER], vec16_, s);
   DD_TRY(mark(7));
   launch_finish(J, nj, s);

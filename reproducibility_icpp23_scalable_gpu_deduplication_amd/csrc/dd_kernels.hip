@@ -12,8 +12,8 @@
 //   pass2 (tiles)  fixed / shifted / done labels with map lookups
 //   roots_*        roots in node order, region sizes, block offsets
 //   sbin_*         shifted roots grouped by source checkpoint (stable)
-//   write_meta     header, first-occurrence list, prior table, shifted list
-//   gather         first-occurrence chunks into the data section
+//   gather         first-occurrence chunks into the data section, then the
+//                  header, first-occurrence list, prior table, shifted list
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -1604,38 +1604,54 @@ __device__ __forceinline__ void export_counters(const JobG& J) {
 }
 
 // One workgroup per job: new map entries of the checkpoint, scan of the
-// shifted-root histogram, the prior table, the diff length.
+// shifted-root histogram, the prior table, the diff length.  Two rounds of
+// loads: the claim counts with the counter fields, then the histogram, which
+// (at most 1024 entries, the usual case) is scanned and turned into per-bin
+// totals in LDS.
 __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ jobs) {
   const JobG& J = jobs[blockIdx.x];
   __shared__ uint32_t sh[17];
   __shared__ unsigned long long shl[16];
+  __shared__ uint32_t hs[1025];
+  __shared__ uint32_t sflags, snshift, snfirst, spend;
+  __shared__ unsigned long long sdchunks;
+  if (threadIdx.x == 0) {
+    sflags = J.ctr->flags;
+    snshift = J.ctr->nshift;
+    snfirst = J.ctr->nfirst;
+    sdchunks = J.ctr->dchunks;
+    spend = J.is_baseline ? 0u : (uint32_t)pending_total(J);
+  }
   unsigned long long claims = 0;
   for (uint32_t i = threadIdx.x; i < J.nwg_ins_leaf; i += blockDim.x) claims += J.blk_claim[i];
   for (uint32_t i = threadIdx.x; i < J.nwg_ins_int; i += blockDim.x) claims += J.int_claim[i];
-  claims = block_sum(claims, shl);
-  const bool skip = (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) != 0;
-  __syncthreads();
-  if (skip) {
+  claims = block_sum(claims, shl);  // its barriers publish the fields above
+  const uint64_t pending = J.is_baseline ? J.C : spend;
+  if (sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) {
     if (threadIdx.x == 0) {
       J.ctr->map_count = J.map_count_at_start + claims;
-      J.ctr->pending = J.is_baseline ? J.C : pending_total(J);
+      J.ctr->pending = pending;
     }
     return;
   }
-  const uint32_t nt = stiles_of(J);
+  const uint32_t S = snshift;
+  const uint32_t nt = (S + kSTile - 1) / kSTile;
   const uint32_t nbins = J.ckpt_id + 1;
-  const uint32_t S = J.ctr->nshift;
   uint32_t nprior = 0;
   if (nt > 0) {
-    // Exclusive scan of the histogram in bin-major order.
     const uint64_t total = (uint64_t)nbins * nt;
+    const bool in_lds = total <= blockDim.x;  // uniform
+    // Exclusive scan of the histogram in bin-major order.
     uint32_t run = 0;
     for (uint64_t base = 0; base < total; base += blockDim.x) {
       const uint64_t i = base + threadIdx.x;
       const uint32_t v = i < total ? J.s_hist[i] : 0;
       uint32_t tot;
       const uint32_t e = block_excl_scan<16>(v, sh, tot);
-      if (i < total) J.s_hist[i] = run + e;
+      if (i < total) {
+        J.s_hist[i] = run + e;
+        if (in_lds) hs[i] = e;
+      }
       run += tot;
     }
     __syncthreads();
@@ -1645,8 +1661,9 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ job
       const uint32_t bn = base + threadIdx.x;
       uint32_t cnt = 0;
       if (bn < nbins) {
-        const uint32_t lo = J.s_hist[(uint64_t)bn * nt];
-        const uint32_t hi = bn + 1 < nbins ? J.s_hist[(uint64_t)(bn + 1) * nt] : S;
+        const uint64_t i0 = (uint64_t)bn * nt, i1 = (uint64_t)(bn + 1) * nt;
+        const uint32_t lo = in_lds ? hs[i0] : J.s_hist[i0];
+        const uint32_t hi = bn + 1 < nbins ? (in_lds ? hs[i1] : J.s_hist[i1]) : S;
         cnt = hi - lo;
       }
       uint32_t tot;
@@ -1661,10 +1678,10 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ job
   }
   if (threadIdx.x == 0) {
     J.ctr->nprior = nprior;
-    const uint64_t meta = 4ull * J.ctr->nfirst + 8ull * nprior + 8ull * J.ctr->nshift;
-    J.ctr->diff_len = 32 + meta + J.ctr->dchunks * (uint64_t)J.chunk_size;
+    const uint64_t meta = 4ull * snfirst + 8ull * nprior + 8ull * S;
+    J.ctr->diff_len = 32 + meta + sdchunks * (uint64_t)J.chunk_size;
     J.ctr->map_count = J.map_count_at_start + claims;
-    J.ctr->pending = J.is_baseline ? J.C : pending_total(J);
+    J.ctr->pending = pending;
   }
 }
 
@@ -1736,14 +1753,11 @@ __global__ __launch_bounds__(64) void k_sbin_scatter(const JobG* __restrict__ jo
 // Diff metadata: header_t, first-occurrence roots, prior table, shifted
 // entries (src/tree_approach.cpp:593,622-706).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_write_meta(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
-  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobG& J = jobs[j];
-  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  const uint32_t F = J.ctr->nfirst, P = J.ctr->nprior, S = J.ctr->nshift;
-  GlobalPtr<uint8_t> out = J.diff;
-  const uint32_t tid = (blockIdx.x - prefix[j]) * blockDim.x + threadIdx.x;
-  const uint32_t nth = (prefix[j + 1] - prefix[j]) * blockDim.x;
+// Written by the gather workgroups (thread tid of nth), after their copies:
+// loads and stores share one in-order counter, so the stores go last.
+__device__ __forceinline__ void write_meta(const JobG& J, uint32_t F, uint32_t P, uint32_t S, uint32_t tid,
+                                           uint32_t nth) {
+  const GlobalPtr<uint8_t> out = J.diff;
   if (tid == 0) {
     store_u32(out + 0, J.baseline_id);
     store_u32(out + 4, J.ckpt_id);
@@ -1815,35 +1829,21 @@ __device__ __forceinline__ void copy_piece(GlobalPtr<uint8_t> dst, GlobalPtr<con
   }
 }
 
-// Groups of 256 data chunks: each lane finds the source chunk of one, then the
-// workgroup copies the group with 16-byte pieces (1-byte pieces when the
-// chunk size is not a multiple of 16 or the input is not 16-byte aligned).
+// The data section: groups of 256 chunks per workgroup iteration.
 template <bool Vec>
-__global__ __launch_bounds__(256) void k_gather(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
-  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
-  const JobG& J = jobs[j];
-  // The counters are final: the job's first workgroup copies them to the
-  // host-mapped block (its slow stores overlap the copy below), also for a
-  // checkpoint that must re-run.
-  if (blockIdx.x == prefix[j] && threadIdx.x == 0) export_counters(J);
-  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
-  // The roots are written: reset the labels of the tiles this checkpoint
-  // visited (gather runs a large grid whatever the diff size).
-  if (!J.list_mode) reset_visited_tiles(J, blockIdx.x - prefix[j], prefix[j + 1] - prefix[j]);
-  const uint32_t F = J.ctr->nfirst, P = J.ctr->nprior, S = J.ctr->nshift;
-  const uint64_t Dq = J.ctr->dchunks;
+__device__ __forceinline__ void gather_data(const JobG& J, uint32_t F, uint32_t P, uint32_t S, uint64_t Dq,
+                                            uint32_t wg, uint32_t nblk) {
   if (Dq == 0) return;
   const TreeGeom g = make_geom(J.C);
   const uint64_t cs = J.chunk_size;
   GlobalPtr<uint8_t> dst_base = J.diff + 32 + 4ull * F + 8ull * P + 8ull * S;
   __shared__ uint64_t src[256];
-  const uint32_t nblk = prefix[j + 1] - prefix[j];
   const uint64_t ngroups = (Dq + 255) / 256;
   const uint32_t piece = Vec ? 16 : 1;
   const uint64_t ppc = cs / piece;  // pieces per chunk
   const bool pow2 = (ppc & (ppc - 1)) == 0;
   const uint32_t sh = pow2 ? (uint32_t)__builtin_ctzll(ppc) : 0;
-  for (uint64_t grp = blockIdx.x - prefix[j]; grp < ngroups; grp += nblk) {
+  for (uint64_t grp = wg; grp < ngroups; grp += nblk) {
     const uint64_t q0 = grp * 256;
     const uint64_t q = q0 + threadIdx.x;
     if (q < Dq) {
@@ -1863,6 +1863,28 @@ __global__ __launch_bounds__(256) void k_gather(const JobG* __restrict__ jobs, c
     }
     __syncthreads();
   }
+}
+
+// Groups of 256 data chunks: each lane finds the source chunk of one, then the
+// workgroup copies the group with 16-byte pieces (1-byte pieces when the
+// chunk size is not a multiple of 16 or the input is not 16-byte aligned).
+template <bool Vec>
+__global__ __launch_bounds__(256) void k_gather(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobG& J = jobs[j];
+  // The counters are final: the job's first workgroup copies them to the
+  // host-mapped block (its slow stores overlap the copy below), also for a
+  // checkpoint that must re-run.
+  if (blockIdx.x == prefix[j] && threadIdx.x == 0) export_counters(J);
+  if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
+  const uint32_t F = J.ctr->nfirst, P = J.ctr->nprior, S = J.ctr->nshift;
+  const uint64_t Dq = J.ctr->dchunks;
+  const uint32_t wg = blockIdx.x - prefix[j], nblk = prefix[j + 1] - prefix[j];
+  gather_data<Vec>(J, F, P, S, Dq, wg, nblk);
+  // Stores only from here on: the metadata, and (the roots are written) the
+  // reset of the labels of the tiles this checkpoint visited.
+  write_meta(J, F, P, S, wg * blockDim.x + threadIdx.x, nblk * blockDim.x);
+  if (!J.list_mode) reset_visited_tiles(J, wg, nblk);
 }
 
 // ---------------------------------------------------------------------------
@@ -2118,10 +2140,6 @@ void launch_sbin_scan(const JobDev* jobs_h, uint32_t njobs, hipStream_t s) {
 void launch_sbin_scatter(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_sbin_scatter, grid, 64, 0, s, jobs, prefix, njobs);
-}
-void launch_write_meta(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
-  const JobG* jobs = gview(jobs_h);
-  if (grid) DD_LAUNCH(k_write_meta, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_gather(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool vec16,
                    hipStream_t s) {

@@ -179,7 +179,6 @@ void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njo
 void launch_sbin_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_sbin_scan(const JobDev* jobs, uint32_t njobs, hipStream_t s);
 void launch_sbin_scatter(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
-void launch_write_meta(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_gather(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool vec16,
                    hipStream_t s);
 

@@ -130,6 +130,18 @@ def test_config1_shape_64mib(torch_gpu):
     check_chain(torch_gpu, 128, datas, restart_ids=[0, 4, 9])
 
 
+def test_more_dirty_tiles_than_tile_workgroups(torch_gpu):
+    # 2^22 + 1000 leaves of 16 B (a chunk count that is not a power of two,
+    # so group-0 tiles hold leaves one level above the deepest), 4
+    # checkpoints with 20 % fresh and 10 % copied chunks scattered over the
+    # buffer: nearly all of the ~16K group-0 tiles are dirty, more than the
+    # 8192 workgroups of a tile pass, so workgroups walk several tiles and
+    # the interior inserts walk ~16K per-tile regions.
+    C, cs = (1 << 22) + 1000, 16
+    datas = W.chain_uniform_changes(77, C * cs, cs, 4, fresh_frac=0.2, copy_frac=0.1, zero_bytes=1 << 16)
+    check_chain(torch_gpu, cs, datas, restart_ids=[0, 3])
+
+
 def test_device_output_matches_host_output(torch_gpu):
     torch = torch_gpu
     datas = W.chain_uniform_changes(8, 1 << 20, 128, 4, zero_bytes=8192)
