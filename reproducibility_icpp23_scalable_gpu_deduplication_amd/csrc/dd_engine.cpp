@@ -203,6 +203,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.is_baseline = baseline ? 1 : 0;
   job.list_mode = method == DD_METHOD_LIST ? 1 : 0;
   job.node_bias = job.list_mode ? (uint32_t)(C - 1) : 0;
+  job.write_all_labels = (baseline || job.list_mode || !labels_fixed) ? 1 : 0;
+  labels_fixed = false;  // until this checkpoint completes
   job.tree = tree.as<Digest>();
   job.labels = labels.as<uint8_t>();
   job.slot_of = slot_of.as<uint32_t>();
@@ -247,6 +249,9 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
 
 void StreamState::commit(const JobCounters& c, bool baseline) {
   if (baseline) baseline_id = current_id;
+  // The gather kernel of a completed tree checkpoint reset every label of
+  // the tiles it visited; the list method keeps its labels.
+  labels_fixed = method == DD_METHOD_TREE;
   nfirst = c.nfirst;
   nshift = c.nshift;
   map_count = c.map_count;

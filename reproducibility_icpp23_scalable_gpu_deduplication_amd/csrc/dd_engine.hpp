@@ -35,6 +35,7 @@ struct StreamState {
   uint32_t current_id = 0, baseline_id = 0;
   bool have_tree = false;
   bool ctr_armed = false;  // the device counters are zero and hold the current map count
+  bool labels_fixed = false;  // every leaf label is FIXED (the last tree checkpoint completed)
   uint64_t data_len = 0, C = 0, N = 0;
   int64_t last_map_ckpt = -1;  // checkpoint id of the last insert into the map
 

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobG* __restrict__ jobs
       pending = true;
     }
     if (pending) store_digest(J.tree + leaf, d);
-    J.labels[leaf] = lab;
+    if (pending || J.write_all_labels) J.labels[leaf] = lab;
   }
   record_pending(J, k, pending);
 }
@@ -294,7 +294,7 @@ __device__ __forceinline__ bool leaf_finish(const JobG& J, uint64_t k, const Dig
     pending = true;
   }
   if (pending) store_digest(J.tree + leaf, d);
-  J.labels[leaf] = lab;
+  if (pending || J.write_all_labels) J.labels[leaf] = lab;
   return pending;
 }
 
@@ -534,11 +534,11 @@ __device__ __forceinline__ InsertCtx insert_ctx(const JobG& J) {
   return InsertCtx{J.map, J.tree, J.C, J.ckpt_id, J.is_baseline != 0};
 }
 
-// Slot s of the map as {k0, k1} and {val, pad}.
-__device__ __forceinline__ void load_slot(const MapG& m, uint64_t s, uint4& key, uint4& val) {
-  const GlobalPtr<const uint8_t> p = (GlobalPtr<const uint8_t>)&m.slots[s];
-  key = gload16((GlobalPtr<const void>)p);
-  val = gload16((GlobalPtr<const void>)(p + 16));
+// Slot s of the map: its key {k0, k1} (one 16-byte load) and its value.
+__device__ __forceinline__ void load_slot(const MapG& m, uint64_t s, uint4& key, unsigned long long& val) {
+  const GlobalPtr<const Slot> sl = &m.slots[s];
+  key = gload16((GlobalPtr<const void>)sl);
+  val = sl->val;
 }
 
 // Settles an insert at slot s whose value is v (not EMPTY): the entry's key
@@ -602,13 +602,14 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
     s = (s + 1) & m.mask;
   }
   for (uint64_t it = 0; it <= m.mask; it += 2) {
-    uint4 kk[2], vv[2];
+    uint4 kk[2];
+    unsigned long long vv[2];
     load_slot(m, s, kk[0], vv[0]);
     load_slot(m, (s + 1) & m.mask, kk[1], vv[1]);
 #pragma unroll
     for (int e = 0; e < 2; e++) {
       const uint64_t se = (s + e) & m.mask;
-      unsigned long long v = vv[e].x | ((unsigned long long)vv[e].y << 32);
+      unsigned long long v = vv[e];
       const Digest key{kk[e].x | ((uint64_t)kk[e].y << 32), kk[e].z | ((uint64_t)kk[e].w << 32)};
       bool have_key = true;
       if (v == SLOT_EMPTY) {
@@ -1141,7 +1142,8 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
     s[n] = map_start(m, key[n]) & m.mask;
   }
   for (uint64_t it = 0; it <= m.mask; it += 2) {
-    uint4 k[NK][2], v[NK][2];
+    uint4 k[NK][2];
+    unsigned long long v[NK][2];
 #pragma unroll
     for (int n = 0; n < NK; n++) {
       if (!todo[n]) continue;
@@ -1155,7 +1157,7 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
 #pragma unroll
       for (int e = 0; e < 2; e++) {
         if (!todo[n]) break;
-        const unsigned long long val = v[n][e].x | ((unsigned long long)v[n][e].y << 32);
+        const unsigned long long val = v[n][e];
         const uint64_t k0 = k[n][e].x | ((uint64_t)k[n][e].y << 32), k1 = k[n][e].z | ((uint64_t)k[n][e].w << 32);
         if (val == SLOT_EMPTY) {
           todo[n] = false;
@@ -1520,12 +1522,12 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobG* __restrict__ jo
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t stiles_of(const JobG& J) { return (J.ctr->nshift + kSTile - 1) / kSTile; }
 
-// Label hygiene, run by k_gather once the roots are written: the interior labels (tile
-// root included) of every group-0 tile visited by this checkpoint are reset to
-// FIXED, which is what pass 2 computes for a tile whose leaves are all fixed.
-// A later checkpoint that does not visit the tile then reads FIXED labels
-// there, so the root scan needs no per-tile check.  Leaf labels are rewritten
-// by every leaf kernel.
+// Label hygiene, run by k_gather once the roots are written: every label of
+// every group-0 tile visited by this checkpoint (leaves, interior nodes, tile
+// root) is reset to FIXED, which is what pass 2 computes for a tile whose
+// leaves are all fixed.  A later checkpoint that does not visit the tile then
+// reads FIXED labels there, so the root scan needs no per-tile check, and its
+// leaf kernel writes the labels of changed leaves only.
 __device__ __forceinline__ void reset_visited_tiles(const JobG& J, uint32_t wg, uint32_t nwg) {
   const uint32_t Lr0 = J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0;
   const uint32_t ntiles0 = 1u << Lr0;
@@ -1538,8 +1540,13 @@ __device__ __forceinline__ void reset_visited_tiles(const JobG& J, uint32_t wg, 
   }
   for (uint32_t it = wg; it < nitems; it += nwg) {
     const uint32_t t = J.is_baseline ? it : list_item(J.dirty_list, J.dirty_cap, pre, it);
-    if (threadIdx.x + 1 < (1u << H)) {
-      const uint64_t x = tile_node(((uint64_t)1 << Lr0) - 1 + t, threadIdx.x);
+    const uint64_t r = ((uint64_t)1 << Lr0) - 1 + t;
+    if (threadIdx.x + 1 < (1u << H)) {  // levels above the bottom (256 threads, at most 255 positions)
+      const uint64_t x = tile_node(r, threadIdx.x);
+      if (x < J.N) J.labels[x] = FIXED_DUPL;
+    }
+    if (threadIdx.x < (1u << H)) {  // the bottom level
+      const uint64_t x = ((r + 1) << H) - 1 + threadIdx.x;
       if (x < J.N) J.labels[x] = FIXED_DUPL;
     }
   }

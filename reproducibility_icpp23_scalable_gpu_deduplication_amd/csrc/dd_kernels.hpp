@@ -60,6 +60,11 @@ struct JobT {
   // root.  Diff node ids of a list job are chunk indices (leaf - node_bias).
   uint32_t list_mode;
   uint32_t node_bias;
+  // 0: every leaf label is FIXED on entry (the previous tree checkpoint of
+  // the handle completed and reset the labels of the tiles it visited), so
+  // the leaf kernel writes the labels of changed leaves only.
+  uint32_t write_all_labels;
+  uint32_t pad0;
   // persistent state
   P<Digest> tree;        // N digests, heap order
   P<uint8_t> labels;     // N labels (padded for 16-byte reads)

@@ -117,6 +117,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   t.nfull = tl / cs;
   t.D = make_geom(C).D;
   t.is_baseline = 1;
+  t.write_all_labels = 1;
   t.epoch = 1;
   t.tree = tree.as<Digest>();
   t.labels = labels.as<uint8_t>();
