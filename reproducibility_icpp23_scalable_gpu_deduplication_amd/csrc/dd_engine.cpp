@@ -150,7 +150,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   const uint64_t Kcap = ((uint64_t)K | 63) + 1;  // ids 0 .. Kcap-1
   DD_TRY(s_hist.ensure(Kcap * ceil_div(C, kSTile) * 4));
   DD_TRY(ptab.ensure(Kcap * 8));
-  DD_TRY(ctr.ensure(sizeof(JobCounters) + 32 * kCntStride * 4, true, s));
+  // Two sets of 32 pending-list counters, one per checkpoint parity.
+  DD_TRY(ctr.ensure(sizeof(JobCounters) + 2 * 32 * kCntStride * 4, true, s));
   const uint64_t nlb = ceil_div(C, 256);
   // blk_claim (one per insert_leaves workgroup) then int_claim (one per
   // insert_interior workgroup); a baseline launches one of each per leaf block.
@@ -179,7 +180,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   }
   epoch += 1;
   const uint64_t dirty_cap = ceil_div(ntiles0, 32);
-  DD_TRY(lists.ensure((32 * kCntStride + 32 * dirty_cap) * 4));
+  DD_TRY(lists.ensure((2 * 32 * kCntStride + 32 * dirty_cap) * 4, true, s));
   const uint64_t need = max_diff_size(len, chunk_size, K);
   if (!out) {
     DD_TRY(diff.ensure(max_diff_size(len, chunk_size, (uint32_t)(Kcap - 1))));
@@ -215,8 +216,12 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.host_ctr = nullptr;  // set by the pipeline
   job.blk_claim = leafblk.as<uint32_t>();
   job.int_claim = job.blk_claim + nclaim;
-  // 32 pending-leaf counters follow the counters
-  job.pend_cnt = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
+  // The counter sets alternate with the checkpoint's parity: this checkpoint
+  // uses one, its k_sbin_scan zeroes the other for the next checkpoint.
+  const uint32_t par = cnt_par ^ 1u;  // adopted by commit() once the checkpoint completes
+  uint32_t* const pend_sets = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
+  job.pend_cnt = pend_sets + par * 32 * kCntStride;
+  job.pend_cnt_next = pend_sets + (1 - par) * 32 * kCntStride;
   job.pend_list = pend_list.as<uint32_t>();
   job.pend_cap = (uint32_t)pend_cap;
   job.fint_list = tiles.as<uint32_t>();
@@ -225,9 +230,10 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.ntiles = (uint32_t)ntiles;
   job.epoch = epoch;
   job.tile_epoch = tile_epoch.as<uint32_t>();
-  job.dirty_cnt = lists.as<uint32_t>();
+  job.dirty_cnt = lists.as<uint32_t>() + par * 32 * kCntStride;
+  job.dirty_cnt_next = lists.as<uint32_t>() + (1 - par) * 32 * kCntStride;
   job.dirty_cap = (uint32_t)dirty_cap;
-  job.dirty_list = job.dirty_cnt + 32 * kCntStride;
+  job.dirty_list = lists.as<uint32_t>() + 2 * 32 * kCntStride;
   job.blk_f = blk.as<uint32_t>();
   job.blk_fc = job.blk_f + nblk;
   job.blk_s = job.blk_fc + nblk;
@@ -252,6 +258,7 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
   // The gather kernel of a completed tree checkpoint reset every label of
   // the tiles it visited; the list method keeps its labels.
   labels_fixed = method == DD_METHOD_TREE;
+  cnt_par ^= 1u;  // this checkpoint's k_sbin_scan zeroed the other counter set
   nfirst = c.nfirst;
   nshift = c.nshift;
   map_count = c.map_count;
@@ -411,7 +418,6 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   DD_TRY(mark(6));
   launch_gather(J, P(S_GATHER), nj, grid_[S_GATHER], vec16_, s);
   DD_TRY(mark(7));
-  launch_finish(J, nj, s);
   DD_HIP(hipGetLastError());
   return DD_OK;
 }
@@ -446,7 +452,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   DD_TRY(st.prepare(reinterpret_cast<const uint8_t*>(dev_data), len, baseline, out, out_cap, job, h->stream));
   std::vector<JobDev> jobs(1, job);
   st.timers[0] = now_s() - t0;
-  // The counters are armed by the previous successful checkpoint (k_finish);
+  // The counters are armed by the previous successful checkpoint (k_sbin_scan);
   // a baseline resets the entry count, so it starts with k_begin.
   const bool begin = baseline || !st.ctr_armed;
   st.ctr_armed = false;
@@ -490,7 +496,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
     }
   }
   st.commit(c, baseline);
-  st.ctr_armed = true;  // k_finish re-armed the counters
+  st.ctr_armed = true;  // k_sbin_scan zeroed the next checkpoint's counter set
   *result = c;
   return DD_OK;
 }

@@ -45,6 +45,7 @@ struct StreamState {
   DevBuf blk, f_list, f_off, s_node, s_prev, s_tree, ss_node, ss_prev, s_hist, ptab;
   DevBuf leafblk, pend_list, tiles, tile_epoch, lists;
   uint32_t epoch = 0;     // stamp of the current checkpoint (tile_epoch)
+  uint32_t cnt_par = 1;   // work-list counter set of the last completed checkpoint (the next uses the other)
   uint64_t tile_epoch_n = 0;
   DevBuf diff;  // internal diff buffer (host-output path)
   DevBuf ctr;   // JobCounters

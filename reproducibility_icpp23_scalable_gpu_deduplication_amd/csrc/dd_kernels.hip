@@ -111,27 +111,6 @@ __global__ void k_begin(const JobG* __restrict__ jobs, uint32_t njobs, int rerun
   c->unresolved = 0;
 }
 
-// finish: after a checkpoint that needs no re-run, re-arm the counters for
-// the next one (the final entry count stays: it is the next checkpoint's
-// starting count).  Runs while the host wakes from its synchronization, so
-// the next checkpoint starts with the leaf kernel and no k_begin.
-__global__ void k_finish(const JobG* __restrict__ jobs, uint32_t njobs) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= njobs) return;
-  const JobG& J = jobs[j];
-  const GlobalPtr<JobCounters> c = J.ctr;
-  if (c->flags) return;  // a re-run or a failed checkpoint: the host re-arms with k_begin
-  for (uint32_t i = 0; i < 32; i++) {
-    J.pend_cnt[i * kCntStride] = 0;
-    J.dirty_cnt[i * kCntStride] = 0;
-  }
-  c->pending = 0;
-  c->nfirst = c->nshift = c->nprior = 0;
-  c->dchunks = 0;
-  c->diff_len = 0;
-  c->unresolved = 0;
-}
-
 // ---------------------------------------------------------------------------
 // leaf_hash: one lane per chunk.  Leaf digests are written in place over the
 // previous checkpoint's digests, only when they change.
@@ -1611,7 +1590,10 @@ __device__ __forceinline__ void export_counters(const JobG& J) {
 }
 
 // One workgroup per job: new map entries of the checkpoint, scan of the
-// shifted-root histogram, the prior table, the diff length.  Two rounds of
+// shifted-root histogram, the prior table, the diff length; and, once the
+// checkpoint needs no re-run, the next checkpoint's work-list counters are
+// zeroed (checkpoints alternate between two counter sets, so no kernel has
+// to re-arm them after the pipeline).  Two rounds of
 // loads: the claim counts with the counter fields, then the histogram, which
 // (at most 1024 entries, the usual case) is scanned and turned into per-bin
 // totals in LDS.
@@ -1640,6 +1622,12 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ job
       J.ctr->pending = pending;
     }
     return;
+  }
+  // The work-list counters of the next checkpoint (the set this checkpoint
+  // does not use) start at zero.
+  if (threadIdx.x < kPendSpread) {
+    J.pend_cnt_next[threadIdx.x * kCntStride] = 0;
+    J.dirty_cnt_next[threadIdx.x * kCntStride] = 0;
   }
   const uint32_t S = snshift;
   const uint32_t nt = (S + kSTile - 1) / kSTile;
@@ -2103,11 +2091,6 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
   }
 }
 #undef DD_LEAF_LAUNCH
-
-void launch_finish(const JobDev* jobs_h, uint32_t njobs, hipStream_t s) {
-  const JobG* jobs = gview(jobs_h);
-  DD_LAUNCH(k_finish, (njobs + 63) / 64, 64, 0, s, jobs, njobs);
-}
 
 void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);

@@ -78,6 +78,7 @@ struct JobT {
   // leaf block or tile index modulo 32) with one counter each, so no single
   // counter is hit by every workgroup; consumers prefix-sum the 32 counts.
   P<uint32_t> pend_cnt;    // 32 counters, kCntStride words apart
+  P<uint32_t> pend_cnt_next;   // the other set (the next checkpoint's), zeroed by k_sbin_scan
   P<uint32_t> pend_list;   // 32 x pend_cap: chunk indices of the leaves that need a map insert
   uint32_t pend_cap;
   // First-occurrence interior nodes found by pass 1, per tile (global tile
@@ -88,6 +89,7 @@ struct JobT {
   uint32_t epoch;        // stamp of this checkpoint (never 0)
   P<uint32_t> tile_epoch;  // per group-0 tile: epoch of the last checkpoint that changed one of its leaves
   P<uint32_t> dirty_cnt;   // 32 counters, kCntStride words apart
+  P<uint32_t> dirty_cnt_next;  // the other set (the next checkpoint's), zeroed by k_sbin_scan
   P<uint32_t> dirty_list;  // 32 x dirty_cap: group-0 tiles with a changed leaf
   uint32_t dirty_cap;
   uint32_t tile_base[4];  // first tile index of each group
@@ -142,8 +144,6 @@ constexpr uint32_t kCoverFirst = 0xffffffffu;
 
 // Launch helpers (dd_kernels.hip) -------------------------------------------
 void launch_begin(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, bool rerun, hipStream_t s);
-// Re-arms the counters of every job that finished without a flag.
-void launch_finish(const JobDev* jobs, uint32_t njobs, hipStream_t s);
 // Chunk-to-lane mappings of the leaf kernel.  DIRECT: one lane per chunk,
 // 16-byte loads, 64 bytes in flight per lane; DEEP: the same with 256 bytes
 // in flight (Murmur3); LDS<seg>: each wavefront loads `seg` bytes of 64

@@ -124,6 +124,8 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   t.ctr = ctr.as<JobCounters>();
   t.pend_cnt = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
   t.dirty_cnt = t.pend_cnt + 32 * kCntStride;
+  t.pend_cnt_next = t.pend_cnt;  // a baseline keeps no lists
+  t.dirty_cnt_next = t.dirty_cnt;
   t.pend_list = plist.as<uint32_t>();
   t.pend_cap = (uint32_t)pend_cap;
   // One workgroup prefix {0, grid} per mapping: the grid-stride mappings
