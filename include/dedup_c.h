@@ -89,7 +89,12 @@ dd_status dd_destroy(dd_dedup* h);
  * buffer against the handle's previous checkpoint, returns the diff on the
  * host in *out (header + metadata + data), and advances the checkpoint id.
  * An incremental checkpoint (make_baseline == 0 after the first call) must
- * keep the buffer length of the chain; otherwise DD_ESTATE. */
+ * keep the buffer length of the chain; otherwise DD_ESTATE.
+ * Errors (every checkpoint entry point, batched ones included): a call
+ * rejected by its argument checks (DD_EINVAL, DD_ESTATE) leaves the handle as
+ * it was.  A call that fails after it started to change the handle's device
+ * state (DD_ENOMEM, DD_ECORRUPT, DD_EHIP) leaves the handle needing a
+ * baseline: its next checkpoint runs as one whatever make_baseline says. */
 dd_status dd_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline,
                         dd_buf* out);
 

@@ -106,17 +106,25 @@ dd_status StreamState::grow_map(uint64_t need, hipStream_t s) {
   return DD_OK;
 }
 
-dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline, uint8_t* out, uint64_t out_cap,
-                               JobDev& job, hipStream_t s) {
-  const uint64_t cs = chunk_size;
-  const uint64_t Cn = ceil_div(len, cs);
+dd_status StreamState::validate(uint64_t len, bool baseline, const uint8_t* out, uint64_t out_cap) const {
+  const uint64_t Cn = ceil_div(len, chunk_size);
   if (len == 0 || Cn > 0x7fffffffull) return DD_EINVAL;
-  const uint64_t Nn = 2 * Cn - 1;
-  const uint32_t K = current_id;
   if (!baseline) {
     if (len != data_len) return DD_ESTATE;
-    if ((int64_t)K <= last_map_ckpt) return DD_ESTATE;
+    if ((int64_t)current_id <= last_map_ckpt) return DD_ESTATE;
   }
+  if (out && out_cap < max_diff_size(len, chunk_size, current_id)) return DD_EINVAL;
+  return DD_OK;
+}
+
+dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline, uint8_t* out, uint64_t out_cap,
+                               JobDev& job, hipStream_t s) {
+  DD_TRY(validate(len, baseline, out, out_cap));
+  const uint64_t cs = chunk_size;
+  const uint64_t Cn = ceil_div(len, cs);
+  const uint64_t Nn = 2 * Cn - 1;
+  const uint32_t K = current_id;
+  need_baseline = true;  // until commit()
   if (baseline) {
     DD_TRY(tree.ensure(Nn * sizeof(Digest)));
     DD_TRY(labels.ensure(ceil_div(Nn, kNodesPerBlock) * kNodesPerBlock + 64));  // root scans read 16 at a time
@@ -181,13 +189,10 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   epoch += 1;
   const uint64_t dirty_cap = ceil_div(ntiles0, 32);
   DD_TRY(lists.ensure((2 * 32 * kCntStride + 32 * dirty_cap) * 4, true, s));
-  const uint64_t need = max_diff_size(len, chunk_size, K);
   if (!out) {
     DD_TRY(diff.ensure(max_diff_size(len, chunk_size, (uint32_t)(Kcap - 1))));
     out = diff.as<uint8_t>();
     out_cap = diff.bytes;
-  } else if (out_cap < need) {
-    return DD_EINVAL;
   }
 
   memset(&job, 0, sizeof job);
@@ -255,6 +260,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
 
 void StreamState::commit(const JobCounters& c, bool baseline) {
   if (baseline) baseline_id = current_id;
+  need_baseline = false;
   // The gather kernel of a completed tree checkpoint reset every label of
   // the tiles it visited; the list method keeps its labels.
   labels_fixed = method == DD_METHOD_TREE;
@@ -447,7 +453,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   const double t0 = now_s();
   DD_HIP(hipSetDevice(h->device));
   StreamState& st = h->st;
-  const bool baseline = make_baseline || !st.have_tree || st.current_id == 0;
+  const bool baseline = st.runs_baseline(make_baseline);
   JobDev job;
   DD_TRY(st.prepare(reinterpret_cast<const uint8_t*>(dev_data), len, baseline, out, out_cap, job, h->stream));
   std::vector<JobDev> jobs(1, job);
@@ -965,10 +971,16 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
   std::vector<JobDev> jobs(n);
   std::vector<char> base(n);
   bool begin = false;  // k_begin for every job unless all counters are armed
+  // Every job is checked before any handle's state changes, so a rejected
+  // call leaves all of them as they were.
   for (size_t i = 0; i < n; i++) {
     if (!dev_data[i] || !dev_out[i]) return DD_EINVAL;
     StreamState& st = b->hs[i]->st;
-    base[i] = make_baseline[i] || !st.have_tree || st.current_id == 0;
+    base[i] = st.runs_baseline(make_baseline[i]);
+    DD_TRY(st.validate(lens[i], base[i], static_cast<const uint8_t*>(dev_out[i]), out_caps[i]));
+  }
+  for (size_t i = 0; i < n; i++) {
+    StreamState& st = b->hs[i]->st;
     begin = begin || base[i] || !st.ctr_armed;
     DD_TRY(st.prepare(static_cast<const uint8_t*>(dev_data[i]), lens[i], base[i], static_cast<uint8_t*>(dev_out[i]),
                       out_caps[i], jobs[i], b->stream));

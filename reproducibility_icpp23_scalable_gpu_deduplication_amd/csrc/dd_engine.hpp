@@ -36,6 +36,9 @@ struct StreamState {
   bool have_tree = false;
   bool ctr_armed = false;  // the device counters are zero and hold the current map count
   bool labels_fixed = false;  // every leaf label is FIXED (the last tree checkpoint completed)
+  // A checkpoint started changing the persistent state (tree, labels, map)
+  // and did not complete: the next checkpoint must be a baseline.
+  bool need_baseline = false;
   uint64_t data_len = 0, C = 0, N = 0;
   int64_t last_map_ckpt = -1;  // checkpoint id of the last insert into the map
 
@@ -54,8 +57,17 @@ struct StreamState {
   double timers[4] = {0, 0, 0, 0};
   double restart_timers[2] = {0, 0};
 
+  // Whether a checkpoint call with `make_baseline` runs as a baseline.
+  bool runs_baseline(int make_baseline) const {
+    return make_baseline || !have_tree || current_id == 0 || need_baseline;
+  }
+  // The checks of prepare() that change nothing: a call they reject leaves
+  // the handle as it was.
+  dd_status validate(uint64_t len, bool baseline, const uint8_t* out, uint64_t out_cap) const;
   // Sizes the state for a checkpoint of `len` bytes (fresh map and zero tree
   // on a baseline) and fills `job`.  `out`/`out_cap` are the diff buffer.
+  // From its first change of the persistent state until commit(), the
+  // handle needs a baseline.
   dd_status prepare(const uint8_t* data, uint64_t len, bool baseline, uint8_t* out, uint64_t out_cap, JobDev& job,
                     hipStream_t s);
   // Called after the checkpoint's counters are back on the host.
@@ -74,7 +86,8 @@ class Pipeline {
   // `stage_ev`, when given, holds DD_NUM_STAGES + 1 events recorded around
   // the stages; `ev_dedup` is recorded after the duplicate forest.
   // `begin` launches k_begin first; without it the counters must be armed
-  // (k_finish of a previous successful run, or zero-filled allocation).
+  // (the work-list counter set of this checkpoint's parity zeroed by the
+  // previous completed checkpoint's k_sbin_scan, or a zero-filled allocation).
   dd_status run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup = nullptr,
                 hipEvent_t* stage_ev = nullptr);
   // The two halves of run(): upload the job table (a copy on `s`), then

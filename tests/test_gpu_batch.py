@@ -54,6 +54,39 @@ def test_batch_map_growth_rerun(torch_gpu):
         assert diffs[i] == od
 
 
+def test_batch_rejected_call_changes_no_handle(torch_gpu):
+    # The second job's output buffer is too small: the whole call is
+    # rejected before any handle's state changes, and the next call continues
+    # both chains.
+    torch = torch_gpu
+    chains = [W.chain_uniform_changes(31 + i, s, 128, 3, fresh_frac=0.03, copy_frac=0.03, zero_bytes=4096)
+              for i, s in enumerate((1 << 18, 5000 * 128))]
+    handles = [dd.TreeDeduplicator(128) for _ in chains]
+    b = dd.Batch(handles, graph=True)
+    diffs = [[] for _ in chains]
+
+    def call(k, caps):
+        bufs = [torch.from_numpy(c[k]).to("cuda") for c in chains]
+        outs = [torch.empty(max(cap, 1), dtype=torch.uint8, device="cuda") for cap in caps]
+        torch.cuda.synchronize()
+        lens = b.checkpoint_device([t.data_ptr() for t in bufs], [t.numel() for t in bufs], [k == 0] * len(chains),
+                                   [o.data_ptr() for o in outs], caps)
+        torch.cuda.synchronize()
+        for i in range(len(chains)):
+            diffs[i].append(outs[i][:lens[i]].cpu().numpy().tobytes())
+
+    full = lambda k: [dd.max_diff_size(c[k].nbytes, 128, k) for c in chains]
+    call(0, full(0))
+    with pytest.raises(dd.DedupError) as e:
+        call(1, [full(1)[0], 16])
+    assert e.value.status == dd.DD_EINVAL
+    call(1, full(1))
+    call(2, full(2))
+    for i, c in enumerate(chains):
+        od, _ = oracle_chain(128, c)
+        assert diffs[i] == od, f"buffer {i}"
+
+
 def test_batch_equals_plain_path(torch_gpu):
     torch = torch_gpu
     sizes = [1 << 19, 9 << 16]

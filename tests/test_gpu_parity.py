@@ -182,6 +182,31 @@ def test_header_overload_does_not_advance(torch_gpu):
     assert hd2["num_shift_dupl"] == 3 and g.current_id == 1
 
 
+def test_rejected_call_leaves_the_chain_intact(torch_gpu):
+    # A call rejected by its checks (here a baseline request whose output
+    # buffer is too small) changes nothing: the next incremental checkpoint
+    # still deduplicates against the chain before it.
+    torch = torch_gpu
+    datas = W.chain_uniform_changes(77, 1 << 18, 128, 4, fresh_frac=0.03, copy_frac=0.03, zero_bytes=4096)
+    g = dd.TreeDeduplicator(128)
+    got = []
+    for k in (0, 1):
+        t = to_dev(torch, datas[k])
+        got.append(g.checkpoint(t.data_ptr(), t.numel(), k == 0))
+    t = to_dev(torch, datas[2])
+    small = torch.empty(64, dtype=torch.uint8, device="cuda")
+    with pytest.raises(dd.DedupError) as e:
+        g.checkpoint_device(t.data_ptr(), t.numel(), small.data_ptr(), small.numel(), True)
+    assert e.value.status == dd.DD_EINVAL
+    assert g.current_id == 2
+    for k in (2, 3):
+        t = to_dev(torch, datas[k])
+        got.append(g.checkpoint(t.data_ptr(), t.numel(), False))
+    od, _ = oracle_chain(128, datas)
+    for k in range(4):
+        assert got[k] == od[k], f"ckpt {k}: {first_mismatch(got[k], od[k])}"
+
+
 def test_error_paths(torch_gpu):
     torch = torch_gpu
     g = dd.TreeDeduplicator(16)
