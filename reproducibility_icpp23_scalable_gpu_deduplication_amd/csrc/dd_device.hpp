@@ -516,18 +516,4 @@ __host__ __device__ inline uint64_t map_start(const M& m, const Digest& k) {
   return ((k.k0 ^ (k.k1 * 0x9e3779b97f4a7c15ull)) * 0xff51afd7ed558ccdull) >> m.shift;
 }
 
-// Read-only lookup (every key of the map is published).  Returns the slot
-// index or ~0 when absent.
-__device__ inline uint64_t map_find(const MapG& m, const Digest& k) {
-  uint64_t s = map_start(m, k) & m.mask;
-  for (uint64_t it = 0; it <= m.mask; it++) {
-    GlobalPtr<const Slot> sl = &m.slots[s];
-    unsigned long long v = sl->val;
-    if (v == SLOT_EMPTY) return ~0ull;
-    if (sl->k0 == k.k0 && sl->k1 == k.k1) return s;
-    s = (s + 1) & m.mask;
-  }
-  return ~0ull;
-}
-
 }  // namespace dd

@@ -168,6 +168,10 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   // Pending leaves: sub-list r takes wavefronts w = r (mod 32), 64 leaves each.
   const uint64_t pend_cap = ceil_div(ceil_div(C, 64), 32) * 64;
   DD_TRY(pend_list.ensure(32 * pend_cap * 4));
+  // Digests beside the first eighth of each sub-list (every entry up to
+  // 12.5 % changed chunks): 2 bytes per chunk instead of 16.
+  const uint64_t dig_cap = std::min<uint64_t>(pend_cap, ceil_div(ceil_div(pend_cap, 8), 64) * 64);
+  DD_TRY(pend_dig.ensure(32 * dig_cap * sizeof(Digest)));
   const TreeGeom gm = make_geom(C);
   uint64_t ntiles = 0;
   uint32_t tile_base[4] = {0, 0, 0, 0};
@@ -229,6 +233,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.pend_cnt_next = pend_sets + (1 - par) * 32 * kCntStride;
   job.pend_list = pend_list.as<uint32_t>();
   job.pend_cap = (uint32_t)pend_cap;
+  job.dig_cap = (uint32_t)dig_cap;
+  job.pend_dig = pend_dig.as<Digest>();
   job.fint_list = tiles.as<uint32_t>();
   job.fint_n = job.fint_list + ntiles * kTileInterior;
   for (int g = 0; g < 4; g++) job.tile_base[g] = tile_base[g];

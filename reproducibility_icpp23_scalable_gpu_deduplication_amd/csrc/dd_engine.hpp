@@ -46,7 +46,7 @@ struct StreamState {
   DevBuf map;
   uint64_t map_cap = 0, map_count = 0;
   DevBuf blk, f_list, f_off, s_node, s_prev, s_tree, ss_node, ss_prev, s_hist, ptab;
-  DevBuf leafblk, pend_list, tiles, tile_epoch, lists;
+  DevBuf leafblk, pend_list, pend_dig, tiles, tile_epoch, lists;
   uint32_t epoch = 0;     // stamp of the current checkpoint (tile_epoch)
   uint32_t cnt_par = 1;   // work-list counter set of the last completed checkpoint (the next uses the other)
   uint64_t tile_epoch_n = 0;

@@ -137,11 +137,12 @@ __device__ __forceinline__ void list_append(GlobalPtr<uint32_t> cnt, GlobalPtr<u
 }
 
 // The leaves of a wavefront (64 consecutive chunks k, k >> 6 = w) that need a
-// map insert are appended to sub-list w % 32 of pend_list: one atomic per
+// map insert are appended to sub-list w % 32 of pend_list (with their digest
+// d in pend_dig while the sub-list is within dig_cap): one atomic per
 // wavefront with a changed leaf, no block barrier, so a block's waves retire
 // independently.  A baseline visits every leaf and tile densely and keeps no
 // lists.  Every lane of the wavefront calls this.
-__device__ __forceinline__ void record_pending(const JobG& J, uint64_t k, bool pending) {
+__device__ __forceinline__ void record_pending(const JobG& J, uint64_t k, bool pending, const Digest& d) {
   if (J.is_baseline) return;  // uniform: a workgroup belongs to one job
   const unsigned long long b = __ballot(pending);
   if (!b) return;
@@ -150,7 +151,11 @@ __device__ __forceinline__ void record_pending(const JobG& J, uint64_t k, bool p
   uint32_t base = 0;
   if (lane_id() == leader) base = g_add(&J.pend_cnt[r * kCntStride], (uint32_t)__popcll(b));
   base = __shfl(base, leader, 64);
-  if (pending) J.pend_list[(uint64_t)r * J.pend_cap + base + __popcll(b & lanemask_lt())] = (uint32_t)k;
+  const uint32_t pos = base + __popcll(b & lanemask_lt());
+  if (pending) {
+    J.pend_list[(uint64_t)r * J.pend_cap + pos] = (uint32_t)k;
+    if (pos < J.dig_cap) store_digest(J.pend_dig + (uint64_t)r * J.dig_cap + pos, d);
+  }
   // Mark the group-0 tiles of changed leaves: the first marker of a tile in
   // this checkpoint appends it to the dirty list.  The list method has no
   // tiles.
@@ -201,6 +206,33 @@ __device__ __forceinline__ uint32_t list_item(GlobalPtr<const uint32_t> list, ui
   return list[(uint64_t)r * cap + (i - pre[r])];
 }
 
+// Sub-list-major walk of a 32-way split work list by P workgroups that take
+// B items per step.  With P >= 32, workgroup w shares sub-list w % 32 with
+// the other workgroups of its residue and takes items q*B .. q*B+B-1 of it,
+// then every Q*B further (q = w / 32, Q workgroups share the sub-list); with
+// P < 32 it takes sub-lists w, w + P, ... whole.  A worker's first item does
+// not depend on the sub-list counts, so it can be loaded together with them.
+struct ListWalk {
+  uint32_t r, r_step;  // first sub-list, step to the next one
+  uint32_t j0, j_step;  // first item (of the workgroup's B) in the sub-list, step
+};
+
+__device__ __forceinline__ ListWalk list_walk(uint32_t w, uint32_t P, uint32_t B) {
+  ListWalk L;
+  if (P >= kPendSpread) {
+    L.r = w % kPendSpread;
+    L.r_step = kPendSpread;
+    L.j0 = (w / kPendSpread) * B;
+    L.j_step = ((P - 1 - L.r) / kPendSpread + 1) * B;
+  } else {
+    L.r = w;
+    L.r_step = P;
+    L.j0 = 0;
+    L.j_step = B;
+  }
+  return L;
+}
+
 __device__ __forceinline__ uint64_t pending_total(const JobG& J) {
   uint64_t t = 0;
 #pragma unroll
@@ -218,6 +250,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobG* __restrict__ jobs
   const JobG& J = jobs[j];
   const uint64_t k = (uint64_t)(blockIdx.x - prefix[j]) * blockDim.x + threadIdx.x;
   bool pending = false;
+  Digest d{0, 0};
   if (k < J.C) {
     const uint64_t cs = J.chunk_size;
     const uint64_t off = k * cs;
@@ -228,7 +261,6 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobG* __restrict__ jobs
     // behind the chunk loads instead of following the hash.
     Digest prev{0, 0};
     if (!J.is_baseline) prev = load_digest(J.tree + leaf);
-    Digest d;
     if (Kind == HASH_MD5) {
       d = Aligned16 ? md5_range<true>(p, nb) : md5_range<false>(p, nb);
     } else {
@@ -255,7 +287,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const JobG* __restrict__ jobs
     if (pending) store_digest(J.tree + leaf, d);
     if (pending || J.write_all_labels) J.labels[leaf] = lab;
   }
-  record_pending(J, k, pending);
+  record_pending(J, k, pending, d);
 }
 
 // Shared tail of both leaf kernels: compare with the previous digest, record
@@ -340,8 +372,8 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
     __syncthreads();
   }
   bool pending = false;
+  Digest d{0, 0};
   if (k < full) {
-    Digest d;
     if (Kind == HASH_MD5) {
       uint32_t X[16] = {0x80u, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       const uint64_t bits = cs * 8;
@@ -357,10 +389,10 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
     const uint64_t off = k * cs;
     const uint8_t* p = (const uint8_t*)(J.data + off);
     const uint64_t nb = J.len - off;
-    const Digest d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
+    d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
     pending = leaf_finish(J, k, d, prev);
   }
-  record_pending(J, k, pending);
+  record_pending(J, k, pending, d);
 }
 
 // leaf_hash, direct-to-LDS: each wavefront owns 64 consecutive chunks and an
@@ -452,8 +484,8 @@ __global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobG* __restrict__ 
     }
   }
   bool pending = false;
+  Digest d{0, 0};
   if (k < full) {
-    Digest d;
     if (Kind == HASH_MD5) {
       uint32_t X[16] = {0x80u, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       const uint64_t bits = cs * 8;
@@ -469,10 +501,10 @@ __global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobG* __restrict__ 
     const uint64_t off = k * cs;
     const uint8_t* p = (const uint8_t*)(J.data + off);
     const uint64_t nb = J.len - off;
-    const Digest d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
+    d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
     pending = leaf_finish(J, k, d, prev);
   }
-  record_pending(J, k, pending);
+  record_pending(J, k, pending, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -563,10 +595,15 @@ __device__ __forceinline__ unsigned long long map_claim(const MapG& m, uint64_t 
 
 // Linear probing; every round loads two consecutive slots (keys and values)
 // before settling either, so an insert that ends within two slots of where
-// the round starts costs one load round trip plus its atomic.  A slot that
-// was EMPTY when loaded is claimed by CAS; a failed CAS returns the value of
-// the lane that won, settled like a loaded one.  In a baseline the map
-// starts empty, so the home slot is claimed by CAS without a load first.
+// the round starts costs one load round trip.  The rounds only load: a lane
+// that reaches an EMPTY slot stops there, and the lanes of a wavefront that
+// stopped claim their slots with one CAS each after the wavefront's probing
+// ends.  A claim inside the rounds would hold every lane of the wavefront for
+// the CAS's round trip in each round where any lane claims (most rounds, with
+// 64 lanes).  A failed CAS returns the value of the lane that won, settled
+// like a loaded one; if that is another key, probing resumes after the slot.
+// In a baseline the map starts empty, so the home slot is claimed by CAS
+// without a load first.
 template <bool Leaf>
 __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, uint32_t node) {
   const MapG& m = J.m;
@@ -580,25 +617,33 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
     if (map_settle<Leaf>(J, k, s, old, none, false, mine, r)) return r;
     s = (s + 1) & m.mask;
   }
-  for (uint64_t it = 0; it <= m.mask; it += 2) {
-    uint4 kk[2];
-    unsigned long long vv[2];
-    load_slot(m, s, kk[0], vv[0]);
-    load_slot(m, (s + 1) & m.mask, kk[1], vv[1]);
+  uint64_t probed = 0;  // slots examined (bounds the walk of a full map)
+  while (probed <= m.mask) {
+    uint64_t cand = ~0ull;  // EMPTY slot to claim
+    for (; probed <= m.mask; probed += 2, s = (s + 2) & m.mask) {
+      uint4 kk[2];
+      unsigned long long vv[2];
+      load_slot(m, s, kk[0], vv[0]);
+      load_slot(m, (s + 1) & m.mask, kk[1], vv[1]);
 #pragma unroll
-    for (int e = 0; e < 2; e++) {
-      const uint64_t se = (s + e) & m.mask;
-      unsigned long long v = vv[e];
-      const Digest key{kk[e].x | ((uint64_t)kk[e].y << 32), kk[e].z | ((uint64_t)kk[e].w << 32)};
-      bool have_key = true;
-      if (v == SLOT_EMPTY) {
-        v = map_claim(m, se, k, mine);
-        if (v == SLOT_EMPTY) return InsertResult{true, true, se, SLOT_EMPTY};
-        have_key = false;  // the loaded key predates the claim
+      for (int e = 0; e < 2; e++) {
+        const uint64_t se = (s + e) & m.mask;
+        if (vv[e] == SLOT_EMPTY) {
+          cand = se;
+          break;
+        }
+        const Digest key{kk[e].x | ((uint64_t)kk[e].y << 32), kk[e].z | ((uint64_t)kk[e].w << 32)};
+        if (map_settle<Leaf>(J, k, se, vv[e], key, true, mine, r)) return r;
       }
-      if (map_settle<Leaf>(J, k, se, v, key, have_key, mine, r)) return r;
+      if (cand != ~0ull) break;
     }
-    s = (s + 2) & m.mask;
+    if (cand == ~0ull) break;
+    const unsigned long long v = map_claim(m, cand, k, mine);
+    if (v == SLOT_EMPTY) return InsertResult{true, true, cand, SLOT_EMPTY};
+    // The loaded key predates the claim: settle with the slot's key.
+    if (map_settle<Leaf>(J, k, cand, v, none, false, mine, r)) return r;
+    s = (cand + 1) & m.mask;
+    probed += 1;
   }
   return InsertResult{false, false, 0, 0};
 }
@@ -629,12 +674,23 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
   // A baseline inserts every leaf: lane k takes chunk k directly (the list
   // would hold every chunk too).
   const bool dense = J.is_baseline != 0;
-  // The flags and the pending-list counts in one round trip.  Other
-  // workgroups of this kernel may raise MAP_FULL while it runs: the decision
-  // is made once, uniform over the workgroup.
+  // The flags, the pending-list counts and (incremental) each lane's first
+  // list entry with its digest, all in one round trip.  Other workgroups of
+  // this kernel may raise MAP_FULL while it runs: the decision is made once,
+  // uniform over the workgroup.
   if (threadIdx.x == 0) sflags = J.ctr->flags;
-  if (!dense) load_list_prefix(J.pend_cnt, pre);
-  else __syncthreads();
+  ListWalk L{};
+  uint32_t spec_chunk = 0;
+  Digest spec_dig{0, 0};
+  if (!dense) {
+    L = list_walk(wg, nwg, blockDim.x);
+    const uint32_t j0 = L.j0 + threadIdx.x;
+    if (j0 < J.pend_cap) spec_chunk = J.pend_list[(uint64_t)L.r * J.pend_cap + j0];
+    if (j0 < J.dig_cap) spec_dig = load_digest(J.pend_dig + (uint64_t)L.r * J.dig_cap + j0);
+    load_list_prefix(J.pend_cnt, pre);
+  } else {
+    __syncthreads();
+  }
   if (sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) {
     if (threadIdx.x == 0) J.blk_claim[wg] = 0;
     return;
@@ -654,26 +710,12 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
   const InsertCtx ic = insert_ctx(J);
   const GlobalPtr<uint8_t> labels = J.labels;
   const GlobalPtr<uint32_t> slot_of = J.slot_of;
-  const GlobalPtr<const uint32_t> pend_list = J.pend_list;
-  const uint32_t pend_cap = J.pend_cap;
   const GlobalPtr<uint32_t> flags = &J.ctr->flags;
   const uint32_t lane = lane_id();
-  // The loop bound is uniform over the workgroup, so every lane of a
-  // wavefront takes part in its shuffles; lanes past the list are inactive.
-  for (uint32_t base = wg * blockDim.x; base < nitems; base += nwg * blockDim.x) {
-    const uint32_t it = base + threadIdx.x;
-    bool act = it < nitems;
-    uint64_t leaf = 0;
-    uint8_t own = 0;
-    Digest k{0, 0};
-    if (act) {
-      leaf = ic.C - 1 + (dense ? it : list_item(pend_list, pend_cap, pre, it));
-      own = labels[leaf];
-      // A list baseline leaves all-zero digests out (they equal the fresh
-      // list's entry); every other leaf of a dense pass needs its insert.
-      if (own == FIXED_DUPL) act = false;
-      else k = load_digest(ic.tree + leaf);
-    }
+  // One item per lane: leaf `leaf` with label `own` and digest `k`.  Called
+  // by every lane of a wavefront together (uniform loop bounds), since it
+  // shuffles; lanes past the list are inactive.
+  auto insert_item = [&](bool act, uint64_t leaf, uint8_t own, const Digest& k) {
     const unsigned long long pk0 = __shfl_up(k.k0, 1, 64), pk1 = __shfl_up(k.k1, 1, 64);
     const unsigned long long pleaf = __shfl_up((unsigned long long)leaf, 1, 64);
     const int pact = __shfl_up((int)act, 1, 64);
@@ -717,6 +759,45 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
       if (own == PENDING) labels[leaf] = SHIFT_DUPL;  // a lower leaf of this run holds the digest
     } else if (act && head && ok) {
       slot_of[leaf] = (uint32_t)slot;
+    }
+  };
+  if (dense) {
+    for (uint32_t base = wg * blockDim.x; base < nitems; base += nwg * blockDim.x) {
+      const uint32_t it = base + threadIdx.x;
+      bool act = it < nitems;
+      const uint64_t leaf = ic.C - 1 + it;
+      uint8_t own = 0;
+      Digest k{0, 0};
+      if (act) {
+        own = labels[leaf];
+        // A list baseline leaves all-zero digests out (they equal the fresh
+        // list's entry); every other leaf of a dense pass needs its insert.
+        if (own == FIXED_DUPL) act = false;
+        else k = load_digest(ic.tree + leaf);
+      }
+      insert_item(act, leaf, own, k);
+    }
+  } else {
+    // Every pending leaf of an incremental checkpoint was labelled PENDING
+    // by the leaf kernel (a re-run after a map growth may find it SHIFT
+    // already, and writing SHIFT again changes nothing).
+    bool first = true;
+    for (uint32_t r = L.r; r < kPendSpread; r += L.r_step) {
+      const uint32_t cnt = pre[r + 1] - pre[r];
+      for (uint32_t jb = L.j0; jb < cnt; jb += L.j_step, first = false) {
+        const uint32_t jj = jb + threadIdx.x;
+        const bool act = jj < cnt;
+        uint64_t leaf = 0;
+        Digest k{0, 0};
+        if (act) {
+          const uint64_t e = (uint64_t)r * J.pend_cap + jj;
+          leaf = ic.C - 1 + (first ? spec_chunk : J.pend_list[e]);
+          if (jj >= J.dig_cap) k = load_digest(ic.tree + leaf);
+          else k = first ? spec_dig : load_digest(J.pend_dig + (uint64_t)r * J.dig_cap + jj);
+        }
+        insert_item(act, leaf, PENDING, k);
+      }
+      first = false;
     }
   }
   const uint32_t n = block_sum<uint32_t>(claims, sh);

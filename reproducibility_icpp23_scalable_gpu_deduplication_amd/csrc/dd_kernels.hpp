@@ -81,6 +81,11 @@ struct JobT {
   P<uint32_t> pend_cnt_next;   // the other set (the next checkpoint's), zeroed by k_sbin_scan
   P<uint32_t> pend_list;   // 32 x pend_cap: chunk indices of the leaves that need a map insert
   uint32_t pend_cap;
+  // Digests of the first dig_cap entries of every pending sub-list (32 x
+  // dig_cap, beside pend_list), so the leaf inserts read an entry and its
+  // digest in one round trip; entries past dig_cap are read from the tree.
+  uint32_t dig_cap;
+  P<Digest> pend_dig;
   // First-occurrence interior nodes found by pass 1, per tile (global tile
   // id T over all groups): fint_n[T] entries at fint_list[T * kTileInterior].
   // Pass 1 writes fint_n for every tile it visits, so no clearing is needed.
