@@ -297,8 +297,10 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   aligned16_ = vec16_ = true;
   hash_kind_ = jobs[0].hash_kind;
   list_mode_ = jobs[0].list_mode != 0;
+  kinds_ = 0;
   for (const JobDev& J : jobs) {
     if (J.hash_kind != hash_kind_ || (J.list_mode != 0) != list_mode_) return DD_EINVAL;
+    kinds_ |= J.is_baseline ? JOBS_BASELINE : JOBS_INCREMENTAL;
     max_groups_ = std::max(max_groups_, groups_of(J.D));
     const bool al = (reinterpret_cast<uintptr_t>(J.data) & 15) == 0 && (J.chunk_size & 15) == 0;
     aligned16_ = aligned16_ && al;
@@ -374,7 +376,7 @@ std::vector<uint64_t> Pipeline::signature() const {
   sig.push_back((uint64_t)max_groups_);
   sig.push_back((uint64_t)leaf_map_);
   sig.push_back((uint64_t)aligned16_ | ((uint64_t)vec16_ << 1));
-  sig.push_back((uint64_t)hash_kind_ | ((uint64_t)list_mode_ << 8));
+  sig.push_back((uint64_t)hash_kind_ | ((uint64_t)list_mode_ << 8) | ((uint64_t)kinds_ << 16));
   sig.push_back(reinterpret_cast<uintptr_t>(dev_table_.p));
   for (uint32_t o : off_) sig.push_back(o);
   for (uint32_t g : grid_) sig.push_back(g);
@@ -406,7 +408,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
                 leaf_events ? stage_ev[0] : nullptr, leaf_events ? stage_ev[1] : nullptr);
   }
   DD_TRY(mark(1));
-  launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], s);
+  launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], kinds_, s);
   DD_TRY(mark(2));
   if (list_mode_) {
     // The list method has no tree: its chunks are classified by the inserts.
@@ -416,7 +418,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   } else {
     for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
     DD_TRY(mark(3));
-    launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], s);
+    launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], kinds_, s);
     DD_TRY(mark(4));
     for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
   }

@@ -118,6 +118,7 @@ class Pipeline {
   bool aligned16_ = true, vec16_ = true;
   int hash_kind_ = 0;
   bool list_mode_ = false;  // list-method jobs: no tree passes
+  uint32_t kinds_ = 0;      // JOBS_BASELINE | JOBS_INCREMENTAL present in the table
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
 
  public:

@@ -593,19 +593,27 @@ __device__ __forceinline__ unsigned long long map_claim(const MapG& m, uint64_t 
   return old;
 }
 
-// Linear probing; every round loads two consecutive slots (keys and values)
-// before settling either, so an insert that ends within two slots of where
-// the round starts costs one load round trip.  The rounds only load: a lane
-// that reaches an EMPTY slot stops there, and the lanes of a wavefront that
-// stopped claim their slots with one CAS each after the wavefront's probing
-// ends.  A claim inside the rounds would hold every lane of the wavefront for
-// the CAS's round trip in each round where any lane claims (most rounds, with
-// 64 lanes).  A failed CAS returns the value of the lane that won, settled
-// like a loaded one; if that is another key, probing resumes after the slot.
-// In a baseline the map starts empty, so the home slot is claimed by CAS
-// without a load first.
-template <bool Leaf>
+// Linear probing.  The rounds only load: a lane that reaches an EMPTY slot
+// stops there, and the lanes of a wavefront that stopped claim their slots
+// with one CAS each after the wavefront's probing ends.  A claim inside the
+// rounds would hold every lane of the wavefront for the CAS's round trip in
+// each round where any lane claims (most rounds, with 64 lanes).  A failed
+// CAS returns the value of the lane that won, settled like a loaded one; if
+// that is another key, probing resumes after the slot.  In a baseline the map
+// starts empty, so the home slot is claimed by CAS without a load first.
+//
+// Round widths.  Narrow (baselines, whose inserts are bound by random-line
+// traffic and want many waves in flight): two consecutive slots per round.
+// Wide (incremental checkpoints, whose few inserts are bound by each
+// wavefront's longest chain, and each round costs the slowest of its lanes'
+// random loads): the rest of the home slot's 128-byte line, then two lines
+// (eight slots) per round.  All slots of a round are loaded (keys and values)
+// before any is settled.
+constexpr uint32_t kLineSlots = 4;  // 32-byte slots per 128-byte line
+
+template <bool Leaf, bool Wide>
 __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, uint32_t node) {
+  constexpr uint32_t RW = Wide ? 2 * kLineSlots : 2;  // slots per round after the first
   const MapG& m = J.m;
   const unsigned long long mine = pack_node(J.ckpt, node);
   uint64_t s = map_start(m, k) & m.mask;
@@ -620,14 +628,21 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
   uint64_t probed = 0;  // slots examined (bounds the walk of a full map)
   while (probed <= m.mask) {
     uint64_t cand = ~0ull;  // EMPTY slot to claim
-    for (; probed <= m.mask; probed += 2, s = (s + 2) & m.mask) {
-      uint4 kk[2];
-      unsigned long long vv[2];
-      load_slot(m, s, kk[0], vv[0]);
-      load_slot(m, (s + 1) & m.mask, kk[1], vv[1]);
+    // Round: slots base+lo .. base+hi-1 (the capacity is a power of two
+    // >= 1024, so base+e wraps with the mask).
+    uint64_t base = Wide ? (s & ~(uint64_t)(kLineSlots - 1)) : s;
+    uint32_t lo = Wide ? (uint32_t)(s & (kLineSlots - 1)) : 0u;
+    uint32_t hi = Wide ? kLineSlots : RW;
+    for (; probed <= m.mask; probed += hi - lo, base = (base + hi) & m.mask, lo = 0, hi = RW) {
+      uint4 kk[RW];
+      unsigned long long vv[RW];
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        const uint64_t se = (s + e) & m.mask;
+      for (uint32_t e = 0; e < RW; e++)
+        if (e >= lo && e < hi) load_slot(m, (base + e) & m.mask, kk[e], vv[e]);
+#pragma unroll
+      for (uint32_t e = 0; e < RW; e++) {
+        if (e < lo || e >= hi) continue;
+        const uint64_t se = (base + e) & m.mask;
         if (vv[e] == SLOT_EMPTY) {
           cand = se;
           break;
@@ -663,9 +678,13 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
 // occurrence, so only the run's first lane touches the map and the others
 // label themselves SHIFT with the first lane's slot.  Long runs of one digest
 // (zero-filled regions) would otherwise serialize on one slot's atomics.
+// Wide: the incremental jobs of the table (wide probe rounds); otherwise the
+// baselines.  A table with both kinds launches both variants.
+template <bool Wide>
 __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
+  if ((J.is_baseline == 0) != Wide) return;  // the other variant's job
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   uint32_t claims = 0;
   __shared__ uint32_t sh[4];
@@ -723,7 +742,7 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
     bool ok = true;
     uint64_t slot = 0;
     if (act && head) {
-      const InsertResult r = map_insert<true>(ic, k, (uint32_t)leaf);
+      const InsertResult r = map_insert<true, Wide>(ic, k, (uint32_t)leaf);
       ok = r.ok;
       slot = r.slot;
       if (!r.ok) {
@@ -862,9 +881,11 @@ __device__ __forceinline__ uint64_t tile_node(uint64_t r, uint32_t i) {
 
 // Map inserts of the first-occurrence interior nodes that pass 1 listed: one
 // lane per list entry (grid-stride).
+template <bool Wide>
 __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
+  if ((J.is_baseline == 0) != Wide) return;  // the other variant's job
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
@@ -883,7 +904,7 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
       const GlobalPtr<const uint8_t> labels = J.labels;
       for (uint64_t x = (uint64_t)wg * blockDim.x + threadIdx.x; x + 1 < J.C; x += (uint64_t)nwg * blockDim.x) {
         if (labels[x] != FIRST_OCUR) continue;
-        const InsertResult r = map_insert<false>(ic, load_digest(ic.tree + x), (uint32_t)x);
+        const InsertResult r = map_insert<false, Wide>(ic, load_digest(ic.tree + x), (uint32_t)x);
         if (!r.ok) g_or(flags, FLAG_MAP_FULL);
         claims += r.claimed ? 1 : 0;
       }
@@ -904,7 +925,7 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
         uint32_t x0 = region[lane];
         for (uint32_t q = lane; q < cnt; q += 64) {
           const uint32_t x = q < 64 ? x0 : region[q];
-          const InsertResult r = map_insert<false>(ic, load_digest(ic.tree + x), x);
+          const InsertResult r = map_insert<false, Wide>(ic, load_digest(ic.tree + x), x);
           if (!r.ok) g_or(flags, FLAG_MAP_FULL);
           claims += r.claimed ? 1 : 0;
         }
@@ -2173,9 +2194,12 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
 }
 #undef DD_LEAF_LAUNCH
 
-void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
+                          hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
-  if (grid) DD_LAUNCH(k_insert_leaves, grid, 256, 0, s, jobs, prefix, njobs);
+  if (!grid) return;
+  if (kinds & JOBS_BASELINE) DD_LAUNCH(k_insert_leaves<false>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_INCREMENTAL) DD_LAUNCH(k_insert_leaves<true>, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_list_resolve(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
@@ -2185,9 +2209,12 @@ void launch_pass1(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, 
   const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_pass1, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
 }
-void launch_insert_interior(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
+void launch_insert_interior(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid,
+                            uint32_t kinds, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
-  if (grid) DD_LAUNCH(k_insert_interior, grid, 256, 0, s, jobs, prefix, njobs);
+  if (!grid) return;
+  if (kinds & JOBS_BASELINE) DD_LAUNCH(k_insert_interior<false>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_INCREMENTAL) DD_LAUNCH(k_insert_interior<true>, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_pass2(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);

@@ -177,12 +177,17 @@ void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uin
 // Workgroups of the leaf stage for a job of C chunks: one per 256 chunks in
 // every mapping.
 inline uint64_t leaf_workgroups(int, uint64_t C) { return (C + 255) / 256; }
-void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+// Kinds of jobs in a table (bit mask): the insert stages launch one kernel
+// variant per kind present.
+enum : uint32_t { JOBS_BASELINE = 1, JOBS_INCREMENTAL = 2 };
+void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
+                          hipStream_t s);
 // List method: the leaves left PENDING by insert_leaves are first
 // occurrences; a baseline also sets every interior label to DONE.
 void launch_list_resolve(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
-void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
+                            hipStream_t s);
 void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
 void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
