@@ -3,6 +3,7 @@ buffer on the plain path and to the oracle, with and without hipGraph replay."""
 import numpy as np
 import pytest
 
+import oracle as O
 import reproducibility_icpp23_scalable_gpu_deduplication_amd as dd
 from reproducibility_icpp23_scalable_gpu_deduplication_amd import workloads as W
 from gpu_util import first_mismatch, gpu_restart, oracle_chain
@@ -85,6 +86,38 @@ def test_batch_rejected_call_changes_no_handle(torch_gpu):
     for i, c in enumerate(chains):
         od, _ = oracle_chain(128, c)
         assert diffs[i] == od, f"buffer {i}"
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("cls", [dd.TreeDeduplicator, dd.ListDeduplicator])
+def test_batch_mixing_baselines_and_incrementals(torch_gpu, graph, cls):
+    # Handle i re-baselines at checkpoint index i + 1, so calls 1-3 mix one
+    # baseline with incremental checkpoints (both insert kernel variants run
+    # in one call).
+    torch = torch_gpu
+    sizes = (1 << 18, 777 * 128, 5000 * 128 + 40)
+    chains = [W.chain_uniform_changes(61 + i, s, 128, 5, fresh_frac=0.03, copy_frac=0.04, zero_bytes=4096)
+              for i, s in enumerate(sizes)]
+    base = [{0, i + 1} for i in range(len(chains))]
+    handles = [cls(128) for _ in chains]
+    b = dd.Batch(handles, graph=graph)
+    diffs = [[] for _ in chains]
+    for k in range(5):
+        bufs = [torch.from_numpy(c[k]).to("cuda") for c in chains]
+        caps = [dd.max_diff_size(c[k].nbytes, 128, k) for c in chains]
+        outs = [torch.empty(cap, dtype=torch.uint8, device="cuda") for cap in caps]
+        torch.cuda.synchronize()
+        lens = b.checkpoint_device([t.data_ptr() for t in bufs], [t.numel() for t in bufs],
+                                   [k in base[i] for i in range(len(chains))], [o.data_ptr() for o in outs], caps)
+        torch.cuda.synchronize()
+        for i in range(len(chains)):
+            diffs[i].append(outs[i][:lens[i]].cpu().numpy().tobytes())
+    ocls = O.OracleListDeduplicator if cls is dd.ListDeduplicator else O.OracleDeduplicator
+    for i, c in enumerate(chains):
+        od, _ = oracle_chain(128, c, baselines=base[i], cls=ocls)
+        for k in range(len(c)):
+            assert diffs[i][k] == od[k], f"buffer {i} ckpt {k}: {first_mismatch(diffs[i][k], od[k])}"
+        assert gpu_restart(torch, handles[i], diffs[i], 4, c[4].nbytes) == c[4].tobytes()
 
 
 def test_batch_equals_plain_path(torch_gpu):
