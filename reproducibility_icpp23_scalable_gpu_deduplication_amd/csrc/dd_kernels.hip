@@ -515,8 +515,8 @@ __global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobG* __restrict__ 
 // read from the tree array (tree[node]), written by an earlier kernel, so no
 // lane ever waits for another lane's key store; the claiming lane stores the
 // key into the slot for later kernels.  Entries of the same checkpoint and
-// digest then agree on one node: the lowest (leaves, UseMax=false) or the
-// highest (interior nodes, UseMax=true), which is the first one the serial
+// digest then agree on one node: the lowest (leaves, atomicMin) or the
+// highest (interior nodes, atomicMax), which is the first one the serial
 // reference visits (include/reference_impl.hpp:43-82).
 // ---------------------------------------------------------------------------
 struct InsertResult {
@@ -571,8 +571,15 @@ __device__ __forceinline__ bool map_settle(const InsertCtx& J, const Digest& k, 
     e.k1 = J.m.slots[s].k1;
   }
   if (!digest_eq(e, k)) return false;
+  // An entry of this checkpoint only moves toward `mine` (down for leaves,
+  // up for interior nodes).  When the value seen is already past `mine`, the
+  // atomic would leave the entry as it is and return a value past `mine` as
+  // well, which the caller treats the same way, so it is skipped: a digest
+  // repeated all over a buffer (zero chunks) would otherwise send one atomic
+  // per run of it to one slot.
   unsigned long long prev = v;
-  if (vt == J.ckpt) prev = Leaf ? g_min(&J.m.slots[s].val, mine) : g_max(&J.m.slots[s].val, mine);
+  if (vt == J.ckpt && (Leaf ? mine < v : mine > v))
+    prev = Leaf ? g_min(&J.m.slots[s].val, mine) : g_max(&J.m.slots[s].val, mine);
   r = InsertResult{true, false, s, prev};
   return true;
 }
