@@ -214,6 +214,13 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.list_mode = method == DD_METHOD_LIST ? 1 : 0;
   job.node_bias = job.list_mode ? (uint32_t)(C - 1) : 0;
   job.write_all_labels = (baseline || job.list_mode || !labels_fixed) ? 1 : 0;
+  // Wide probe rounds while the changed leaves stay under 1/8 of the chunks
+  // (the last incremental checkpoint's count predicts this one's; a handle
+  // with none yet starts wide): a wavefront then waits for its longest
+  // chain.  Above that the inserts are bound by random-line traffic, where
+  // narrow rounds keep more waves in flight (an MD5 field with a fifth of
+  // its chunks changed: 7.8 ms per checkpoint instead of 8.1-20.8 ms).
+  job.wide_probe = (!baseline && (last_pending == ~0ull || last_pending * 8 < Cn)) ? 1 : 0;
   labels_fixed = false;  // until this checkpoint completes
   job.tree = tree.as<Digest>();
   job.labels = labels.as<uint8_t>();
@@ -271,6 +278,7 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
   // the tiles it visited; the list method keeps its labels.
   labels_fixed = method == DD_METHOD_TREE;
   cnt_par ^= 1u;  // this checkpoint's k_sbin_scan zeroed the other counter set
+  if (!baseline) last_pending = c.pending;
   nfirst = c.nfirst;
   nshift = c.nshift;
   map_count = c.map_count;
@@ -300,7 +308,7 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   kinds_ = 0;
   for (const JobDev& J : jobs) {
     if (J.hash_kind != hash_kind_ || (J.list_mode != 0) != list_mode_) return DD_EINVAL;
-    kinds_ |= J.is_baseline ? JOBS_BASELINE : JOBS_INCREMENTAL;
+    kinds_ |= J.wide_probe ? JOBS_WIDE : JOBS_NARROW;
     max_groups_ = std::max(max_groups_, groups_of(J.D));
     const bool al = (reinterpret_cast<uintptr_t>(J.data) & 15) == 0 && (J.chunk_size & 15) == 0;
     aligned16_ = aligned16_ && al;

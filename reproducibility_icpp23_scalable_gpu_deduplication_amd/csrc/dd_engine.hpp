@@ -49,6 +49,7 @@ struct StreamState {
   DevBuf leafblk, pend_list, pend_dig, tiles, tile_epoch, lists;
   uint32_t epoch = 0;     // stamp of the current checkpoint (tile_epoch)
   uint32_t cnt_par = 1;   // work-list counter set of the last completed checkpoint (the next uses the other)
+  uint64_t last_pending = ~0ull;  // pending leaves of the last completed incremental checkpoint (~0: none yet)
   uint64_t tile_epoch_n = 0;
   DevBuf diff;  // internal diff buffer (host-output path)
   DevBuf ctr;   // JobCounters
@@ -118,7 +119,7 @@ class Pipeline {
   bool aligned16_ = true, vec16_ = true;
   int hash_kind_ = 0;
   bool list_mode_ = false;  // list-method jobs: no tree passes
-  uint32_t kinds_ = 0;      // JOBS_BASELINE | JOBS_INCREMENTAL present in the table
+  uint32_t kinds_ = 0;      // JOBS_NARROW | JOBS_WIDE present in the table
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
 
  public:

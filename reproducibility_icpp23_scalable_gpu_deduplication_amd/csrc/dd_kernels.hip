@@ -683,13 +683,13 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
 // occurrence, so only the run's first lane touches the map and the others
 // label themselves SHIFT with the first lane's slot.  Long runs of one digest
 // (zero-filled regions) would otherwise serialize on one slot's atomics.
-// Wide: the incremental jobs of the table (wide probe rounds); otherwise the
-// baselines.  A table with both kinds launches both variants.
+// Wide: the jobs of the table that probe in wide rounds (wide_probe);
+// otherwise the others.  A table with both kinds launches both variants.
 template <bool Wide>
 __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
-  if ((J.is_baseline == 0) != Wide) return;  // the other variant's job
+  if ((J.wide_probe != 0) != Wide) return;  // the other variant's job
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   uint32_t claims = 0;
   __shared__ uint32_t sh[4];
@@ -890,7 +890,7 @@ template <bool Wide>
 __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
-  if ((J.is_baseline == 0) != Wide) return;  // the other variant's job
+  if ((J.wide_probe != 0) != Wide) return;  // the other variant's job
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
@@ -2203,8 +2203,8 @@ void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t
                           hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (!grid) return;
-  if (kinds & JOBS_BASELINE) DD_LAUNCH(k_insert_leaves<false>, grid, 256, 0, s, jobs, prefix, njobs);
-  if (kinds & JOBS_INCREMENTAL) DD_LAUNCH(k_insert_leaves<true>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_NARROW) DD_LAUNCH(k_insert_leaves<false>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_WIDE) DD_LAUNCH(k_insert_leaves<true>, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_list_resolve(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
@@ -2218,8 +2218,8 @@ void launch_insert_interior(const JobDev* jobs_h, const uint32_t* prefix, uint32
                             uint32_t kinds, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (!grid) return;
-  if (kinds & JOBS_BASELINE) DD_LAUNCH(k_insert_interior<false>, grid, 256, 0, s, jobs, prefix, njobs);
-  if (kinds & JOBS_INCREMENTAL) DD_LAUNCH(k_insert_interior<true>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_NARROW) DD_LAUNCH(k_insert_interior<false>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_WIDE) DD_LAUNCH(k_insert_interior<true>, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_pass2(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);

@@ -64,7 +64,10 @@ struct JobT {
   // the handle completed and reset the labels of the tiles it visited), so
   // the leaf kernel writes the labels of changed leaves only.
   uint32_t write_all_labels;
-  uint32_t pad0;
+  // 1: the map inserts probe in wide rounds (few inserts expected, each
+  // wavefront waiting for its longest chain); 0: narrow rounds (baselines
+  // and change-heavy checkpoints, bound by random-line traffic).
+  uint32_t wide_probe;
   // persistent state
   P<Digest> tree;        // N digests, heap order
   P<uint8_t> labels;     // N labels (padded for 16-byte reads)
@@ -177,9 +180,9 @@ void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uin
 // Workgroups of the leaf stage for a job of C chunks: one per 256 chunks in
 // every mapping.
 inline uint64_t leaf_workgroups(int, uint64_t C) { return (C + 255) / 256; }
-// Kinds of jobs in a table (bit mask): the insert stages launch one kernel
-// variant per kind present.
-enum : uint32_t { JOBS_BASELINE = 1, JOBS_INCREMENTAL = 2 };
+// Probe widths of the jobs in a table (bit mask): the insert stages launch
+// one kernel variant per width present.
+enum : uint32_t { JOBS_NARROW = 1, JOBS_WIDE = 2 };
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
                           hipStream_t s);
 // List method: the leaves left PENDING by insert_leaves are first
