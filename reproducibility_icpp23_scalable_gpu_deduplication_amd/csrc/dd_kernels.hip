@@ -696,8 +696,9 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
   __shared__ uint32_t pre[kPendSpread + 1];
   __shared__ uint32_t sflags;
   // A baseline inserts every leaf: lane k takes chunk k directly (the list
-  // would hold every chunk too).
-  const bool dense = J.is_baseline != 0;
+  // would hold every chunk too).  A wide-probe job is never a baseline, so
+  // the wide variant has no dense path (fewer registers).
+  const bool dense = !Wide && J.is_baseline != 0;
   // The flags, the pending-list counts and (incremental) each lane's first
   // list entry with its digest, all in one round trip.  Other workgroups of
   // this kernel may raise MAP_FULL while it runs: the decision is made once,
@@ -903,7 +904,7 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
   if (!(sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
     const InsertCtx ic = insert_ctx(J);
     const GlobalPtr<uint32_t> flags = &J.ctr->flags;
-    if (J.is_baseline) {
+    if (!Wide && J.is_baseline) {  // a wide-probe job is never a baseline
       // A baseline visits every interior node 0 .. C-2 by label (most of
       // them are first occurrences).
       const GlobalPtr<const uint8_t> labels = J.labels;
