@@ -308,7 +308,7 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   kinds_ = 0;
   for (const JobDev& J : jobs) {
     if (J.hash_kind != hash_kind_ || (J.list_mode != 0) != list_mode_) return DD_EINVAL;
-    kinds_ |= J.wide_probe ? JOBS_WIDE : JOBS_NARROW;
+    kinds_ |= J.is_baseline ? JOBS_DENSE : J.wide_probe ? JOBS_LIST_WIDE : JOBS_LIST_NARROW;
     max_groups_ = std::max(max_groups_, groups_of(J.D));
     const bool al = (reinterpret_cast<uintptr_t>(J.data) & 15) == 0 && (J.chunk_size & 15) == 0;
     aligned16_ = aligned16_ && al;

@@ -119,7 +119,7 @@ class Pipeline {
   bool aligned16_ = true, vec16_ = true;
   int hash_kind_ = 0;
   bool list_mode_ = false;  // list-method jobs: no tree passes
-  uint32_t kinds_ = 0;      // JOBS_NARROW | JOBS_WIDE present in the table
+  uint32_t kinds_ = 0;      // insert modes (JOBS_DENSE | JOBS_LIST_NARROW | JOBS_LIST_WIDE) in the table
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
 
  public:

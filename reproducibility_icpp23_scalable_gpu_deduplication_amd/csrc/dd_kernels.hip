@@ -683,22 +683,27 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
 // occurrence, so only the run's first lane touches the map and the others
 // label themselves SHIFT with the first lane's slot.  Long runs of one digest
 // (zero-filled regions) would otherwise serialize on one slot's atomics.
-// Wide: the jobs of the table that probe in wide rounds (wide_probe);
-// otherwise the others.  A table with both kinds launches both variants.
-template <bool Wide>
+// Insert mode of a job (one of JOBS_DENSE, JOBS_LIST_NARROW, JOBS_LIST_WIDE);
+// each insert kernel variant takes the jobs of its mode, so a variant holds
+// only the code (and registers) of its own path.
+__device__ __forceinline__ uint32_t insert_mode(const JobG& J) {
+  return J.is_baseline ? JOBS_DENSE : J.wide_probe ? JOBS_LIST_WIDE : JOBS_LIST_NARROW;
+}
+
+template <uint32_t Mode>
 __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  constexpr bool Wide = Mode == JOBS_LIST_WIDE;
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
-  if ((J.wide_probe != 0) != Wide) return;  // the other variant's job
+  if (insert_mode(J) != Mode) return;  // another variant's job
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   uint32_t claims = 0;
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
   __shared__ uint32_t sflags;
   // A baseline inserts every leaf: lane k takes chunk k directly (the list
-  // would hold every chunk too).  A wide-probe job is never a baseline, so
-  // the wide variant has no dense path (fewer registers).
-  const bool dense = !Wide && J.is_baseline != 0;
+  // would hold every chunk too).
+  constexpr bool dense = Mode == JOBS_DENSE;
   // The flags, the pending-list counts and (incremental) each lane's first
   // list entry with its digest, all in one round trip.  Other workgroups of
   // this kernel may raise MAP_FULL while it runs: the decision is made once,
@@ -887,11 +892,12 @@ __device__ __forceinline__ uint64_t tile_node(uint64_t r, uint32_t i) {
 
 // Map inserts of the first-occurrence interior nodes that pass 1 listed: one
 // lane per list entry (grid-stride).
-template <bool Wide>
+template <uint32_t Mode>
 __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  constexpr bool Wide = Mode == JOBS_LIST_WIDE;
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
-  if ((J.wide_probe != 0) != Wide) return;  // the other variant's job
+  if (insert_mode(J) != Mode) return;  // another variant's job
   const uint32_t wg = blockIdx.x - prefix[j], nwg = prefix[j + 1] - prefix[j];
   __shared__ uint32_t sh[4];
   __shared__ uint32_t pre[kPendSpread + 1];
@@ -899,12 +905,12 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
   uint32_t claims = 0;
   // The flags and (incremental) the dirty-list counts in one round trip.
   if (threadIdx.x == 0) sflags = J.ctr->flags;
-  if (!J.is_baseline) load_list_prefix(J.dirty_cnt, pre);
+  if (Mode != JOBS_DENSE) load_list_prefix(J.dirty_cnt, pre);
   else __syncthreads();
   if (!(sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
     const InsertCtx ic = insert_ctx(J);
     const GlobalPtr<uint32_t> flags = &J.ctr->flags;
-    if (!Wide && J.is_baseline) {  // a wide-probe job is never a baseline
+    if (Mode == JOBS_DENSE) {
       // A baseline visits every interior node 0 .. C-2 by label (most of
       // them are first occurrences).
       const GlobalPtr<const uint8_t> labels = J.labels;
@@ -2204,8 +2210,9 @@ void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t
                           hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (!grid) return;
-  if (kinds & JOBS_NARROW) DD_LAUNCH(k_insert_leaves<false>, grid, 256, 0, s, jobs, prefix, njobs);
-  if (kinds & JOBS_WIDE) DD_LAUNCH(k_insert_leaves<true>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_DENSE) DD_LAUNCH(k_insert_leaves<JOBS_DENSE>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_LIST_NARROW) DD_LAUNCH(k_insert_leaves<JOBS_LIST_NARROW>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_LIST_WIDE) DD_LAUNCH(k_insert_leaves<JOBS_LIST_WIDE>, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_list_resolve(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
@@ -2219,8 +2226,9 @@ void launch_insert_interior(const JobDev* jobs_h, const uint32_t* prefix, uint32
                             uint32_t kinds, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (!grid) return;
-  if (kinds & JOBS_NARROW) DD_LAUNCH(k_insert_interior<false>, grid, 256, 0, s, jobs, prefix, njobs);
-  if (kinds & JOBS_WIDE) DD_LAUNCH(k_insert_interior<true>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_DENSE) DD_LAUNCH(k_insert_interior<JOBS_DENSE>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_LIST_NARROW) DD_LAUNCH(k_insert_interior<JOBS_LIST_NARROW>, grid, 256, 0, s, jobs, prefix, njobs);
+  if (kinds & JOBS_LIST_WIDE) DD_LAUNCH(k_insert_interior<JOBS_LIST_WIDE>, grid, 256, 0, s, jobs, prefix, njobs);
 }
 void launch_pass2(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);

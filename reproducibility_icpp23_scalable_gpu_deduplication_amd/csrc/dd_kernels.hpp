@@ -180,9 +180,11 @@ void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uin
 // Workgroups of the leaf stage for a job of C chunks: one per 256 chunks in
 // every mapping.
 inline uint64_t leaf_workgroups(int, uint64_t C) { return (C + 255) / 256; }
-// Probe widths of the jobs in a table (bit mask): the insert stages launch
-// one kernel variant per width present.
-enum : uint32_t { JOBS_NARROW = 1, JOBS_WIDE = 2 };
+// Insert modes of the jobs in a table (bit mask): the insert stages launch
+// one kernel variant per mode present.  Dense: a baseline (every leaf and
+// interior node, narrow probe rounds); list, narrow or wide rounds: an
+// incremental checkpoint (its work lists; wide_probe picks the width).
+enum : uint32_t { JOBS_DENSE = 1, JOBS_LIST_NARROW = 2, JOBS_LIST_WIDE = 4 };
 void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
                           hipStream_t s);
 // List method: the leaves left PENDING by insert_leaves are first
