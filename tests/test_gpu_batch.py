@@ -92,13 +92,17 @@ def test_batch_rejected_call_changes_no_handle(torch_gpu):
 @pytest.mark.parametrize("cls", [dd.TreeDeduplicator, dd.ListDeduplicator])
 def test_batch_mixing_baselines_and_incrementals(torch_gpu, graph, cls):
     # Handle i re-baselines at checkpoint index i + 1, so calls 1-3 mix one
-    # baseline with incremental checkpoints (both insert kernel variants run
-    # in one call).
+    # baseline with incremental checkpoints; the last handle gets fresh bytes
+    # every checkpoint, so from its second incremental checkpoint on it
+    # probes in narrow rounds while the others probe wide (every insert
+    # kernel variant runs in one call).
     torch = torch_gpu
     sizes = (1 << 18, 777 * 128, 5000 * 128 + 40)
     chains = [W.chain_uniform_changes(61 + i, s, 128, 5, fresh_frac=0.03, copy_frac=0.04, zero_bytes=4096)
               for i, s in enumerate(sizes)]
-    base = [{0, i + 1} for i in range(len(chains))]
+    rng = np.random.default_rng(67)
+    chains.append([rng.integers(0, 256, 3 << 16, dtype=np.uint8) for _ in range(5)])
+    base = [{0, i + 1} for i in range(len(sizes))] + [{0}]
     handles = [cls(128) for _ in chains]
     b = dd.Batch(handles, graph=graph)
     diffs = [[] for _ in chains]
