@@ -46,12 +46,13 @@ struct Request {
 
 // Device bytes a handle keeps for a buffer of `len` bytes: tree (16 B/node),
 // labels and slot index (5 B/node), the digest map (2N slots of 32 B) and the
-// per-chunk scratch arrays (about 40 B/chunk).
+// per-chunk scratch arrays (root lists, work lists with their digests, tile
+// regions: about 42 B/chunk, counted as 44).
 uint64_t state_bytes(uint64_t len, uint32_t chunk) {
   const uint64_t C = (len + chunk - 1) / chunk, N = 2 * C - 1;
   uint64_t cap = 1;
   while (cap < 2 * N) cap <<= 1;
-  return 21 * N + 32 * cap + 40 * C;
+  return 21 * N + 32 * cap + 44 * C;
 }
 
 }  // namespace
