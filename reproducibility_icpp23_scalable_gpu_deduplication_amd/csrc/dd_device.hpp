@@ -24,7 +24,8 @@ enum : uint8_t {
 enum : int { HASH_MURMUR3 = 0, HASH_MD5 = 1 };
 
 // Map / pipeline status flags.
-enum : uint32_t { FLAG_NEED_GROW = 1u, FLAG_MAP_FULL = 2u, FLAG_BAD_SOURCE = 4u };
+// FLAG_BAD_DIFF: a restart's parse found a diff inconsistent.
+enum : uint32_t { FLAG_NEED_GROW = 1u, FLAG_MAP_FULL = 2u, FLAG_BAD_SOURCE = 4u, FLAG_BAD_DIFF = 8u };
 
 struct Digest {
   uint64_t k0, k1;

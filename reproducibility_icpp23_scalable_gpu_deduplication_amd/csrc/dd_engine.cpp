@@ -564,198 +564,363 @@ uint32_t rd32(const uint8_t* p) {
   return v;
 }
 
-struct HostCkpt {
-  dd_header hd;
-  uint64_t meta_bytes, data_bytes, total;
-  std::vector<uint8_t> meta;   // first list, table, shift list
-  std::vector<uint32_t> f_off, s_off, s_tree;
-  uint64_t nfl = 0, nsl = 0;   // leaves in first / shifted regions
-};
-
-// Validates one diff of the chain and builds its region prefixes.  Every node
-// id, leaf count and data range the kernels will touch is checked here.
-dd_status parse_ckpt(const uint8_t* hdr32, const uint8_t* meta, uint64_t total, uint32_t idx, uint64_t len,
-                     uint32_t cs, uint32_t bias, HostCkpt& hc) {
-  if (total < 32) return DD_ECORRUPT;
-  memcpy(&hc.hd, hdr32, 32);
-  const dd_header& hd = hc.hd;
-  if (hd.datalen != len || hd.chunk_size != cs || hd.chkpt_id != idx || hd.ref_id > hd.chkpt_id) return DD_ECORRUPT;
-  hc.meta_bytes = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
-  if (32 + hc.meta_bytes > total) return DD_ECORRUPT;
-  hc.total = total;
-  hc.data_bytes = total - 32 - hc.meta_bytes;
-  hc.meta.assign(meta, meta + hc.meta_bytes);
-  const TreeGeom g = make_geom(ceil_div(len, cs));
-  const uint8_t* fl = hc.meta.data();
-  const uint8_t* tb = fl + 4ull * hd.num_first_ocur;
-  const uint8_t* sl = tb + 8ull * hd.num_prior_chkpts;
-  hc.f_off.resize((size_t)hd.num_first_ocur + 1);
-  uint64_t acc = 0;
-  for (uint32_t i = 0; i < hd.num_first_ocur; i++) {
-    const uint64_t node = (uint64_t)rd32(fl + 4ull * i) + bias;
-    if (node >= g.N) return DD_ECORRUPT;
-    hc.f_off[i] = (uint32_t)acc;
-    const LeafRuns r = subtree_runs(g, node);
-    acc += r.n0 + r.n1;
-    if (acc > g.C) return DD_ECORRUPT;
-  }
-  hc.f_off[hd.num_first_ocur] = (uint32_t)acc;
-  hc.nfl = acc;
-  if (acc * cs > hc.data_bytes) return DD_ECORRUPT;
-  // Prior table -> source checkpoint of each shifted entry (cumulative
-  // counts, src/tree_approach.cpp:868-898).
-  std::vector<uint64_t> cnt((size_t)hd.chkpt_id + 1, 0);
-  uint64_t tot = 0;
-  for (uint32_t i = 0; i < hd.num_prior_chkpts; i++) {
-    const uint32_t ck = rd32(tb + 8ull * i), n = rd32(tb + 8ull * i + 4);
-    if (ck > hd.chkpt_id) return DD_ECORRUPT;
-    cnt[ck] = n;
-  }
-  for (uint64_t v : cnt) tot += v;
-  if (tot != hd.num_shift_dupl) return DD_ECORRUPT;
-  hc.s_tree.resize(hd.num_shift_dupl);
-  hc.s_off.resize((size_t)hd.num_shift_dupl + 1);
-  {
-    uint64_t k = 0;
-    for (uint32_t t = 0; t <= hd.chkpt_id; t++)
-      for (uint64_t q = 0; q < cnt[t]; q++) hc.s_tree[k++] = t;
-  }
-  acc = 0;
-  for (uint32_t i = 0; i < hd.num_shift_dupl; i++) {
-    const uint64_t node = (uint64_t)rd32(sl + 8ull * i) + bias, prev = (uint64_t)rd32(sl + 8ull * i + 4) + bias;
-    if (node >= g.N || prev >= g.N) return DD_ECORRUPT;
-    const LeafRuns a = subtree_runs(g, node), b = subtree_runs(g, prev);
-    if (a.n0 + a.n1 != b.n0 + b.n1) return DD_ECORRUPT;
-    hc.s_off[i] = (uint32_t)acc;
-    acc += a.n0 + a.n1;
-    if (acc > g.C) return DD_ECORRUPT;
-  }
-  hc.s_off[hd.num_shift_dupl] = (uint32_t)acc;
-  hc.nsl = acc;
-  return DD_OK;
-}
-
 uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
-// Walks the chain newest to oldest on the device.  `dev_diffs[k]` is the
-// device copy of ckpts[k]'s diff (or null: it is uploaded from host_diffs).
-dd_status restart_walk(dd_dedup* h, std::vector<HostCkpt>& ckpts, const std::vector<const uint8_t*>& host_diffs,
-                       const std::vector<const uint8_t*>& dev_diffs, uint32_t chkpt_id, void* dev_out, uint64_t len) {
-  const double t0 = now_s();
-  hipStream_t s = h->stream;
-  const uint32_t cs = ckpts[0].hd.chunk_size;
-  const TreeGeom g = make_geom(ceil_div(len, cs));
-  // Arena: [diff copies][f_off][s_off][s_tree] per checkpoint.
-  uint64_t arena = 0;
-  for (size_t k = 0; k < ckpts.size(); k++) {
-    if (!dev_diffs[k]) arena += align256(ckpts[k].total);
-    arena += align256(ckpts[k].f_off.size() * 4) + align256(ckpts[k].s_off.size() * 4) +
-             align256(ckpts[k].s_tree.size() * 4 + 4);
-  }
-  // Grows by at least half its size: restarts of chains that grow by one
-  // diff at a time would otherwise reallocate on every call.
-  if (arena > h->rs_arena.bytes) DD_TRY(h->rs_arena.ensure(std::max<uint64_t>(arena, h->rs_arena.bytes * 3 / 2)));
-  DD_TRY(h->rs_desc.ensure(ckpts.size() * sizeof(RestartCkpt)));
-  DD_TRY(h->ref.ensure(g.C * 8));
-  if (h->cover.bytes < g.C * sizeof(CoverEntry)) {
-    h->cover.release();
-    DD_TRY(h->cover.ensure(g.C * sizeof(CoverEntry), true, s));
-    h->stamp = 0;
-  }
-  DD_TRY(h->rs_ctr.ensure(sizeof(JobCounters), true, s));
-  if (!h->rs_ctr_host) DD_HIP(hipHostMalloc((void**)&h->rs_ctr_host, sizeof(JobCounters), hipHostMallocDefault));
-  if (h->stamp > 0xfffff000u) {  // stamps are about to wrap: start over from a zeroed array
-    DD_HIP(hipMemsetAsync(h->cover.p, 0, h->cover.bytes, s));
-    h->stamp = 0;
-  }
-  std::vector<RestartCkpt> desc(ckpts.size());
-  uint8_t* base = h->rs_arena.as<uint8_t>();
-  uint64_t off = 0;
-  const double c0 = now_s();
-  for (size_t k = 0; k < ckpts.size(); k++) {
-    HostCkpt& hc = ckpts[k];
-    RestartCkpt& rc = desc[k];
-    if (dev_diffs[k]) {
-      rc.diff = dev_diffs[k];
-    } else {
-      DD_HIP(hipMemcpyAsync(base + off, host_diffs[k], hc.total, hipMemcpyHostToDevice, s));
-      rc.diff = base + off;
-      off += align256(hc.total);
-    }
-    DD_HIP(hipMemcpyAsync(base + off, hc.f_off.data(), hc.f_off.size() * 4, hipMemcpyHostToDevice, s));
-    rc.f_off = reinterpret_cast<const uint32_t*>(base + off);
-    off += align256(hc.f_off.size() * 4);
-    DD_HIP(hipMemcpyAsync(base + off, hc.s_off.data(), hc.s_off.size() * 4, hipMemcpyHostToDevice, s));
-    rc.s_off = reinterpret_cast<const uint32_t*>(base + off);
-    off += align256(hc.s_off.size() * 4);
-    if (!hc.s_tree.empty())
-      DD_HIP(hipMemcpyAsync(base + off, hc.s_tree.data(), hc.s_tree.size() * 4, hipMemcpyHostToDevice, s));
-    rc.s_tree = reinterpret_cast<const uint32_t*>(base + off);
-    off += align256(hc.s_tree.size() * 4 + 4);
-    rc.nfirst = hc.hd.num_first_ocur;
-    rc.nshift = hc.hd.num_shift_dupl;
-    rc.ckpt_id = hc.hd.chkpt_id;
-    rc.first_off = 32;
-    rc.shift_off = 32 + 4ull * hc.hd.num_first_ocur + 8ull * hc.hd.num_prior_chkpts;
-    rc.data_off = 32 + hc.meta_bytes;
-    rc.node_bias = h->st.method == DD_METHOD_LIST ? (uint32_t)(g.C - 1) : 0;
-    rc.pad = 0;
-  }
-  DD_HIP(hipMemcpyAsync(h->rs_desc.p, desc.data(), desc.size() * sizeof(RestartCkpt), hipMemcpyHostToDevice, s));
-  DD_HIP(hipStreamSynchronize(s));
-  const double c1 = now_s();
-  JobCounters* ctr = h->rs_ctr.as<JobCounters>();
-  launch_restart_init(h->ref.as<uint64_t>(), g.C, chkpt_id, ctr, s);
-  const RestartCkpt* dd = h->rs_desc.as<RestartCkpt>();
-  for (size_t k = 0; k < ckpts.size(); k++) {
-    const uint32_t stamp = ++h->stamp;
-    launch_restart_cover(dd + k, g, h->cover.as<CoverEntry>(), stamp, ckpts[k].nfl, ckpts[k].nsl, cs, s);
-    launch_restart_resolve(dd + k, g, h->cover.as<CoverEntry>(), stamp, h->ref.as<uint64_t>(),
-                           reinterpret_cast<uint8_t*>(dev_out), len, cs, ctr, s);
-  }
-  launch_restart_check(h->ref.as<uint64_t>(), g.C, ctr, s);
-  DD_HIP(hipGetLastError());
-  DD_HIP(hipMemcpyAsync(h->rs_ctr_host, ctr, sizeof(JobCounters), hipMemcpyDeviceToHost, s));
-  DD_HIP(hipStreamSynchronize(s));
-  h->st.restart_timers[0] = c1 - c0;
-  h->st.restart_timers[1] = now_s() - t0;
-  if (h->rs_ctr_host->unresolved || (h->rs_ctr_host->flags & FLAG_BAD_SOURCE)) return DD_ECORRUPT;
-  return DD_OK;
-}
+// Device memory and host staging of one restart: the handle's own buffers,
+// or (for callers that cannot allocate) the library's static scratch.
+class RestartSpace {
+ public:
+  virtual ~RestartSpace() = default;
+  // Device staging for kHdrWindow headers.
+  virtual dd_status headers(uint8_t** dev) = 0;
+  // Cover table, pending sources and pending list for C chunks, and the
+  // stamp counter of that cover table.  A cover table that is new (or whose
+  // contents are unknown) reads as all zero, with the counter at zero.
+  virtual dd_status tables(uint64_t C, uint2** cover, uint64_t** ref, uint32_t** pend, uint32_t** stamp) = 0;
+  // Device arena of at least `bytes` (called after tables()).
+  virtual dd_status arena(uint64_t bytes, uint8_t** dev) = 0;
+  // Host staging of at least `bytes`; pinned when `pinned()`.
+  virtual dd_status host(uint64_t bytes, uint8_t** p) = 0;
+  virtual bool pinned() const = 0;
+};
 
-// Collects the checkpoints the walk visits: chkpt_id down to the reference id
-// named by each visited header (src/tree_approach.cpp:943).
-// `list`: the diffs are list-method diffs, whose node ids are chunk indices.
-template <typename GetHdr>
-dd_status collect_chain(uint32_t nchain, uint32_t chkpt_id, uint64_t len, bool list, GetHdr get,
-                        std::vector<HostCkpt>& out, std::vector<uint32_t>& ids) {
-  if (chkpt_id >= nchain) return DD_EINVAL;
-  uint8_t hdr[32];
-  uint64_t total = 0;
-  std::vector<uint8_t> meta;
-  DD_TRY(get(chkpt_id, hdr, meta, total));
-  dd_header h0;
-  if (total < 32) return DD_ECORRUPT;
-  memcpy(&h0, hdr, 32);
-  if (h0.datalen != len) return DD_EINVAL;
-  if (h0.chunk_size == 0) return DD_ECORRUPT;
-  const uint32_t cs = h0.chunk_size;
-  const uint64_t Cn = ceil_div(len, cs);
-  if (Cn > 0x7fffffffull) return DD_ECORRUPT;
-  const uint32_t bias = list ? (uint32_t)(Cn - 1) : 0;
-  int64_t idx = chkpt_id;
-  int64_t ref = h0.ref_id;
+constexpr uint32_t kHdrWindow = 64;
+
+// The handle's restart buffers: grow-only device buffers and pinned staging.
+class HandleSpace : public RestartSpace {
+ public:
+  HandleSpace(dd_dedup* h) : h_(h) {}
+  dd_status headers(uint8_t** dev) override {
+    DD_TRY(h_->rs_hdr.ensure(kHdrWindow * 32));
+    *dev = h_->rs_hdr.as<uint8_t>();
+    return DD_OK;
+  }
+  dd_status tables(uint64_t C, uint2** cover, uint64_t** ref, uint32_t** pend, uint32_t** stamp) override {
+    if (h_->cover.bytes < C * sizeof(uint2)) {
+      h_->cover.release();
+      DD_TRY(h_->cover.ensure(C * sizeof(uint2), true, h_->stream));
+      h_->stamp = 0;
+    }
+    DD_TRY(h_->ref.ensure(C * 8));
+    DD_TRY(h_->pend.ensure(C * 4));
+    *cover = h_->cover.as<uint2>();
+    *ref = h_->ref.as<uint64_t>();
+    *pend = h_->pend.as<uint32_t>();
+    *stamp = &h_->stamp;
+    return DD_OK;
+  }
+  dd_status arena(uint64_t bytes, uint8_t** dev) override {
+    // Grows by at least half its size: restarts of chains that grow by one
+    // diff at a time would otherwise reallocate on every call.
+    if (bytes > h_->rs_arena.bytes) DD_TRY(h_->rs_arena.ensure(std::max<uint64_t>(bytes, h_->rs_arena.bytes * 3 / 2)));
+    *dev = h_->rs_arena.as<uint8_t>();
+    return DD_OK;
+  }
+  dd_status host(uint64_t bytes, uint8_t** p) override {
+    if (bytes > h_->rs_host_cap) {
+      // the previous restart's copies from the staging have completed (it synchronized)
+      if (h_->rs_host) (void)hipHostFree(h_->rs_host);
+      h_->rs_host = nullptr;
+      h_->rs_host_cap = 0;
+      const uint64_t n = std::max<uint64_t>(bytes, 64 << 10);
+      DD_HIP(hipHostMalloc((void**)&h_->rs_host, n, hipHostMallocDefault));
+      h_->rs_host_cap = n;
+    }
+    *p = h_->rs_host;
+    return DD_OK;
+  }
+  bool pinned() const override { return true; }
+
+ private:
+  dd_dedup* h_;
+};
+
+// Validated headers of the checkpoints a restart visits: chkpt_id down to the
+// reference id named by each visited header (src/tree_approach.cpp:943).
+// `hdr_of(i, hdr)` returns diff i's header.
+template <typename HdrOf>
+dd_status walk_headers(uint32_t chkpt_id, uint64_t len, HdrOf hdr_of, std::vector<uint32_t>& ids,
+                       std::vector<dd_header>& hdrs) {
+  int64_t idx = chkpt_id, ref = 0;
   bool first = true;
-  while (idx >= ref) {
-    if (!first) DD_TRY(get((uint32_t)idx, hdr, meta, total));
-    HostCkpt hc;
-    DD_TRY(parse_ckpt(hdr, meta.data(), total, (uint32_t)idx, len, cs, bias, hc));
-    ref = hc.hd.ref_id;
-    out.push_back(std::move(hc));
+  while (first || idx >= ref) {
+    dd_header hd;
+    DD_TRY(hdr_of((uint32_t)idx, hd));
+    if (first) {
+      if (hd.datalen != len) return DD_EINVAL;
+      if (hd.chunk_size == 0 || ceil_div(len, hd.chunk_size) > 0x7fffffffull) return DD_ECORRUPT;
+    } else if (hd.datalen != len || hd.chunk_size != hdrs[0].chunk_size) {
+      return DD_ECORRUPT;
+    }
+    if (hd.chkpt_id != (uint32_t)idx || hd.ref_id > hd.chkpt_id) return DD_ECORRUPT;
     ids.push_back((uint32_t)idx);
+    hdrs.push_back(hd);
+    ref = hd.ref_id;
     first = false;
     idx--;
   }
+  return DD_OK;
+}
+
+// Copies device byte ranges into one device buffer with k_fetch launches
+// (kFetchRanges per launch), then into host memory; waits for the copy.
+dd_status fetch_to_host(const std::vector<const uint8_t*>& src, const std::vector<uint64_t>& n,
+                        const std::vector<uint64_t>& off, uint8_t* dev_dst, uint8_t* host_dst, uint64_t total,
+                        hipStream_t s) {
+  for (size_t b = 0; b < src.size(); b += kFetchRanges) {
+    FetchArgs a;
+    memset(&a, 0, sizeof a);
+    uint64_t mx = 0;
+    for (size_t i = b; i < src.size() && i < b + kFetchRanges; i++) {
+      a.src[a.n] = src[i];
+      a.len[a.n] = n[i];
+      a.dst_off[a.n] = off[i];
+      mx = std::max(mx, n[i]);
+      a.n++;
+    }
+    launch_fetch(a, mx, dev_dst, s);
+  }
+  DD_HIP(hipGetLastError());
+  if (total) DD_HIP(hipMemcpyAsync(host_dst, dev_dst, total, hipMemcpyDeviceToHost, s));
+  DD_HIP(hipStreamSynchronize(s));
+  return DD_OK;
+}
+
+// Restart of checkpoint chkpt_id from diffs in host memory (`host_chain`) or
+// device memory (`dev_chain`); exactly one of them is given.
+dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf* host_chain,
+                      const void* const* dev_chain, const uint64_t* dev_lens, uint32_t nchain, uint32_t chkpt_id,
+                      void* dev_out, uint64_t len, double timers[2]) {
+  const double t0 = now_s();
+  if (chkpt_id >= nchain) return DD_EINVAL;
+  auto total_of = [&](uint32_t i) -> uint64_t { return host_chain ? host_chain[i].len : dev_lens[i]; };
+  auto valid = [&](uint32_t i) -> bool {
+    return (host_chain ? host_chain[i].data != nullptr : dev_chain[i] != nullptr) && total_of(i) >= 32;
+  };
+  // 1. Headers of the visited checkpoints (device diffs: read in windows of
+  // kHdrWindow ids per round trip).
+  std::vector<uint32_t> ids;
+  std::vector<dd_header> hdrs;
+  if (host_chain) {
+    DD_TRY(walk_headers(chkpt_id, len, [&](uint32_t i, dd_header& hd) -> dd_status {
+      if (!valid(i)) return DD_ECORRUPT;
+      memcpy(&hd, host_chain[i].data, 32);
+      return DD_OK;
+    }, ids, hdrs));
+  } else {
+    uint8_t* stage = nullptr;
+    DD_TRY(sp.headers(&stage));
+    std::vector<dd_header> known((size_t)chkpt_id + 1);
+    std::vector<char> have((size_t)chkpt_id + 1, 0);
+    DD_TRY(walk_headers(chkpt_id, len, [&](uint32_t i, dd_header& hd) -> dd_status {
+      if (!valid(i)) return DD_ECORRUPT;
+      if (!have[i]) {
+        const uint32_t lo = i + 1 >= kHdrWindow ? i + 1 - kHdrWindow : 0;
+        std::vector<const uint8_t*> src;
+        std::vector<uint64_t> n, off;
+        std::vector<uint32_t> got;
+        for (uint32_t k = lo; k <= i; k++) {
+          if (have[k] || !valid(k)) continue;
+          src.push_back(static_cast<const uint8_t*>(dev_chain[k]));
+          n.push_back(32);
+          off.push_back(32ull * got.size());
+          got.push_back(k);
+        }
+        uint8_t* hst = nullptr;
+        DD_TRY(sp.host(32ull * got.size(), &hst));
+        DD_TRY(fetch_to_host(src, n, off, stage, hst, 32ull * got.size(), s));
+        for (size_t k = 0; k < got.size(); k++) {
+          memcpy(&known[got[k]], hst + 32 * k, 32);
+          have[got[k]] = 1;
+        }
+      }
+      hd = known[i];
+      return DD_OK;
+    }, ids, hdrs));
+  }
+  const uint32_t cs = hdrs[0].chunk_size;
+  const uint64_t C = ceil_div(len, cs);
+  const TreeGeom g = make_geom(C);
+  const uint32_t bias = method == DD_METHOD_LIST ? (uint32_t)(C - 1) : 0;
+  const size_t M = ids.size();
+  uint2* cover = nullptr;
+  uint64_t* ref = nullptr;
+  uint32_t* pend = nullptr;
+  uint32_t* stamp = nullptr;
+  DD_TRY(sp.tables(C, &cover, &ref, &pend, &stamp));
+  // 2. Layout.  Region lists are parsed on the device; the host reads only
+  // the prior tables (8 bytes per source checkpoint).
+  std::vector<uint64_t> meta_bytes(M), tab_bytes(M), tab_off(M);
+  uint64_t tab_total = 0, nblk = 0;
+  std::vector<RestartList> lists(2 * M);
+  std::vector<uint32_t> list_blk(2 * M + 1);
+  for (size_t k = 0; k < M; k++) {
+    const dd_header& hd = hdrs[k];
+    meta_bytes[k] = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
+    if (32 + meta_bytes[k] > total_of(ids[k])) return DD_ECORRUPT;
+    if (hd.num_first_ocur > C || hd.num_shift_dupl > C) return DD_ECORRUPT;  // regions are disjoint subtrees
+    tab_bytes[k] = 8ull * hd.num_prior_chkpts;
+    tab_off[k] = tab_total;
+    tab_total += tab_bytes[k];
+    for (int sh = 0; sh < 2; sh++) {
+      RestartList& L = lists[2 * k + sh];
+      L.ckpt = (uint32_t)k;
+      L.shift = (uint32_t)sh;
+      L.n = sh ? hd.num_shift_dupl : hd.num_first_ocur;
+      L.blk0 = (uint32_t)nblk;
+      list_blk[2 * k + sh] = (uint32_t)nblk;
+      nblk += ceil_div(L.n, kParseBlock);
+    }
+  }
+  list_blk[2 * M] = (uint32_t)nblk;
+  if (nblk > 0x7fffffffull) return DD_ECORRUPT;
+  // Prior tables -> per checkpoint, the end of each source checkpoint's run
+  // of the shifted list (cumulative counts in checkpoint order).
+  std::vector<std::vector<uint32_t>> pb_end(M), pb_ck(M);
+  uint64_t pb_total = 0;
+  {
+    uint8_t* stage = nullptr;
+    uint8_t* hst = nullptr;
+    if (dev_chain) {
+      DD_TRY(sp.arena(align256(tab_total) + 4096, &stage));
+      DD_TRY(sp.host(tab_total, &hst));
+      std::vector<const uint8_t*> src(M);
+      for (size_t k = 0; k < M; k++)
+        src[k] = static_cast<const uint8_t*>(dev_chain[ids[k]]) + 32 + 4ull * hdrs[k].num_first_ocur;
+      DD_TRY(fetch_to_host(src, tab_bytes, tab_off, stage, hst, tab_total, s));
+    }
+    for (size_t k = 0; k < M; k++) {
+      const dd_header& hd = hdrs[k];
+      const uint8_t* tb = dev_chain ? hst + tab_off[k] : host_chain[ids[k]].data + 32 + 4ull * hd.num_first_ocur;
+      std::vector<uint64_t> cnt((size_t)hd.chkpt_id + 1, 0);
+      for (uint32_t i = 0; i < hd.num_prior_chkpts; i++) {
+        const uint32_t ck = rd32(tb + 8ull * i), n = rd32(tb + 8ull * i + 4);
+        if (ck > hd.chkpt_id) return DD_ECORRUPT;
+        cnt[ck] = n;
+      }
+      uint64_t cum = 0;
+      for (uint32_t t = 0; t <= hd.chkpt_id; t++) {
+        if (!cnt[t]) continue;
+        cum += cnt[t];
+        if (cum > hd.num_shift_dupl) return DD_ECORRUPT;
+        pb_end[k].push_back((uint32_t)cum);
+        pb_ck[k].push_back(t);
+      }
+      if (cum != hd.num_shift_dupl) return DD_ECORRUPT;
+      pb_total += align256(8ull * pb_end[k].size() + 8);
+    }
+  }
+  // Arena: [upload block: counters, descriptors, lists, list prefixes, prior
+  // boundaries][block sums][f_off and s_off of every checkpoint][host diff copies].
+  const uint64_t up_desc = sizeof(RestartCtr);
+  const uint64_t up_lists = up_desc + align256(M * sizeof(RestartCkpt));
+  const uint64_t up_lblk = up_lists + align256(2 * M * sizeof(RestartList));
+  const uint64_t up_pb = up_lblk + align256((2 * M + 1) * 4);
+  const uint64_t upload = up_pb + pb_total;
+  const uint64_t blk_at = align256(upload);
+  uint64_t off_at = blk_at + align256(8 * nblk + 8);
+  uint64_t end = off_at;
+  const uint64_t idx_bytes = align256(4 * ceil_div(C, 256));
+  for (size_t k = 0; k < M; k++)
+    end += align256(4ull * (hdrs[k].num_first_ocur + 1)) + align256(4ull * (hdrs[k].num_shift_dupl + 1)) +
+           2 * idx_bytes;
+  const uint64_t copies_at = end;
+  if (host_chain)
+    for (size_t k = 0; k < M; k++) end += align256(total_of(ids[k]));
+  uint8_t* ar = nullptr;
+  DD_TRY(sp.arena(end, &ar));
+  uint8_t* hst = nullptr;
+  DD_TRY(sp.host(upload, &hst));
+  const double c0 = now_s();
+  // 3. Diff bytes on the device (host diffs are copied into the arena).
+  std::vector<const uint8_t*> diff(M);
+  {
+    uint64_t off = copies_at;
+    for (size_t k = 0; k < M; k++) {
+      if (dev_chain) {
+        diff[k] = static_cast<const uint8_t*>(dev_chain[ids[k]]);
+      } else {
+        DD_HIP(hipMemcpyAsync(ar + off, host_chain[ids[k]].data, total_of(ids[k]), hipMemcpyHostToDevice, s));
+        diff[k] = ar + off;
+        off += align256(total_of(ids[k]));
+      }
+    }
+  }
+  // 4. Upload block.
+  memset(hst, 0, up_lists);
+  RestartCkpt* desc = reinterpret_cast<RestartCkpt*>(hst + up_desc);
+  {
+    uint64_t pb = up_pb;
+    for (size_t k = 0; k < M; k++) {
+      const dd_header& hd = hdrs[k];
+      RestartCkpt& rc = desc[k];
+      rc.diff = diff[k];
+      rc.f_off = reinterpret_cast<uint32_t*>(ar + off_at);
+      off_at += align256(4ull * (hd.num_first_ocur + 1));
+      rc.s_off = reinterpret_cast<uint32_t*>(ar + off_at);
+      off_at += align256(4ull * (hd.num_shift_dupl + 1));
+      rc.f_idx = reinterpret_cast<uint32_t*>(ar + off_at);
+      off_at += idx_bytes;
+      rc.s_idx = reinterpret_cast<uint32_t*>(ar + off_at);
+      off_at += idx_bytes;
+      const size_t np = pb_end[k].size();
+      if (np) {
+        memcpy(hst + pb, pb_end[k].data(), 4 * np);
+        memcpy(hst + pb + 4 * np, pb_ck[k].data(), 4 * np);
+      }
+      rc.pb_end = reinterpret_cast<const uint32_t*>(ar + pb);
+      rc.pb_ck = reinterpret_cast<const uint32_t*>(ar + pb + 4 * np);
+      rc.npb = (uint32_t)np;
+      pb += align256(8ull * np + 8);
+      rc.nfirst = hd.num_first_ocur;
+      rc.nshift = hd.num_shift_dupl;
+      rc.ckpt_id = hd.chkpt_id;
+      rc.node_bias = bias;
+      rc.first_off = 32;
+      rc.shift_off = 32 + 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts;
+      rc.data_off = 32 + meta_bytes[k];
+      rc.data_bytes = total_of(ids[k]) - 32 - meta_bytes[k];
+    }
+  }
+  memcpy(hst + up_lists, lists.data(), lists.size() * sizeof(RestartList));
+  memcpy(hst + up_lblk, list_blk.data(), list_blk.size() * 4);
+  DD_HIP(hipMemcpyAsync(ar, hst, upload, hipMemcpyHostToDevice, s));
+  RestartCtr* ctr = reinterpret_cast<RestartCtr*>(ar);
+  RestartCkpt* ddesc = reinterpret_cast<RestartCkpt*>(ar + up_desc);
+  launch_restart_parse(ddesc, reinterpret_cast<const RestartList*>(ar + up_lists), (uint32_t)(2 * M),
+                       reinterpret_cast<const uint32_t*>(ar + up_lblk), (uint32_t)nblk,
+                       reinterpret_cast<uint32_t*>(ar + blk_at), g, cs, ctr, s);
+  // Copy width: 16-byte pieces need 16-byte chunks and output and 4-byte
+  // aligned data sections (16-byte loads at 4-byte alignment).
+  bool src4 = true;
+  for (size_t k = 0; k < M; k++) src4 = src4 && ((reinterpret_cast<uintptr_t>(diff[k]) + 32 + meta_bytes[k]) & 3) == 0;
+  const uintptr_t op = reinterpret_cast<uintptr_t>(dev_out);
+  const int piece = (src4 && cs % 16 == 0 && op % 16 == 0) ? 16 : (src4 && cs % 4 == 0 && op % 4 == 0) ? 4 : 1;
+  // 5. The walk, newest first: two launches per checkpoint.
+  if ((uint64_t)*stamp + M + 1 > 0xfffffff0ull) {  // stamps would wrap: start over from a cleared table
+    launch_fill_u32(reinterpret_cast<uint32_t*>(cover), 2 * C, 0, s);
+    *stamp = 0;
+  }
+  const uint32_t base = *stamp;
+  uint8_t* out = static_cast<uint8_t*>(dev_out);
+  bool any_shift = false;  // the pending list stays empty until a checkpoint has shifted entries
+  for (size_t k = 0; k < M; k++) {
+    const uint32_t st = base + 1 + (uint32_t)k;
+    launch_restart_regions(ddesc + k, g, cover, base, st, ref, pend, ctr, out, len, cs, piece, s);
+    any_shift = any_shift || hdrs[k].num_shift_dupl > 0;
+    if (any_shift) launch_restart_pending(ddesc + k, g, cover, st, ref, pend, ctr, out, len, cs, piece, s);
+  }
+  *stamp = base + (uint32_t)M;
+  DD_HIP(hipGetLastError());
+  RestartCtr res;
+  DD_HIP(hipMemcpyAsync(hst, ctr, sizeof(RestartCtr), hipMemcpyDeviceToHost, s));
+  DD_HIP(hipStreamSynchronize(s));
+  memcpy(&res, hst, sizeof res);
+  timers[0] = c0 - t0;
+  timers[1] = now_s() - t0;
+  if (res.flags & FLAG_BAD_DIFF) return DD_ECORRUPT;
+  const uint64_t unresolved = (C - std::min<uint64_t>(C, res.claimed)) + (res.npend - res.ndone);
+  if (unresolved || (res.flags & FLAG_BAD_SOURCE)) return DD_ECORRUPT;
   return DD_OK;
 }
 
@@ -824,7 +989,7 @@ dd_status dd_destroy(dd_dedup* h) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : h->pev)
     if (e) (void)hipEventDestroy(e);
-  if (h->rs_ctr_host) (void)hipHostFree(h->rs_ctr_host);
+  if (h->rs_host) (void)hipHostFree(h->rs_host);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return DD_OK;
@@ -888,54 +1053,18 @@ dd_status dd_restart(dd_dedup* h, const dd_buf* chain, uint32_t nchain, uint32_t
                      uint64_t len) {
   if (!h || !chain || !dev_out) return DD_EINVAL;
   if (hipSetDevice(h->device) != hipSuccess) return DD_EHIP;
-  auto get = [&](uint32_t i, uint8_t* hdr, std::vector<uint8_t>& meta, uint64_t& total) -> dd_status {
-    total = chain[i].len;
-    if (!chain[i].data || total < 32) return DD_ECORRUPT;
-    memcpy(hdr, chain[i].data, 32);
-    dd_header hd;
-    memcpy(&hd, hdr, 32);
-    const uint64_t mb = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
-    if (32 + mb > total) return DD_ECORRUPT;
-    meta.assign(chain[i].data + 32, chain[i].data + 32 + mb);
-    return DD_OK;
-  };
-  std::vector<HostCkpt> ck;
-  std::vector<uint32_t> ids;
-  DD_TRY(collect_chain(nchain, chkpt_id, len, h->st.method == DD_METHOD_LIST, get, ck, ids));
-  std::vector<const uint8_t*> hd(ck.size()), dv(ck.size(), nullptr);
-  for (size_t k = 0; k < ck.size(); k++) hd[k] = chain[ids[k]].data;
-  return restart_walk(h, ck, hd, dv, chkpt_id, dev_out, len);
+  HandleSpace sp(h);
+  return restart_run(sp, h->stream, h->st.method, chain, nullptr, nullptr, nchain, chkpt_id, dev_out, len,
+                     h->st.restart_timers);
 }
 
 dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uint64_t* lens, uint32_t nchain,
                             uint32_t chkpt_id, void* dev_out, uint64_t len) {
   if (!h || !dev_chain || !lens || !dev_out) return DD_EINVAL;
   if (hipSetDevice(h->device) != hipSuccess) return DD_EHIP;
-  auto get = [&](uint32_t i, uint8_t* hdr, std::vector<uint8_t>& meta, uint64_t& total) -> dd_status {
-    total = lens[i];
-    if (!dev_chain[i] || total < 32) return DD_ECORRUPT;
-    // Reads go through the handle's stream so a restart does not wait for
-    // work of other streams (the legacy null stream would).
-    if (hipMemcpyAsync(hdr, dev_chain[i], 32, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-      return DD_EHIP;
-    dd_header hd;
-    memcpy(&hd, hdr, 32);
-    const uint64_t mb = 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts + 8ull * hd.num_shift_dupl;
-    if (32 + mb > total) return DD_ECORRUPT;
-    meta.resize(mb);
-    if (mb && (hipMemcpyAsync(meta.data(), static_cast<const uint8_t*>(dev_chain[i]) + 32, mb,
-                              hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-               hipStreamSynchronize(h->stream) != hipSuccess))
-      return DD_EHIP;
-    return DD_OK;
-  };
-  std::vector<HostCkpt> ck;
-  std::vector<uint32_t> ids;
-  DD_TRY(collect_chain(nchain, chkpt_id, len, h->st.method == DD_METHOD_LIST, get, ck, ids));
-  std::vector<const uint8_t*> hd(ck.size(), nullptr), dv(ck.size());
-  for (size_t k = 0; k < ck.size(); k++) dv[k] = static_cast<const uint8_t*>(dev_chain[ids[k]]);
-  return restart_walk(h, ck, hd, dv, chkpt_id, dev_out, len);
+  HandleSpace sp(h);
+  return restart_run(sp, h->stream, h->st.method, nullptr, dev_chain, lens, nchain, chkpt_id, dev_out, len,
+                     h->st.restart_timers);
 }
 
 dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out) {

@@ -145,11 +145,13 @@ struct dd_dedup {
   int profile = 0;  // 0 off, 1 leaf-hash events only, 2 every stage
   hipEvent_t pev[DD_NUM_STAGES + 1] = {};
   double stage_ms[DD_NUM_STAGES] = {};
-  // restart scratch
-  dd::DevBuf ref, cover, rs_arena, rs_desc;
-  uint32_t stamp = 0;
-  dd::DevBuf rs_ctr;
-  dd::JobCounters* rs_ctr_host = nullptr;
+  // restart buffers: cover table (8 B/chunk), pending sources (8 B/chunk),
+  // pending list (4 B/chunk), header staging, arena (metadata staging, upload
+  // block, host diff copies) and pinned host staging
+  dd::DevBuf cover, ref, pend, rs_hdr, rs_arena;
+  uint32_t stamp = 0;  // last stamp written into `cover`
+  uint8_t* rs_host = nullptr;
+  uint64_t rs_host_cap = 0;
 };
 
 struct dd_batch {

@@ -108,7 +108,6 @@ __global__ void k_begin(const JobG* __restrict__ jobs, uint32_t njobs, int rerun
   c->nfirst = c->nshift = c->nprior = 0;
   c->dchunks = 0;
   c->diff_len = 0;
-  c->unresolved = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2032,123 +2031,432 @@ __global__ void k_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView m,
 }
 
 // ---------------------------------------------------------------------------
-// Restart (src/tree_approach.cpp:712-1128).  ref[c] holds the pending source
-// of chunk c as (ckpt << 32 | leaf node), or ~0 once the chunk is written.
-// Checkpoints are visited newest to oldest; for each, `cover` maps the leaves
-// of its first-occurrence and shifted regions (stamped with a generation so
-// the array is never cleared).
+// Restart (src/tree_approach.cpp:712-1128, include/reference_impl.hpp's
+// restart semantics as restated by the oracle).
+//
+// The reference walks the chain newest to oldest and, for every older
+// checkpoint, passes over all C chunks to see which still need data from it
+// (:1083-1118).  Here the walk touches only what each checkpoint carries:
+// - A chunk that no newer checkpoint re-sourced takes its data from the
+//   newest checkpoint whose regions cover it.  So each step enumerates the
+//   leaves of its own regions (k_restart_regions) and claims those that no
+//   newer step covered; the cover table doubles as the claim record (its
+//   stamps grow by one per step and per restart, so no clearing is needed).
+// - A claimed leaf of a shifted region names a source (checkpoint, chunk);
+//   the chunk goes on a pending list, and the step of the source checkpoint
+//   resolves it from that step's cover entries (k_restart_pending).
+// Work per restart is the regions' leaves plus the pending chunks, not
+// C x chain length; the data moves once, 256 chunks per workgroup round in
+// 16-byte pieces.
 // ---------------------------------------------------------------------------
-constexpr unsigned long long kRefDone = ~0ull;
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));  // a cover entry {stamp, region}
 
-__global__ void k_restart_init(uint64_t* ref, uint64_t C, uint32_t ckpt, JobCounters* ctr) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i0 == 0) {
-    ctr->unresolved = 0;
-    ctr->flags = 0;
-  }
-  for (uint64_t c = i0; c < C; c += stride) ref[c] = ((uint64_t)ckpt << 32) | (C - 1 + c);
-}
-
-__device__ __forceinline__ uint32_t load_u32(const uint8_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
-
-__global__ __launch_bounds__(256) void k_restart_cover(const RestartCkpt* rcp, TreeGeom g, CoverEntry* cover,
-                                                       uint32_t stamp, uint64_t nfl, uint64_t nsl, uint32_t cs) {
-  const RestartCkpt rc = *rcp;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nfl + nsl; e += stride) {
-    if (e < nfl) {
-      const uint32_t r = region_of(rc.f_off, rc.nfirst, e);
-      const uint32_t root = load_u32(rc.diff + rc.first_off + 4ull * r) + rc.node_bias;
-      const uint64_t leaf = run_leaf(subtree_runs(g, root), e - rc.f_off[r]);
-      CoverEntry ce;
-      ce.stamp = stamp;
-      ce.src_ckpt = kCoverFirst;
-      ce.payload = e * (uint64_t)cs;
-      cover[leaf - (g.C - 1)] = ce;
+template <int PIECE>
+__device__ __forceinline__ void copy_chunk_piece(uint8_t* __restrict__ out, uint64_t dst, const uint8_t* src,
+                                                 uint64_t len) {
+  GlobalPtr<uint8_t> o = (GlobalPtr<uint8_t>)out;
+  GlobalPtr<const uint8_t> in = (GlobalPtr<const uint8_t>)src;
+  if (dst + PIECE <= len) {
+    if (PIECE == 16) {
+      // the source is 4-byte aligned (data sections follow 4-byte ids)
+      const u32x4a4 v = *(GlobalPtr<const u32x4a4>)in;
+      *(GlobalPtr<u32x4_t>)(o + dst) = u32x4_t{v.x, v.y, v.z, v.w};
+    } else if (PIECE == 4) {
+      *(GlobalPtr<uint32_t>)(o + dst) = *(GlobalPtr<const uint32_t>)in;
     } else {
-      const uint64_t q = e - nfl;
-      const uint32_t r = region_of(rc.s_off, rc.nshift, q);
-      const uint32_t node = load_u32(rc.diff + rc.shift_off + 8ull * r) + rc.node_bias;
-      const uint32_t prev = load_u32(rc.diff + rc.shift_off + 8ull * r + 4) + rc.node_bias;
-      const uint64_t idx = q - rc.s_off[r];
-      const uint64_t leaf = run_leaf(subtree_runs(g, node), idx);
-      const uint64_t src = run_leaf(subtree_runs(g, prev), idx);
-      CoverEntry ce;
-      ce.stamp = stamp;
-      ce.src_ckpt = rc.s_tree[r];
-      ce.payload = src;
-      cover[leaf - (g.C - 1)] = ce;
-    }
-  }
-}
-
-// Each lane resolves one chunk; chunks whose data is in this checkpoint are
-// then copied by the whole workgroup, 4-byte words across lanes.
-__global__ __launch_bounds__(256) void k_restart_resolve(const RestartCkpt* rcp, TreeGeom g, const CoverEntry* cover,
-                                                         uint32_t stamp, uint64_t* ref, uint8_t* out, uint64_t len,
-                                                         uint32_t cs, JobCounters* ctr) {
-  const RestartCkpt rc = *rcp;
-  __shared__ uint64_t src_off[256];
-  const uint32_t cur = rc.ckpt_id;
-  const uint64_t c0 = (uint64_t)blockIdx.x * 256;
-  const uint64_t c = c0 + threadIdx.x;
-  uint64_t so = ~0ull;
-  if (c < g.C) {
-    const uint64_t r = ref[c];
-    if (r != kRefDone && (uint32_t)(r >> 32) == cur) {
-      const uint64_t x = (uint32_t)r;
-      const CoverEntry e = cover[x - (g.C - 1)];
-      if (e.stamp == stamp) {
-        if (e.src_ckpt == kCoverFirst) {
-          so = e.payload;
-        } else if (e.src_ckpt == cur) {
-          const CoverEntry e2 = cover[e.payload - (g.C - 1)];
-          if (e2.stamp == stamp && e2.src_ckpt == kCoverFirst) so = e2.payload;
-          else atomicOr(&ctr->flags, FLAG_BAD_SOURCE);
-        } else {
-          ref[c] = ((uint64_t)e.src_ckpt << 32) | e.payload;
-        }
-      } else {
-        ref[c] = ((uint64_t)(cur - 1) << 32) | x;  // fixed: same leaf, one checkpoint back
-      }
-      if (so != ~0ull) ref[c] = kRefDone;
-    }
-  }
-  src_off[threadIdx.x] = so;
-  __syncthreads();
-  const uint8_t* data = rc.diff + rc.data_off;
-  const uint64_t nchunks = min((uint64_t)256, g.C - c0);
-  if ((cs & 3) == 0) {
-    const uint64_t wpc = cs / 4;
-    for (uint64_t w = threadIdx.x; w < nchunks * wpc; w += 256) {
-      const uint64_t i = w / wpc, wi = w - i * wpc;
-      const uint64_t s = src_off[i];
-      if (s == ~0ull) continue;
-      const uint64_t dst = (c0 + i) * cs + 4 * wi;
-      if (dst + 4 <= len) {
-        *reinterpret_cast<uint32_t*>(out + dst) = load_u32(data + s + 4 * wi);
-      } else if (dst < len) {
-        for (uint64_t b = dst; b < len; b++) out[b] = data[s + 4 * wi + (b - dst)];
-      }
+      o[dst] = in[0];
     }
   } else {
-    for (uint64_t w = threadIdx.x; w < nchunks * cs; w += 256) {
-      const uint64_t i = w / cs, bi = w - i * cs;
-      const uint64_t s = src_off[i];
-      if (s == ~0ull) continue;
-      const uint64_t dst = (c0 + i) * cs + bi;
-      if (dst < len) out[dst] = data[s + bi];
-    }
+    for (uint64_t b = dst; b < len; b++) o[b] = in[b - dst];
   }
 }
 
-__global__ void k_restart_check(const uint64_t* ref, uint64_t C, JobCounters* ctr) {
+// Copies the chunks staged in LDS by one workgroup round: chunk i of the
+// round goes from data + src[i] to out + dst[i] * cs (src[i] == ~0: none).
+template <int PIECE>
+__device__ __forceinline__ void copy_round(const uint64_t* s_src, const uint32_t* s_dst, const uint8_t* data,
+                                           uint8_t* out, uint64_t len, uint32_t cs) {
+  const uint32_t ppc = cs / PIECE;  // pieces per chunk
+  const bool pow2 = (ppc & (ppc - 1)) == 0;
+  const uint32_t sh = pow2 ? __builtin_ctz(ppc) : 0;
+  const uint32_t total = 256 * ppc;
+  for (uint32_t w = threadIdx.x; w < total; w += 256) {
+    const uint32_t i = pow2 ? (w >> sh) : (w / ppc);
+    const uint64_t so = s_src[i];
+    if (so == ~0ull) continue;
+    const uint32_t pi = w - i * ppc;
+    copy_chunk_piece<PIECE>(out, (uint64_t)s_dst[i] * cs + (uint64_t)pi * PIECE, data + so + (uint64_t)pi * PIECE, len);
+  }
+}
+
+// Position of leaf `leaf` in the left-to-right enumeration of a subtree.
+__device__ __forceinline__ uint64_t run_index(const LeafRuns& r, uint64_t leaf) {
+  return (leaf >= r.a0 && leaf < r.a0 + r.n0) ? leaf - r.a0 : r.n0 + (leaf - r.a1);
+}
+
+__device__ __forceinline__ uint32_t rs_load32(const uint8_t* p) {
+  return *(GlobalPtr<const uint32_t>)(GlobalPtr<const uint8_t>)p;
+}
+
+// Data offset (in the data section) of chunk `c` of first-occurrence region r.
+__device__ __forceinline__ uint64_t first_src(const RestartCkpt& rc, const TreeGeom& g, uint32_t r, uint64_t c,
+                                              uint32_t cs) {
+  const uint64_t root = (uint64_t)rs_load32(rc.diff + rc.first_off + 4ull * r) + rc.node_bias;
+  const uint64_t idx = run_index(subtree_runs(g, root), c + g.C - 1);
+  return ((uint64_t)rc.f_off[r] + idx) * cs;
+}
+
+// Source chunk of chunk `c` in shifted region r: the leaf at the same
+// position of the region's source subtree.
+__device__ __forceinline__ uint64_t shift_src(const RestartCkpt& rc, const TreeGeom& g, uint32_t r, uint64_t c) {
+  const uint64_t node = (uint64_t)rs_load32(rc.diff + rc.shift_off + 8ull * r) + rc.node_bias;
+  const uint64_t prev = (uint64_t)rs_load32(rc.diff + rc.shift_off + 8ull * r + 4) + rc.node_bias;
+  const uint64_t idx = run_index(subtree_runs(g, node), c + g.C - 1);
+  return run_leaf(subtree_runs(g, prev), idx) - (g.C - 1);
+}
+
+// Appends the workgroup's pushed chunks to the pending list (capacity `cap`)
+// with one atomic per workgroup round (a counter hit once per wavefront
+// serializes hundreds of same-address atomics per step).  Each chunk is
+// claimed once per restart, so the list cannot pass C entries unless a
+// diff's regions overlap; such a diff is flagged.  `sh` holds 5 words of LDS.
+__device__ __forceinline__ void pend_push(bool push, uint32_t c, uint32_t* pend, uint64_t cap, RestartCtr* ctr,
+                                          uint32_t* sh) {
+  const unsigned long long m = __ballot(push);
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 0) sh[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t n = sh[0] + sh[1] + sh[2] + sh[3];
+    sh[4] = n ? g_add((GlobalPtr<uint32_t>)&ctr->npend, n) : 0u;
+  }
+  __syncthreads();
+  if (push) {
+    uint32_t before = 0;
+    for (uint32_t i = 0; i < w; i++) before += sh[i];
+    const uint64_t at = (uint64_t)sh[4] + before + __popcll(m & lanemask_lt());
+    if (at < cap) ((GlobalPtr<uint32_t>)pend)[at] = c;
+    else g_or((GlobalPtr<uint32_t>)&ctr->flags, FLAG_BAD_SOURCE);
+  }
+}
+
+// Source checkpoint of entry r of a shifted list (cumulative prior-table
+// counts, src/tree_approach.cpp:868-898): the first j with r < pb_end[j].
+template <class Ptr>
+__device__ __forceinline__ uint32_t prior_search(Ptr pb_end, Ptr pb_ck, uint32_t npb, uint32_t r) {
+  uint32_t lo = 0, hi = npb;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pb_end[mid] <= r) lo = mid + 1;
+    else hi = mid;
+  }
+  return pb_ck[lo < npb ? lo : npb - 1];
+}
+__device__ __forceinline__ uint32_t prior_of(const RestartCkpt& rc, uint32_t r) {
+  return prior_search((GlobalPtr<const uint32_t>)rc.pb_end, (GlobalPtr<const uint32_t>)rc.pb_ck, rc.npb, r);
+}
+
+// The prior boundaries of a checkpoint in LDS (up to kPriorLds of them).
+constexpr uint32_t kPriorLds = 128;
+struct PriorLds {
+  uint32_t end[kPriorLds], ck[kPriorLds];
+};
+__device__ __forceinline__ void prior_to_lds(const RestartCkpt& rc, PriorLds& p) {
+  if (rc.npb > kPriorLds) return;
+  for (uint32_t i = threadIdx.x; i < rc.npb; i += blockDim.x) {
+    p.end[i] = ((GlobalPtr<const uint32_t>)rc.pb_end)[i];
+    p.ck[i] = ((GlobalPtr<const uint32_t>)rc.pb_ck)[i];
+  }
+}
+__device__ __forceinline__ uint32_t prior_from(const RestartCkpt& rc, const PriorLds& p, uint32_t r) {
+  if (rc.npb <= kPriorLds) return prior_search(p.end, p.ck, rc.npb, r);
+  return prior_of(rc, r);
+}
+
+__device__ __forceinline__ uint64_t leaves_of(const TreeGeom& g, uint64_t v) {
+  const LeafRuns r = subtree_runs(g, v);
+  return r.n0 + r.n1;
+}
+
+// Parse, step 1: each block of 1024 entries of a list counts its regions'
+// leaves, checks the node ids, and writes the block-local exclusive prefix
+// into f_off / s_off and its total into blk_sum.
+__global__ __launch_bounds__(256) void k_rs_count(RestartCkpt* __restrict__ desc, const RestartList* __restrict__ lists,
+                                                  uint32_t nlists, const uint32_t* __restrict__ list_blk,
+                                                  unsigned long long* __restrict__ blk_sum, TreeGeom g,
+                                                  RestartCtr* __restrict__ ctr) {
+  __shared__ unsigned long long sh[5];
+  const uint32_t L = find_job(list_blk, nlists, blockIdx.x);
+  const RestartList li = lists[L];
+  const RestartCkpt& rc = desc[li.ckpt];
+  const uint64_t e0 = (uint64_t)(blockIdx.x - list_blk[L]) * kParseBlock + threadIdx.x * 4;
+  uint64_t cnt[4];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const uint64_t e = e0 + j;
+    cnt[j] = 0;
+    if (e >= li.n) continue;
+    if (!li.shift) {
+      const uint64_t node = (uint64_t)rs_load32(rc.diff + rc.first_off + 4 * e) + rc.node_bias;
+      if (node >= g.N) bad = true;
+      else cnt[j] = leaves_of(g, node);
+    } else {
+      const uint64_t node = (uint64_t)rs_load32(rc.diff + rc.shift_off + 8 * e) + rc.node_bias;
+      const uint64_t prev = (uint64_t)rs_load32(rc.diff + rc.shift_off + 8 * e + 4) + rc.node_bias;
+      if (node >= g.N || prev >= g.N) {
+        bad = true;
+      } else {
+        cnt[j] = leaves_of(g, node);
+        if (leaves_of(g, prev) != cnt[j]) bad = true;
+      }
+    }
+  }
+  const unsigned long long t = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+  unsigned long long total;
+  unsigned long long x = block_excl_scan<4>(t, sh, total);
+  GlobalPtr<uint32_t> off = (GlobalPtr<uint32_t>)(li.shift ? rc.s_off : rc.f_off);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (e0 + j < li.n) off[e0 + j] = (uint32_t)x;
+    x += cnt[j];
+  }
+  if (threadIdx.x == 0) ((GlobalPtr<unsigned long long>)blk_sum)[blockIdx.x] = total;
+  if (bad) g_or((GlobalPtr<uint32_t>)&ctr->flags, FLAG_BAD_DIFF);
+}
+
+// Parse, step 2: one workgroup per list scans its block totals into block
+// offsets, stores the list's leaf total (nfl / nsl and the prefix's last
+// entry) and checks it against the chunk count and the data section.
+__global__ __launch_bounds__(256) void k_rs_scan(RestartCkpt* __restrict__ desc, const RestartList* __restrict__ lists,
+                                                 unsigned long long* __restrict__ blk_sum, TreeGeom g, uint32_t cs,
+                                                 RestartCtr* __restrict__ ctr) {
+  __shared__ unsigned long long sh[5];
+  const RestartList li = lists[blockIdx.x];
+  RestartCkpt& rc = desc[li.ckpt];
+  const uint32_t nb = (li.n + kParseBlock - 1) / kParseBlock;
+  GlobalPtr<unsigned long long> bs = (GlobalPtr<unsigned long long>)blk_sum + li.blk0;
+  unsigned long long run = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+    const uint32_t b = b0 + threadIdx.x;
+    const unsigned long long v = b < nb ? bs[b] : 0ull;
+    unsigned long long tot;
+    const unsigned long long x = block_excl_scan<4>(v, sh, tot);
+    if (b < nb) bs[b] = run + x;
+    run += tot;
+  }
+  if (threadIdx.x == 0) {
+    bool bad = run > g.C;
+    if (!li.shift) {
+      bad = bad || run * cs > rc.data_bytes;
+      ((GlobalPtr<uint32_t>)rc.f_off)[li.n] = (uint32_t)run;
+      rc.nfl = run;
+    } else {
+      ((GlobalPtr<uint32_t>)rc.s_off)[li.n] = (uint32_t)run;
+      rc.nsl = run;
+    }
+    if (bad) g_or((GlobalPtr<uint32_t>)&ctr->flags, FLAG_BAD_DIFF);
+  }
+}
+
+// Parse, step 3: block offsets added to the block-local prefixes.
+__global__ __launch_bounds__(256) void k_rs_add(const RestartCkpt* __restrict__ desc,
+                                                const RestartList* __restrict__ lists, uint32_t nlists,
+                                                const uint32_t* __restrict__ list_blk,
+                                                const unsigned long long* __restrict__ blk_sum) {
+  const uint32_t L = find_job(list_blk, nlists, blockIdx.x);
+  const uint32_t add = (uint32_t)((GlobalPtr<const unsigned long long>)blk_sum)[blockIdx.x];
+  if (add == 0) return;
+  const RestartList li = lists[L];
+  const RestartCkpt& rc = desc[li.ckpt];
+  GlobalPtr<uint32_t> off = (GlobalPtr<uint32_t>)(li.shift ? rc.s_off : rc.f_off);
+  const uint64_t e0 = (uint64_t)(blockIdx.x - list_blk[L]) * kParseBlock + threadIdx.x * 4;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (e0 + j < li.n) off[e0 + j] += add;
+}
+
+// Parse, step 4: for every round of 256 leaves of a list, the region that
+// holds its first leaf (the walk then finds each leaf's region among the next
+// 256 entries in LDS, not by a search in global memory).
+__global__ __launch_bounds__(256) void k_rs_index(const RestartCkpt* __restrict__ desc,
+                                                  const RestartList* __restrict__ lists,
+                                                  RestartCtr* __restrict__ ctr) {
+  if (((GlobalPtr<const uint32_t>)&ctr->flags)[0] & FLAG_BAD_DIFF) return;
+  const RestartList li = lists[blockIdx.y];
+  const RestartCkpt& rc = desc[li.ckpt];
+  const uint64_t nl = li.shift ? rc.nsl : rc.nfl;
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t * 256 >= nl) return;
+  const GlobalPtr<const uint32_t> off = (GlobalPtr<const uint32_t>)(li.shift ? rc.s_off : rc.f_off);
+  const GlobalPtr<uint32_t> idx = (GlobalPtr<uint32_t>)(li.shift ? rc.s_idx : rc.f_idx);
+  idx[t] = region_of(off, li.n, t * 256);
+}
+
+template <int PIECE>
+__global__ __launch_bounds__(256) void k_restart_regions(const RestartCkpt* __restrict__ rcp, TreeGeom g,
+                                                         uint2* __restrict__ cover, uint32_t base, uint32_t stamp,
+                                                         uint64_t* __restrict__ ref, uint32_t* __restrict__ pend,
+                                                         RestartCtr* __restrict__ ctr, uint8_t* __restrict__ out,
+                                                         uint64_t len, uint32_t cs) {
+  const RestartCkpt rc = *rcp;
+  if (((GlobalPtr<const uint32_t>)&ctr->flags)[0] & FLAG_BAD_DIFF) return;
+  __shared__ uint64_t s_src[256];
+  __shared__ uint32_t s_dst[256];
+  __shared__ uint32_t s_off[257];
+  __shared__ uint32_t s_cnt[5];
+  __shared__ PriorLds s_pb;
+  prior_to_lds(rc, s_pb);
+  // Rounds of 256 leaves: first the first-occurrence list's, then the shifted list's.
+  const uint64_t rf = (rc.nfl + 255) / 256, rs = (rc.nsl + 255) / 256;
+  const GlobalPtr<u32x2_t> cv = (GlobalPtr<u32x2_t>)cover;
+  uint32_t claimed = 0;
+  for (uint64_t t = blockIdx.x; t < rf + rs; t += gridDim.x) {
+    const bool sh = t >= rf;
+    const uint64_t tl = sh ? t - rf : t;
+    const uint64_t nl = sh ? rc.nsl : rc.nfl;
+    const uint32_t nreg = sh ? rc.nshift : rc.nfirst;
+    const GlobalPtr<const uint32_t> off = (GlobalPtr<const uint32_t>)(sh ? rc.s_off : rc.f_off);
+    const uint32_t r0 = ((GlobalPtr<const uint32_t>)(sh ? rc.s_idx : rc.f_idx))[tl];
+    // Prefix entries r0 .. r0+256 (past the list: above every leaf index).
+    for (uint32_t i = threadIdx.x; i < 257; i += 256) s_off[i] = (uint64_t)r0 + i <= nreg ? off[r0 + i] : 0xffffffffu;
+    __syncthreads();
+    const uint64_t e = tl * 256 + threadIdx.x;
+    uint64_t so = ~0ull;
+    uint32_t c = 0;
+    bool push = false;
+    if (e < nl) {
+      uint32_t lo = 0, hi = 256;  // s_off[lo] <= e < s_off[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t r = r0 + lo;
+      const uint64_t idx = e - s_off[lo];
+      uint64_t leaf, src = 0;
+      uint32_t info, src_ck = 0;
+      if (!sh) {
+        const uint64_t root = (uint64_t)rs_load32(rc.diff + rc.first_off + 4ull * r) + rc.node_bias;
+        leaf = run_leaf(subtree_runs(g, root), idx);
+        info = r;
+      } else {
+        const uint64_t node = (uint64_t)rs_load32(rc.diff + rc.shift_off + 8ull * r) + rc.node_bias;
+        const uint64_t prev = (uint64_t)rs_load32(rc.diff + rc.shift_off + 8ull * r + 4) + rc.node_bias;
+        leaf = run_leaf(subtree_runs(g, node), idx);
+        src = run_leaf(subtree_runs(g, prev), idx) - (g.C - 1);
+        src_ck = prior_from(rc, s_pb, r);
+        info = r | kCoverShift;
+      }
+      c = (uint32_t)(leaf - (g.C - 1));
+      const u32x2_t old = cv[c];
+      cv[c] = u32x2_t{stamp, info};
+      if (old.x <= base) {  // no newer checkpoint of this restart covered the chunk
+        claimed++;
+        if (!sh) {
+          so = e * cs;  // first-occurrence data is stored in region-leaf order
+        } else {
+          push = true;
+          ((GlobalPtr<uint64_t>)ref)[c] =
+              ((uint64_t)src_ck << 32) | (src_ck == rc.ckpt_id ? kRefSameCkpt : 0u) | (uint32_t)src;
+        }
+      }
+    }
+    pend_push(push, c, pend, g.C, ctr, s_cnt);
+    s_src[threadIdx.x] = so;
+    s_dst[threadIdx.x] = c;
+    __syncthreads();
+    copy_round<PIECE>(s_src, s_dst, rc.diff + rc.data_off, out, len, cs);
+    __syncthreads();
+  }
+  const uint32_t tot = wave_sum(claimed);
+  if (lane_id() == 0 && tot) g_add((GlobalPtr<unsigned long long>)&ctr->claimed, (unsigned long long)tot);
+}
+
+template <int PIECE>
+__global__ __launch_bounds__(256) void k_restart_pending(const RestartCkpt* __restrict__ rcp, TreeGeom g,
+                                                         const uint2* __restrict__ cover, uint32_t stamp,
+                                                         uint64_t* __restrict__ ref, const uint32_t* __restrict__ pend,
+                                                         RestartCtr* __restrict__ ctr, uint8_t* __restrict__ out,
+                                                         uint64_t len, uint32_t cs) {
+  const RestartCkpt rc = *rcp;
+  if (((GlobalPtr<const uint32_t>)&ctr->flags)[0] & FLAG_BAD_DIFF) return;
+  __shared__ uint64_t s_src[256];
+  __shared__ uint32_t s_dst[256];
+  __shared__ PriorLds s_pb;
+  prior_to_lds(rc, s_pb);
+  __syncthreads();
+  const uint64_t n = min((uint64_t)((GlobalPtr<const uint32_t>)&ctr->npend)[0], g.C);
+  const uint32_t cur = rc.ckpt_id;
+  const GlobalPtr<const u32x2_t> cv = (GlobalPtr<const u32x2_t>)cover;
+  const GlobalPtr<uint64_t> rf = (GlobalPtr<uint64_t>)ref;
+  uint32_t done = 0, bad = 0;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < n; i0 += (uint64_t)gridDim.x * 256) {
+    const uint64_t i = i0 + threadIdx.x;
+    uint64_t so = ~0ull;
+    uint32_t c = 0;
+    if (i < n) {
+      c = ((GlobalPtr<const uint32_t>)pend)[i];
+      const uint64_t r = rf[c];
+      if (r != kRefDone && (uint32_t)(r >> 32) == cur) {
+        const bool same = ((uint32_t)r & kRefSameCkpt) != 0;
+        const uint64_t x = (uint32_t)r & ~kRefSameCkpt;
+        const u32x2_t e = cv[x];
+        if (e.x == stamp && !(e.y & kCoverShift)) {
+          so = first_src(rc, g, e.y, x, cs);
+        } else if (e.x == stamp && !same) {
+          const uint32_t sr = e.y & ~kCoverShift;
+          const uint32_t s2 = prior_from(rc, s_pb, sr);
+          const uint64_t x2 = shift_src(rc, g, sr, x);
+          if (s2 < cur) {
+            rf[c] = ((uint64_t)s2 << 32) | (uint32_t)x2;
+          } else {  // a source in this checkpoint must be a first occurrence here
+            const u32x2_t e2 = cv[x2];
+            if (e2.x == stamp && !(e2.y & kCoverShift)) so = first_src(rc, g, e2.y, x2, cs);
+            else bad = 1;
+          }
+        } else if (same) {
+          bad = 1;
+        } else if (cur > 0) {
+          rf[c] = ((uint64_t)(cur - 1) << 32) | (uint32_t)x;  // not here: same chunk, one checkpoint back
+        }
+        if (so != ~0ull) {
+          rf[c] = kRefDone;
+          done++;
+        }
+      }
+    }
+    s_src[threadIdx.x] = so;
+    s_dst[threadIdx.x] = c;
+    __syncthreads();
+    copy_round<PIECE>(s_src, s_dst, rc.diff + rc.data_off, out, len, cs);
+    __syncthreads();
+  }
+  const uint32_t tot = wave_sum(done);
+  if (lane_id() == 0 && tot) g_add((GlobalPtr<uint32_t>)&ctr->ndone, tot);
+  if (bad) g_or((GlobalPtr<uint32_t>)&ctr->flags, FLAG_BAD_SOURCE);
+}
+
+// Copies up to kFetchRanges device byte ranges into dst (blockIdx.y: range).
+__global__ __launch_bounds__(256) void k_fetch(FetchArgs a, uint8_t* __restrict__ dst) {
+  const uint32_t r = blockIdx.y;
+  if (r >= a.n) return;
+  const GlobalPtr<const uint8_t> src = (GlobalPtr<const uint8_t>)a.src[r];
+  const GlobalPtr<uint8_t> d = (GlobalPtr<uint8_t>)dst + a.dst_off[r];
+  const uint64_t n = a.len[r];
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  if (((reinterpret_cast<uintptr_t>(a.src[r]) | a.dst_off[r] | n) & 3) == 0) {
+    for (uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x; w < n / 4; w += stride)
+      ((GlobalPtr<uint32_t>)d)[w] = ((GlobalPtr<const uint32_t>)src)[w];
+  } else {
+    for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < n; b += stride) d[b] = src[b];
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  uint32_t n = 0;
-  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < C; c += stride) n += ref[c] != kRefDone;
-  n = wave_sum(n);
-  if (lane_id() == 0 && n) atomicAdd(&ctr->unresolved, n);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    ((GlobalPtr<uint32_t>)p)[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -2273,31 +2581,57 @@ void launch_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView nmap, ui
   DD_LAUNCH(k_map_rehash, (uint32_t)blocks, 256, 0, s, old_slots, old_cap, nmap, flags);
 }
 
-void launch_restart_init(uint64_t* ref, uint64_t C, uint32_t ckpt, JobCounters* ctr, hipStream_t s) {
-  uint64_t blocks = (C + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  DD_LAUNCH(k_restart_init, (uint32_t)blocks, 256, 0, s, ref, C, ckpt, ctr);
+void launch_fetch(const FetchArgs& a, uint64_t max_len, uint8_t* dst, hipStream_t s) {
+  if (a.n == 0) return;
+  uint64_t bx = (max_len + 4095) / 4096;
+  if (bx > 256) bx = 256;
+  if (bx == 0) bx = 1;
+  hipLaunchKernelGGL(k_fetch, dim3((uint32_t)bx, a.n), dim3(256), 0, s, a, dst);
 }
 
-void launch_restart_cover(const RestartCkpt* rc, TreeGeom g, CoverEntry* cover, uint32_t stamp, uint64_t nfl,
-                          uint64_t nsl, uint32_t cs, hipStream_t s) {
-  uint64_t n = nfl + nsl;
+void launch_restart_parse(RestartCkpt* desc, const RestartList* lists, uint32_t nlists, const uint32_t* list_blk,
+                          uint32_t nblk, uint32_t* blk_sum, TreeGeom g, uint32_t cs, RestartCtr* ctr,
+                          hipStream_t s) {
+  unsigned long long* bs = reinterpret_cast<unsigned long long*>(blk_sum);
+  if (nblk) DD_LAUNCH(k_rs_count, nblk, 256, 0, s, desc, lists, nlists, list_blk, bs, g, ctr);
+  DD_LAUNCH(k_rs_scan, nlists, 256, 0, s, desc, lists, bs, g, cs, ctr);
+  if (nblk) DD_LAUNCH(k_rs_add, nblk, 256, 0, s, desc, lists, nlists, list_blk, bs);
+  const uint32_t rounds = (uint32_t)((g.C + 255) / 256);  // rounds of 256 leaves per list, at most
+  DD_LAUNCH(k_rs_index, dim3((rounds + 255) / 256, nlists), dim3(256), 0, s, desc, lists, ctr);
+}
+
+void launch_restart_regions(const RestartCkpt* rc, TreeGeom g, uint2* cover, uint32_t base, uint32_t stamp,
+                            uint64_t* ref, uint32_t* pend, RestartCtr* ctr, uint8_t* out, uint64_t len, uint32_t cs,
+                            int piece, hipStream_t s) {
+  // The leaf count is computed on the device: a grid-stride launch of at
+  // most 2048 workgroups (8 per CU).
+  uint64_t blocks = (g.C + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  const uint32_t b = (uint32_t)blocks;
+  if (piece == 16) DD_LAUNCH(k_restart_regions<16>, b, 256, 0, s, rc, g, cover, base, stamp, ref, pend, ctr, out, len, cs);
+  else if (piece == 4) DD_LAUNCH(k_restart_regions<4>, b, 256, 0, s, rc, g, cover, base, stamp, ref, pend, ctr, out, len, cs);
+  else DD_LAUNCH(k_restart_regions<1>, b, 256, 0, s, rc, g, cover, base, stamp, ref, pend, ctr, out, len, cs);
+}
+
+void launch_restart_pending(const RestartCkpt* rc, TreeGeom g, const uint2* cover, uint32_t stamp, uint64_t* ref,
+                            const uint32_t* pend, RestartCtr* ctr, uint8_t* out, uint64_t len, uint32_t cs, int piece,
+                            hipStream_t s) {
+  // The list length is only known on the device: a grid-stride launch sized
+  // for a tenth of the chunks (workgroups past the list return at once).
+  uint64_t blocks = (g.C / 10 + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  if (blocks == 0) blocks = 1;
+  const uint32_t b = (uint32_t)blocks;
+  if (piece == 16) DD_LAUNCH(k_restart_pending<16>, b, 256, 0, s, rc, g, cover, stamp, ref, pend, ctr, out, len, cs);
+  else if (piece == 4) DD_LAUNCH(k_restart_pending<4>, b, 256, 0, s, rc, g, cover, stamp, ref, pend, ctr, out, len, cs);
+  else DD_LAUNCH(k_restart_pending<1>, b, 256, 0, s, rc, g, cover, stamp, ref, pend, ctr, out, len, cs);
+}
+
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
   if (n == 0) return;
   uint64_t blocks = (n + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  DD_LAUNCH(k_restart_cover, (uint32_t)blocks, 256, 0, s, rc, g, cover, stamp, nfl, nsl, cs);
-}
-
-void launch_restart_resolve(const RestartCkpt* rc, TreeGeom g, const CoverEntry* cover, uint32_t stamp, uint64_t* ref,
-                            uint8_t* out, uint64_t len, uint32_t cs, JobCounters* ctr, hipStream_t s) {
-  uint64_t blocks = (g.C + 255) / 256;
-  DD_LAUNCH(k_restart_resolve, (uint32_t)blocks, 256, 0, s, rc, g, cover, stamp, ref, out, len, cs, ctr);
-}
-
-void launch_restart_check(const uint64_t* ref, uint64_t C, JobCounters* ctr, hipStream_t s) {
-  uint64_t blocks = (C + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  DD_LAUNCH(k_restart_check, (uint32_t)blocks, 256, 0, s, ref, C, ctr);
+  DD_LAUNCH(k_fill_u32, (uint32_t)blocks, 256, 0, s, p, n, v);
 }
 
 }  // namespace dd

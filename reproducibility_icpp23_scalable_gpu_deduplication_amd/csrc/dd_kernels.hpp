@@ -36,8 +36,6 @@ struct JobCounters {
   uint32_t nprior;                // prior-checkpoint table entries
   uint64_t dchunks;               // chunks in first-occurrence regions
   uint64_t diff_len;              // bytes of the diff
-  uint32_t unresolved;            // restart: chunks left without a source
-  uint32_t pad;
 };
 
 // One job.  The host fills it as JobDev (plain pointers); kernels read the
@@ -129,26 +127,70 @@ using JobG = JobT<GlobalPtr>;
 static_assert(sizeof(JobDev) == sizeof(JobG), "one layout for both pointer kinds");
 
 // Restart descriptors -------------------------------------------------------
+// One checkpoint of the chain a restart walks (newest first).  The host fills
+// everything but f_off, s_off, f_idx, s_idx, nfl and nsl, which the parse
+// kernels compute from the diff's region lists.
 struct RestartCkpt {
   const uint8_t* diff;      // device copy of the diff
-  const uint32_t* f_off;    // exclusive prefix of first-region leaf counts (nfirst+1)
-  const uint32_t* s_off;    // exclusive prefix of shifted-region leaf counts (nshift+1)
-  const uint32_t* s_tree;   // source checkpoint per shifted entry
+  uint32_t* f_off;          // exclusive prefix of first-region leaf counts (nfirst+1)
+  uint32_t* s_off;          // exclusive prefix of shifted-region leaf counts (nshift+1)
+  uint32_t* f_idx;          // per round of 256 first-region leaves: the region of its first leaf
+  uint32_t* s_idx;          // the same for the shifted regions (ceil(C/256) entries each)
+  const uint32_t* pb_end;   // source checkpoints of the shifted list: entries below pb_end[j]
+  const uint32_t* pb_ck;    //   (and at or above pb_end[j-1]) come from checkpoint pb_ck[j]
+  uint32_t npb;
   uint32_t nfirst, nshift;
   uint32_t ckpt_id;
+  uint32_t node_bias;       // added to every node id of the diff (C-1 for list diffs: chunk -> leaf)
+  uint32_t pad;
   uint64_t first_off;       // byte offset of the first-occurrence list
   uint64_t shift_off;       // byte offset of the shifted list
   uint64_t data_off;        // byte offset of the data section
-  uint32_t node_bias;       // added to every node id of the diff (C-1 for list diffs: chunk -> leaf)
-  uint32_t pad;
+  uint64_t data_bytes;      // bytes of the data section
+  uint64_t nfl, nsl;        // leaves in first-occurrence / shifted regions
 };
 
-struct CoverEntry {
-  uint32_t stamp;     // generation that wrote the entry
-  uint32_t src_ckpt;  // kCoverFirst for data in this checkpoint
-  uint64_t payload;   // data byte offset, or source leaf node
+// A region list of the chain (first-occurrence or shifted list of one
+// checkpoint) in the parse kernels: `blk0` is its first block of 1024 entries
+// in the launch; blk_sum[blk0 + b] holds block b's leaf count, then its
+// offset.
+struct RestartList {
+  uint32_t ckpt;   // descriptor index
+  uint32_t shift;  // 0 first-occurrence list, 1 shifted list
+  uint32_t n;      // entries
+  uint32_t blk0;
 };
-constexpr uint32_t kCoverFirst = 0xffffffffu;
+constexpr uint32_t kParseBlock = 1024;  // region entries per parse block (256 lanes x 4)
+
+// Counters of one restart (device), zeroed by the host's upload.
+struct RestartCtr {
+  unsigned long long claimed;  // chunks whose newest covering checkpoint was found
+  uint32_t npend;              // entries of the pending list (chunks that name an older source)
+  uint32_t ndone;              // pending entries resolved
+  uint32_t flags;              // FLAG_BAD_SOURCE, FLAG_BAD_DIFF
+  uint32_t pad[3];
+};
+
+// Cover table entry, one per chunk: {stamp of the checkpoint step that wrote
+// it, region}.  Region: bit 31 set for a shifted region, low 31 bits its index
+// in the checkpoint's first-occurrence or shifted list.
+constexpr uint32_t kCoverShift = 0x80000000u;
+// Pending source of a chunk: ckpt << 32 | kRefSameCkpt? | source chunk.
+// kRefSameCkpt marks a source in the same checkpoint as the shifted region
+// that named it, which must be a first occurrence there.
+constexpr uint64_t kRefDone = ~0ull;
+constexpr uint32_t kRefSameCkpt = 0x80000000u;
+
+// Up to kFetchRanges device byte ranges copied into one buffer by one launch
+// (headers and metadata of the diffs of a chain).
+constexpr int kFetchRanges = 32;
+struct FetchArgs {
+  uint32_t n;
+  uint32_t pad;
+  const uint8_t* src[kFetchRanges];
+  uint64_t len[kFetchRanges];
+  uint64_t dst_off[kFetchRanges];
+};
 
 // Launch helpers (dd_kernels.hip) -------------------------------------------
 void launch_begin(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, bool rerun, hipStream_t s);
@@ -205,12 +247,30 @@ void launch_gather(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, u
 void launch_map_clear(Slot* slots, uint64_t cap, hipStream_t s);
 void launch_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView nmap, uint32_t* flags, hipStream_t s);
 
-void launch_restart_init(uint64_t* ref, uint64_t C, uint32_t ckpt, JobCounters* ctr, hipStream_t s);
-void launch_restart_cover(const RestartCkpt* rc, TreeGeom g, CoverEntry* cover, uint32_t stamp, uint64_t nfirst_leaves,
-                          uint64_t nshift_leaves, uint32_t chunk_size, hipStream_t s);
-void launch_restart_resolve(const RestartCkpt* rc, TreeGeom g, const CoverEntry* cover, uint32_t stamp,
-                            uint64_t* ref, uint8_t* out, uint64_t len, uint32_t chunk_size, JobCounters* ctr,
-                            hipStream_t s);
-void launch_restart_check(const uint64_t* ref, uint64_t C, JobCounters* ctr, hipStream_t s);
+void launch_fetch(const FetchArgs& a, uint64_t max_len, uint8_t* dst, hipStream_t s);
+// Parse of every region list of a chain: leaf counts and their exclusive
+// prefixes per list (RestartCkpt f_off/s_off, nfl/nsl), with the checks that
+// keep the walk in bounds (node ids, equal sizes of a shifted region and its
+// source, totals within the chunk count and the data section).  A failed
+// check sets FLAG_BAD_DIFF and the walk kernels then do nothing.
+void launch_restart_parse(RestartCkpt* desc, const RestartList* lists, uint32_t nlists, const uint32_t* list_blk,
+                          uint32_t nblk, uint32_t* blk_sum, TreeGeom g, uint32_t chunk_size, RestartCtr* ctr,
+                          hipStream_t s);
+// Restart step of one checkpoint (newest first): every leaf of its regions
+// gets a cover entry stamped `stamp`; a leaf whose cover entry is older than
+// this restart (stamp <= base) has this checkpoint as its newest cover: data
+// of a first-occurrence region is copied to `out`, a shifted region puts the
+// chunk on the pending list with its source.  `piece`: 16, 4 or 1 byte
+// copies (chosen by the host from the alignments).
+void launch_restart_regions(const RestartCkpt* rc, TreeGeom g, uint2* cover, uint32_t base, uint32_t stamp,
+                            uint64_t* ref, uint32_t* pend, RestartCtr* ctr, uint8_t* out, uint64_t len,
+                            uint32_t chunk_size, int piece, hipStream_t s);
+// Resolves the pending chunks whose source is in this checkpoint (read from
+// the cover entries the step wrote), copying their data or moving them to an
+// older checkpoint.
+void launch_restart_pending(const RestartCkpt* rc, TreeGeom g, const uint2* cover, uint32_t stamp, uint64_t* ref,
+                            const uint32_t* pend, RestartCtr* ctr, uint8_t* out, uint64_t len, uint32_t chunk_size,
+                            int piece, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
 }  // namespace dd
