@@ -29,7 +29,8 @@ typedef enum dd_status {
   DD_ENOMEM = 2,    /* host or device allocation failed */
   DD_ECORRUPT = 3,  /* a diff in the chain is inconsistent or the chain is short */
   DD_EHIP = 4,      /* a HIP runtime call failed */
-  DD_ESTATE = 5     /* call order violates the checkpoint chain (see dd_checkpoint) */
+  DD_ESTATE = 5,    /* call order violates the checkpoint chain (see dd_checkpoint) */
+  DD_ELIMIT = 6     /* the request exceeds the fixed scratch of an allocation-free entry point */
 } dd_status;
 
 typedef enum dd_hash_kind {
@@ -145,6 +146,26 @@ dd_status dd_restart(dd_dedup* h, const dd_buf* chain, uint32_t nchain, uint32_t
  * (dev_chain[i] holds chain_lens[i] bytes; both arrays are host arrays). */
 dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uint64_t* chain_lens,
                             uint32_t nchain, uint32_t chkpt_id, void* dev_out, uint64_t len);
+
+/* Allocation-free restart (no counterpart in the reference): for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, checks and bytes
+ * as dd_restart_device, but every device buffer the walk needs (cover table,
+ * pending chunks, staging, parse arrays) comes from one static device array
+ * of DD_NOALLOC_SCRATCH_BYTES per device, which the library holds until the
+ * process ends; the call allocates nothing.  Limits: at most
+ * DD_NOALLOC_MAX_CHUNKS chunks (ceil(len / chunk_size); 1 GiB at 128-byte
+ * chunks fits) and DD_NOALLOC_ARENA_BYTES of per-chain arrays (about 4 bytes
+ * per region entry plus 8 bytes per 256 chunks for each checkpoint walked);
+ * past either limit the call returns DD_ELIMIT before any kernel runs.
+ * Calls on one device share its array, so the library serializes them (a
+ * process-wide lock per device); calls from different processes use their
+ * own copies. */
+#define DD_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define DD_NOALLOC_MAX_CHUNKS (10ull << 20)
+#define DD_NOALLOC_ARENA_BYTES (DD_NOALLOC_SCRATCH_BYTES - 4096ull - 20ull * DD_NOALLOC_MAX_CHUNKS)
+dd_status dd_restart_device_noalloc(dd_dedup* h, const void* const* dev_chain, const uint64_t* chain_lens,
+                                    uint32_t nchain, uint32_t chkpt_id, void* dev_out, uint64_t len);
 
 /* Batched checkpointing (no counterpart in the reference, whose multi-rank
  * driver src/dedup_files_mpi.cpp:54-120 runs one TreeDeduplicator per MPI

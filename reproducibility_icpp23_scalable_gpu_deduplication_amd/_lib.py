@@ -10,7 +10,11 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdedup_hip.so")
 
-DD_OK, DD_EINVAL, DD_ENOMEM, DD_ECORRUPT, DD_EHIP, DD_ESTATE = range(6)
+DD_OK, DD_EINVAL, DD_ENOMEM, DD_ECORRUPT, DD_EHIP, DD_ESTATE, DD_ELIMIT = range(7)
+# Limits of dd_restart_device_noalloc (include/dedup_c.h).
+NOALLOC_SCRATCH_BYTES = 256 << 20
+NOALLOC_MAX_CHUNKS = 10 << 20
+
 HASH_MURMUR3 = 0
 HASH_MD5 = 1
 METHOD_TREE = 0
@@ -20,7 +24,7 @@ METHOD_LIST = 1
 EXPORTED = [
     "dd_create", "dd_create_method", "dd_get_method", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
     "dd_max_diff_size", "dd_counts", "dd_get_ids", "dd_set_current_id", "dd_get_timers",
-    "dd_restart", "dd_restart_device", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
+    "dd_restart", "dd_restart_device", "dd_restart_device_noalloc", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
     "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
     "dd_server_submit_restart", "dd_server_wait", "dd_server_usage", "dd_now", "dd_set_profiling",
     "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_buf_free", "dd_strerror",
@@ -79,6 +83,7 @@ def load():
         "dd_get_timers": ([vp, P(ctypes.c_double), P(ctypes.c_double)], i32),
         "dd_restart": ([vp, P(dd_buf), u32, u32, vp, u64], i32),
         "dd_restart_device": ([vp, P(vp), P(u64), u32, u32, vp, u64], i32),
+        "dd_restart_device_noalloc": ([vp, P(vp), P(u64), u32, u32, vp, u64], i32),
         "dd_batch_create": ([P(vp), u32, P(vp)], i32),
         "dd_batch_destroy": ([vp], i32),
         "dd_batch_use_graph": ([vp, i32], i32),
@@ -198,6 +203,14 @@ class TreeDeduplicator:
         ptrs = (ctypes.c_void_p * n)(*dev_ptrs)
         ls = (ctypes.c_uint64 * n)(*lens)
         _check(load().dd_restart_device(self._h, ptrs, ls, n, ckpt_id, out_ptr, length), "dd_restart_device")
+
+    def restart_device_noalloc(self, dev_ptrs, lens, ckpt_id, out_ptr, length):
+        """dd_restart_device_noalloc: the restart in the library's static
+        scratch (no device allocation); returns the dd_status."""
+        n = len(dev_ptrs)
+        ptrs = (ctypes.c_void_p * n)(*dev_ptrs)
+        ls = (ctypes.c_uint64 * n)(*lens)
+        return load().dd_restart_device_noalloc(self._h, ptrs, ls, n, ckpt_id, out_ptr, length)
 
     # -- state ------------------------------------------------------------
     def counts(self):

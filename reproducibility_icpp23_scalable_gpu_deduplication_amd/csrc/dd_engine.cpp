@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
+#include <mutex>
 #include <new>
 
 #define DD_HIP(x)                         \
@@ -635,6 +636,62 @@ class HandleSpace : public RestartSpace {
   dd_dedup* h_;
 };
 
+// The allocation-free restart's buffers: one static device array per device,
+// laid out independently of the request, so that no part of a cover table
+// ever holds another request's pending list or arena bytes:
+// [headers 4 KiB][cover 8 B x max chunks][pending sources 8 B x max][pending
+// list 4 B x max][arena].  Host staging is ordinary (pageable) memory.
+static __device__ __align__(256) uint8_t scratch[DD_NOALLOC_SCRATCH_BYTES];
+constexpr uint64_t kNaCover = 4096;
+constexpr uint64_t kNaRef = kNaCover + 8 * DD_NOALLOC_MAX_CHUNKS;
+constexpr uint64_t kNaPend = kNaRef + 8 * DD_NOALLOC_MAX_CHUNKS;
+constexpr uint64_t kNaArena = kNaPend + 4 * DD_NOALLOC_MAX_CHUNKS;
+static_assert(kNaArena + DD_NOALLOC_ARENA_BYTES == DD_NOALLOC_SCRATCH_BYTES, "scratch layout");
+static_assert(kHdrWindow * 32 <= kNaCover, "header staging");
+constexpr int kNaDevices = 64;
+
+// Per device: the device address of `scratch` (from hipGetSymbolAddress,
+// with that device current), the last stamp written into its cover table,
+// and the lock that serializes the calls using it.
+struct NoAllocDevice {
+  std::mutex mu;
+  uint8_t* ptr = nullptr;
+  uint32_t stamp = 0;
+};
+NoAllocDevice na_dev[kNaDevices];
+
+class NoAllocSpace : public RestartSpace {
+ public:
+  explicit NoAllocSpace(NoAllocDevice& d) : d_(d) {}
+  dd_status headers(uint8_t** dev) override {
+    *dev = d_.ptr;
+    return DD_OK;
+  }
+  dd_status tables(uint64_t C, uint2** cover, uint64_t** ref, uint32_t** pend, uint32_t** stamp) override {
+    if (C > DD_NOALLOC_MAX_CHUNKS) return DD_ELIMIT;
+    *cover = reinterpret_cast<uint2*>(d_.ptr + kNaCover);
+    *ref = reinterpret_cast<uint64_t*>(d_.ptr + kNaRef);
+    *pend = reinterpret_cast<uint32_t*>(d_.ptr + kNaPend);
+    *stamp = &d_.stamp;
+    return DD_OK;
+  }
+  dd_status arena(uint64_t bytes, uint8_t** dev) override {
+    if (bytes > DD_NOALLOC_ARENA_BYTES) return DD_ELIMIT;
+    *dev = d_.ptr + kNaArena;
+    return DD_OK;
+  }
+  dd_status host(uint64_t bytes, uint8_t** p) override {
+    if (host_.size() < bytes) host_.resize(bytes);
+    *p = host_.data();
+    return DD_OK;
+  }
+  bool pinned() const override { return false; }
+
+ private:
+  NoAllocDevice& d_;
+  std::vector<uint8_t> host_;
+};
+
 // Validated headers of the checkpoints a restart visits: chkpt_id down to the
 // reference id named by each visited header (src/tree_approach.cpp:943).
 // `hdr_of(i, hdr)` returns diff i's header.
@@ -1067,6 +1124,33 @@ dd_status dd_restart_device(dd_dedup* h, const void* const* dev_chain, const uin
                      h->st.restart_timers);
 }
 
+dd_status dd_restart_device_noalloc(dd_dedup* h, const void* const* dev_chain, const uint64_t* lens,
+                                    uint32_t nchain, uint32_t chkpt_id, void* dev_out, uint64_t len) {
+  if (!h || !dev_chain || !lens || !dev_out) return DD_EINVAL;
+  if (h->device < 0 || h->device >= kNaDevices) return DD_ELIMIT;
+  if (hipSetDevice(h->device) != hipSuccess) return DD_EHIP;
+  NoAllocDevice& d = na_dev[h->device];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (!d.ptr) {
+    // Set-up, once per device with the device current: the array's device
+    // address (the symbol's host-side name is not one), then a cleared
+    // cover table.
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return DD_EHIP;
+    }
+    launch_fill_u32(reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(p) + kNaCover), 2 * DD_NOALLOC_MAX_CHUNKS, 0,
+                    h->stream);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) return DD_EHIP;
+    d.ptr = static_cast<uint8_t*>(p);
+    d.stamp = 0;
+  }
+  NoAllocSpace sp(d);
+  return restart_run(sp, h->stream, h->st.method, nullptr, dev_chain, lens, nchain, chkpt_id, dev_out, len,
+                     h->st.restart_timers);
+}
+
 dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out) {
   if (!handles || n == 0 || !out) return DD_EINVAL;
   *out = nullptr;
@@ -1225,6 +1309,7 @@ const char* dd_strerror(int s) {
     case DD_ECORRUPT: return "corrupt or incomplete checkpoint chain";
     case DD_EHIP: return "HIP runtime error";
     case DD_ESTATE: return "call violates the checkpoint chain";
+    case DD_ELIMIT: return "request exceeds the allocation-free scratch";
     default: return "unknown status";
   }
 }
