@@ -174,6 +174,24 @@ static uint64_t map_find(const om_map* m, uint64_t k0, uint64_t k1, uint32_t K, 
   }
 }
 
+/* Leaf `idx` of the subtree of `node` in or_subtree_leaves' order: the run on
+ * the deepest level, then the run one level up. */
+static uint64_t subtree_leaf(uint64_t node, uint64_t C, uint64_t idx) {
+  const uint64_t N = 2 * C - 1;
+  if (node >= C - 1) return node;
+  unsigned D = 0, d = 0;
+  while ((N >> (D + 1)) > 0) D++;
+  while (((node + 1) >> (d + 1)) > 0) d++;
+  const uint64_t lo = ((node + 1) << (D - d)) - 1;
+  uint64_t hi = ((node + 2) << (D - d)) - 2;
+  if (hi > N - 1) hi = N - 1;
+  const uint64_t n0 = lo <= N - 1 ? hi - lo + 1 : 0;
+  if (idx < n0) return lo + idx;
+  uint64_t lo1 = ((node + 1) << (D - 1 - d)) - 1;
+  if (lo1 < C - 1) lo1 = C - 1;
+  return lo1 + (idx - n0);
+}
+
 static void put32(uint8_t* p, uint32_t v) { memcpy(p, &v, 4); }
 static void put64(uint8_t* p, uint64_t v) { memcpy(p, &v, 8); }
 
@@ -491,26 +509,23 @@ int om_checkpoint(om_dedup* d, const uint8_t* data, uint64_t len, int make_basel
       put32(ps + 8 * i, ss_node[i]);
       put32(ps + 8 * i + 4, ss_prev[i]);
     }
-#pragma omp parallel
-    {
-      uint32_t* leaves = (uint32_t*)malloc(C * 4);
-#pragma omp for schedule(dynamic, 64)
-      for (uint64_t i = 0; i < F; i++) {
-        if (!leaves) continue;
-        const uint64_t n = or_subtree_leaves(f_list[i], (uint32_t)C, leaves);
-        uint8_t* p = pd + f_off[i] * cs;
-        for (uint64_t q = 0; q < n; q++, p += cs) {
-          const uint64_t src = (leaves[q] - L0) * cs;
-          const uint64_t w = src + cs > len ? len - src : cs;
-          memcpy(p, data + src, w);
-          if (w < cs) memset(p + w, 0, cs - w);
-        }
+    /* Data section: data chunk q lies in region r with f_off[r] <= q <
+     * f_off[r+1]; chunks are split over the threads, not regions (one region
+     * can hold the whole buffer). */
+#pragma omp parallel for schedule(static, 4096)
+    for (uint64_t q = 0; q < DQ; q++) {
+      uint64_t lo = 0, hi = F; /* f_off[lo] <= q < f_off[hi] */
+      while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (f_off[mid] <= q) lo = mid;
+        else hi = mid;
       }
-      if (!leaves) {
-#pragma omp atomic write
-        rc = OR_ENOMEM;
-      }
-      free(leaves);
+      const uint64_t leaf = subtree_leaf(f_list[lo], C, q - f_off[lo]);
+      const uint64_t src = (leaf - L0) * cs;
+      const uint64_t w = src + cs > len ? len - src : cs;
+      uint8_t* p = pd + q * cs;
+      memcpy(p, data + src, w);
+      if (w < cs) memset(p + w, 0, cs - w);
     }
   }
   free(cnt); free(f_list); free(f_off); free(s_node); free(s_prev); free(s_tree); free(ss_node); free(ss_prev);
