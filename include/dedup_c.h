@@ -242,6 +242,23 @@ dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]);
  * hash, alignment and chunk-size combination (DD_LEAF_KERNEL=direct|deep|
  * lds64|lds128|lds256|short|dma forces one). */
 dd_status dd_get_leaf_mapping(dd_dedup* h, int* mapping);
+/* Profiling of a batch's stage sequence (no counterpart in the reference):
+ * as dd_set_profiling / dd_get_stage_times for the last batch call; the
+ * events are recorded on the batch's stream (inside its graph when graphs
+ * are enabled). */
+dd_status dd_batch_set_profiling(dd_batch* b, int level);
+dd_status dd_batch_get_stage_times(dd_batch* b, double ms[DD_NUM_STAGES]);
+/* Sizes of a handle's last completed checkpoint (no counterpart in the
+ * reference; what the algorithmic byte counts of a roofline need). */
+typedef struct dd_ckpt_stats {
+  uint64_t chunks;        /* chunks of the buffer */
+  uint64_t changed;       /* leaves whose digest was rewritten: changed leaves, every leaf at a baseline */
+  uint64_t first_chunks;  /* chunks copied into the diff's data section */
+  uint64_t diff_bytes;    /* bytes of the diff */
+  uint32_t baseline;      /* 1 when it ran as a baseline */
+  uint32_t pad;
+} dd_ckpt_stats;
+dd_status dd_get_ckpt_stats(dd_dedup* h, dd_ckpt_stats* out);
 dd_status dd_batch_get_leaf_mapping(dd_batch* b, int* mapping);
 
 void dd_buf_free(dd_buf* b);

@@ -27,7 +27,8 @@ EXPORTED = [
     "dd_restart", "dd_restart_device", "dd_restart_device_noalloc", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
     "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
     "dd_server_submit_restart", "dd_server_wait", "dd_server_usage", "dd_now", "dd_set_profiling",
-    "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_buf_free", "dd_strerror",
+    "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_batch_set_profiling",
+    "dd_batch_get_stage_times", "dd_get_ckpt_stats", "dd_buf_free", "dd_strerror",
 ]
 LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma"]
 STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
@@ -48,6 +49,11 @@ class dd_header(ctypes.Structure):
     _fields_ = [("ref_id", ctypes.c_uint32), ("chkpt_id", ctypes.c_uint32), ("datalen", ctypes.c_uint64),
                 ("chunk_size", ctypes.c_uint32), ("num_first_ocur", ctypes.c_uint32),
                 ("num_prior_chkpts", ctypes.c_uint32), ("num_shift_dupl", ctypes.c_uint32)]
+
+
+class dd_ckpt_stats(ctypes.Structure):
+    _fields_ = [("chunks", ctypes.c_uint64), ("changed", ctypes.c_uint64), ("first_chunks", ctypes.c_uint64),
+                ("diff_bytes", ctypes.c_uint64), ("baseline", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 class dd_request_result(ctypes.Structure):
@@ -99,6 +105,9 @@ def load():
         "dd_get_stage_times": ([vp, P(ctypes.c_double)], i32),
         "dd_get_leaf_mapping": ([vp, P(i32)], i32),
         "dd_batch_get_leaf_mapping": ([vp, P(i32)], i32),
+        "dd_batch_set_profiling": ([vp, i32], i32),
+        "dd_batch_get_stage_times": ([vp, P(ctypes.c_double)], i32),
+        "dd_get_ckpt_stats": ([vp, P(dd_ckpt_stats)], i32),
         "dd_buf_free": ([P(dd_buf)], None),
         "dd_strerror": ([i32], ctypes.c_char_p),
     }
@@ -250,6 +259,14 @@ class TreeDeduplicator:
         _check(load().dd_get_stage_times(self._h, ms), "dd_get_stage_times")
         return dict(zip(STAGES, list(ms)))
 
+    def ckpt_stats(self):
+        """Sizes of the last completed checkpoint: chunks, changed leaves
+        (every leaf at a baseline), first-occurrence chunks copied, diff
+        bytes, baseline flag."""
+        st = dd_ckpt_stats()
+        _check(load().dd_get_ckpt_stats(self._h, ctypes.byref(st)), "dd_get_ckpt_stats")
+        return {k: getattr(st, k) for k, _ in dd_ckpt_stats._fields_ if k != "pad"}
+
     def leaf_mapping(self):
         """Name of the leaf kernel mapping of the last checkpoint."""
         m = ctypes.c_int()
@@ -302,6 +319,15 @@ class Batch:
         m = ctypes.c_int()
         _check(load().dd_batch_get_leaf_mapping(self._h, ctypes.byref(m)), "dd_batch_get_leaf_mapping")
         return LEAF_MAPPINGS[m.value]
+
+    def set_profiling(self, level=1):
+        """0 off, 1 leaf-hash kernel events, 2 events around every stage."""
+        _check(load().dd_batch_set_profiling(self._h, int(level)), "dd_batch_set_profiling")
+
+    def stage_times(self):
+        ms = (ctypes.c_double * len(STAGES))()
+        _check(load().dd_batch_get_stage_times(self._h, ms), "dd_batch_get_stage_times")
+        return dict(zip(STAGES, list(ms)))
 
     def checkpoint_device(self, dev_ptrs, lens, make_baseline, out_ptrs, out_caps):
         n = len(self._dedups)

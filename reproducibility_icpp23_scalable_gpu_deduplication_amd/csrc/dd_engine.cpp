@@ -280,6 +280,11 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
   labels_fixed = method == DD_METHOD_TREE;
   cnt_par ^= 1u;  // this checkpoint's k_sbin_scan zeroed the other counter set
   if (!baseline) last_pending = c.pending;
+  stats.chunks = C;
+  stats.changed = baseline ? C : c.pending;
+  stats.first_chunks = c.dchunks;
+  stats.diff_bytes = c.diff_len;
+  stats.baseline = baseline ? 1 : 0;
   nfirst = c.nfirst;
   nshift = c.nshift;
   map_count = c.map_count;
@@ -399,7 +404,8 @@ dd_status Pipeline::run(const std::vector<JobDev>& jobs, hipStream_t s, bool rer
   return launch_all(s, rerun, begin, ev_dedup, stage_ev);
 }
 
-dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup, hipEvent_t* stage_ev) {
+dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup, hipEvent_t* stage_ev,
+                               int part) {
   const uint32_t nj = njobs_;
   const JobDev* J = dev_jobs_;
   auto P = [&](int st) { return dev_prefix_ + off_[st]; };
@@ -410,13 +416,19 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
     return DD_OK;
   };
   const bool leaf_events = stage_ev && profile_level_ == 1;
-  if (rerun || begin) launch_begin(J, nullptr, nj, rerun, s);
-  DD_TRY(mark(0));
-  if (!rerun) {
-    launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s,
-                leaf_events ? stage_ev[0] : nullptr, leaf_events ? stage_ev[1] : nullptr);
+  if (part != PART_TAIL) {
+    if (rerun || begin) launch_begin(J, nullptr, nj, rerun, s);
+    DD_TRY(mark(0));
+    if (!rerun) {
+      launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s,
+                  leaf_events ? stage_ev[0] : nullptr, leaf_events ? stage_ev[1] : nullptr);
+    }
+    DD_TRY(mark(1));
+    if (part == PART_HEAD) {
+      DD_HIP(hipGetLastError());
+      return DD_OK;
+    }
   }
-  DD_TRY(mark(1));
   launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], kinds_, s);
   DD_TRY(mark(2));
   if (list_mode_) {
@@ -1170,6 +1182,12 @@ dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out) 
     delete b;
     return DD_EHIP;
   }
+  for (auto& e : b->pev) {
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+      dd_batch_destroy(b);
+      return DD_EHIP;
+    }
+  }
   *out = b;
   return DD_OK;
 }
@@ -1178,6 +1196,8 @@ dd_status dd_batch_destroy(dd_batch* b) {
   if (!b) return DD_EINVAL;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& e : b->pev)
+    if (e) (void)hipEventDestroy(e);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1216,16 +1236,23 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
   }
   for (size_t i = 0; i < n; i++) b->hs[i]->st.ctr_armed = false;
   DD_TRY(b->pipe.upload(jobs, b->stream));
-  if (b->use_graph) {
+  b->pipe.profile_level_ = b->profile;
+  hipEvent_t* pev = b->profile ? b->pev : nullptr;
+  // The counter reset and the leaf kernel are launched directly (events
+  // attach to the leaf kernel's dispatch: events recorded inside a graph
+  // cannot be timed); the stages after it are replayed from the graph.
+  // Profiling level 2 records events between all stages and runs without
+  // the graph.
+  if (b->use_graph && b->profile < 2) {
     std::vector<uint64_t> sig = b->pipe.signature();
-    sig.push_back(begin ? 1 : 0);
+    DD_TRY(b->pipe.launch_all(b->stream, false, begin, nullptr, pev, Pipeline::PART_HEAD));
     if (!b->exec || sig != b->sig) {
       if (b->exec) (void)hipGraphExecDestroy(b->exec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->exec = nullptr;
       b->graph = nullptr;
       DD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      const dd_status rc = b->pipe.launch_all(b->stream, false, begin);
+      const dd_status rc = b->pipe.launch_all(b->stream, false, begin, nullptr, nullptr, Pipeline::PART_TAIL);
       const hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
       if (rc != DD_OK) return rc;
       if (ec != hipSuccess) return DD_EHIP;
@@ -1234,9 +1261,16 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
     }
     DD_HIP(hipGraphLaunch(b->exec, b->stream));
   } else {
-    DD_TRY(b->pipe.launch_all(b->stream, false, begin));
+    DD_TRY(b->pipe.launch_all(b->stream, false, begin, nullptr, pev));
   }
   DD_HIP(hipStreamSynchronize(b->stream));
+  if (b->profile) {
+    for (int i = 0; i < DD_NUM_STAGES; i++) {
+      float ms = 0;
+      if (b->profile >= 2 || i == 0) (void)hipEventElapsedTime(&ms, b->pev[i], b->pev[i + 1]);
+      b->stage_ms[i] = ms;
+    }
+  }
   std::vector<JobCounters> c(b->pipe.host_ctr, b->pipe.host_ctr + n);
   for (int attempt = 0; attempt < 4; attempt++) {
     std::vector<size_t> redo;
@@ -1291,6 +1325,24 @@ dd_status dd_set_profiling(dd_dedup* h, int enable) {
 dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]) {
   if (!h || !ms) return DD_EINVAL;
   memcpy(ms, h->stage_ms, sizeof h->stage_ms);
+  return DD_OK;
+}
+
+dd_status dd_get_ckpt_stats(dd_dedup* h, dd_ckpt_stats* out) {
+  if (!h || !out) return DD_EINVAL;
+  *out = h->st.stats;
+  return DD_OK;
+}
+
+dd_status dd_batch_set_profiling(dd_batch* b, int level) {
+  if (!b) return DD_EINVAL;
+  b->profile = level < 0 ? 0 : (level > 2 ? 2 : level);
+  return DD_OK;
+}
+
+dd_status dd_batch_get_stage_times(dd_batch* b, double ms[DD_NUM_STAGES]) {
+  if (!b || !ms) return DD_EINVAL;
+  memcpy(ms, b->stage_ms, sizeof b->stage_ms);
   return DD_OK;
 }
 

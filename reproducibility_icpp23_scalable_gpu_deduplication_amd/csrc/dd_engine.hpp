@@ -55,6 +55,7 @@ struct StreamState {
   DevBuf ctr;   // JobCounters
 
   uint64_t nfirst = 0, nshift = 0, last_diff_len = 0;
+  dd_ckpt_stats stats{};  // of the last completed checkpoint
   double timers[4] = {0, 0, 0, 0};
   double restart_timers[2] = {0, 0};
 
@@ -95,8 +96,11 @@ class Pipeline {
   // enqueue the kernels.  launch_all() enqueues nothing but kernels and event
   // records, so it can be captured into a hipGraph.
   dd_status upload(const std::vector<JobDev>& jobs, hipStream_t s);
+  // `part`: PART_ALL, or PART_HEAD (counter reset and leaf kernel) and
+  // PART_TAIL (the stages after the leaf kernel) in two calls.
+  enum { PART_ALL = 0, PART_HEAD = 1, PART_TAIL = 2 };
   dd_status launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup = nullptr,
-                       hipEvent_t* stage_ev = nullptr);
+                       hipEvent_t* stage_ev = nullptr, int part = PART_ALL);
   // Identifies the launch sequence of the last upload (grids, kernel
   // variants, table address): a captured graph is valid while it is equal.
   std::vector<uint64_t> signature() const;
@@ -163,4 +167,7 @@ struct dd_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> sig;
+  int profile = 0;  // as dd_dedup::profile, for the batch's sequence (level 2 runs without the graph)
+  hipEvent_t pev[DD_NUM_STAGES + 1] = {};
+  double stage_ms[DD_NUM_STAGES] = {};
 };
