@@ -81,6 +81,19 @@ dd_status dd_create(uint32_t chunk_size, int hash_kind, void* hip_stream, dd_ded
  * read list diffs.  The chunk count of a list chain is fixed between
  * baselines (DD_ESTATE otherwise), as for the tree. */
 dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void* hip_stream, dd_dedup** out);
+/* Options of a handle (no counterpart in the reference), applied from its
+ * next checkpoint on.  DD_OPT_EMIT_ROOT_SHIFT: when a whole buffer equals an
+ * earlier checkpoint its root (node 0) is a shifted duplicate, which the
+ * reference's labeling does not emit (include/reference_impl.hpp:105-106:
+ * node 0 becomes a root only as a first occurrence), so the diff carries no
+ * region and a restart of that checkpoint returns its predecessor's bytes;
+ * with the option node 0 is emitted as a shifted root and the checkpoint
+ * restarts to its own bytes.  The reference's restart reads such an entry
+ * like any other (src/tree_approach.cpp:887-925).  Default 0: the
+ * reference's bytes. */
+#define DD_OPT_EMIT_ROOT_SHIFT 1u
+dd_status dd_set_options(dd_dedup* h, uint32_t options);
+dd_status dd_get_options(dd_dedup* h, uint32_t* options);
 /* The method a handle was created with. */
 dd_status dd_get_method(dd_dedup* h, int* method);
 dd_status dd_destroy(dd_dedup* h);
@@ -215,7 +228,11 @@ dd_status dd_server_submit_checkpoint(dd_server* s, dd_dedup* h, const void* dev
 dd_status dd_server_submit_restart(dd_server* s, dd_dedup* h, const void* const* dev_chain,
                                    const uint64_t* chain_lens, uint32_t nchain, uint32_t chkpt_id,
                                    void* dev_out, uint64_t len, uint64_t* ticket);
-/* Blocks until the request is done; each ticket can be waited once. */
+/* Blocks until the request is done; each ticket can be waited once (a second
+ * wait, also one from another thread while the first blocks, returns
+ * DD_EINVAL).  The budget charges each handle, after each of its requests,
+ * the device bytes it holds (state, scratch and restart buffers); a handle's
+ * first checkpoint is admitted with an estimate of that state. */
 dd_status dd_server_wait(dd_server* s, uint64_t ticket, dd_request_result* out);
 /* Bytes currently reserved against the budget, and the peak so far. */
 dd_status dd_server_usage(dd_server* s, uint64_t* in_use, uint64_t* peak);
@@ -259,6 +276,10 @@ typedef struct dd_ckpt_stats {
   uint32_t pad;
 } dd_ckpt_stats;
 dd_status dd_get_ckpt_stats(dd_dedup* h, dd_ckpt_stats* out);
+/* Device bytes the handle holds now: tree, digest map, scratch arrays and
+ * restart buffers (no counterpart in the reference; what dd_server charges a
+ * handle against its budget). */
+dd_status dd_get_device_bytes(dd_dedup* h, uint64_t* bytes);
 dd_status dd_batch_get_leaf_mapping(dd_batch* b, int* mapping);
 
 void dd_buf_free(dd_buf* b);

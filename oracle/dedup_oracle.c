@@ -267,7 +267,10 @@ struct or_dedup {
   uint8_t* labels;   /* last checkpoint's labels */
   dmap map;
   uint64_t nfirst, nshift;
+  uint32_t options;  /* OR_OPT_* */
 };
+
+void or_set_options(or_dedup* d, uint32_t options) { d->options = options; }
 
 or_dedup* or_create(uint32_t chunk_size, int hash_kind) {
   if (chunk_size == 0) return NULL;
@@ -412,6 +415,10 @@ int or_checkpoint(or_dedup* d, const uint8_t* data, uint64_t len, int make_basel
     }
   }
   if (lab[0] == OR_FIRST_OCUR) is_root[0] = 1; /* :105-106 */
+  /* Option of this build: a shifted node 0 (the whole buffer equals an
+   * earlier checkpoint) is emitted too, so the checkpoint restarts to its own
+   * bytes. */
+  if ((d->options & OR_OPT_EMIT_ROOT_SHIFT) && lab[0] == OR_SHIFT_DUPL) is_root[0] = 1;
 
   /* Root lists ascending by node id (std::set order, :108-119). */
   uint64_t nf = 0, ns = 0;

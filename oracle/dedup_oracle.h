@@ -52,6 +52,10 @@ void or_hash(int kind, const void* data, uint64_t len, uint8_t out[16]);
 
 or_dedup* or_create(uint32_t chunk_size, int hash_kind);
 void or_destroy(or_dedup* d);
+/* Options of this build (no counterpart in the reference).
+ * OR_OPT_EMIT_ROOT_SHIFT: emit node 0 as a shifted root when it is one. */
+enum { OR_OPT_EMIT_ROOT_SHIFT = 1 };
+void or_set_options(or_dedup* d, uint32_t options);
 
 /* TreeDeduplicator::checkpoint(data, len, diff_h, make_baseline)
  * (src/tree_approach.cpp:1740-1749): builds the diff and advances the

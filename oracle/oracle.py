@@ -45,6 +45,8 @@ def lib():
         L.or_create.restype = ctypes.c_void_p
         L.or_destroy.argtypes = [ctypes.c_void_p]
         L.or_destroy.restype = None
+        L.or_set_options.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.or_set_options.restype = None
         L.or_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
                                     ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_uint64)]
         L.or_checkpoint.restype = ctypes.c_int
@@ -130,14 +132,21 @@ def subtree_leaves(node, num_chunks):
     return list(arr[: int(n)])
 
 
-class OracleDeduplicator:
-    """Serial CPU oracle with the TreeDeduplicator checkpoint surface."""
+OPT_EMIT_ROOT_SHIFT = 1
 
-    def __init__(self, chunk_size, hash_kind=MURMUR3):
+
+class OracleDeduplicator:
+    """Serial CPU oracle with the TreeDeduplicator checkpoint surface.
+    `options`: OPT_EMIT_ROOT_SHIFT emits a shifted node 0 (an option of this
+    build; the reference emits node 0 only as a first occurrence)."""
+
+    def __init__(self, chunk_size, hash_kind=MURMUR3, options=0):
         self._h = lib().or_create(chunk_size, hash_kind)
         if not self._h:
             raise ValueError("bad chunk size")
         self.chunk_size = chunk_size
+        if options:
+            lib().or_set_options(self._h, options)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -17,18 +17,19 @@ NOALLOC_MAX_CHUNKS = 10 << 20
 
 HASH_MURMUR3 = 0
 HASH_MD5 = 1
+OPT_EMIT_ROOT_SHIFT = 1
 METHOD_TREE = 0
 METHOD_LIST = 1
 
 # Every symbol include/dedup_c.h declares.
 EXPORTED = [
-    "dd_create", "dd_create_method", "dd_get_method", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
+    "dd_create", "dd_create_method", "dd_get_method", "dd_set_options", "dd_get_options", "dd_destroy", "dd_checkpoint", "dd_checkpoint_header", "dd_checkpoint_device",
     "dd_max_diff_size", "dd_counts", "dd_get_ids", "dd_set_current_id", "dd_get_timers",
     "dd_restart", "dd_restart_device", "dd_restart_device_noalloc", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
     "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
     "dd_server_submit_restart", "dd_server_wait", "dd_server_usage", "dd_now", "dd_set_profiling",
     "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_batch_set_profiling",
-    "dd_batch_get_stage_times", "dd_get_ckpt_stats", "dd_buf_free", "dd_strerror",
+    "dd_batch_get_stage_times", "dd_get_ckpt_stats", "dd_get_device_bytes", "dd_buf_free", "dd_strerror",
 ]
 LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma"]
 STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
@@ -78,6 +79,8 @@ def load():
         "dd_create": ([u32, i32, vp, P(vp)], i32),
         "dd_create_method": ([u32, i32, i32, vp, P(vp)], i32),
         "dd_get_method": ([vp, P(i32)], i32),
+        "dd_set_options": ([vp, u32], i32),
+        "dd_get_options": ([vp, P(u32)], i32),
         "dd_destroy": ([vp], i32),
         "dd_checkpoint": ([vp, vp, u64, i32, P(dd_buf)], i32),
         "dd_checkpoint_header": ([vp, P(dd_header), vp, u64, i32, P(dd_buf)], i32),
@@ -108,6 +111,7 @@ def load():
         "dd_batch_set_profiling": ([vp, i32], i32),
         "dd_batch_get_stage_times": ([vp, P(ctypes.c_double)], i32),
         "dd_get_ckpt_stats": ([vp, P(dd_ckpt_stats)], i32),
+        "dd_get_device_bytes": ([vp, P(u64)], i32),
         "dd_buf_free": ([P(dd_buf)], None),
         "dd_strerror": ([i32], ctypes.c_char_p),
     }
@@ -146,6 +150,17 @@ class TreeDeduplicator:
         _check(L.dd_create_method(chunk_size, hash_kind, self.METHOD, stream, ctypes.byref(h)), "dd_create_method")
         self._h = h
         self.chunk_size = chunk_size
+
+    @property
+    def options(self):
+        """DD_OPT_* flags (OPT_EMIT_ROOT_SHIFT)."""
+        o = ctypes.c_uint32()
+        _check(load().dd_get_options(self._h, ctypes.byref(o)), "dd_get_options")
+        return o.value
+
+    @options.setter
+    def options(self, flags):
+        _check(load().dd_set_options(self._h, int(flags)), "dd_set_options")
 
     @property
     def method(self):
@@ -266,6 +281,12 @@ class TreeDeduplicator:
         st = dd_ckpt_stats()
         _check(load().dd_get_ckpt_stats(self._h, ctypes.byref(st)), "dd_get_ckpt_stats")
         return {k: getattr(st, k) for k, _ in dd_ckpt_stats._fields_ if k != "pad"}
+
+    def device_bytes(self):
+        """Device bytes the handle holds (state, scratch, restart buffers)."""
+        n = ctypes.c_uint64()
+        _check(load().dd_get_device_bytes(self._h, ctypes.byref(n)), "dd_get_device_bytes")
+        return n.value
 
     def leaf_mapping(self):
         """Name of the leaf kernel mapping of the last checkpoint."""

@@ -57,6 +57,17 @@ static uint64_t next_pow2(uint64_t x) {
 
 static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+uint64_t resident_bytes(const dd_dedup* h) {
+  const StreamState& st = h->st;
+  const DevBuf* bufs[] = {&st.tree, &st.labels, &st.slot_of, &st.map, &st.blk, &st.f_list, &st.f_off, &st.s_node,
+                          &st.s_prev, &st.s_tree, &st.ss_node, &st.ss_prev, &st.s_hist, &st.ptab, &st.leafblk,
+                          &st.pend_list, &st.pend_dig, &st.tiles, &st.tile_epoch, &st.lists, &st.diff, &st.ctr,
+                          &h->cover, &h->ref, &h->pend, &h->rs_hdr, &h->rs_arena};
+  uint64_t n = h->pipe.device_bytes();
+  for (const DevBuf* b : bufs) n += b->bytes;
+  return n;
+}
+
 uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id) {
   if (chunk_size == 0) return 0;
   const uint64_t C = ceil_div(len, chunk_size);
@@ -222,6 +233,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   // narrow rounds keep more waves in flight (an MD5 field with a fifth of
   // its chunks changed: 7.8 ms per checkpoint instead of 8.1-20.8 ms).
   job.wide_probe = (!baseline && (last_pending == ~0ull || last_pending * 8 < Cn)) ? 1 : 0;
+  job.root_shift = (options & DD_OPT_EMIT_ROOT_SHIFT) ? 1 : 0;
   labels_fixed = false;  // until this checkpoint completes
   job.tree = tree.as<Digest>();
   job.labels = labels.as<uint8_t>();
@@ -491,13 +503,16 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   // a baseline resets the entry count, so it starts with k_begin.
   const bool begin = baseline || !st.ctr_armed;
   st.ctr_armed = false;
+  // The upload (and, on a table's first use, the leaf mapping's tuning) runs
+  // before the timers' first event.
+  DD_TRY(h->pipe.upload(jobs, h->stream));
   DD_HIP(hipEventRecord(h->ev[0], h->stream));
   h->pipe.profile_level_ = h->profile;
   // Events in the middle of the stream cost the host tens of microseconds per
   // checkpoint: the dedup/gather split is recorded only at profiling level 2,
   // and level 1 attaches its two events to the leaf kernel's dispatch.
-  DD_TRY(h->pipe.run(jobs, h->stream, false, begin, h->profile >= 2 ? h->ev[1] : nullptr,
-                     h->profile ? h->pev : nullptr));
+  DD_TRY(h->pipe.launch_all(h->stream, false, begin, h->profile >= 2 ? h->ev[1] : nullptr,
+                            h->profile ? h->pev : nullptr));
   DD_HIP(hipEventRecord(h->ev[2], h->stream));
   DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
   JobCounters c = h->pipe.host_ctr[0];
@@ -1044,6 +1059,18 @@ dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void*
   return DD_OK;
 }
 
+dd_status dd_set_options(dd_dedup* h, uint32_t options) {
+  if (!h || (options & ~(uint32_t)DD_OPT_EMIT_ROOT_SHIFT)) return DD_EINVAL;
+  h->st.options = options;
+  return DD_OK;
+}
+
+dd_status dd_get_options(dd_dedup* h, uint32_t* options) {
+  if (!h || !options) return DD_EINVAL;
+  *options = h->st.options;
+  return DD_OK;
+}
+
 dd_status dd_get_method(dd_dedup* h, int* method) {
   if (!h || !method) return DD_EINVAL;
   *method = h->st.method;
@@ -1331,6 +1358,12 @@ dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]) {
 dd_status dd_get_ckpt_stats(dd_dedup* h, dd_ckpt_stats* out) {
   if (!h || !out) return DD_EINVAL;
   *out = h->st.stats;
+  return DD_OK;
+}
+
+dd_status dd_get_device_bytes(dd_dedup* h, uint64_t* bytes) {
+  if (!h || !bytes) return DD_EINVAL;
+  *bytes = resident_bytes(h);
   return DD_OK;
 }
 

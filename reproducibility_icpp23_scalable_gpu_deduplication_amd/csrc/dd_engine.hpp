@@ -32,6 +32,7 @@ struct StreamState {
   uint32_t chunk_size = 0;
   int hash_kind = 0;
   int method = DD_METHOD_TREE;  // DD_METHOD_TREE or DD_METHOD_LIST
+  uint32_t options = 0;         // DD_OPT_* flags
   uint32_t current_id = 0, baseline_id = 0;
   bool have_tree = false;
   bool ctr_armed = false;  // the device counters are zero and hold the current map count
@@ -128,6 +129,7 @@ class Pipeline {
 
  public:
   int leaf_mapping() const { return leaf_map_; }
+  uint64_t device_bytes() const { return dev_table_.bytes; }
 };
 
 uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
@@ -157,6 +159,11 @@ struct dd_dedup {
   uint8_t* rs_host = nullptr;
   uint64_t rs_host_cap = 0;
 };
+
+namespace dd {
+// Device bytes a handle holds now (its state, scratch and restart buffers).
+uint64_t resident_bytes(const dd_dedup* h);
+}  // namespace dd
 
 struct dd_batch {
   int device = 0;

@@ -1432,9 +1432,10 @@ __device__ __forceinline__ void scan16(const JobG& J, uint64_t v0, const uint4 w
     if (L != FIRST_OCUR && L != SHIFT_DUPL) continue;
     bool root;
     if (v == 0) {
-      // A tree emits node 0 only as a first occurrence; in a list job node 0
-      // is a candidate only when it is the single chunk.
-      root = L == FIRST_OCUR || J.list_mode;
+      // A tree emits node 0 only as a first occurrence (or, with the
+      // root-shift option, also as a shifted root); in a list job node 0 is
+      // a candidate only when it is the single chunk.
+      root = L == FIRST_OCUR || J.list_mode || J.root_shift;
     } else {
       const uint8_t P = t == 0 ? p_first : (uint8_t)(pw >> (8 * ((t - 1) >> 1)));
       root = P == DONE;

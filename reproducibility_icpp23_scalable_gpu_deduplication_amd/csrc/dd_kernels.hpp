@@ -66,6 +66,11 @@ struct JobT {
   // wavefront waiting for its longest chain); 0: narrow rounds (baselines
   // and change-heavy checkpoints, bound by random-line traffic).
   uint32_t wide_probe;
+  // 1: node 0 is emitted as a shifted root when the whole buffer equals an
+  // earlier checkpoint (DD_OPT_EMIT_ROOT_SHIFT); 0: only as a first
+  // occurrence, as the reference does (include/reference_impl.hpp:105-106).
+  uint32_t root_shift;
+  uint32_t pad_opts;
   // persistent state
   P<Digest> tree;        // N digests, heap order
   P<uint8_t> labels;     // N labels (padded for 16-byte reads)

@@ -81,8 +81,10 @@ struct dd_server {
     for (auto it = queue.begin(); it != queue.end(); ++it) {
       Request* r = *it;
       if (busy.count(r->h)) continue;
-      const bool new_state = r->kind == 0 && !charged.count(r->h);
-      const uint64_t state = new_state ? state_bytes(r->len, r->h->st.chunk_size) : 0;
+      const bool new_state = !charged.count(r->h);
+      // A handle's first request is charged an estimate of its state; after
+      // every request the charge becomes the bytes it actually holds.
+      const uint64_t state = new_state ? (r->kind == 0 ? state_bytes(r->len, r->h->st.chunk_size) : 0) : 0;
       const uint64_t need = r->footprint + state;
       if (in_use + need > budget && !busy.empty()) return nullptr;
       queue.erase(it);
@@ -116,9 +118,16 @@ struct dd_server {
       }
       r->res.t_end = steady_now();
       r->res.status = st;
+      const uint64_t resident = dd::resident_bytes(r->h);  // the handle is not used elsewhere meanwhile
       lk.lock();
       busy.erase(r->h);
-      in_use -= r->footprint;  // the handle's state stays charged
+      in_use -= r->footprint;
+      // The handle's buffers stay allocated after the request (its state, and
+      // restart buffers that grow with the chain): charge what it holds now.
+      uint64_t& ch = charged[r->h];
+      in_use = in_use - ch + resident;
+      ch = resident;
+      if (in_use > peak) peak = in_use;
       r->done = true;
       cv_done.notify_all();
       cv_work.notify_all();
@@ -216,6 +225,8 @@ dd_status dd_server_wait(dd_server* s, uint64_t ticket, dd_request_result* out) 
   auto it = s->by_ticket.find(ticket);
   if (it == s->by_ticket.end()) return DD_EINVAL;
   Request* r = it->second;
+  if (r->waited) return DD_EINVAL;  // one waiter per ticket: another thread waits for it
+  r->waited = true;
   s->cv_done.wait(lk, [&] { return r->done; });
   *out = r->res;
   s->by_ticket.erase(it);

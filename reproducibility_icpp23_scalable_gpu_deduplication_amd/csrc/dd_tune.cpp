@@ -9,11 +9,15 @@
 // a new (device, hash, alignment, chunk sizes) key times every valid mapping
 // on the largest job's first 256 MiB and keeps the fastest for the process.
 // The timed launches run as baselines on scratch tree, label and list arrays,
-// so the handles' state is not touched.  DD_LEAF_KERNEL=direct|deep|lds64|
+// so the handles' state is not touched.  The mappings are timed round-robin
+// (each round launches every mapping once) and each keeps its median, so a
+// drift of the device's clock or of other work affects all alike.  One
+// thread tunes a key; callers that need the same key meanwhile wait for it.  DD_LEAF_KERNEL=direct|deep|lds64|
 // lds128|lds256|short|dma forces a mapping (when it is valid for the table).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -29,6 +33,8 @@ namespace {
 using Key = std::tuple<int, int, int, uint32_t, uint32_t>;  // device, hash, aligned16, min chunk, max chunk
 
 std::mutex g_mu;
+std::condition_variable g_cv;
+constexpr int kTuning = -1;  // g_choice value while one thread tunes the key
 std::map<Key, int> g_choice;
 
 bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) {
@@ -56,21 +62,17 @@ int forced_mapping() {
   return -1;
 }
 
-// Mean time of `reps` launches of one mapping on the scratch job (after one
-// untimed launch), in milliseconds; negative on a HIP error.
+// Time of one launch of a mapping on the scratch job, in milliseconds;
+// negative on a HIP error.
 double time_mapping(const JobDev* dj, const uint32_t* dprefix, uint32_t grid, int m, bool al16, int kind, hipStream_t s,
-                    hipEvent_t e0, hipEvent_t e1, int reps) {
-  double tot = 0;
-  for (int r = 0; r <= reps; r++) {
-    launch_begin(dj, nullptr, 1, false, s);
-    if (hipEventRecord(e0, s) != hipSuccess) return -1;
-    launch_leaf(dj, dprefix, 1, grid, m, al16, kind, s);
-    if (hipEventRecord(e1, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return -1;
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1;
-    if (r > 0) tot += ms;
-  }
-  return tot / reps;
+                    hipEvent_t e0, hipEvent_t e1) {
+  launch_begin(dj, nullptr, 1, false, s);
+  if (hipEventRecord(e0, s) != hipSuccess) return -1;
+  launch_leaf(dj, dprefix, 1, grid, m, al16, kind, s);
+  if (hipEventRecord(e1, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return -1;
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1;
+  return ms;
 }
 
 }  // namespace
@@ -90,10 +92,24 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   (void)hipGetDevice(&dev);
   const Key key{dev, kind, al16 ? 1 : 0, cmin, cmax};
   {
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::unique_lock<std::mutex> lk(g_mu);
+    g_cv.wait(lk, [&] {
+      auto it = g_choice.find(key);
+      return it == g_choice.end() || it->second != kTuning;
+    });
     auto it = g_choice.find(key);
     if (it != g_choice.end()) return it->second;
+    g_choice[key] = kTuning;  // this thread tunes the key
   }
+  // Publishes the result (or, when tuning could not run, removes the marker
+  // so a later call tries again) and wakes the waiting callers.
+  auto publish = [&](int m, bool keep) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (keep) g_choice[key] = m;
+    else g_choice.erase(key);
+    g_cv.notify_all();
+    return m;
+  };
   // Scratch baseline job over the first (at most) 256 MiB of the largest job.
   const uint64_t cs = R->chunk_size;
   uint64_t tl = std::min<uint64_t>(R->len, 256ull << 20);
@@ -105,7 +121,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   if (tree.ensure(16 * N) != DD_OK || labels.ensure(N + 64) != DD_OK ||
       ctr.ensure(sizeof(JobCounters) + 2 * 32 * kCntStride * 4) != DD_OK || plist.ensure(32 * pend_cap * 4) != DD_OK ||
       table.ensure(sizeof(JobDev) + LEAF_NMAP * 2 * 4) != DD_OK)
-    return best;  // no memory to tune: keep the default, do not cache
+    return publish(best, false);  // no memory to tune: keep the default, do not cache
   JobDev t;
   memset(&t, 0, sizeof t);
   t.data = R->data;
@@ -143,27 +159,37 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
       hipStreamSynchronize(s) != hipSuccess || hipEventCreateWithFlags(&e0, hipEventDisableSystemFence) != hipSuccess ||
       hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
-    return best;
+    return publish(best, false);
   }
   const JobDev* dj = table.as<JobDev>();
   const uint32_t* dprefix = reinterpret_cast<const uint32_t*>(table.as<uint8_t>() + sizeof(JobDev));
+  // One untimed launch per mapping, then kRounds rounds of one timed launch
+  // each; the fastest median wins.
+  constexpr int kRounds = 7;
+  std::vector<double> times[LEAF_NMAP];
+  bool ok = true;
+  for (int r = 0; r <= kRounds && ok; r++) {
+    for (int m = 0; m < LEAF_NMAP && ok; m++) {
+      if (!mapping_valid(m, al16, kind, jobs)) continue;
+      const double ms = time_mapping(dj, dprefix + 2 * m, grids[m], m, al16, kind, s, e0, e1);
+      if (ms < 0) ok = false;
+      else if (r > 0) times[m].push_back(ms);
+    }
+  }
   double best_ms = -1;
-  for (int m = 0; m < LEAF_NMAP; m++) {
-    if (!mapping_valid(m, al16, kind, jobs)) continue;
-    const double ms = time_mapping(dj, dprefix + 2 * m, grids[m], m, al16, kind, s, e0, e1, 3);
-    if (ms >= 0 && (best_ms < 0 || ms < best_ms)) {
-      best_ms = ms;
+  for (int m = 0; m < LEAF_NMAP && ok; m++) {
+    if (times[m].empty()) continue;
+    std::sort(times[m].begin(), times[m].end());
+    const double med = times[m][times[m].size() / 2];
+    if (best_ms < 0 || med < best_ms) {
+      best_ms = med;
       best = m;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipGetLastError();
-  if (best_ms >= 0) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_choice[key] = best;
-  }
-  return best;
+  return publish(best, ok && best_ms >= 0);
 }
 
 }  // namespace dd

@@ -123,6 +123,29 @@ def test_whole_buffer_repeat_matches_oracle(torch_gpu):
     assert gpu_restart(torch_gpu, g, gd, 2, 8) == b"IJKLMNOP"
 
 
+def test_whole_buffer_repeat_root_shift_option(torch_gpu):
+    # With DD_OPT_EMIT_ROOT_SHIFT node 0 is emitted as a shifted root: the
+    # diff equals the oracle's with the same option and the repeated
+    # checkpoint restarts to its own bytes (chunk 1 and a 64 KiB chain).
+    rng = np.random.default_rng(3)
+    x0, x1 = (rng.integers(0, 256, 1 << 16, dtype=np.uint8) for _ in range(2))
+    for chunk, datas in ((1, [b"ABCDEFGH", b"IJKLMNOP", b"ABCDEFGH"]), (64, [x0, x1, x0.copy()])):
+        g = dd.TreeDeduplicator(chunk)
+        g.options = dd.OPT_EMIT_ROOT_SHIFT
+        assert g.options == dd.OPT_EMIT_ROOT_SHIFT
+        o = O.OracleDeduplicator(chunk, options=O.OPT_EMIT_ROOT_SHIFT)
+        gd = []
+        for i, x in enumerate(datas):
+            t = to_dev(torch_gpu, x)
+            gd.append(g.checkpoint(t.data_ptr(), t.numel(), i == 0))
+            want = o.checkpoint(bytes(x), i == 0)
+            assert gd[-1] == want, f"chunk {chunk} checkpoint {i}"
+        assert struct.unpack_from("<IIQIIII", gd[2])[4:] == (0, 1, 1)
+        assert gpu_restart(torch_gpu, g, gd, 2, len(datas[2])) == bytes(datas[2])
+    with pytest.raises(dd.DedupError):
+        g.options = 2
+
+
 def test_config1_shape_64mib(torch_gpu):
     # 64 MiB, 128 B chunks, 2^19 leaves, 10 checkpoints: 1 % fresh chunks,
     # 2 % copied chunks, one 1 MiB zero region per checkpoint.

@@ -89,7 +89,10 @@ def test_server_mix_equals_alone(torch_gpu):
     in_use, peak = srv.usage()
     srv.close()
     assert peak <= budget
-    assert in_use > 0  # the handles' device state stays charged
+    # with nothing in flight the budget holds exactly what the handles hold
+    # (their state and their restart buffers)
+    assert in_use == sum(h.device_bytes() for h in handles)
+    assert in_use > 0
 
 
 def test_server_oversized_request_runs_alone(torch_gpu):
@@ -113,3 +116,20 @@ def test_server_oversized_request_runs_alone(torch_gpu):
     in_use, peak = srv.usage()
     srv.close()
     assert peak > (1 << 16)  # the oversized request was admitted alone
+
+
+def test_server_ticket_waited_once(torch_gpu):
+    torch = torch_gpu
+    x = np.random.default_rng(4).integers(0, 256, 1 << 16, dtype=np.uint8)
+    t = torch.from_numpy(x).to("cuda")
+    cap = dd.max_diff_size(x.nbytes, CHUNK, 0)
+    o = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    srv = dd.Server(1 << 30, slots=1)
+    g = dd.TreeDeduplicator(CHUNK)
+    tk = srv.submit_checkpoint(g, t.data_ptr(), t.numel(), o.data_ptr(), cap, True)
+    assert srv.wait(tk)["status"] == dd.DD_OK
+    with pytest.raises(dd.DedupError) as e:
+        srv.wait(tk)
+    assert e.value.status == dd.DD_EINVAL
+    srv.close()

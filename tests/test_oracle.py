@@ -291,6 +291,24 @@ def test_shift_root_zero_is_not_emitted():
     assert O.restart([c0, c1, c2], 2, 8) == b"IJKLMNOP"
 
 
+def test_shift_root_zero_emitted_with_the_option():
+    # Option of this build: node 0 is emitted as a shifted root, so the
+    # repeated buffer restarts to its own bytes.  Hand-derived diff: header
+    # (ref 0, ckpt 2, datalen 8, chunk 1, F 0, P 1, S 1), prior table
+    # {(0, 1)}, shifted entry (node 0, source node 0 of checkpoint 0).
+    d = O.OracleDeduplicator(1, options=O.OPT_EMIT_ROOT_SHIFT)
+    c0 = d.checkpoint(b"ABCDEFGH", True)
+    c1 = d.checkpoint(b"IJKLMNOP", False)
+    c2 = d.checkpoint(b"ABCDEFGH", False)
+    assert c2 == struct.pack("<IIQIIII", 0, 2, 8, 1, 0, 1, 1) + struct.pack("<II", 0, 1) + struct.pack("<II", 0, 0)
+    assert O.restart([c0, c1, c2], 2, 8) == b"ABCDEFGH"
+    # the option changes nothing else: the same chain without a repeat
+    plain = O.OracleDeduplicator(1)
+    opt = O.OracleDeduplicator(1, options=O.OPT_EMIT_ROOT_SHIFT)
+    for i, x in enumerate([b"ABCDEFGH", b"ABCDEFGX", b"XBCDEFGH", b"ABCDABCD"]):
+        assert plain.checkpoint(x, i == 0) == opt.checkpoint(x, i == 0)
+
+
 def test_restart_errors():
     d = O.OracleDeduplicator(4)
     c0 = d.checkpoint(bytes(range(64)), True)
