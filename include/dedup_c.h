@@ -41,12 +41,22 @@ typedef enum dd_hash_kind {
 /* Deduplication method of a handle (the reference's DedupMode,
  * include/utils.hpp:40-49, restricted to the approaches built here). */
 typedef enum dd_method {
-  DD_METHOD_TREE = 0, /* TreeDeduplicator, src/tree_approach.cpp; also
-                         TreeLowRootDeduplicator, whose checkpoint runs the same
-                         labeling (src/tree_low_root_approach.cpp:393-399) */
-  DD_METHOD_LIST = 1  /* ListDeduplicator, src/list_approach.cpp:23-256: one
+  DD_METHOD_TREE = 0, /* TreeDeduplicator, src/tree_approach.cpp: low-offset
+                         first occurrences */
+  DD_METHOD_LIST = 1, /* ListDeduplicator, src/list_approach.cpp:23-256: one
                          digest per chunk, no tree; diff node ids are chunk
                          indices */
+  DD_METHOD_TREE_LOW_ROOT = 2 /* TreeLowRootDeduplicator: the tree method with
+                         low-root first occurrences, the rule of
+                         dedup_low_root_ref (include/reference_impl.hpp:265-472)
+                         made deterministic: among the changed leaves whose
+                         digest has no entry from an earlier checkpoint, the
+                         first occurrence of a digest is the leaf whose highest
+                         ancestor with only such leaves below has the smallest
+                         node id, ties to the lowest leaf.  Tree diffs and
+                         restarts.  (The reference's TreeLowRootDeduplicator
+                         runs the low-offset labeling,
+                         src/tree_low_root_approach.cpp:393-399.) */
 } dd_method;
 
 /* header_t, include/utils.hpp:29-37 (32 bytes, little-endian). */

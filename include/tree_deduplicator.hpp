@@ -89,14 +89,15 @@ class TreeDeduplicator {
   std::pair<uint64_t, uint64_t> datasizes_{0, 0};
 };
 
-// TreeLowRootDeduplicator(uint32_t), src/tree_low_root_approach.cpp:8-12.  Its
-// checkpoint runs the same labeling as TreeDeduplicator
-// (src/tree_low_root_approach.cpp:393-399; dedup_data_low_root is never
-// called), so it is the tree method with the tree's files and logs.
+// TreeLowRootDeduplicator(uint32_t), src/tree_low_root_approach.cpp:8-12:
+// the tree method with low-root first occurrences (DD_METHOD_TREE_LOW_ROOT,
+// the rule of dedup_low_root_ref, include/reference_impl.hpp:265-472, made
+// deterministic; the reference's own class runs the low-offset labeling,
+// :393-399).  Tree diffs, ".hashtree.incr_chkpt" files and "Tree" log rows.
 class TreeLowRootDeduplicator : public TreeDeduplicator {
  public:
   explicit TreeLowRootDeduplicator(uint32_t chunk_size, HashKind kind = HashKind::Murmur3, void* hip_stream = nullptr)
-      : TreeDeduplicator(chunk_size, kind, hip_stream) {}
+      : TreeDeduplicator(chunk_size, kind, hip_stream, DD_METHOD_TREE_LOW_ROOT, "Tree", ".hashtree.incr_chkpt") {}
 };
 
 // ListDeduplicator(uint32_t bytes_per_chunk), src/list_approach.cpp:5-9: the

@@ -321,6 +321,61 @@ static void put64(uint8_t* p, uint64_t v) { for (int i = 0; i < 8; i++) p[i] = (
 static uint32_t get32(const uint8_t* p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24); }
 static uint64_t get64(const uint8_t* p) { return (uint64_t)get32(p) | ((uint64_t)get32(p + 4) << 32); }
 
+/* Low-root first occurrences: the TreeLowRootDeduplicator labeling
+ * (include/reference_impl.hpp:265-472, dedup_low_root_ref), restated as a
+ * deterministic rule.  A leaf is new when it changed and its digest has no
+ * entry from an earlier checkpoint (the reference's FIRST_DUPL, :307-323); an
+ * interior node is new when both children are (:327-344).  Among the new
+ * leaves with one digest, the first occurrence is the leaf whose highest new
+ * ancestor (the reference's possible_root, :374-399: walk up while the parent
+ * is new) has the smallest node id, ties to the lowest leaf (the reference
+ * keeps the first of its list with a strictly smaller root, and lists leaves
+ * in ascending order); the others are shifted duplicates of it.  Not kept:
+ * the reference's pass that erases roots holding duplicate leaves
+ * (:347-372) while iterating a std::unordered_map, whose result depends on
+ * that map's iteration order.  Runs after the low-offset leaf pass, whose
+ * labels it corrects: the map entry of each new digest (this checkpoint's)
+ * is moved to the selected leaf. */
+static int low_root_select(or_dedup* d, uint8_t* lab, const uint8_t* curr, uint64_t C, uint32_t K) {
+  const uint64_t N = 2 * C - 1;
+  uint8_t* nw = (uint8_t*)calloc(N, 1);
+  uint64_t* slot = (uint64_t*)malloc(N * sizeof(uint64_t));
+  uint64_t* best = (uint64_t*)malloc(d->map.cap * sizeof(uint64_t));
+  if (!nw || !slot || !best) { free(nw); free(slot); free(best); return OR_ENOMEM; }
+  for (uint64_t leaf = C - 1; leaf < N; leaf++) {
+    if (lab[leaf] != OR_FIRST_OCUR && lab[leaf] != OR_SHIFT_DUPL) continue;
+    const uint64_t s = dmap_find(&d->map, curr + 16 * leaf);
+    if (s == ~(uint64_t)0 || d->map.vals[s].tree != K) continue; /* an earlier checkpoint's digest */
+    nw[leaf] = 1;
+    slot[leaf] = s;
+    best[s] = ~(uint64_t)0;
+  }
+  if (C >= 2)
+    for (uint64_t node = C - 2;; node--) {
+      nw[node] = nw[2 * node + 1] && nw[2 * node + 2];
+      if (node == 0) break;
+    }
+  for (uint64_t leaf = C - 1; leaf < N; leaf++) {
+    if (!nw[leaf]) continue;
+    uint64_t t = leaf;
+    while (t > 0 && nw[(t - 1) / 2]) t = (t - 1) / 2;
+    const uint64_t key = (t << 32) | leaf;
+    if (key < best[slot[leaf]]) best[slot[leaf]] = key;
+  }
+  for (uint64_t leaf = C - 1; leaf < N; leaf++) {
+    if (!nw[leaf]) continue;
+    const uint64_t s = slot[leaf];
+    if ((uint32_t)best[s] == (uint32_t)leaf) {
+      lab[leaf] = OR_FIRST_OCUR;
+      d->map.vals[s].node = (uint32_t)leaf;
+    } else {
+      lab[leaf] = OR_SHIFT_DUPL;
+    }
+  }
+  free(nw); free(slot); free(best);
+  return OR_OK;
+}
+
 int or_checkpoint(or_dedup* d, const uint8_t* data, uint64_t len, int make_baseline,
                   uint8_t** out, uint64_t* out_len) {
   *out = NULL;
@@ -380,6 +435,10 @@ int or_checkpoint(or_dedup* d, const uint8_t* data, uint64_t len, int make_basel
         lab[leaf] = OR_SHIFT_DUPL;
       }
     }
+  }
+  if (d->options & OR_OPT_LOW_ROOT) {
+    int lr = low_root_select(d, lab, curr, C64, K);
+    if (lr != OR_OK) { free(curr); free(lab); free(is_root); return lr; }
   }
   if (C64 >= 2) {
     for (uint64_t node = C64 - 2;; node--) { /* first-occurrence forest, :73-82 */

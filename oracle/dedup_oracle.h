@@ -53,8 +53,11 @@ void or_hash(int kind, const void* data, uint64_t len, uint8_t out[16]);
 or_dedup* or_create(uint32_t chunk_size, int hash_kind);
 void or_destroy(or_dedup* d);
 /* Options of this build (no counterpart in the reference).
- * OR_OPT_EMIT_ROOT_SHIFT: emit node 0 as a shifted root when it is one. */
-enum { OR_OPT_EMIT_ROOT_SHIFT = 1 };
+ * OR_OPT_EMIT_ROOT_SHIFT: emit node 0 as a shifted root when it is one.
+ * OR_OPT_LOW_ROOT: the low-root first-occurrence rule of
+ * TreeLowRootDeduplicator (include/reference_impl.hpp:265-472), restated as a
+ * deterministic rule (dedup_oracle.c, low_root_select). */
+enum { OR_OPT_EMIT_ROOT_SHIFT = 1, OR_OPT_LOW_ROOT = 2 };
 void or_set_options(or_dedup* d, uint32_t options);
 
 /* TreeDeduplicator::checkpoint(data, len, diff_h, make_baseline)

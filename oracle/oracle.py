@@ -133,6 +133,7 @@ def subtree_leaves(node, num_chunks):
 
 
 OPT_EMIT_ROOT_SHIFT = 1
+OPT_LOW_ROOT = 2  # the low-root first-occurrence rule (TreeLowRootDeduplicator)
 
 
 class OracleDeduplicator:

@@ -20,6 +20,7 @@ HASH_MD5 = 1
 OPT_EMIT_ROOT_SHIFT = 1
 METHOD_TREE = 0
 METHOD_LIST = 1
+METHOD_TREE_LOW_ROOT = 2
 
 # Every symbol include/dedup_c.h declares.
 EXPORTED = [
@@ -302,9 +303,12 @@ class TreeDeduplicator:
 
 
 class TreeLowRootDeduplicator(TreeDeduplicator):
-    """TreeLowRootDeduplicator (include/tree_approach.hpp:207-218): its
-    checkpoint runs the tree's low-offset labeling
-    (src/tree_low_root_approach.cpp:393-399), so it is the tree method."""
+    """TreeLowRootDeduplicator (include/tree_approach.hpp:207-218): the tree
+    method with low-root first occurrences (DD_METHOD_TREE_LOW_ROOT): the
+    rule of dedup_low_root_ref (include/reference_impl.hpp:265-472) made
+    deterministic.  Tree diffs and restarts."""
+
+    METHOD = METHOD_TREE_LOW_ROOT
 
 
 class ListDeduplicator(TreeDeduplicator):

@@ -59,7 +59,7 @@ static uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 uint64_t resident_bytes(const dd_dedup* h) {
   const StreamState& st = h->st;
-  const DevBuf* bufs[] = {&st.tree, &st.labels, &st.slot_of, &st.map, &st.blk, &st.f_list, &st.f_off, &st.s_node,
+  const DevBuf* bufs[] = {&st.tree, &st.labels, &st.slot_of, &st.lr_new, &st.map, &st.blk, &st.f_list, &st.f_off, &st.s_node,
                           &st.s_prev, &st.s_tree, &st.ss_node, &st.ss_prev, &st.s_hist, &st.ptab, &st.leafblk,
                           &st.pend_list, &st.pend_dig, &st.tiles, &st.tile_epoch, &st.lists, &st.diff, &st.ctr,
                           &h->cover, &h->ref, &h->pend, &h->rs_hdr, &h->rs_arena};
@@ -138,6 +138,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   const uint32_t K = current_id;
   need_baseline = true;  // until commit()
   if (baseline) {
+    if (method == DD_METHOD_TREE_LOW_ROOT) DD_TRY(lr_new.ensure(Nn));
     DD_TRY(tree.ensure(Nn * sizeof(Digest)));
     DD_TRY(labels.ensure(ceil_div(Nn, kNodesPerBlock) * kNodesPerBlock + 64));  // root scans read 16 at a time
     DD_TRY(slot_of.ensure(Nn * 4));
@@ -234,6 +235,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   // its chunks changed: 7.8 ms per checkpoint instead of 8.1-20.8 ms).
   job.wide_probe = (!baseline && (last_pending == ~0ull || last_pending * 8 < Cn)) ? 1 : 0;
   job.root_shift = (options & DD_OPT_EMIT_ROOT_SHIFT) ? 1 : 0;
+  job.low_root = method == DD_METHOD_TREE_LOW_ROOT ? 1 : 0;
+  job.lr_new = job.low_root ? lr_new.as<uint8_t>() : nullptr;
   labels_fixed = false;  // until this checkpoint completes
   job.tree = tree.as<Digest>();
   job.labels = labels.as<uint8_t>();
@@ -289,7 +292,7 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
   need_baseline = false;
   // The gather kernel of a completed tree checkpoint reset every label of
   // the tiles it visited; the list method keeps its labels.
-  labels_fixed = method == DD_METHOD_TREE;
+  labels_fixed = method != DD_METHOD_LIST;
   cnt_par ^= 1u;  // this checkpoint's k_sbin_scan zeroed the other counter set
   if (!baseline) last_pending = c.pending;
   stats.chunks = C;
@@ -323,10 +326,14 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   aligned16_ = vec16_ = true;
   hash_kind_ = jobs[0].hash_kind;
   list_mode_ = jobs[0].list_mode != 0;
+  low_root_ = false;
+  max_c_ = 0;
   kinds_ = 0;
   for (const JobDev& J : jobs) {
     if (J.hash_kind != hash_kind_ || (J.list_mode != 0) != list_mode_) return DD_EINVAL;
     kinds_ |= J.is_baseline ? JOBS_DENSE : J.wide_probe ? JOBS_LIST_WIDE : JOBS_LIST_NARROW;
+    low_root_ = low_root_ || J.low_root;
+    max_c_ = std::max<uint64_t>(max_c_, J.C);
     max_groups_ = std::max(max_groups_, groups_of(J.D));
     const bool al = (reinterpret_cast<uintptr_t>(J.data) & 15) == 0 && (J.chunk_size & 15) == 0;
     aligned16_ = aligned16_ && al;
@@ -402,7 +409,9 @@ std::vector<uint64_t> Pipeline::signature() const {
   sig.push_back((uint64_t)max_groups_);
   sig.push_back((uint64_t)leaf_map_);
   sig.push_back((uint64_t)aligned16_ | ((uint64_t)vec16_ << 1));
-  sig.push_back((uint64_t)hash_kind_ | ((uint64_t)list_mode_ << 8) | ((uint64_t)kinds_ << 16));
+  sig.push_back((uint64_t)hash_kind_ | ((uint64_t)list_mode_ << 8) | ((uint64_t)kinds_ << 16) |
+                ((uint64_t)low_root_ << 24));
+  sig.push_back(low_root_ ? max_c_ : 0);
   sig.push_back(reinterpret_cast<uintptr_t>(dev_table_.p));
   for (uint32_t o : off_) sig.push_back(o);
   for (uint32_t g : grid_) sig.push_back(g);
@@ -442,6 +451,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
     }
   }
   launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], kinds_, s);
+  if (low_root_) launch_low_root(J, nj, max_c_, (uint32_t)ilog2_64(2 * max_c_ - 1), s);
   DD_TRY(mark(2));
   if (list_mode_) {
     // The list method has no tree: its chunks are classified by the inserts.
@@ -1020,7 +1030,7 @@ dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void*
   if (!out) return DD_EINVAL;
   *out = nullptr;
   if (chunk_size == 0 || (hash_kind != DD_HASH_MURMUR3 && hash_kind != DD_HASH_MD5)) return DD_EINVAL;
-  if (method != DD_METHOD_TREE && method != DD_METHOD_LIST) return DD_EINVAL;
+  if (method != DD_METHOD_TREE && method != DD_METHOD_LIST && method != DD_METHOD_TREE_LOW_ROOT) return DD_EINVAL;
   dd_dedup* h = new (std::nothrow) dd_dedup;
   if (!h) return DD_ENOMEM;
   h->st.method = method;

@@ -31,7 +31,7 @@ struct DevBuf {
 struct StreamState {
   uint32_t chunk_size = 0;
   int hash_kind = 0;
-  int method = DD_METHOD_TREE;  // DD_METHOD_TREE or DD_METHOD_LIST
+  int method = DD_METHOD_TREE;  // DD_METHOD_TREE, DD_METHOD_LIST or DD_METHOD_TREE_LOW_ROOT
   uint32_t options = 0;         // DD_OPT_* flags
   uint32_t current_id = 0, baseline_id = 0;
   bool have_tree = false;
@@ -43,7 +43,7 @@ struct StreamState {
   uint64_t data_len = 0, C = 0, N = 0;
   int64_t last_map_ckpt = -1;  // checkpoint id of the last insert into the map
 
-  DevBuf tree, labels, slot_of;
+  DevBuf tree, labels, slot_of, lr_new;
   DevBuf map;
   uint64_t map_cap = 0, map_count = 0;
   DevBuf blk, f_list, f_off, s_node, s_prev, s_tree, ss_node, ss_prev, s_hist, ptab;
@@ -124,6 +124,8 @@ class Pipeline {
   bool aligned16_ = true, vec16_ = true;
   int hash_kind_ = 0;
   bool list_mode_ = false;  // list-method jobs: no tree passes
+  bool low_root_ = false;   // some job uses the low-root rule
+  uint64_t max_c_ = 0;      // chunks of the largest job
   uint32_t kinds_ = 0;      // insert modes (JOBS_DENSE | JOBS_LIST_NARROW | JOBS_LIST_WIDE) in the table
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
 

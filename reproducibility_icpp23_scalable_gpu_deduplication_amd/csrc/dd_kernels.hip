@@ -834,6 +834,84 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Low-root first occurrences (TreeLowRootDeduplicator; dedup_low_root_ref,
+// include/reference_impl.hpp:265-472, restated as a deterministic rule).
+// After the leaf inserts, whose first occurrences follow the low-offset rule,
+// each new digest's first occurrence moves to the leaf whose highest new
+// ancestor has the smallest node id, ties to the lowest leaf.  New: a changed
+// leaf whose digest has no entry from an earlier checkpoint (PENDING, or
+// SHIFT with an entry of this checkpoint); an interior node whose children
+// are both new.  The selection key (top << 32 | leaf) is lowered with an
+// atomic in the slot's spare word.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool lr_skip(const JobG& J) {
+  return !J.low_root || (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL));
+}
+
+// Leaves: new flags; each group's low-offset first occurrence (the one
+// PENDING leaf of its digest) resets the group's key to ~0.
+__global__ __launch_bounds__(256) void k_lr_leaves(const JobG* __restrict__ jobs) {
+  const JobG& J = jobs[blockIdx.y];
+  if (lr_skip(J)) return;
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < J.C; k += (uint64_t)gridDim.x * 256) {
+    const uint64_t leaf = J.C - 1 + k;
+    const uint8_t lab = J.labels[leaf];
+    uint8_t nw = 0;
+    if (lab == PENDING) {
+      nw = 1;
+      *(GlobalPtr<uint64_t>)&J.map.slots[J.slot_of[leaf]].pad = ~0ull;
+    } else if (lab == SHIFT_DUPL) {
+      nw = (uint32_t)(J.map.slots[J.slot_of[leaf]].val >> 32) == J.ckpt_id ? 1 : 0;
+    }
+    J.lr_new[leaf] = nw;
+  }
+}
+
+// Interior nodes of level d (all of them, deepest level first).
+__global__ __launch_bounds__(256) void k_lr_level(const JobG* __restrict__ jobs, uint32_t d) {
+  const JobG& J = jobs[blockIdx.y];
+  if (lr_skip(J) || J.C < 2) return;
+  const uint64_t lo = (1ull << d) - 1;
+  const uint64_t hi = min((2ull << d) - 1, J.C - 1);  // interior nodes only
+  for (uint64_t v = lo + (uint64_t)blockIdx.x * 256 + threadIdx.x; v < hi; v += (uint64_t)gridDim.x * 256)
+    J.lr_new[v] = J.lr_new[2 * v + 1] & J.lr_new[2 * v + 2];
+}
+
+// Each new leaf lowers its group's key with (highest new ancestor, leaf).
+__global__ __launch_bounds__(256) void k_lr_key(const JobG* __restrict__ jobs) {
+  const JobG& J = jobs[blockIdx.y];
+  if (lr_skip(J)) return;
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < J.C; k += (uint64_t)gridDim.x * 256) {
+    const uint64_t leaf = J.C - 1 + k;
+    if (!J.lr_new[leaf]) continue;
+    uint64_t t = leaf;
+    while (t > 0 && J.lr_new[(t - 1) >> 1]) t = (t - 1) >> 1;
+    const unsigned long long key = ((unsigned long long)t << 32) | (uint32_t)leaf;
+    const GlobalPtr<unsigned long long> pad = (GlobalPtr<unsigned long long>)&J.map.slots[J.slot_of[leaf]].pad;
+    if (key < *pad) g_min(pad, key);  // a key already below cannot change
+  }
+}
+
+// The group's winner becomes its first occurrence (and the map entry's
+// node); the other new leaves of the group are shifted duplicates of it.
+__global__ __launch_bounds__(256) void k_lr_select(const JobG* __restrict__ jobs) {
+  const JobG& J = jobs[blockIdx.y];
+  if (lr_skip(J)) return;
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < J.C; k += (uint64_t)gridDim.x * 256) {
+    const uint64_t leaf = J.C - 1 + k;
+    if (!J.lr_new[leaf]) continue;
+    const GlobalPtr<Slot> sl = &J.map.slots[J.slot_of[leaf]];
+    const unsigned long long key = *(GlobalPtr<unsigned long long>)&sl->pad;
+    if ((uint32_t)key == (uint32_t)leaf) {
+      J.labels[leaf] = PENDING;
+      *(GlobalPtr<unsigned long long>)&sl->val = pack_node(J.ckpt_id, (uint32_t)leaf);
+    } else {
+      J.labels[leaf] = SHIFT_DUPL;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // List method (src/list_approach.cpp:23-62): a changed chunk is a first
 // occurrence when its insert created the entry or lowered it to this chunk
 // (the serial loop's first chunk of a digest), and a shifted duplicate
@@ -2526,6 +2604,20 @@ void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t
 void launch_list_resolve(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (grid) DD_LAUNCH(k_list_resolve, grid, 256, 0, s, jobs, prefix, njobs);
+}
+void launch_low_root(const JobDev* jobs_h, uint32_t njobs, uint64_t max_c, uint32_t max_depth, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
+  uint64_t bx = (max_c + 255) / 256;
+  if (bx > 1024) bx = 1024;
+  const dim3 leaf_grid((uint32_t)bx, njobs);
+  hipLaunchKernelGGL(k_lr_leaves, leaf_grid, dim3(256), 0, s, jobs);
+  for (int d = (int)max_depth - 1; d >= 0; d--) {
+    uint64_t n = 1ull << d, b = (n + 255) / 256;
+    if (b > 1024) b = 1024;
+    hipLaunchKernelGGL(k_lr_level, dim3((uint32_t)b, njobs), dim3(256), 0, s, jobs, (uint32_t)d);
+  }
+  hipLaunchKernelGGL(k_lr_key, leaf_grid, dim3(256), 0, s, jobs);
+  hipLaunchKernelGGL(k_lr_select, leaf_grid, dim3(256), 0, s, jobs);
 }
 void launch_pass1(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);

@@ -70,11 +70,14 @@ struct JobT {
   // earlier checkpoint (DD_OPT_EMIT_ROOT_SHIFT); 0: only as a first
   // occurrence, as the reference does (include/reference_impl.hpp:105-106).
   uint32_t root_shift;
-  uint32_t pad_opts;
+  // 1: low-root first occurrences (DD_METHOD_TREE_LOW_ROOT); lr_new holds N
+  // new-node flags of this checkpoint.
+  uint32_t low_root;
   // persistent state
   P<Digest> tree;        // N digests, heap order
   P<uint8_t> labels;     // N labels (padded for 16-byte reads)
   P<uint32_t> slot_of;   // N: map slot of pending leaves and shifted interior nodes
+  P<uint8_t> lr_new;     // N (low-root jobs only)
   MapT<P> map;
   uint64_t map_count_at_start;  // entries before this checkpoint
   uint64_t map_limit;           // max entries the map may hold
@@ -237,6 +240,10 @@ void launch_insert_leaves(const JobDev* jobs, const uint32_t* prefix, uint32_t n
 // List method: the leaves left PENDING by insert_leaves are first
 // occurrences; a baseline also sets every interior label to DONE.
 void launch_list_resolve(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
+// Low-root method: after the leaf inserts, moves each new digest's first
+// occurrence to the leaf whose highest new ancestor has the smallest node id
+// (jobs without low_root are skipped).  `max_c` / `max_depth`: over the jobs.
+void launch_low_root(const JobDev* jobs, uint32_t njobs, uint64_t max_c, uint32_t max_depth, hipStream_t s);
 void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
 void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
                             hipStream_t s);

@@ -19,18 +19,22 @@ namespace dd_drivers {
 
 // The reference's mode flags (src/utils.cpp:15-37).  Its drivers hand every
 // tree mode to TreeDeduplicator (src/dedup_files.cpp:61-63) and the list
-// mode to ListDeduplicator (:59-60), and so do these; the full and basic
-// methods are not part of this library.
-enum class Mode { Unknown, Tree, List, Other };
+// mode to ListDeduplicator (:59-60).  Here the low-root flags select the
+// low-root method (TreeLowRootDeduplicator, which labels with the low-root
+// rule); the other tree flags the tree method.  The full and basic methods
+// are not part of this library.
+enum class Mode { Unknown, Tree, TreeLowRoot, List, Other };
 
 inline Mode get_mode(int argc, char** argv) {
   static const char* tree_flags[] = {"--run-tree-chkpt", "--run-tree-low-offset-ref-chkpt",
-                                     "--run-tree-low-offset-chkpt", "--run-tree-low-root-ref-chkpt",
-                                     "--run-tree-low-root-chkpt"};
+                                     "--run-tree-low-offset-chkpt"};
+  static const char* low_root_flags[] = {"--run-tree-low-root-ref-chkpt", "--run-tree-low-root-chkpt"};
   static const char* other_flags[] = {"--run-full-chkpt", "--run-basic-chkpt"};
   for (int i = 0; i < argc; i++) {
     for (const char* f : tree_flags)
       if (!strcmp(argv[i], f)) return Mode::Tree;
+    for (const char* f : low_root_flags)
+      if (!strcmp(argv[i], f)) return Mode::TreeLowRoot;
     if (!strcmp(argv[i], "--run-list-chkpt")) return Mode::List;
     for (const char* f : other_flags)
       if (!strcmp(argv[i], f)) return Mode::Other;
@@ -39,7 +43,7 @@ inline Mode get_mode(int argc, char** argv) {
 }
 
 inline void print_mode_help() {
-  printf("Modes (every tree mode runs the tree approach with lowest-offset first occurrence):\n");
+  printf("Modes (the low-root modes pick low-root first occurrences, the other tree modes lowest offset):\n");
   printf("List checkpoint:                                   --run-list-chkpt\n");
   printf("Tree checkpoint:                                   --run-tree-chkpt\n");
   printf("Tree checkpoint lowest offset reference algorithm: --run-tree-low-offset-ref-chkpt\n");
@@ -53,6 +57,7 @@ inline void print_mode_help() {
 // (src/restart_chkpt_files.cpp:165,194).
 inline std::unique_ptr<dd::TreeDeduplicator> make_dedup(Mode m, uint32_t chunk_size) {
   if (m == Mode::List) return std::unique_ptr<dd::TreeDeduplicator>(new dd::ListDeduplicator(chunk_size));
+  if (m == Mode::TreeLowRoot) return std::unique_ptr<dd::TreeDeduplicator>(new dd::TreeLowRootDeduplicator(chunk_size));
   return std::unique_ptr<dd::TreeDeduplicator>(new dd::TreeDeduplicator(chunk_size));
 }
 inline const char* file_suffix(Mode m) { return m == Mode::List ? ".hashlist.incr_chkpt" : ".hashtree.incr_chkpt"; }

@@ -33,7 +33,7 @@ int main(int argc, char** argv) {
   const uint32_t num_tests = (uint32_t)strtoul(argv[3], nullptr, 10);
   const uint32_t chunk_size = (uint32_t)strtoul(argv[4], nullptr, 10);
   const Mode mode = get_mode(argc, argv);
-  if (mode != Mode::Tree && mode != Mode::List) {
+  if (mode != Mode::Tree && mode != Mode::TreeLowRoot && mode != Mode::List) {
     fprintf(stderr, "ERROR: %s mode\n", mode == Mode::Other ? "unsupported (not tree or list)" : "incorrect");
     print_mode_help();
     return 2;
