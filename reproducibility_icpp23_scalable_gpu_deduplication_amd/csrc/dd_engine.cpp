@@ -85,6 +85,9 @@ static MapView make_view(void* slots, uint64_t cap) {
 
 static uint64_t map_limit_of(uint64_t cap) { return cap / 2 + cap / 4; }
 
+// Inserts per job table above which every job probes in narrow rounds.
+constexpr uint64_t kWideInsertLimit = 256 * 1024;
+
 // Tree tiles: group g of a tree of depth D covers levels D-8g down to
 // max(D-8(g+1), 0); it has one tile per node of its top level.
 static int groups_of(uint32_t D) { return D == 0 ? 1 : (int)((D + kTileLevels - 1) / kTileLevels); }
@@ -234,6 +237,9 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   // narrow rounds keep more waves in flight (an MD5 field with a fifth of
   // its chunks changed: 7.8 ms per checkpoint instead of 8.1-20.8 ms).
   job.wide_probe = (!baseline && (last_pending == ~0ull || last_pending * 8 < Cn)) ? 1 : 0;
+  // DD_INSERT_ROUNDS=narrow|wide forces the width of incremental probe rounds.
+  static const char* rounds = std::getenv("DD_INSERT_ROUNDS");
+  if (!baseline && rounds) job.wide_probe = strcmp(rounds, "wide") == 0 ? 1 : 0;
   job.root_shift = (options & DD_OPT_EMIT_ROOT_SHIFT) ? 1 : 0;
   job.low_root = method == DD_METHOD_TREE_LOW_ROOT ? 1 : 0;
   job.lr_new = job.low_root ? lr_new.as<uint8_t>() : nullptr;
@@ -1272,6 +1278,19 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
                       out_caps[i], jobs[i], b->stream));
   }
   for (size_t i = 0; i < n; i++) b->hs[i]->st.ctr_armed = false;
+  // Probe-round width over the whole table: wide rounds suit a few inserts
+  // (a wavefront waits for its longest chain), narrow rounds many (bound by
+  // random-line traffic, more waves in flight).  The 64-buffer workload, 1.6M
+  // inserts per call, runs 3.6 % faster with narrow rounds for every job.
+  if (!std::getenv("DD_INSERT_ROUNDS")) {
+    uint64_t expect = 0;
+    for (size_t i = 0; i < n; i++) {
+      const StreamState& st = b->hs[i]->st;
+      if (!base[i] && st.last_pending != ~0ull) expect += st.last_pending;
+    }
+    if (expect > kWideInsertLimit)
+      for (JobDev& J : jobs) J.wide_probe = 0;
+  }
   DD_TRY(b->pipe.upload(jobs, b->stream));
   b->pipe.profile_level_ = b->profile;
   hipEvent_t* pev = b->profile ? b->pev : nullptr;

@@ -418,13 +418,18 @@ __device__ __forceinline__ uint32_t dma_slot(uint32_t i, uint32_t j, uint32_t p)
   return ((i & 1) ? 8 * (p ^ 1) : 8 * p) + ((j + p) & 7);
 }
 
-template <int Kind>
-__global__ __launch_bounds__(256) void k_leaf_hash_dma(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
-  __shared__ __attribute__((aligned(16))) uint8_t lds[4][8192];
+// WPG wavefronts per workgroup (4: 256 chunks; 1: 64 chunks).  A wavefront
+// has one 8 KiB round of loads in flight in LDS, so the LDS of a CU (160 KB)
+// bounds the bytes in flight; with long chunks (many rounds per wavefront,
+// few wavefronts) one-wavefront workgroups let 20 rather than 16 wavefronts
+// share a CU.
+template <int Kind, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_leaf_hash_dma(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  __shared__ __attribute__((aligned(16))) uint8_t lds[WPG][8192];
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint64_t k0 = ((uint64_t)(blockIdx.x - prefix[j]) * 4 + wave) * 64;  // first chunk of this wave
+  const uint32_t wave = WPG == 1 ? 0 : threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t k0 = ((uint64_t)(blockIdx.x - prefix[j]) * WPG + wave) * 64;  // first chunk of this wave
   const uint64_t k = k0 + lane;
   const uint64_t cs = J.chunk_size;
   const uint64_t full = J.nfull;  // chunks that are complete
@@ -2555,13 +2560,14 @@ void launch_begin(const JobDev* jobs_h, const uint32_t*, uint32_t njobs, bool re
 // With timing events the leaf kernel is launched with them attached to its
 // own dispatch (hipExtLaunchKernelGGL) instead of separate event records
 // around it, which cost the host tens of microseconds per checkpoint.
-#define DD_LEAF_LAUNCH(kern)                                                                           \
-  do {                                                                                                 \
-    if (ev_start) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, ev_start, ev_stop, 0, jobs, \
-                                        prefix, njobs);                                                \
-    else DD_LAUNCH(kern, grid, 256, 0, s, jobs, prefix, njobs);                                        \
-    return;                                                                                            \
+#define DD_LEAF_LAUNCH_T(kern, threads)                                                                    \
+  do {                                                                                                     \
+    if (ev_start) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, s, ev_start, ev_stop, 0, jobs, \
+                                        prefix, njobs);                                                    \
+    else DD_LAUNCH(kern, grid, threads, 0, s, jobs, prefix, njobs);                                        \
+    return;                                                                                                \
   } while (0)
+#define DD_LEAF_LAUNCH(kern) DD_LEAF_LAUNCH_T(kern, 256)
 
 void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping, bool al16,
                  int kind, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
@@ -2573,7 +2579,8 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
       case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>));
       case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>));
       case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>));
-      case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5>));
+      case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5, 4>));
+      case LEAF_DMA1: DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MD5, 1>), 64);
       default:
         if (al16) DD_LEAF_LAUNCH((k_leaf_hash<HASH_MD5, true, LEAF_DIRECT>));
         DD_LEAF_LAUNCH((k_leaf_hash<HASH_MD5, false, LEAF_DIRECT>));
@@ -2585,13 +2592,15 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
     case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>));
     case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>));
     case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>));
-    case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3>));
+    case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3, 4>));
+    case LEAF_DMA1: DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MURMUR3, 1>), 64);
     default:
       if (al16) DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DIRECT>));
       DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, LEAF_DIRECT>));
   }
 }
 #undef DD_LEAF_LAUNCH
+#undef DD_LEAF_LAUNCH_T
 
 void launch_insert_leaves(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
                           hipStream_t s) {

@@ -210,8 +210,9 @@ void launch_begin(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, bo
 // per chunk of at most 128 bytes, the whole chunk in flight (Murmur3); DMA:
 // each wavefront loads 128-byte segments of its 64 chunks straight into LDS
 // with contiguous loads (global_load_lds) and each lane hashes its chunk
-// from LDS (needs chunk_size % 128 == 0).  All but DIRECT need 16-byte
-// aligned buffers; one workgroup covers 256 chunks in every case.
+// from LDS (needs chunk_size % 128 == 0); DMA1: DMA with one-wavefront
+// workgroups.  All but DIRECT need 16-byte aligned buffers; one workgroup
+// covers 256 chunks (64 for DMA1).
 enum LeafMapping : int {
   LEAF_DIRECT = 0,
   LEAF_DEEP = 1,
@@ -220,16 +221,16 @@ enum LeafMapping : int {
   LEAF_LDS256 = 4,
   LEAF_SHORT = 5,
   LEAF_DMA = 6,
-  LEAF_NMAP = 7
+  LEAF_DMA1 = 7,
+  LEAF_NMAP = 8
 };
 // ev_start/ev_stop (optional) are recorded at the start and the end of the
 // leaf kernel's own dispatch.
 void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping,
                  bool all_aligned16, int hash_kind, hipStream_t s, hipEvent_t ev_start = nullptr,
                  hipEvent_t ev_stop = nullptr);
-// Workgroups of the leaf stage for a job of C chunks: one per 256 chunks in
-// every mapping.
-inline uint64_t leaf_workgroups(int, uint64_t C) { return (C + 255) / 256; }
+// Workgroups of the leaf stage for a job of C chunks.
+inline uint64_t leaf_workgroups(int mapping, uint64_t C) { return mapping == LEAF_DMA1 ? (C + 63) / 64 : (C + 255) / 256; }
 // Insert modes of the jobs in a table (bit mask): the insert stages launch
 // one kernel variant per mode present.  Dense: a baseline (every leaf and
 // interior node, narrow probe rounds); list, narrow or wide rounds: an

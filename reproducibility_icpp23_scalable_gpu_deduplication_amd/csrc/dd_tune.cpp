@@ -47,7 +47,7 @@ bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) 
       if (J.chunk_size > 128) return false;
     return true;
   }
-  const uint32_t seg = m == LEAF_LDS64 ? 64 : (m == LEAF_LDS128 || m == LEAF_DMA) ? 128 : 256;
+  const uint32_t seg = m == LEAF_LDS64 ? 64 : (m == LEAF_LDS128 || m == LEAF_DMA || m == LEAF_DMA1) ? 128 : 256;
   for (const JobDev& J : jobs)
     if (J.chunk_size % seg) return false;
   return true;
@@ -56,7 +56,7 @@ bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) 
 int forced_mapping() {
   const char* e = std::getenv("DD_LEAF_KERNEL");
   if (!e) return -1;
-  static const char* names[LEAF_NMAP] = {"direct", "deep", "lds64", "lds128", "lds256", "short", "dma"};
+  static const char* names[LEAF_NMAP] = {"direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1"};
   for (int m = 0; m < LEAF_NMAP; m++)
     if (!strcmp(e, names[m])) return m;
   return -1;

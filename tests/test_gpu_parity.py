@@ -254,7 +254,7 @@ def test_error_paths(torch_gpu):
     assert out.cpu().numpy().tobytes() == bytes(range(256)) * 4
 
 
-@pytest.mark.parametrize("mapping", ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma", None])
+@pytest.mark.parametrize("mapping", ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1", None])
 def test_leaf_mappings_match_oracle(torch_gpu, mapping, monkeypatch):
     # Every leaf kernel mapping (forced through DD_LEAF_KERNEL) and the tuned
     # choice (None) give the oracle's diffs; 256-byte chunks admit all but
@@ -267,7 +267,7 @@ def test_leaf_mappings_match_oracle(torch_gpu, mapping, monkeypatch):
     cases = ((256, dd.HASH_MURMUR3), (256, dd.HASH_MD5))
     if mapping in ("short", None):
         cases += ((128, dd.HASH_MURMUR3), (48, dd.HASH_MURMUR3), (16, dd.HASH_MURMUR3))
-    if mapping in ("dma", None):
+    if mapping in ("dma", "dma1", None):
         cases += ((128, dd.HASH_MURMUR3), (128, dd.HASH_MD5), (4096, dd.HASH_MURMUR3))
     for cs, kind in cases:
         datas = W.chain_uniform_changes(17 + cs, 3000 * cs + 7, cs, 4, fresh_frac=0.03, copy_frac=0.05,
@@ -275,7 +275,7 @@ def test_leaf_mappings_match_oracle(torch_gpu, mapping, monkeypatch):
         g, _ = check_chain(torch_gpu, cs, datas, kind, restart_ids=[0, 3])
         got = g.leaf_mapping()
         valid = (not (mapping in ("deep", "short") and kind == dd.HASH_MD5) and not (mapping == "short" and cs > 128)
-                 and not (mapping == "dma" and cs % 128))
+                 and not (mapping in ("dma", "dma1") and cs % 128))
         if mapping and valid:
             assert got == mapping
         else:
