@@ -318,6 +318,18 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
 // ---------------------------------------------------------------------------
 enum Stage { S_LEAF = 0, S_INSL, S_TILES, S_NODEBLK, S_STILE, S_GATHER, S_TILE0 };
 
+// Batch-wide workgroup budget of an incremental grid-stride stage of
+// 256-lane workgroups (DD_STAGE_BUDGET overrides it): 8192 is 32 per CU of
+// the 256 CUs, four rounds of the eight such workgroups a CU holds at once.
+static uint64_t stage_budget() {
+  static const uint64_t b = [] {
+    const char* e = std::getenv("DD_STAGE_BUDGET");
+    const long long v = e ? std::atoll(e) : 0;
+    return v > 0 ? (uint64_t)v : (uint64_t)8192;
+  }();
+  return b;
+}
+
 
 Pipeline::~Pipeline() {
   if (host_table_) (void)hipHostFree(host_table_);
@@ -364,12 +376,13 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   uint32_t* hprefix = reinterpret_cast<uint32_t*>(host_table_ + jobs_bytes);
   off_.assign(nstages, 0);
   grid_.assign(nstages, 0);
+  std::vector<uint64_t> nwg(nj);
+  const uint64_t budget = stage_budget();
   for (size_t st = 0; st < nstages; st++) {
     off_[st] = (uint32_t)(st * (nj + 1));
-    uint64_t acc = 0;
+    uint64_t incr = 0;  // workgroups of the incremental jobs
     for (size_t j = 0; j < nj; j++) {
       const JobDev& J = jobs[j];
-      hprefix[off_[st] + j] = (uint32_t)acc;
       uint64_t n = 0;
       switch (st) {
         case S_LEAF: n = leaf_workgroups(leaf_map_, J.C); break;
@@ -388,6 +401,21 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
           break;
         }
       }
+      nwg[j] = n;
+      if (!J.is_baseline && st != S_LEAF) incr += n;
+    }
+    // The incremental stages stride over work lists, so any workgroup count
+    // gives the same result; their per-job caps add up over a batch of many
+    // jobs to far more workgroups than the device holds, most of which only
+    // find that the lists are done.  Over a batch-wide budget (four times for
+    // the one-wavefront tile passes), each incremental job keeps its share in
+    // proportion, at least one workgroup.
+    const uint64_t cap = st >= S_TILE0 ? 4 * budget : budget;
+    uint64_t acc = 0;
+    for (size_t j = 0; j < nj; j++) {
+      uint64_t n = nwg[j];
+      if (incr > cap && !jobs[j].is_baseline && st != S_LEAF) n = std::max<uint64_t>(1, n * cap / incr);
+      hprefix[off_[st] + j] = (uint32_t)acc;
       acc += n;
     }
     if (acc > 0x7fffffffull) return DD_EINVAL;

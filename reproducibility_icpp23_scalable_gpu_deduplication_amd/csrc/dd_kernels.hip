@@ -1061,7 +1061,7 @@ __device__ __forceinline__ uint32_t group_items(const JobG& J, int g, uint32_t* 
 }
 
 // Both passes process a tile with one wavefront (a 64-lane workgroup) so that
-// many tiles are in flight.  Lane i owns bottom positions 4i .. 4i+3 of the
+// many tiles are in flight (six wavefronts per SIMD: at most 80 registers).  Lane i owns bottom positions 4i .. 4i+3 of the
 // tile, the two level-1 nodes above them and the level-2 node above those,
 // so the first two levels are computed in registers; the levels above (at
 // most 63 nodes) go through LDS, one barrier per level.  Every global load
@@ -1155,6 +1155,32 @@ __device__ __forceinline__ void tile_digests(GlobalPtr<const Digest> tree, const
     if (ul[k] != want) ud[k] = Digest{0, 0};
 }
 
+// Tiles of one workgroup of a tile pass: iterations n = 0 .. niter-1 take
+// item wg + n * nwg of the group.  The tile ids of 64 iterations are loaded
+// together into LDS (one per lane of the one-wavefront workgroup), so a
+// tile's id is an LDS read rather than a global load that would wait behind
+// the previous tile's stores.
+__device__ __forceinline__ uint32_t tile_item(const JobG& J, bool listed, const uint32_t* pre, uint32_t it) {
+  return listed ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
+}
+
+__device__ __forceinline__ uint32_t tile_iterations(uint32_t wg, uint32_t nwg, uint32_t nitems) {
+  return wg < nitems ? (nitems - wg + nwg - 1) / nwg : 0u;
+}
+
+// Tile id of iteration n (every lane calls it with the same n, iterations in
+// increasing order from 0): refills `ids` when n starts a block of 64.
+__device__ __forceinline__ uint64_t tile_at(const JobG& J, bool listed, const uint32_t* pre, uint32_t* ids,
+                                            uint32_t wg, uint32_t nwg, uint32_t niter, uint32_t n) {
+  if ((n & 63) == 0) {
+    __syncthreads();  // the previous block's ids are read
+    const uint32_t m = n + threadIdx.x;
+    if (m < niter) ids[threadIdx.x] = tile_item(J, listed, pre, wg + m * nwg);
+    __syncthreads();
+  }
+  return ids[n & 63];
+}
+
 // Pass 1, the first-occurrence forest (include/reference_impl.hpp:73-82).
 // Group 0 visits the dirty tiles only; the groups above visit every tile.
 // In an incremental checkpoint the first-occurrence interior nodes of a tile
@@ -1181,8 +1207,11 @@ __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__
   const GlobalPtr<uint32_t> fint_list = J.fint_list;
   const GlobalPtr<uint32_t> fint_n = J.fint_n;
   const uint32_t tbase = J.tile_base[g];
-  for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint64_t t = (g == 0 && listing) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
+  const bool listed = g == 0 && listing;
+  __shared__ uint32_t ids[64];
+  const uint32_t niter = tile_iterations(wg, nwg, nitems);
+  for (uint32_t n = 0; n < niter; n++) {
+    const uint64_t t = tile_at(J, listed, pre, ids, wg, nwg, niter, n);
     const TileLane T = tile_lane(tile_ref(J, g, t), N, i);
     // Loads: labels of the bottom nodes and of the level-1 leaves ...
     uint8_t raw[4], rawu[2], bl[4], ul[2];
@@ -1348,7 +1377,7 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
   }
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_pass2(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pass2(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
                                                         int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
@@ -1367,9 +1396,11 @@ __global__ __launch_bounds__(kTileThreads) void k_pass2(const JobG* __restrict__
   const GlobalPtr<uint32_t> slot_of = J.slot_of;
   const int kind = J.hash_kind;
   const MapG map = J.map;
-  for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint64_t t = (g == 0 && !J.is_baseline) ? list_item(J.dirty_list, J.dirty_cap, pre, it) : it;
-    const TileLane T = tile_lane(tile_ref(J, g, t), N, i);
+  const bool listed = g == 0 && !J.is_baseline;
+  __shared__ uint32_t ids[64];
+  const uint32_t niter = tile_iterations(wg, nwg, nitems);
+  for (uint32_t n = 0; n < niter; n++) {
+    const TileLane T = tile_lane(tile_ref(J, g, tile_at(J, listed, pre, ids, wg, nwg, niter, n)), N, i);
     // Loads: labels (leaves were resolved by pass 1), then the digests of
     // the shifted ones.
     uint8_t bl[4], ul[2];
@@ -1709,21 +1740,33 @@ __device__ __forceinline__ void reset_visited_tiles(const JobG& J, uint32_t wg, 
   const uint32_t ntiles0 = 1u << Lr0;
   const uint32_t H = J.D - Lr0;
   __shared__ uint32_t pre[kPendSpread + 1];
+  __shared__ uint32_t tids[256];
   uint32_t nitems = ntiles0;
   if (!J.is_baseline) {
     load_list_prefix(J.dirty_cnt, pre);
     nitems = pre[kPendSpread];
   }
-  for (uint32_t it = wg; it < nitems; it += nwg) {
-    const uint32_t t = J.is_baseline ? it : list_item(J.dirty_list, J.dirty_cap, pre, it);
-    const uint64_t r = ((uint64_t)1 << Lr0) - 1 + t;
-    if (threadIdx.x + 1 < (1u << H)) {  // levels above the bottom (256 threads, at most 255 positions)
-      const uint64_t x = tile_node(r, threadIdx.x);
-      if (x < J.N) J.labels[x] = FIXED_DUPL;
+  // The tile ids of up to 256 iterations are loaded together, then their
+  // labels stored: a list load between stores would wait for them.
+  const uint32_t niter = wg < nitems ? (nitems - wg + nwg - 1) / nwg : 0;
+  for (uint32_t n0 = 0; n0 < niter; n0 += 256) {
+    const uint32_t nb = min(256u, niter - n0);
+    __syncthreads();  // the previous block's ids are read
+    if (threadIdx.x < nb) {
+      const uint32_t it = wg + (n0 + threadIdx.x) * nwg;
+      tids[threadIdx.x] = J.is_baseline ? it : list_item(J.dirty_list, J.dirty_cap, pre, it);
     }
-    if (threadIdx.x < (1u << H)) {  // the bottom level
-      const uint64_t x = ((r + 1) << H) - 1 + threadIdx.x;
-      if (x < J.N) J.labels[x] = FIXED_DUPL;
+    __syncthreads();
+    for (uint32_t b = 0; b < nb; b++) {
+      const uint64_t r = ((uint64_t)1 << Lr0) - 1 + tids[b];
+      if (threadIdx.x + 1 < (1u << H)) {  // levels above the bottom (256 threads, at most 255 positions)
+        const uint64_t x = tile_node(r, threadIdx.x);
+        if (x < J.N) J.labels[x] = FIXED_DUPL;
+      }
+      if (threadIdx.x < (1u << H)) {  // the bottom level
+        const uint64_t x = ((r + 1) << H) - 1 + threadIdx.x;
+        if (x < J.N) J.labels[x] = FIXED_DUPL;
+      }
     }
   }
 }
@@ -1994,31 +2037,31 @@ __device__ __forceinline__ uint32_t region_of(Ptr f_off, uint32_t F, uint64_t q)
 
 typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
-// Copies `n` bytes of chunk data from `src` (16-byte aligned when Vec) to the
-// 4-byte aligned `dst`, zero-filling the bytes at and past `avail`.
+// One piece of chunk data at `src` (16 bytes, 16-byte aligned, when Vec;
+// else one byte, in .x), zero at and past `avail` bytes.
 template <bool Vec>
-__device__ __forceinline__ void copy_piece(GlobalPtr<uint8_t> dst, GlobalPtr<const uint8_t> src, uint64_t avail) {
-  if (Vec) {
-    if (avail >= 16) {
-      const uint4 v = gload16((GlobalPtr<const void>)src);
-      *(GlobalPtr<u32x4a4>)dst = u32x4a4{v.x, v.y, v.z, v.w};
-      return;
-    }
-    uint32_t w[4];
+__device__ __forceinline__ uint4 load_piece(GlobalPtr<const uint8_t> src, uint64_t avail) {
+  if (!Vec) return make_uint4(avail > 0 ? *src : 0u, 0, 0, 0);
+  if (avail >= 16) return gload16((GlobalPtr<const void>)src);
+  uint32_t w[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      uint32_t x = 0;
+  for (int k = 0; k < 4; k++) {
+    uint32_t x = 0;
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
-        const uint64_t o = 4 * k + b;
-        if (o < avail) x |= (uint32_t)src[o] << (8 * b);
-      }
-      w[k] = x;
+    for (int b = 0; b < 4; b++) {
+      const uint64_t o = 4 * k + b;
+      if (o < avail) x |= (uint32_t)src[o] << (8 * b);
     }
-    *(GlobalPtr<u32x4a4>)dst = u32x4a4{w[0], w[1], w[2], w[3]};
-  } else {
-    *dst = avail > 0 ? *src : 0;
+    w[k] = x;
   }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Stores a piece to the 4-byte aligned `dst`.
+template <bool Vec>
+__device__ __forceinline__ void store_piece(GlobalPtr<uint8_t> dst, const uint4& v) {
+  if (Vec) *(GlobalPtr<u32x4a4>)dst = u32x4a4{v.x, v.y, v.z, v.w};
+  else *dst = (uint8_t)v.x;
 }
 
 // The data section: groups of 256 chunks per workgroup iteration.
@@ -2046,12 +2089,26 @@ __device__ __forceinline__ void gather_data(const JobG& J, uint32_t F, uint32_t 
     __syncthreads();
     const uint64_t nq = min((uint64_t)256, Dq - q0);
     const uint64_t total = nq * ppc;
-    for (uint64_t w = threadIdx.x; w < total; w += 256) {
-      const uint64_t i = pow2 ? (w >> sh) : (w / ppc);
-      const uint64_t pi = w - i * ppc;
-      const uint64_t so = src[i] * cs + pi * piece;
-      const uint64_t avail = so < J.len ? J.len - so : 0;
-      copy_piece<Vec>(dst_base + (q0 + i) * cs + pi * piece, J.data + so, avail);
+    // Four pieces per lane are loaded before any is stored: a load issued
+    // after a store waits for it.
+    constexpr int U = 4;
+    for (uint64_t w0 = threadIdx.x; w0 < total; w0 += 256 * U) {
+      uint4 v[U];
+      uint64_t doff[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint64_t w = w0 + 256 * u;
+        const uint64_t i = pow2 ? (w >> sh) : (w / ppc);
+        const uint64_t pi = w - i * ppc;
+        doff[u] = (q0 + i) * cs + pi * piece;
+        if (w < total) {
+          const uint64_t so = src[i] * cs + pi * piece;
+          v[u] = load_piece<Vec>(J.data + so, so < J.len ? J.len - so : 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (w0 + 256 * u < total) store_piece<Vec>(dst_base + doff[u], v[u]);
     }
     __syncthreads();
   }

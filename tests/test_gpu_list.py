@@ -33,7 +33,7 @@ def check_list_chain(torch, chunk, datas, kind=dd.HASH_MURMUR3, baselines=(0,), 
 def test_method_and_hand_derived_chain(torch_gpu):
     g, gd = check_list_chain(torch_gpu, 1, [b"ABCA", b"ABDB", b"DDDD"])
     assert g.method == dd.METHOD_LIST and dd.TreeDeduplicator(1).method == dd.METHOD_TREE
-    assert dd.TreeLowRootDeduplicator(1).method == dd.METHOD_TREE
+    assert dd.TreeLowRootDeduplicator(1).method == dd.METHOD_TREE_LOW_ROOT
     assert gd[0] == diff_bytes(0, 0, 4, 1, [0, 1, 2], [(0, 1)], [(3, 0)], b"ABC")
     assert gd[2] == diff_bytes(0, 2, 4, 1, [], [(1, 3)], [(0, 2), (1, 2), (3, 2)], b"")
 
