@@ -4,16 +4,20 @@
 // the checkpoint's bandwidth: one lane per chunk with 64 or 256 bytes of loads
 // in flight, or wavefront-cooperative loads staged through LDS in 64/128/256
 // byte segments (LeafMapping, dd_kernels.hpp).  Which is fastest depends on
-// the chunk size and the hash (MI355X, 1 GiB: lane-per-chunk 4.9 TB/s at
-// 128 B, LDS-staged 2.9 TB/s at 4096 B).  The first upload of a job table with
-// a new (device, hash, alignment, chunk sizes) key times every valid mapping
-// on the largest job's first 256 MiB and keeps the fastest for the process.
-// The timed launches run as baselines on scratch tree, label and list arrays,
-// so the handles' state is not touched.  The mappings are timed round-robin
+// the chunk size, the hash and the number of jobs in the table (MI355X:
+// lane-per-chunk 5.9 TB/s for one 1 GiB buffer at 128 B, direct-to-LDS 5.6
+// TB/s over a batch of 64 buffers).  The first upload of a job table with a
+// new (device, hash, alignment, chunk sizes, one job or several) key times
+// every valid mapping on a sample of the table (every job, at most 64, over
+// the first bytes of its buffer, at most 4 GiB in all) and keeps the fastest
+// for the process.  The timed launches are incremental checkpoints of
+// unchanged data (after one baseline launch) on scratch tree, label and list
+// arrays, so the handles' state is not touched.  The mappings are timed round-robin
 // (each round launches every mapping once) and each keeps its median, so a
 // drift of the device's clock or of other work affects all alike.  One
-// thread tunes a key; callers that need the same key meanwhile wait for it.  DD_LEAF_KERNEL=direct|deep|lds64|
-// lds128|lds256|short|dma forces a mapping (when it is valid for the table).
+// thread tunes a key; callers that need the same key meanwhile wait for it.
+// DD_LEAF_KERNEL=direct|deep|lds64|lds128|lds256|short|dma|dma1 forces a
+// mapping (when it is valid for the table).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,7 +34,7 @@
 namespace dd {
 namespace {
 
-using Key = std::tuple<int, int, int, uint32_t, uint32_t>;  // device, hash, aligned16, min chunk, max chunk
+using Key = std::tuple<int, int, int, uint32_t, uint32_t, int>;  // device, hash, aligned16, min/max chunk, batch
 
 std::mutex g_mu;
 std::condition_variable g_cv;
@@ -64,11 +68,11 @@ int forced_mapping() {
 
 // Time of one launch of a mapping on the scratch job, in milliseconds;
 // negative on a HIP error.
-double time_mapping(const JobDev* dj, const uint32_t* dprefix, uint32_t grid, int m, bool al16, int kind, hipStream_t s,
-                    hipEvent_t e0, hipEvent_t e1) {
-  launch_begin(dj, nullptr, 1, false, s);
+double time_mapping(const JobDev* dj, const uint32_t* dprefix, uint32_t njobs, uint32_t grid, int m, bool al16, int kind,
+                    hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  launch_begin(dj, nullptr, njobs, false, s);
   if (hipEventRecord(e0, s) != hipSuccess) return -1;
-  launch_leaf(dj, dprefix, 1, grid, m, al16, kind, s);
+  launch_leaf(dj, dprefix, njobs, grid, m, al16, kind, s);
   if (hipEventRecord(e1, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return -1;
   float ms = 0;
   if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1;
@@ -82,15 +86,13 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   if (f >= 0 && mapping_valid(f, al16, kind, jobs)) return f;
   if (!al16) return LEAF_DIRECT;
   uint32_t cmin = ~0u, cmax = 0;
-  const JobDev* R = &jobs[0];
   for (const JobDev& J : jobs) {
     cmin = std::min(cmin, J.chunk_size);
     cmax = std::max(cmax, J.chunk_size);
-    if (J.C > R->C) R = &J;
   }
   int dev = 0;
   (void)hipGetDevice(&dev);
-  const Key key{dev, kind, al16 ? 1 : 0, cmin, cmax};
+  const Key key{dev, kind, al16 ? 1 : 0, cmin, cmax, jobs.size() > 1 ? 1 : 0};
   {
     std::unique_lock<std::mutex> lk(g_mu);
     g_cv.wait(lk, [&] {
@@ -110,59 +112,114 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
     g_cv.notify_all();
     return m;
   };
-  // Scratch baseline job over the first (at most) 256 MiB of the largest job.
-  const uint64_t cs = R->chunk_size;
-  uint64_t tl = std::min<uint64_t>(R->len, 256ull << 20);
-  if (tl >= cs) tl = tl / cs * cs;
-  const uint64_t C = (tl + cs - 1) / cs, N = 2 * C - 1;
-  const uint64_t pend_cap = ((C + 63) / 64 + 31) / 32 * 64;  // unused: a baseline keeps no pending list
+  // Scratch jobs: every job of the table (at most kSampleJobs, the largest)
+  // over the first bytes of its buffer, scaled so that the sample holds at
+  // most kSampleBytes, each at least one chunk.  A table of many small jobs
+  // and one of a single large job favour different mappings.
+  constexpr uint64_t kSampleBytes = 4ull << 30;
+  constexpr size_t kSampleJobs = 64;
+  std::vector<const JobDev*> src;
+  for (const JobDev& J : jobs) src.push_back(&J);
+  std::sort(src.begin(), src.end(), [](const JobDev* a, const JobDev* b) { return a->len > b->len; });
+  if (src.size() > kSampleJobs) src.resize(kSampleJobs);
+  uint64_t stotal = 0;
+  for (const JobDev* J : src) stotal += J->len;
+  const double scale = stotal > kSampleBytes ? (double)kSampleBytes / (double)stotal : 1.0;
+  const size_t nj = src.size();
+  std::vector<JobDev> sj(nj);
+  std::vector<uint64_t> tree_off(nj), lab_off(nj), list_off(nj);
+  uint64_t tree_bytes = 0, lab_bytes = 0, list_bytes = 0;
+  for (size_t j = 0; j < nj; j++) {
+    const JobDev& R = *src[j];
+    const uint64_t cs = R.chunk_size;
+    uint64_t tl = R.len;
+    if (scale < 1.0) tl = std::max<uint64_t>(std::min<uint64_t>(cs, R.len), (uint64_t)((double)R.len * scale) / cs * cs);
+    const uint64_t C = (tl + cs - 1) / cs, N = 2 * C - 1;
+    JobDev& t = sj[j];
+    memset(&t, 0, sizeof t);
+    t.data = R.data;
+    t.len = tl;
+    t.chunk_size = (uint32_t)cs;
+    t.hash_kind = kind;
+    t.C = C;
+    t.N = N;
+    t.nfull = tl / cs;
+    t.D = make_geom(C).D;
+    t.is_baseline = 1;
+    t.write_all_labels = 1;
+    t.epoch = 1;
+    // List mode: a leaf that a timed launch finds changed (none, as the data
+    // does not change) would be listed without marking tiles.
+    t.list_mode = 1;
+    t.pend_cap = (uint32_t)(((C + 63) / 64 + 31) / 32 * 64);
+    tree_off[j] = tree_bytes;
+    lab_off[j] = lab_bytes;
+    list_off[j] = list_bytes;
+    tree_bytes += 16 * N;
+    lab_bytes += (N + 64 + 255) / 256 * 256;
+    list_bytes += 32ull * t.pend_cap * 4;
+  }
+  // Counters: one block per job (JobCounters, then 2 x 32 list counts).
+  const size_t ctr_stride = (sizeof(JobCounters) + 2 * 32 * kCntStride * 4 + 255) / 256 * 256;
   DevBuf tree, labels, ctr, plist, table;
   int best = LEAF_DIRECT;
-  if (tree.ensure(16 * N) != DD_OK || labels.ensure(N + 64) != DD_OK ||
-      ctr.ensure(sizeof(JobCounters) + 2 * 32 * kCntStride * 4) != DD_OK || plist.ensure(32 * pend_cap * 4) != DD_OK ||
-      table.ensure(sizeof(JobDev) + LEAF_NMAP * 2 * 4) != DD_OK)
+  const size_t jobs_bytes = (nj * sizeof(JobDev) + 255) / 256 * 256;
+  const size_t table_bytes = jobs_bytes + LEAF_NMAP * (nj + 1) * 4;
+  if (tree.ensure(tree_bytes) != DD_OK || labels.ensure(lab_bytes) != DD_OK || ctr.ensure(nj * ctr_stride) != DD_OK ||
+      plist.ensure(list_bytes) != DD_OK || table.ensure(table_bytes) != DD_OK)
     return publish(best, false);  // no memory to tune: keep the default, do not cache
-  JobDev t;
-  memset(&t, 0, sizeof t);
-  t.data = R->data;
-  t.len = tl;
-  t.chunk_size = (uint32_t)cs;
-  t.hash_kind = kind;
-  t.C = C;
-  t.N = N;
-  t.nfull = tl / cs;
-  t.D = make_geom(C).D;
-  t.is_baseline = 1;
-  t.write_all_labels = 1;
-  t.epoch = 1;
-  t.tree = tree.as<Digest>();
-  t.labels = labels.as<uint8_t>();
-  t.ctr = ctr.as<JobCounters>();
-  t.pend_cnt = reinterpret_cast<uint32_t*>(ctr.as<uint8_t>() + sizeof(JobCounters));
-  t.dirty_cnt = t.pend_cnt + 32 * kCntStride;
-  t.pend_cnt_next = t.pend_cnt;  // a baseline keeps no lists
-  t.dirty_cnt_next = t.dirty_cnt;
-  t.pend_list = plist.as<uint32_t>();
-  t.pend_cap = (uint32_t)pend_cap;
-  // One workgroup prefix {0, grid} per mapping: the grid-stride mappings
-  // read their stride from it.
-  uint8_t host[sizeof(JobDev) + LEAF_NMAP * 8];
-  memcpy(host, &t, sizeof t);
-  uint32_t grids[LEAF_NMAP];
-  for (int m = 0; m < LEAF_NMAP; m++) {
-    grids[m] = (uint32_t)leaf_workgroups(m, C);
-    const uint32_t pre[2] = {0, grids[m]};
-    memcpy(host + sizeof(JobDev) + 8 * m, pre, sizeof pre);
+  std::vector<uint8_t> host(table_bytes, 0);
+  std::vector<uint32_t> grids(LEAF_NMAP);
+  for (size_t j = 0; j < nj; j++) {
+    JobDev& t = sj[j];
+    t.tree = reinterpret_cast<Digest*>(tree.as<uint8_t>() + tree_off[j]);
+    t.labels = labels.as<uint8_t>() + lab_off[j];
+    uint8_t* cb = ctr.as<uint8_t>() + j * ctr_stride;
+    t.ctr = reinterpret_cast<JobCounters*>(cb);
+    t.pend_cnt = reinterpret_cast<uint32_t*>(cb + sizeof(JobCounters));
+    t.dirty_cnt = t.pend_cnt + 32 * kCntStride;
+    t.pend_cnt_next = t.pend_cnt;
+    t.dirty_cnt_next = t.dirty_cnt;
+    t.pend_list = reinterpret_cast<uint32_t*>(plist.as<uint8_t>() + list_off[j]);
   }
+  // One workgroup prefix over the jobs per mapping.
+  for (int m = 0; m < LEAF_NMAP; m++) {
+    uint32_t* pre = reinterpret_cast<uint32_t*>(host.data() + jobs_bytes) + m * (nj + 1);
+    uint64_t acc = 0;
+    for (size_t j = 0; j < nj; j++) {
+      pre[j] = (uint32_t)acc;
+      acc += leaf_workgroups(m, sj[j].C);
+    }
+    pre[nj] = (uint32_t)acc;
+    grids[m] = (uint32_t)acc;
+  }
+  auto upload = [&]() {
+    memcpy(host.data(), sj.data(), nj * sizeof(JobDev));
+    return hipMemcpyAsync(table.p, host.data(), table_bytes, hipMemcpyHostToDevice, s) == hipSuccess &&
+           hipStreamSynchronize(s) == hipSuccess;
+  };
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (hipMemcpyAsync(table.p, host, sizeof host, hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess || hipEventCreateWithFlags(&e0, hipEventDisableSystemFence) != hipSuccess ||
+  if (!upload() || hipEventCreateWithFlags(&e0, hipEventDisableSystemFence) != hipSuccess ||
       hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
     return publish(best, false);
   }
   const JobDev* dj = table.as<JobDev>();
-  const uint32_t* dprefix = reinterpret_cast<const uint32_t*>(table.as<uint8_t>() + sizeof(JobDev));
+  const uint32_t* dprefix = reinterpret_cast<const uint32_t*>(table.as<uint8_t>() + jobs_bytes);
+  // One baseline launch stores every leaf digest; the timed launches are then
+  // incremental checkpoints of unchanged data (read the chunks and the
+  // previous digests, store nothing), the common case of a checkpoint chain.
+  launch_begin(dj, nullptr, (uint32_t)nj, false, s);
+  launch_leaf(dj, dprefix + LEAF_DIRECT * (nj + 1), (uint32_t)nj, grids[LEAF_DIRECT], LEAF_DIRECT, al16, kind, s);
+  for (JobDev& t : sj) {
+    t.is_baseline = 0;
+    t.write_all_labels = 0;
+  }
+  if (!upload()) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return publish(best, false);
+  }
   // One untimed launch per mapping, then kRounds rounds of one timed launch
   // each; the fastest median wins.
   constexpr int kRounds = 7;
@@ -171,7 +228,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   for (int r = 0; r <= kRounds && ok; r++) {
     for (int m = 0; m < LEAF_NMAP && ok; m++) {
       if (!mapping_valid(m, al16, kind, jobs)) continue;
-      const double ms = time_mapping(dj, dprefix + 2 * m, grids[m], m, al16, kind, s, e0, e1);
+      const double ms = time_mapping(dj, dprefix + m * (nj + 1), (uint32_t)nj, grids[m], m, al16, kind, s, e0, e1);
       if (ms < 0) ok = false;
       else if (r > 0) times[m].push_back(ms);
     }
