@@ -458,9 +458,11 @@ def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, 
     `versions` checkpoints.  Arrivals are Poisson at `rate` req/s for
     `seconds` (85 % checkpoints, 15 % restarts of a random checkpoint of the
     rank's current chain); a second, saturated pass submits a fixed number of
-    requests at once.  Every restart is compared byte for byte with the data
-    it restores.  Latency = completion - submission (steady clock of the
-    library)."""
+    requests at once, and a third does the same with batching off (every
+    request alone on its handle's stream).  Checkpoints admitted together run
+    as one batched job table (at most 64).  Every restart is compared byte for
+    byte with the data it restores.  Latency = completion - submission (steady
+    clock of the library)."""
     import queue
     import threading
 
@@ -508,7 +510,8 @@ def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, 
     def run_pass(arrivals):
         """arrivals: list of (time offset s or None, rank, is_restart)."""
         done = queue.Queue()
-        stats = dict(lat_ck=[], lat_rs=[], wait=[], ck_bytes=0, rs_bytes=0, bad=0, errors=0, t0=None, t1=0.0)
+        stats = dict(lat_ck=[], lat_rs=[], wait=[], groups=[], ck_bytes=0, rs_bytes=0, bad=0, errors=0, t0=None,
+                     t1=0.0, error_status=[])
 
         def collector(nreq):
             for _ in range(nreq):
@@ -516,6 +519,7 @@ def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, 
                 res = srv.wait(item[0])
                 if res["status"] != dd.DD_OK:
                     stats["errors"] += 1
+                    stats["error_status"].append(dd.strerror(res["status"]))
                     continue
                 lat = res["t_end"] - res["t_submit"]
                 stats["wait"].append(res["t_start"] - res["t_submit"])
@@ -523,6 +527,7 @@ def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, 
                 if item[1] is None:
                     stats["lat_ck"].append(lat)
                     stats["ck_bytes"] += item[2]
+                    stats["groups"].append(res["group"])
                 else:
                     r, j, o = item[1], item[2], item[3]
                     v = j % versions
@@ -559,7 +564,9 @@ def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, 
             restart_p50_ms=_pct(stats["lat_rs"], 50), restart_p99_ms=_pct(stats["lat_rs"], 99),
             queue_wait_p50_ms=_pct(stats["wait"], 50), queue_wait_p99_ms=_pct(stats["wait"], 99),
             restarts_verified=len(stats["lat_rs"]) - stats["bad"], restart_mismatches=stats["bad"],
-            errors=stats["errors"])
+            checkpoint_group_mean=round(float(np.mean(stats["groups"])), 2) if stats["groups"] else None,
+            checkpoint_group_max=max(stats["groups"]) if stats["groups"] else None,
+            errors=stats["errors"], error_status=sorted(set(stats["error_status"])))
 
     n = int(rate * seconds)
     gaps = rng.exponential(1.0 / rate, n)
@@ -572,6 +579,11 @@ def run_serving(torch, dd, W, ranks, rate, seconds, budget, slots, chunk, kind, 
     sat = [(None, int(rng.integers(0, ranks)), bool(rng.random() < 0.15)) for _ in range(nsat)]
     out["saturated"] = run_pass(sat)
     log({"serving_saturated": out["saturated"]})
+    # The same saturated load with every request alone on its handle's stream.
+    srv.set_batching(1)
+    sat = [(None, int(rng.integers(0, ranks)), bool(rng.random() < 0.15)) for _ in range(nsat)]
+    out["saturated_unbatched"] = run_pass(sat)
+    log({"serving_saturated_unbatched": out["saturated_unbatched"]})
     in_use, peak = srv.usage()
     out["hbm_reserved_peak_bytes"] = peak
     srv.close()

@@ -216,8 +216,9 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
  * budget; a request that does not fit waits for completions and holds back
  * the requests behind it.  With nothing in flight the first eligible request
  * always starts, so a request larger than the budget runs alone.  Admitted
- * requests run concurrently, each on its handle's HIP stream, so their
- * kernels overlap on the device.  Results are the same bytes
+ * work runs concurrently on the execution slots: checkpoint requests admitted
+ * together run as one batched job table (dd_server_set_batching), a restart
+ * or a lone checkpoint on its handle's HIP stream.  Results are the same bytes
  * dd_checkpoint_device / dd_restart_device return.  While a handle has a
  * request queued or in flight, the caller must not use it directly. */
 typedef struct dd_server dd_server;
@@ -228,6 +229,8 @@ typedef struct dd_request_result {
   double t_submit;     /* seconds, steady clock */
   double t_start;      /* admitted and started */
   double t_end;        /* finished */
+  uint32_t group;      /* checkpoint requests run in one batched job table with it (1: alone) */
+  uint32_t pad;
 } dd_request_result;
 /* `footprint` of each request: its input and output buffers plus, on a
  * handle's first request, the handle's state estimate. */
@@ -244,6 +247,12 @@ dd_status dd_server_submit_restart(dd_server* s, dd_dedup* h, const void* const*
  * the device bytes it holds (state, scratch and restart buffers); a handle's
  * first checkpoint is admitted with an estimate of that state. */
 dd_status dd_server_wait(dd_server* s, uint64_t ticket, dd_request_result* out);
+/* Checkpoint requests admitted together into one batched job table (no
+ * counterpart in the reference, whose ranks each run their own checkpoint):
+ * a slot that admits a checkpoint also admits the queued checkpoints of other
+ * idle handles of the same device, hash and method, in queue order while they
+ * fit the budget, at most max_group (default 64; 1 runs every request alone). */
+dd_status dd_server_set_batching(dd_server* s, uint32_t max_group);
 /* Bytes currently reserved against the budget, and the peak so far. */
 dd_status dd_server_usage(dd_server* s, uint64_t* in_use, uint64_t* peak);
 double dd_now(void); /* the steady clock of dd_request_result */

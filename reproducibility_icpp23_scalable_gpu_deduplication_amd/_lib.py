@@ -28,7 +28,8 @@ EXPORTED = [
     "dd_max_diff_size", "dd_counts", "dd_get_ids", "dd_set_current_id", "dd_get_timers",
     "dd_restart", "dd_restart_device", "dd_restart_device_noalloc", "dd_batch_create", "dd_batch_destroy", "dd_batch_use_graph",
     "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
-    "dd_server_submit_restart", "dd_server_wait", "dd_server_usage", "dd_now", "dd_set_profiling",
+    "dd_server_submit_restart", "dd_server_wait", "dd_server_set_batching", "dd_server_usage", "dd_now",
+    "dd_set_profiling",
     "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_batch_set_profiling",
     "dd_batch_get_stage_times", "dd_get_ckpt_stats", "dd_get_device_bytes", "dd_buf_free", "dd_strerror",
 ]
@@ -60,7 +61,8 @@ class dd_ckpt_stats(ctypes.Structure):
 
 class dd_request_result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("kind", ctypes.c_uint32), ("out_len", ctypes.c_uint64),
-                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double)]
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("group", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 _lib = None
@@ -103,6 +105,7 @@ def load():
         "dd_server_submit_checkpoint": ([vp, vp, vp, u64, i32, vp, u64, P(u64)], i32),
         "dd_server_submit_restart": ([vp, vp, P(vp), P(u64), u32, u32, vp, u64, P(u64)], i32),
         "dd_server_wait": ([vp, u64, P(dd_request_result)], i32),
+        "dd_server_set_batching": ([vp, u32], i32),
         "dd_server_usage": ([vp, P(u64), P(u64)], i32),
         "dd_now": ([], ctypes.c_double),
         "dd_set_profiling": ([vp, i32], i32),
@@ -369,10 +372,12 @@ class Batch:
 
 class Server:
     """dd_server: checkpoint and restart requests over many handles, admitted
-    in arrival order by execution slots and an HBM budget, run concurrently on
-    the handles' streams.  ``submit_*`` return a ticket; ``wait`` returns the
-    request's result as a dict (status, kind, out_len, t_submit, t_start,
-    t_end; times in seconds of the steady clock ``now()``)."""
+    in arrival order by execution slots and an HBM budget; checkpoints
+    admitted together run as one batched job table, the rest on the handles'
+    streams.  ``submit_*`` return a ticket; ``wait`` returns the request's
+    result as a dict (status, kind, out_len, t_submit, t_start, t_end, group;
+    times in seconds of the steady clock ``now()``, group the number of
+    checkpoints batched with it)."""
 
     def __init__(self, hbm_budget_bytes, slots=8):
         h = ctypes.c_void_p()
@@ -407,7 +412,12 @@ class Server:
     def wait(self, ticket):
         r = dd_request_result()
         _check(load().dd_server_wait(self._h, ticket, ctypes.byref(r)), "dd_server_wait")
-        return {k: getattr(r, k) for k, _ in dd_request_result._fields_}
+        return {k: getattr(r, k) for k, _ in dd_request_result._fields_ if k != "pad"}
+
+    def set_batching(self, max_group):
+        """Checkpoint requests admitted together run as one batched job table,
+        at most ``max_group`` of them (1: every request alone)."""
+        _check(load().dd_server_set_batching(self._h, int(max_group)), "dd_server_set_batching")
 
     def usage(self):
         a, b = ctypes.c_uint64(), ctypes.c_uint64()

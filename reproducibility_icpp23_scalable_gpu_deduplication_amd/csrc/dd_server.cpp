@@ -1,8 +1,12 @@
 // dd_server.cpp -- request loop over many checkpoint streams (include/dedup_c.h,
-// dd_server_*).  Host threads run admitted requests concurrently; each request
-// runs on its handle's HIP stream, so work of different ranks overlaps on the
-// device.  Admission is in arrival order, bounded by execution slots, one
-// request in flight per handle, and an HBM budget.
+// dd_server_*).  Host threads (execution slots) run admitted work
+// concurrently.  A slot that admits a checkpoint request also admits the
+// checkpoint requests queued behind it for other idle handles (same device,
+// hash and method, up to the group limit) and runs them as one batched job
+// table (dd_batch: one launch per stage over all of them); a restart, or a
+// checkpoint with no companion, runs alone on its handle's stream.
+// Admission is in arrival order, one request in flight per handle, bounded by
+// the slots and an HBM budget.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -44,6 +48,12 @@ struct Request {
   bool waited = false;
 };
 
+// Checkpoint requests that may share a job table with `a`.
+bool groupable(const Request* a, const Request* b) {
+  return b->kind == 0 && a->kind == 0 && a->h->device == b->h->device && a->h->st.hash_kind == b->h->st.hash_kind &&
+         a->h->st.method == b->h->st.method;
+}
+
 // Device bytes a handle keeps for a buffer of `len` bytes: tree (16 B/node),
 // labels and slot index (5 B/node), the digest map (2N slots of 32 B) and the
 // per-chunk scratch arrays (root lists, work lists with their digests, tile
@@ -60,6 +70,7 @@ uint64_t state_bytes(uint64_t len, uint32_t chunk) {
 struct dd_server {
   uint64_t budget = 0, in_use = 0, peak = 0;
   uint32_t slots = 0;
+  uint32_t max_group = 64;  // checkpoint requests per batched job table (1: each alone)
   int device = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -71,67 +82,152 @@ struct dd_server {
   bool stopping = false;
   std::vector<std::thread> workers;
 
-  // First queued request that may start now, or nullptr.  Requests of a busy
-  // rank are skipped (they wait for their predecessor); the first request of
-  // an idle rank that does not fit the budget stops the scan, so large
-  // requests are not starved.  When nothing is in flight the first eligible
-  // request always starts, even if it (with the device state the handles
-  // keep charged) exceeds the budget, so the loop never stalls.
-  Request* pick() {
-    for (auto it = queue.begin(); it != queue.end(); ++it) {
+  // Requests that start now together (empty: none may start).  The first
+  // queued request of an idle handle leads; requests of a busy handle are
+  // skipped (they wait for their predecessor).  A request that does not fit
+  // the budget stops the scan, so large requests are not starved; when
+  // nothing is in flight the leader always starts, even if it (with the
+  // device state the handles keep charged) exceeds the budget, so the loop
+  // never stalls.  A checkpoint leader takes along the queued checkpoint
+  // requests of other idle handles that may share its job table, in queue
+  // order, while they fit the budget.  A handle whose request is passed over
+  // (a restart, another hash) has none of its later requests taken, so each
+  // handle's requests run in submission order.
+  std::vector<Request*> pick() {
+    std::vector<Request*> g;
+    std::unordered_set<dd_dedup*> passed;
+    for (auto it = queue.begin(); it != queue.end();) {
       Request* r = *it;
-      if (busy.count(r->h)) continue;
+      if (busy.count(r->h) || passed.count(r->h)) {
+        ++it;
+        continue;
+      }
+      if (!g.empty() && !groupable(g[0], r)) {
+        passed.insert(r->h);
+        ++it;
+        continue;
+      }
       const bool new_state = !charged.count(r->h);
       // A handle's first request is charged an estimate of its state; after
       // every request the charge becomes the bytes it actually holds.
       const uint64_t state = new_state ? (r->kind == 0 ? state_bytes(r->len, r->h->st.chunk_size) : 0) : 0;
       const uint64_t need = r->footprint + state;
-      if (in_use + need > budget && !busy.empty()) return nullptr;
-      queue.erase(it);
+      if (in_use + need > budget && !busy.empty()) break;
+      it = queue.erase(it);
       if (new_state) charged[r->h] = state;
       in_use += need;
       if (in_use > peak) peak = in_use;
       busy.insert(r->h);
-      return r;
+      g.push_back(r);
+      if (r->kind != 0 || g.size() >= max_group) break;
     }
-    return nullptr;
+    return g;
+  }
+
+  // One request alone, on its handle's stream.
+  static void run_one(Request* r) {
+    if (r->kind == 0) {
+      uint64_t n = 0;
+      r->res.status = dd_checkpoint_device(r->h, r->data, r->len, r->make_baseline, r->out, r->out_cap, &n);
+      r->res.out_len = n;
+    } else {
+      r->res.status =
+          dd_restart_device(r->h, r->chain.data(), r->lens.data(), (uint32_t)r->chain.size(), r->id, r->out, r->len);
+      r->res.out_len = r->res.status == DD_OK ? r->len : 0;
+    }
+    r->res.group = 1;
+  }
+
+  // Checkpoint requests of distinct idle handles as one batched job table on
+  // the slot's batch (created on first use, its handle list replaced per
+  // group).  Requests that the batch would reject (invalid arguments or
+  // state) are answered alone first, so one bad request does not fail the
+  // others.
+  static void run_group(std::vector<Request*>& g, dd_batch*& b) {
+    std::vector<Request*> ok;
+    for (Request* r : g) {
+      const dd::StreamState& st = r->h->st;
+      const dd_status v = st.validate(r->len, st.runs_baseline(r->make_baseline), static_cast<const uint8_t*>(r->out),
+                                      r->out_cap);
+      if (v == DD_OK) {
+        ok.push_back(r);
+      } else {
+        r->res.status = v;
+        r->res.out_len = 0;
+        r->res.group = 1;
+      }
+    }
+    if (ok.size() == 1) run_one(ok[0]);
+    if (ok.size() < 2) return;
+    const size_t n = ok.size();
+    std::vector<dd_dedup*> hs(n);
+    std::vector<const void*> data(n);
+    std::vector<uint64_t> lens(n), caps(n), outl(n, 0);
+    std::vector<int> base(n);
+    std::vector<void*> outs(n);
+    for (size_t i = 0; i < n; i++) {
+      hs[i] = ok[i]->h;
+      data[i] = ok[i]->data;
+      lens[i] = ok[i]->len;
+      base[i] = ok[i]->make_baseline;
+      outs[i] = ok[i]->out;
+      caps[i] = ok[i]->out_cap;
+    }
+    dd_status st = DD_OK;
+    if (b && b->device != hs[0]->device) {
+      (void)dd_batch_destroy(b);
+      b = nullptr;
+    }
+    if (!b) st = dd_batch_create(hs.data(), (uint32_t)n, &b);
+    if (st == DD_OK) {
+      b->hs = hs;
+      st = dd_batch_checkpoint_device(b, data.data(), lens.data(), base.data(), outs.data(), caps.data(), outl.data());
+    }
+    for (size_t i = 0; i < n; i++) {
+      ok[i]->res.status = st;
+      ok[i]->res.out_len = st == DD_OK ? outl[i] : 0;
+      ok[i]->res.group = (uint32_t)n;
+    }
   }
 
   void worker() {
     (void)hipSetDevice(device);
+    dd_batch* batch = nullptr;  // this slot's batched job table
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
-      Request* r = nullptr;
-      cv_work.wait(lk, [&] { return (r = pick()) != nullptr || (stopping && queue.empty()); });
-      if (!r) return;  // stopping and nothing left to start
+      std::vector<Request*> g;
+      cv_work.wait(lk, [&] { return !(g = pick()).empty() || (stopping && queue.empty()); });
+      if (g.empty()) break;  // stopping and nothing left to start
       lk.unlock();
-      r->res.t_start = steady_now();
-      dd_status st;
-      if (r->kind == 0) {
-        uint64_t n = 0;
-        st = dd_checkpoint_device(r->h, r->data, r->len, r->make_baseline, r->out, r->out_cap, &n);
-        r->res.out_len = n;
-      } else {
-        st = dd_restart_device(r->h, r->chain.data(), r->lens.data(), (uint32_t)r->chain.size(), r->id, r->out,
-                               r->len);
-        r->res.out_len = st == DD_OK ? r->len : 0;
+      const double t0 = steady_now();
+      for (Request* r : g) r->res.t_start = t0;
+      if (g.size() == 1) run_one(g[0]);
+      else run_group(g, batch);
+      const double t1 = steady_now();
+      std::vector<uint64_t> resident(g.size());
+      for (size_t i = 0; i < g.size(); i++) {
+        g[i]->res.t_end = t1;
+        resident[i] = dd::resident_bytes(g[i]->h);  // the handle is not used elsewhere meanwhile
       }
-      r->res.t_end = steady_now();
-      r->res.status = st;
-      const uint64_t resident = dd::resident_bytes(r->h);  // the handle is not used elsewhere meanwhile
       lk.lock();
-      busy.erase(r->h);
-      in_use -= r->footprint;
-      // The handle's buffers stay allocated after the request (its state, and
-      // restart buffers that grow with the chain): charge what it holds now.
-      uint64_t& ch = charged[r->h];
-      in_use = in_use - ch + resident;
-      ch = resident;
-      if (in_use > peak) peak = in_use;
-      r->done = true;
+      for (size_t i = 0; i < g.size(); i++) {
+        Request* r = g[i];
+        busy.erase(r->h);
+        in_use -= r->footprint;
+        // The handle's buffers stay allocated after the request (its state,
+        // and restart buffers that grow with the chain): charge what it holds
+        // now.
+        uint64_t& ch = charged[r->h];
+        in_use = in_use - ch + resident[i];
+        ch = resident[i];
+        if (in_use > peak) peak = in_use;
+        r->done = true;
+      }
       cv_done.notify_all();
       cv_work.notify_all();
     }
+    lk.unlock();
+    if (batch) (void)dd_batch_destroy(batch);
   }
 
   dd_status submit(Request* r, uint64_t* ticket) {
@@ -231,6 +327,13 @@ dd_status dd_server_wait(dd_server* s, uint64_t ticket, dd_request_result* out) 
   *out = r->res;
   s->by_ticket.erase(it);
   delete r;
+  return DD_OK;
+}
+
+dd_status dd_server_set_batching(dd_server* s, uint32_t max_group) {
+  if (!s || max_group == 0) return DD_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->max_group = max_group;
   return DD_OK;
 }
 
