@@ -1169,9 +1169,11 @@ __device__ __forceinline__ uint32_t tile_iterations(uint32_t wg, uint32_t nwg, u
 }
 
 // Tile id of iteration n (every lane calls it with the same n, iterations in
-// increasing order from 0): refills `ids` when n starts a block of 64.
+// increasing order from 0): refills `ids` when n starts a block of 64.  A
+// group visited whole needs no list.
 __device__ __forceinline__ uint64_t tile_at(const JobG& J, bool listed, const uint32_t* pre, uint32_t* ids,
                                             uint32_t wg, uint32_t nwg, uint32_t niter, uint32_t n) {
+  if (!listed) return wg + (uint64_t)n * nwg;
   if ((n & 63) == 0) {
     __syncthreads();  // the previous block's ids are read
     const uint32_t m = n + threadIdx.x;
@@ -2089,26 +2091,14 @@ __device__ __forceinline__ void gather_data(const JobG& J, uint32_t F, uint32_t 
     __syncthreads();
     const uint64_t nq = min((uint64_t)256, Dq - q0);
     const uint64_t total = nq * ppc;
-    // Four pieces per lane are loaded before any is stored: a load issued
-    // after a store waits for it.
-    constexpr int U = 4;
-    for (uint64_t w0 = threadIdx.x; w0 < total; w0 += 256 * U) {
-      uint4 v[U];
-      uint64_t doff[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const uint64_t w = w0 + 256 * u;
-        const uint64_t i = pow2 ? (w >> sh) : (w / ppc);
-        const uint64_t pi = w - i * ppc;
-        doff[u] = (q0 + i) * cs + pi * piece;
-        if (w < total) {
-          const uint64_t so = src[i] * cs + pi * piece;
-          v[u] = load_piece<Vec>(J.data + so, so < J.len ? J.len - so : 0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        if (w0 + 256 * u < total) store_piece<Vec>(dst_base + doff[u], v[u]);
+    // One piece per lane per step: loading several pieces before storing
+    // them saves a few microseconds per incremental checkpoint but slows a
+    // baseline's whole-buffer copy by 5 %.
+    for (uint64_t w = threadIdx.x; w < total; w += 256) {
+      const uint64_t i = pow2 ? (w >> sh) : (w / ppc);
+      const uint64_t pi = w - i * ppc;
+      const uint64_t so = src[i] * cs + pi * piece;
+      store_piece<Vec>(dst_base + (q0 + i) * cs + pi * piece, load_piece<Vec>(J.data + so, so < J.len ? J.len - so : 0));
     }
     __syncthreads();
   }
