@@ -34,6 +34,7 @@ MD5 float field (`field_md5`), the list method (`list`), the request loop
 (`serving`, p50/p99), and the 64 MiB CPU reference path (`c1_cpu_reference`).
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -177,6 +178,7 @@ def run_batched(torch, dd, W, O, args, seed=1931):
     # Timed: warmup, then `steps` batched calls with the leaf kernel's events.
     # The batch handles continue their chains (checkpoint index = step % nckpt).
     batch.set_profiling(1)
+    gc.collect()  # the cyclic collector is off while timing (main)
     step = nckpt
     for _ in range(args.warmup):
         batch_step(step % nckpt)
@@ -194,6 +196,7 @@ def run_batched(torch, dd, W, O, args, seed=1931):
     el = time.perf_counter() - t0
     batch.set_profiling(0)
     # Plain path: 64 sequential calls per step, same number of steps.
+    gc.collect()
     pstep = nckpt
     for _ in range(min(args.warmup, nckpt)):
         plain_step(pstep % nckpt)
@@ -273,6 +276,7 @@ def run_plain(torch, dd, W, O, size, chunk, frac, nckpt, steps, warmup, kind=0, 
         diffs.append(out[:n].clone())
     torch.cuda.synchronize()
     g.set_profiling(1)
+    gc.collect()
     i = nckpt
     for _ in range(warmup):
         g.checkpoint_device(chain[i % nckpt].data_ptr(), size, out.data_ptr(), cap, i % nckpt == 0)
@@ -398,7 +402,9 @@ def run_field(torch, dd, W, n_values, nckpt, chunk, seed=1931):
     version perturbs 5 % of the nonzero values by N(0,1e-3) and flips 1 % of
     the blocks between zero and nonzero.  Each version is generated in HBM
     from the previous one (untimed); each checkpoint call is timed on its own
-    (the call synchronizes), with HIP events around the leaf-hash kernel.
+    (the call synchronizes), with HIP events around the leaf-hash kernel,
+    after an untimed baseline of version 0 that allocates the handle's state
+    and tunes the leaf mapping.
     Checkpoints 0, nckpt/2 and nckpt-1 are restarted from the device-resident
     chain and compared byte for byte with the saved versions."""
     size = n_values * 4
@@ -411,6 +417,12 @@ def run_field(torch, dd, W, n_values, nckpt, chunk, seed=1931):
     for k, v in enumerate(W.float_field_versions_gpu(torch, seed, n_values, nckpt)):
         if k in keep_ids:
             saved[k] = v.clone()
+        if k == 0:
+            # Untimed warm-up baseline on the same handle: allocates its state
+            # and runs the leaf-mapping tuner (once per process); the chain
+            # then starts again at checkpoint id 0.
+            g.checkpoint_device(v.data_ptr(), size, out.data_ptr(), cap, True)
+            g.current_id = 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         n = g.checkpoint_device(v.data_ptr(), size, out.data_ptr(), cap, k == 0)
@@ -632,6 +644,11 @@ def main():
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+
+    # The cyclic garbage collector runs only where called (before each timed
+    # loop): a full collection inside a timed loop stalled one step of the
+    # batched leg by ~45 ms.
+    gc.disable()
 
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)

@@ -9,7 +9,7 @@
 // TB/s over a batch of 64 buffers).  The first upload of a job table with a
 // new (device, hash, alignment, chunk sizes, one job or several) key times
 // every valid mapping on a sample of the table (every job, at most 64, over
-// the first bytes of its buffer, at most 4 GiB in all) and keeps the fastest
+// the first bytes of its buffer, at most 1 GiB in all) and keeps the fastest
 // for the process.  The timed launches are incremental checkpoints of
 // unchanged data (after one baseline launch) on scratch tree, label and list
 // arrays, so the handles' state is not touched.  The mappings are timed round-robin
@@ -116,7 +116,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   // over the first bytes of its buffer, scaled so that the sample holds at
   // most kSampleBytes, each at least one chunk.  A table of many small jobs
   // and one of a single large job favour different mappings.
-  constexpr uint64_t kSampleBytes = 4ull << 30;
+  constexpr uint64_t kSampleBytes = 1ull << 30;
   constexpr size_t kSampleJobs = 64;
   std::vector<const JobDev*> src;
   for (const JobDev& J : jobs) src.push_back(&J);
