@@ -334,6 +334,8 @@ static uint64_t stage_budget() {
 Pipeline::~Pipeline() {
   if (host_table_) (void)hipHostFree(host_table_);
   if (host_ctr) (void)hipHostFree(host_ctr);
+  for (hipEvent_t e : trial_ev_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
@@ -357,7 +359,26 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
     aligned16_ = aligned16_ && al;
     vec16_ = vec16_ && al;
   }
-  leaf_map_ = leaf_mapping_for(jobs, aligned16_, hash_kind_, s);
+  {
+    std::vector<uint64_t> key;
+    std::vector<int> cand;
+    leaf_map_ = leaf_mapping_for(jobs, aligned16_, hash_kind_, s, &key, &cand);
+    trial_.current = -1;
+    if (cand.size() < 2) {
+      trial_ = LeafTrial{};
+    } else {
+      if (trial_.key != key || trial_.cand != cand) trial_ = LeafTrial{key, cand, std::vector<std::vector<float>>(cand.size()), -1};
+      bool incremental = true;  // a baseline's leaf launch writes every digest: not comparable
+      for (const JobDev& J : jobs) incremental = incremental && !J.is_baseline;
+      if (incremental) {
+        size_t i = 0;
+        for (size_t k = 1; k < cand.size(); k++)
+          if (trial_.ms[k].size() < trial_.ms[i].size()) i = k;
+        trial_.current = (int)i;
+        leaf_map_ = cand[i];
+      }
+    }
+  }
   const size_t nstages = S_TILE0 + max_groups_;
   const size_t need_prefix = nstages * (nj + 1);
   const size_t jobs_bytes = (nj * sizeof(JobDev) + 255) & ~size_t(255);
@@ -439,17 +460,46 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
 
 std::vector<uint64_t> Pipeline::signature() const {
   std::vector<uint64_t> sig;
+  // The graph holds the stages after the leaf kernel: the leaf mapping and
+  // grid are not part of it.
   sig.push_back(njobs_);
   sig.push_back((uint64_t)max_groups_);
-  sig.push_back((uint64_t)leaf_map_);
   sig.push_back((uint64_t)aligned16_ | ((uint64_t)vec16_ << 1));
   sig.push_back((uint64_t)hash_kind_ | ((uint64_t)list_mode_ << 8) | ((uint64_t)kinds_ << 16) |
                 ((uint64_t)low_root_ << 24));
   sig.push_back(low_root_ ? max_c_ : 0);
   sig.push_back(reinterpret_cast<uintptr_t>(dev_table_.p));
   for (uint32_t o : off_) sig.push_back(o);
-  for (uint32_t g : grid_) sig.push_back(g);
+  for (size_t st = 0; st < grid_.size(); st++)
+    if (st != S_LEAF) sig.push_back(grid_[st]);
   return sig;
+}
+
+void Pipeline::trial_done() {
+  if (trial_.current < 0) return;
+  float ms = 0;
+  const size_t i = (size_t)trial_.current;
+  trial_.current = -1;
+  if (hipEventElapsedTime(&ms, trial_used_[0], trial_used_[1]) != hipSuccess || ms <= 0) {
+    (void)hipGetLastError();
+    return;
+  }
+  trial_.ms[i].push_back(ms);
+  for (const auto& v : trial_.ms)
+    if (v.size() < kTrialRounds) return;
+  size_t best = 0;
+  float best_ms = 0;
+  for (size_t k = 0; k < trial_.cand.size(); k++) {
+    std::vector<float> v = trial_.ms[k];
+    std::sort(v.begin(), v.end());
+    if (k == 0 || v[v.size() / 2] < best_ms) {
+      best_ms = v[v.size() / 2];
+      best = k;
+    }
+  }
+  leaf_mapping_settle(trial_.key, trial_.cand[best]);
+  leaf_map_ = trial_.cand[best];
+  trial_ = LeafTrial{};
 }
 
 dd_status Pipeline::run(const std::vector<JobDev>& jobs, hipStream_t s, bool rerun, bool begin, hipEvent_t ev_dedup,
@@ -474,9 +524,18 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   if (part != PART_TAIL) {
     if (rerun || begin) launch_begin(J, nullptr, nj, rerun, s);
     DD_TRY(mark(0));
+    if (rerun) trial_.current = -1;  // no leaf kernel in a re-run
     if (!rerun) {
-      launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s,
-                  leaf_events ? stage_ev[0] : nullptr, leaf_events ? stage_ev[1] : nullptr);
+      hipEvent_t e0 = leaf_events ? stage_ev[0] : nullptr, e1 = leaf_events ? stage_ev[1] : nullptr;
+      if (trial_.current >= 0 && !e0) {
+        for (hipEvent_t& e : trial_ev_)
+          if (!e) DD_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+        e0 = trial_ev_[0];
+        e1 = trial_ev_[1];
+      }
+      trial_used_[0] = e0;
+      trial_used_[1] = e1;
+      launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s, e0, e1);
     }
     DD_TRY(mark(1));
     if (part == PART_HEAD) {
@@ -559,6 +618,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
                             h->profile ? h->pev : nullptr));
   DD_HIP(hipEventRecord(h->ev[2], h->stream));
   DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
+  h->pipe.trial_done();
   JobCounters c = h->pipe.host_ctr[0];
   for (int attempt = 0; (c.flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) && attempt < 4; attempt++) {
     const uint64_t need = c.map_count + 2 * c.pending;
@@ -1348,6 +1408,7 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
     DD_TRY(b->pipe.launch_all(b->stream, false, begin, nullptr, pev));
   }
   DD_HIP(hipStreamSynchronize(b->stream));
+  b->pipe.trial_done();
   if (b->profile) {
     for (int i = 0; i < DD_NUM_STAGES; i++) {
       float ms = 0;

@@ -128,18 +128,40 @@ class Pipeline {
   uint64_t max_c_ = 0;      // chunks of the largest job
   uint32_t kinds_ = 0;      // insert modes (JOBS_DENSE | JOBS_LIST_NARROW | JOBS_LIST_WIDE) in the table
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
+  // Comparison of the tuner's candidate mappings on this pipeline's own
+  // incremental checkpoints: each launch of a table with no baseline job
+  // runs the candidate with the fewest samples, timed with events on the
+  // leaf kernel's dispatch.
+  struct LeafTrial {
+    std::vector<uint64_t> key;
+    std::vector<int> cand;
+    std::vector<std::vector<float>> ms;
+    int current = -1;  // candidate of the launch in flight
+  } trial_;
+  hipEvent_t trial_ev_[2] = {nullptr, nullptr};
+  hipEvent_t trial_used_[2] = {nullptr, nullptr};  // the events on the leaf dispatch in flight
+  static constexpr size_t kTrialRounds = 3;
 
  public:
   int leaf_mapping() const { return leaf_map_; }
   uint64_t device_bytes() const { return dev_table_.bytes; }
+  // After the stream has completed a launch_all: records the trial launch's
+  // leaf time and settles the mapping once every candidate has its rounds.
+  void trial_done();
 };
 
 uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
 
 // Leaf kernel mapping for a job table (dd_tune.cpp): the fastest mapping
-// measured on this device for the table's hash, alignment and chunk sizes,
-// timed on first use and cached for the process.
-int leaf_mapping_for(const std::vector<JobDev>& jobs, bool aligned16, int hash_kind, hipStream_t s);
+// measured on this device for the table's hash, alignment, chunk sizes, job
+// count and size class, timed on first use and cached for the process.
+// `key` receives the table's tuning key; `cand`, while the key is not
+// settled, the mappings still to be compared on checkpoints (fastest on the
+// tuner's sample first).
+int leaf_mapping_for(const std::vector<JobDev>& jobs, bool aligned16, int hash_kind, hipStream_t s,
+                     std::vector<uint64_t>* key = nullptr, std::vector<int>* cand = nullptr);
+// Settles a key's mapping after its comparison on checkpoints.
+void leaf_mapping_settle(const std::vector<uint64_t>& key, int mapping);
 
 }  // namespace dd
 

@@ -12,7 +12,11 @@
 // the first bytes of its buffer, at most 1 GiB in all) and keeps the fastest
 // for the process.  The timed launches are incremental checkpoints of
 // unchanged data (after one baseline launch) on scratch tree, label and list
-// arrays, so the handles' state is not touched.  The mappings are timed round-robin
+// arrays, so the handles' state is not touched.  The sample has no changed
+// chunks; the mappings within 15 % of the fastest on it (at most 3) are then
+// compared on the first incremental checkpoints of tables of the key, timed
+// on the leaf kernel's own dispatch (Pipeline::trial_done), and the fastest
+// median of three launches each is settled for the process.  The mappings are timed round-robin
 // (each round launches every mapping once) and each keeps its median, so a
 // drift of the device's clock or of other work affects all alike.  One
 // thread tunes a key; callers that need the same key meanwhile wait for it.
@@ -34,12 +38,36 @@
 namespace dd {
 namespace {
 
-using Key = std::tuple<int, int, int, uint32_t, uint32_t, int>;  // device, hash, aligned16, min/max chunk, batch
+// device, hash, aligned16, min/max chunk, several jobs, size class of the table
+using Key = std::tuple<int, int, int, uint32_t, uint32_t, int, int>;
+
+// A key's result: the fastest mapping on the sample and the mappings close to
+// it, which the first checkpoints of the table's kind then compare on their
+// own data (LeafTrial) until one is settled.
+struct Entry {
+  int best = LEAF_DIRECT;
+  std::vector<int> cand;
+  bool settled = false;
+};
 
 std::mutex g_mu;
 std::condition_variable g_cv;
-constexpr int kTuning = -1;  // g_choice value while one thread tunes the key
-std::map<Key, int> g_choice;
+std::map<Key, Entry> g_choice;
+std::map<Key, bool> g_tuning;  // keys one thread is tuning
+
+std::vector<uint64_t> key_vec(const Key& k) {
+  return {(uint64_t)std::get<0>(k), (uint64_t)std::get<1>(k), (uint64_t)std::get<2>(k), std::get<3>(k), std::get<4>(k),
+          (uint64_t)std::get<5>(k), (uint64_t)std::get<6>(k)};
+}
+
+Key key_of(const std::vector<uint64_t>& v) {
+  return Key{(int)v[0], (int)v[1], (int)v[2], (uint32_t)v[3], (uint32_t)v[4], (int)v[5], (int)v[6]};
+}
+
+// Candidates within kNear of the fastest median, at most kMaxCand, fastest
+// first.
+constexpr double kNear = 1.15;
+constexpr size_t kMaxCand = 3;
 
 bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) {
   if (m == LEAF_DIRECT) return true;
@@ -81,37 +109,50 @@ double time_mapping(const JobDev* dj, const uint32_t* dprefix, uint32_t njobs, u
 
 }  // namespace
 
-int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipStream_t s) {
+int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipStream_t s, std::vector<uint64_t>* key_out,
+                     std::vector<int>* cand_out) {
+  if (cand_out) cand_out->clear();
   const int f = forced_mapping();
   if (f >= 0 && mapping_valid(f, al16, kind, jobs)) return f;
   if (!al16) return LEAF_DIRECT;
   uint32_t cmin = ~0u, cmax = 0;
+  uint64_t total = 0;
   for (const JobDev& J : jobs) {
     cmin = std::min(cmin, J.chunk_size);
     cmax = std::max(cmax, J.chunk_size);
+    total += J.len;
   }
+  const int size_class = total < (64ull << 20) ? 0 : total < (512ull << 20) ? 1 : total < (4ull << 30) ? 2 : 3;
   int dev = 0;
   (void)hipGetDevice(&dev);
-  const Key key{dev, kind, al16 ? 1 : 0, cmin, cmax, jobs.size() > 1 ? 1 : 0};
+  const Key key{dev, kind, al16 ? 1 : 0, cmin, cmax, jobs.size() > 1 ? 1 : 0, size_class};
+  if (key_out) *key_out = key_vec(key);
   {
     std::unique_lock<std::mutex> lk(g_mu);
-    g_cv.wait(lk, [&] {
-      auto it = g_choice.find(key);
-      return it == g_choice.end() || it->second != kTuning;
-    });
+    g_cv.wait(lk, [&] { return !g_tuning.count(key); });
     auto it = g_choice.find(key);
-    if (it != g_choice.end()) return it->second;
-    g_choice[key] = kTuning;  // this thread tunes the key
+    if (it != g_choice.end()) {
+      if (cand_out && !it->second.settled) *cand_out = it->second.cand;
+      return it->second.best;
+    }
+    g_tuning[key] = true;  // this thread tunes the key
   }
   // Publishes the result (or, when tuning could not run, removes the marker
   // so a later call tries again) and wakes the waiting callers.
-  auto publish = [&](int m, bool keep) {
+  auto publish = [&](int m, bool keep, const std::vector<int>& cand) {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (keep) g_choice[key] = m;
-    else g_choice.erase(key);
+    g_tuning.erase(key);
+    if (keep) {
+      Entry& e = g_choice[key];
+      e.best = m;
+      e.cand = cand;
+      e.settled = cand.size() < 2;
+      if (cand_out && !e.settled) *cand_out = cand;
+    }
     g_cv.notify_all();
     return m;
   };
+  const std::vector<int> none;
   // Scratch jobs: every job of the table (at most kSampleJobs, the largest)
   // over the first bytes of its buffer, scaled so that the sample holds at
   // most kSampleBytes, each at least one chunk.  A table of many small jobs
@@ -167,7 +208,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   const size_t table_bytes = jobs_bytes + LEAF_NMAP * (nj + 1) * 4;
   if (tree.ensure(tree_bytes) != DD_OK || labels.ensure(lab_bytes) != DD_OK || ctr.ensure(nj * ctr_stride) != DD_OK ||
       plist.ensure(list_bytes) != DD_OK || table.ensure(table_bytes) != DD_OK)
-    return publish(best, false);  // no memory to tune: keep the default, do not cache
+    return publish(best, false, none);  // no memory to tune: keep the default, do not cache
   std::vector<uint8_t> host(table_bytes, 0);
   std::vector<uint32_t> grids(LEAF_NMAP);
   for (size_t j = 0; j < nj; j++) {
@@ -202,7 +243,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   if (!upload() || hipEventCreateWithFlags(&e0, hipEventDisableSystemFence) != hipSuccess ||
       hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
-    return publish(best, false);
+    return publish(best, false, none);
   }
   const JobDev* dj = table.as<JobDev>();
   const uint32_t* dprefix = reinterpret_cast<const uint32_t*>(table.as<uint8_t>() + jobs_bytes);
@@ -218,7 +259,7 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   if (!upload()) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return publish(best, false);
+    return publish(best, false, none);
   }
   // One untimed launch per mapping, then kRounds rounds of one timed launch
   // each; the fastest median wins.
@@ -234,19 +275,34 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
     }
   }
   double best_ms = -1;
+  double med[LEAF_NMAP];
   for (int m = 0; m < LEAF_NMAP && ok; m++) {
+    med[m] = -1;
     if (times[m].empty()) continue;
     std::sort(times[m].begin(), times[m].end());
-    const double med = times[m][times[m].size() / 2];
-    if (best_ms < 0 || med < best_ms) {
-      best_ms = med;
+    med[m] = times[m][times[m].size() / 2];
+    if (best_ms < 0 || med[m] < best_ms) {
+      best_ms = med[m];
       best = m;
     }
   }
+  std::vector<int> cand;
+  for (int m = 0; m < LEAF_NMAP && ok && best_ms > 0; m++)
+    if (med[m] >= 0 && med[m] <= kNear * best_ms) cand.push_back(m);
+  std::sort(cand.begin(), cand.end(), [&](int a, int b) { return med[a] < med[b]; });
+  if (cand.size() > kMaxCand) cand.resize(kMaxCand);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipGetLastError();
-  return publish(best, ok && best_ms >= 0);
+  return publish(best, ok && best_ms >= 0, cand);
+}
+
+void leaf_mapping_settle(const std::vector<uint64_t>& key, int m) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_choice.find(key_of(key));
+  if (it == g_choice.end() || it->second.settled) return;
+  it->second.best = m;
+  it->second.settled = true;
 }
 
 }  // namespace dd
