@@ -274,10 +274,11 @@ dd_status dd_get_stage_times(dd_dedup* h, double ms[DD_NUM_STAGES]);
  * deeper load pipelining, 2/3/4 LDS-staged in 64/128/256-byte segments, 5 one
  * lane per chunk of at most 128 bytes with the whole chunk in flight, 6
  * 128-byte segments loaded straight into LDS with contiguous loads, 7 the
- * same with one-wavefront workgroups.  The
+ * same with one-wavefront workgroups, 8 LDS-staged 128-byte segments with
+ * the next segment's loads in flight.  The
  * library picks it by timing the valid mappings once per process for each
  * hash, alignment and chunk-size combination (DD_LEAF_KERNEL=direct|deep|
- * lds64|lds128|lds256|short|dma|dma1 forces one). */
+ * lds64|lds128|lds256|short|dma|dma1|lds128p forces one). */
 dd_status dd_get_leaf_mapping(dd_dedup* h, int* mapping);
 /* Profiling of a batch's stage sequence (no counterpart in the reference):
  * as dd_set_profiling / dd_get_stage_times for the last batch call; the

@@ -33,7 +33,7 @@ EXPORTED = [
     "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_batch_set_profiling",
     "dd_batch_get_stage_times", "dd_get_ckpt_stats", "dd_get_device_bytes", "dd_buf_free", "dd_strerror",
 ]
-LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1"]
+LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1", "lds128p"]
 STAGES = ["leaf_hash", "insert_leaves", "first_forest", "insert_interior", "dup_forest", "roots_group",
           "meta_gather"]
 

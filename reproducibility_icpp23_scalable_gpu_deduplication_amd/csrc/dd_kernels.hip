@@ -314,7 +314,10 @@ __device__ __forceinline__ bool leaf_finish(const JobG& J, uint64_t k, const Dig
 // pad per chunk (conflict-free 128-bit reads), and each lane then hashes the
 // next SEG bytes of its own chunk out of LDS.  Needs chunk % SEG == 0 and a
 // 16-byte aligned buffer; the partial last chunk is hashed from global memory.
-template <int Kind, int SEG>
+// PF: the loads of round r+1 go to registers before round r is hashed, and
+// are stored to LDS at the start of the next round (long chunks: a wavefront
+// keeps a round of loads in flight while it hashes).
+template <int Kind, int SEG, bool PF>
 __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   constexpr int PITCH = SEG + 16;
   constexpr int VPC = SEG / 16;        // 16-byte vectors per chunk per round
@@ -335,18 +338,39 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
   Murmur m;
   uint32_t st[4];
   md5_init(st);
-  for (uint32_t r = 0; r < rounds; r++) {
-    // cooperative load: vector v of this round covers chunk v / VPC, piece v % VPC
+  // Cooperative load of round r: vector v covers chunk v / VPC, piece v % VPC
+  // (chunks past the complete ones load zeros and are never hashed).
+  uint4 nx[PF ? LOADS : 1];
+  auto load_round = [&](uint32_t r) {
 #pragma unroll
     for (int i = 0; i < LOADS; i++) {
       const uint32_t v = i * 64 + lane;
       const uint32_t c = v / VPC, piece = v % VPC;
-      if (k0 + c < full) {
-        const uint4 x = gload16(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16);
-        *reinterpret_cast<uint4*>(L + c * PITCH + piece * 16) = x;
-      }
+      nx[i] = k0 + c < full ? gload16(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16) : make_uint4(0, 0, 0, 0);
     }
-    __syncthreads();
+  };
+  if (PF && rounds > 0) load_round(0);
+  for (uint32_t r = 0; r < rounds; r++) {
+    if (PF) {
+#pragma unroll
+      for (int i = 0; i < LOADS; i++) {
+        const uint32_t v = i * 64 + lane;
+        *reinterpret_cast<uint4*>(L + (v / VPC) * PITCH + (v % VPC) * 16) = nx[i];
+      }
+      __syncthreads();
+      if (r + 1 < rounds) load_round(r + 1);
+    } else {
+#pragma unroll
+      for (int i = 0; i < LOADS; i++) {
+        const uint32_t v = i * 64 + lane;
+        const uint32_t c = v / VPC, piece = v % VPC;
+        if (k0 + c < full) {
+          const uint4 x = gload16(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16);
+          *reinterpret_cast<uint4*>(L + c * PITCH + piece * 16) = x;
+        }
+      }
+      __syncthreads();
+    }
     if (k < full) {
       const uint8_t* my = L + lane * PITCH;
       if (Kind == HASH_MD5) {
@@ -2623,9 +2647,10 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
   if (!al16) mapping = LEAF_DIRECT;  // the other mappings need 16-byte aligned buffers
   if (kind == HASH_MD5) {
     switch (mapping) {
-      case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64>));
-      case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128>));
-      case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256>));
+      case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64, false>));
+      case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, false>));
+      case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256, false>));
+      case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, true>));
       case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5, 4>));
       case LEAF_DMA1: DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MD5, 1>), 64);
       default:
@@ -2636,9 +2661,10 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
   switch (mapping) {
     case LEAF_DEEP: DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DEEP>));
     case LEAF_SHORT: DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_SHORT>));
-    case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64>));
-    case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128>));
-    case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256>));
+    case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64, false>));
+    case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, false>));
+    case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256, false>));
+    case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, true>));
     case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3, 4>));
     case LEAF_DMA1: DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MURMUR3, 1>), 64);
     default:

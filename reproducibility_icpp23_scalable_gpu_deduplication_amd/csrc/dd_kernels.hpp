@@ -222,7 +222,8 @@ enum LeafMapping : int {
   LEAF_SHORT = 5,
   LEAF_DMA = 6,
   LEAF_DMA1 = 7,
-  LEAF_NMAP = 8
+  LEAF_LDS128P = 8,
+  LEAF_NMAP = 9
 };
 // ev_start/ev_stop (optional) are recorded at the start and the end of the
 // leaf kernel's own dispatch.

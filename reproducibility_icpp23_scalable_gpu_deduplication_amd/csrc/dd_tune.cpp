@@ -20,7 +20,7 @@
 // (each round launches every mapping once) and each keeps its median, so a
 // drift of the device's clock or of other work affects all alike.  One
 // thread tunes a key; callers that need the same key meanwhile wait for it.
-// DD_LEAF_KERNEL=direct|deep|lds64|lds128|lds256|short|dma|dma1 forces a
+// DD_LEAF_KERNEL=direct|deep|lds64|lds128|lds256|short|dma|dma1|lds128p forces a
 // mapping (when it is valid for the table).
 #include <hip/hip_runtime.h>
 
@@ -79,7 +79,7 @@ bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) 
       if (J.chunk_size > 128) return false;
     return true;
   }
-  const uint32_t seg = m == LEAF_LDS64 ? 64 : (m == LEAF_LDS128 || m == LEAF_DMA || m == LEAF_DMA1) ? 128 : 256;
+  const uint32_t seg = m == LEAF_LDS64 ? 64 : (m == LEAF_LDS128 || m == LEAF_LDS128P || m == LEAF_DMA || m == LEAF_DMA1) ? 128 : 256;
   for (const JobDev& J : jobs)
     if (J.chunk_size % seg) return false;
   return true;
@@ -88,7 +88,7 @@ bool mapping_valid(int m, bool al16, int kind, const std::vector<JobDev>& jobs) 
 int forced_mapping() {
   const char* e = std::getenv("DD_LEAF_KERNEL");
   if (!e) return -1;
-  static const char* names[LEAF_NMAP] = {"direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1"};
+  static const char* names[LEAF_NMAP] = {"direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1", "lds128p"};
   for (int m = 0; m < LEAF_NMAP; m++)
     if (!strcmp(e, names[m])) return m;
   return -1;

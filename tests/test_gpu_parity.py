@@ -254,7 +254,8 @@ def test_error_paths(torch_gpu):
     assert out.cpu().numpy().tobytes() == bytes(range(256)) * 4
 
 
-@pytest.mark.parametrize("mapping", ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1", None])
+@pytest.mark.parametrize("mapping", ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1", "lds128p",
+                                     None])
 def test_leaf_mappings_match_oracle(torch_gpu, mapping, monkeypatch):
     # Every leaf kernel mapping (forced through DD_LEAF_KERNEL) and the tuned
     # choice (None) give the oracle's diffs; 256-byte chunks admit all but
@@ -267,7 +268,7 @@ def test_leaf_mappings_match_oracle(torch_gpu, mapping, monkeypatch):
     cases = ((256, dd.HASH_MURMUR3), (256, dd.HASH_MD5))
     if mapping in ("short", None):
         cases += ((128, dd.HASH_MURMUR3), (48, dd.HASH_MURMUR3), (16, dd.HASH_MURMUR3))
-    if mapping in ("dma", "dma1", None):
+    if mapping in ("dma", "dma1", "lds128p", None):
         cases += ((128, dd.HASH_MURMUR3), (128, dd.HASH_MD5), (4096, dd.HASH_MURMUR3))
     for cs, kind in cases:
         datas = W.chain_uniform_changes(17 + cs, 3000 * cs + 7, cs, 4, fresh_frac=0.03, copy_frac=0.05,
@@ -275,8 +276,32 @@ def test_leaf_mappings_match_oracle(torch_gpu, mapping, monkeypatch):
         g, _ = check_chain(torch_gpu, cs, datas, kind, restart_ids=[0, 3])
         got = g.leaf_mapping()
         valid = (not (mapping in ("deep", "short") and kind == dd.HASH_MD5) and not (mapping == "short" and cs > 128)
-                 and not (mapping in ("dma", "dma1") and cs % 128))
+                 and not (mapping in ("dma", "dma1", "lds128p") and cs % 128))
         if mapping and valid:
             assert got == mapping
         else:
             assert got in dd.LEAF_MAPPINGS
+
+
+def test_leaf_mapping_trials_keep_bytes(torch_gpu, monkeypatch):
+    # With no forced mapping, the first incremental checkpoints of a key run
+    # the tuner's near-best candidate mappings in turn until one is settled;
+    # every checkpoint's diff is the oracle's meanwhile, and once settled the
+    # reported mapping stays the same.  A buffer size of its own (37 MiB) and
+    # 384-byte chunks give a key no other test uses.
+    monkeypatch.delenv("DD_LEAF_KERNEL", raising=False)
+    cs = 384
+    datas = W.chain_uniform_changes(4242, (37 << 20) + 5 * cs, cs, 14, fresh_frac=0.01, copy_frac=0.02,
+                                    zero_bytes=64 * cs)
+    g, gd, _ = gpu_chain(torch_gpu, cs, datas)
+    od, _ = oracle_chain(cs, datas)
+    for k in range(len(datas)):
+        assert gd[k] == od[k], f"checkpoint {k}: {first_mismatch(gd[k], od[k])}"
+    maps = []
+    for k, x in enumerate(datas):  # a second chain on a fresh handle: the key is settled by now
+        if k == 0:
+            h = dd.TreeDeduplicator(cs)
+        t = to_dev(torch_gpu, x)
+        assert h.checkpoint(t.data_ptr(), t.numel(), k == 0) == od[k]
+        maps.append(h.leaf_mapping())
+    assert len(set(maps[1:])) == 1, maps
