@@ -85,6 +85,9 @@ static MapView make_view(void* slots, uint64_t cap) {
 
 static uint64_t map_limit_of(uint64_t cap) { return cap / 2 + cap / 4; }
 
+// Checkpoints of growth a map growth makes room for.
+constexpr uint64_t kGrowHorizon = 16;
+
 // Inserts per job table above which every job probes in narrow rounds.
 constexpr uint64_t kWideInsertLimit = 256 * 1024;
 
@@ -103,7 +106,16 @@ static uint64_t tiles_in_group(uint32_t D, int g) {
 // StreamState
 // ---------------------------------------------------------------------------
 dd_status StreamState::grow_map(uint64_t need, hipStream_t s) {
+  // Room for `need` at half load, and for the entries that kGrowHorizon more
+  // checkpoints would add at the rate the chain has added them since its
+  // baseline, under the load limit: a chain that adds many entries per
+  // checkpoint grows in fewer, larger steps (each growth rewrites the whole
+  // table).
   uint64_t cap = next_pow2(std::max<uint64_t>(2 * need, 1024));
+  if (ckpts_since_base > 0 && map_count > base_map_count) {
+    const uint64_t rate = (map_count - base_map_count) / ckpts_since_base;
+    cap = std::max(cap, next_pow2((need + kGrowHorizon * rate) / 3 * 4));
+  }
   if (cap <= map_cap) cap = map_cap * 2;
   DevBuf fresh;
   DD_TRY(fresh.ensure(cap * sizeof(Slot)));
@@ -309,6 +321,12 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
   nfirst = c.nfirst;
   nshift = c.nshift;
   map_count = c.map_count;
+  if (baseline) {
+    base_map_count = map_count;
+    ckpts_since_base = 0;
+  } else {
+    ckpts_since_base++;
+  }
   last_diff_len = c.diff_len;
   last_map_ckpt = current_id;
 }

@@ -46,6 +46,8 @@ struct StreamState {
   DevBuf tree, labels, slot_of, lr_new;
   DevBuf map;
   uint64_t map_cap = 0, map_count = 0;
+  uint64_t base_map_count = 0;    // map entries after the chain's baseline
+  uint32_t ckpts_since_base = 0;  // completed incremental checkpoints since it
   DevBuf blk, f_list, f_off, s_node, s_prev, s_tree, ss_node, ss_prev, s_hist, ptab;
   DevBuf leafblk, pend_list, pend_dig, tiles, tile_epoch, lists;
   uint32_t epoch = 0;     // stamp of the current checkpoint (tile_epoch)
