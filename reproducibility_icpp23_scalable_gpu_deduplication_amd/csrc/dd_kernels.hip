@@ -1797,8 +1797,13 @@ __device__ __forceinline__ void reset_visited_tiles(const JobG& J, uint32_t wg, 
   }
 }
 
-// Workgroups stride over the sort tiles (kSTile shifted roots each).
+// Workgroups stride over the sort tiles (kSTile shifted roots each, kSTile /
+// 256 per lane).  Each lane loads its roots' nodes, then their map slots, then
+// the map entries (three rounds of loads for the whole tile) before any store
+// of the tile: a load issued after a store waits for it.
 __global__ __launch_bounds__(256) void k_sbin_count(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  constexpr uint32_t Q = kSTile / 256;
+  static_assert(kSTile % 256 == 0, "whole rounds of 256 roots per tile");
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
   if (J.ctr->flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) return;
@@ -1807,38 +1812,55 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobG* __restrict__ job
   const uint32_t nbins = J.ckpt_id + 1;
   const uint32_t S = J.ctr->nshift;
   for (uint32_t t = wg; t < nt; t += nwg) {
-  __syncthreads();  // the previous tile's atomics are done before its bins are reused
-  for (uint32_t bn = threadIdx.x; bn < nbins; bn += blockDim.x) J.s_hist[(uint64_t)bn * nt + t] = 0;
-  __syncthreads();
-  const uint32_t e1 = min(S, (t + 1) * kSTile);
-  for (uint32_t base = t * kSTile; base < e1; base += blockDim.x) {
-    const uint32_t e = base + threadIdx.x;
-    uint32_t bn = 0;
-    bool todo = e < e1;
-    if (todo) {
-      // The root's source: the map entry its digest matched (slot recorded
-      // by pass 2, or by the leaf inserts for a shifted leaf).
-      const unsigned long long val = J.map.slots[J.slot_of[J.s_node[e]]].val;
-      J.s_prev[e] = (uint32_t)val;
-      J.s_tree[e] = (uint32_t)(val >> 32);
-      bn = (uint32_t)(val >> 32);
-      if (bn >= nbins) {
-        g_or(&J.ctr->flags, FLAG_BAD_SOURCE);
-        bn = nbins - 1;
+    const uint32_t e0 = t * kSTile, e1 = min(S, e0 + kSTile);
+    // The roots' sources: the map entries their digests matched (slots
+    // recorded by pass 2, or by the leaf inserts for shifted leaves).
+    uint32_t node[Q], slot[Q];
+    unsigned long long val[Q];
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+      const uint32_t e = e0 + q * 256 + threadIdx.x;
+      node[q] = e < e1 ? J.s_node[e] : 0;
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+      const uint32_t e = e0 + q * 256 + threadIdx.x;
+      slot[q] = e < e1 ? J.slot_of[node[q]] : 0;
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+      const uint32_t e = e0 + q * 256 + threadIdx.x;
+      val[q] = e < e1 ? J.map.slots[slot[q]].val : 0;
+    }
+    __syncthreads();  // the previous tile's atomics are done
+    for (uint32_t bn = threadIdx.x; bn < nbins; bn += blockDim.x) J.s_hist[(uint64_t)bn * nt + t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+      const uint32_t e = e0 + q * 256 + threadIdx.x;
+      bool todo = e < e1;
+      uint32_t bn = 0;
+      if (todo) {
+        J.s_prev[e] = (uint32_t)val[q];
+        J.s_tree[e] = (uint32_t)(val[q] >> 32);
+        bn = (uint32_t)(val[q] >> 32);
+        if (bn >= nbins) {
+          g_or(&J.ctr->flags, FLAG_BAD_SOURCE);
+          bn = nbins - 1;
+        }
+      }
+      // one atomic per distinct bin of the wavefront (shifted roots mostly
+      // share one or two source checkpoints)
+      while (true) {
+        const unsigned long long act = __ballot(todo);
+        if (!act) break;
+        const uint32_t leader = __ffsll((long long)act) - 1;
+        const uint32_t lb = __shfl(bn, leader, 64);
+        const unsigned long long same = __ballot(todo && bn == lb);
+        if (lane_id() == leader) g_add(&J.s_hist[(uint64_t)lb * nt + t], (uint32_t)__popcll(same));
+        if (todo && bn == lb) todo = false;
       }
     }
-    // one atomic per distinct bin of the wavefront (shifted roots mostly
-    // share one or two source checkpoints)
-    while (true) {
-      const unsigned long long act = __ballot(todo);
-      if (!act) break;
-      const uint32_t leader = __ffsll((long long)act) - 1;
-      const uint32_t lb = __shfl(bn, leader, 64);
-      const unsigned long long same = __ballot(todo && bn == lb);
-      if (lane_id() == leader) g_add(&J.s_hist[(uint64_t)lb * nt + t], (uint32_t)__popcll(same));
-      if (todo && bn == lb) todo = false;
-    }
-  }
   }
 }
 
