@@ -987,7 +987,7 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
   {
     uint8_t* stage = nullptr;
     uint8_t* hst = nullptr;
-    if (dev_chain) {
+    if (dev_chain && tab_total) {  // no prior table in the walk (a baseline alone): no round trip
       DD_TRY(sp.arena(align256(tab_total) + 4096, &stage));
       DD_TRY(sp.host(tab_total, &hst));
       std::vector<const uint8_t*> src(M);
@@ -997,7 +997,9 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
     }
     for (size_t k = 0; k < M; k++) {
       const dd_header& hd = hdrs[k];
-      const uint8_t* tb = dev_chain ? hst + tab_off[k] : host_chain[ids[k]].data + 32 + 4ull * hd.num_first_ocur;
+      const uint8_t* tb = !tab_bytes[k] ? nullptr
+                          : dev_chain ? hst + tab_off[k]
+                                      : host_chain[ids[k]].data + 32 + 4ull * hd.num_first_ocur;
       std::vector<uint64_t> cnt((size_t)hd.chkpt_id + 1, 0);
       for (uint32_t i = 0; i < hd.num_prior_chkpts; i++) {
         const uint32_t ck = rd32(tb + 8ull * i), n = rd32(tb + 8ull * i + 4);

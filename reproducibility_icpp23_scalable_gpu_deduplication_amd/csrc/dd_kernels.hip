@@ -2257,6 +2257,41 @@ __device__ __forceinline__ void copy_round(const uint64_t* s_src, const uint32_t
   const bool pow2 = (ppc & (ppc - 1)) == 0;
   const uint32_t sh = pow2 ? __builtin_ctz(ppc) : 0;
   const uint32_t total = 256 * ppc;
+  if (PIECE == 16) {
+    // Four 16-byte pieces per lane loaded before any is stored (a load
+    // issued after a store waits for it).
+    GlobalPtr<uint8_t> o = (GlobalPtr<uint8_t>)out;
+    constexpr uint32_t U = 4;
+    for (uint32_t w0 = threadIdx.x; w0 < total; w0 += 256 * U) {
+      u32x4_t v[U];
+      uint64_t dst[U];
+      bool full[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        const uint32_t w = w0 + 256 * u;
+        full[u] = false;
+        dst[u] = 0;
+        if (w >= total) continue;
+        const uint32_t i = pow2 ? (w >> sh) : (w / ppc);
+        const uint64_t so = s_src[i];
+        if (so == ~0ull) continue;
+        const uint32_t pi = w - i * ppc;
+        dst[u] = (uint64_t)s_dst[i] * cs + (uint64_t)pi * 16;
+        const GlobalPtr<const uint8_t> in = (GlobalPtr<const uint8_t>)(data + so + (uint64_t)pi * 16);
+        if (dst[u] + 16 <= len) {
+          const u32x4a4 x = *(GlobalPtr<const u32x4a4>)in;  // 4-byte aligned source
+          v[u] = u32x4_t{x.x, x.y, x.z, x.w};
+          full[u] = true;
+        } else {
+          copy_chunk_piece<16>(out, dst[u], data + so + (uint64_t)pi * 16, len);  // the tail of the buffer
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++)
+        if (full[u]) *(GlobalPtr<u32x4_t>)(o + dst[u]) = v[u];
+    }
+    return;
+  }
   for (uint32_t w = threadIdx.x; w < total; w += 256) {
     const uint32_t i = pow2 ? (w >> sh) : (w / ppc);
     const uint64_t so = s_src[i];
