@@ -212,7 +212,7 @@ def run_batched(torch, dd, W, O, args, seed=1931):
     res = dict(buffers=nbuf, total_bytes=total, checkpoints=nckpt, chunk_size=chunk, steps=args.steps,
                gbs=total * args.steps / el / 1e9, ms_per_step=el * 1e3 / args.steps,
                plain_gbs=total * args.steps / el_plain / 1e9, plain_ms_per_step=el_plain * 1e3 / args.steps,
-               mismatches=mismatches, leaf_mapping=batch.leaf_mapping())
+               mismatches=mismatches, leaf_mapping=batch.leaf_mapping(), stage_budget=batch.stage_budget())
     lm = sum(leaf_ms) / len(leaf_ms)
     ab = sum(alg_leaf[k] for k in idx) / len(idx)
     res["leaf_ms"], res["leaf_alg_bytes"] = lm, ab
@@ -718,6 +718,7 @@ def main():
             "step": "one dd_batch_checkpoint_device call (hipGraph replay) over all buffers",
             "parallelism": "single GPU",
             "leaf_mapping": b["leaf_mapping"],
+            "stage_budget": b["stage_budget"],
         },
         "bit_exact": b["bit_exact"],
         "bit_exact_checks": {"batched_vs_plain_mismatches": b["mismatches"],

@@ -302,6 +302,15 @@ dd_status dd_get_ckpt_stats(dd_dedup* h, dd_ckpt_stats* out);
  * handle against its budget). */
 dd_status dd_get_device_bytes(dd_dedup* h, uint64_t* bytes);
 dd_status dd_batch_get_leaf_mapping(dd_batch* b, int* mapping);
+/* Workgroup budget (no counterpart in the reference) that the last
+ * checkpoint of a handle or batch gave each incremental stage over all its
+ * buffers: a stage whose per-buffer workgroups add up to more is scaled down
+ * to it.  Compared on the first incremental checkpoints of a kind of table
+ * (8192, 16384 or 4096 workgroups of 256 lanes, four times that for the
+ * tree-tile passes) and then fixed for the process; DD_STAGE_BUDGET forces
+ * one. */
+dd_status dd_get_stage_budget(dd_dedup* h, uint64_t* workgroups);
+dd_status dd_batch_get_stage_budget(dd_batch* b, uint64_t* workgroups);
 
 void dd_buf_free(dd_buf* b);
 const char* dd_strerror(int status);

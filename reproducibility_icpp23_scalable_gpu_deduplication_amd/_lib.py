@@ -30,7 +30,8 @@ EXPORTED = [
     "dd_batch_checkpoint_device", "dd_server_create", "dd_server_destroy", "dd_server_submit_checkpoint",
     "dd_server_submit_restart", "dd_server_wait", "dd_server_set_batching", "dd_server_usage", "dd_now",
     "dd_set_profiling",
-    "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_batch_set_profiling",
+    "dd_get_stage_times", "dd_get_leaf_mapping", "dd_batch_get_leaf_mapping", "dd_get_stage_budget",
+    "dd_batch_get_stage_budget", "dd_batch_set_profiling",
     "dd_batch_get_stage_times", "dd_get_ckpt_stats", "dd_get_device_bytes", "dd_buf_free", "dd_strerror",
 ]
 LEAF_MAPPINGS = ["direct", "deep", "lds64", "lds128", "lds256", "short", "dma", "dma1", "lds128p"]
@@ -112,6 +113,8 @@ def load():
         "dd_get_stage_times": ([vp, P(ctypes.c_double)], i32),
         "dd_get_leaf_mapping": ([vp, P(i32)], i32),
         "dd_batch_get_leaf_mapping": ([vp, P(i32)], i32),
+        "dd_get_stage_budget": ([vp, P(u64)], i32),
+        "dd_batch_get_stage_budget": ([vp, P(u64)], i32),
         "dd_batch_set_profiling": ([vp, i32], i32),
         "dd_batch_get_stage_times": ([vp, P(ctypes.c_double)], i32),
         "dd_get_ckpt_stats": ([vp, P(dd_ckpt_stats)], i32),
@@ -298,6 +301,12 @@ class TreeDeduplicator:
         _check(load().dd_get_leaf_mapping(self._h, ctypes.byref(m)), "dd_get_leaf_mapping")
         return LEAF_MAPPINGS[m.value]
 
+    def stage_budget(self):
+        """Batch-wide workgroup budget of the incremental stages at the last checkpoint."""
+        n = ctypes.c_uint64()
+        _check(load().dd_get_stage_budget(self._h, ctypes.byref(n)), "dd_get_stage_budget")
+        return n.value
+
     def timers(self):
         t = (ctypes.c_double * 4)()
         r = (ctypes.c_double * 2)()
@@ -347,6 +356,11 @@ class Batch:
         m = ctypes.c_int()
         _check(load().dd_batch_get_leaf_mapping(self._h, ctypes.byref(m)), "dd_batch_get_leaf_mapping")
         return LEAF_MAPPINGS[m.value]
+
+    def stage_budget(self):
+        n = ctypes.c_uint64()
+        _check(load().dd_batch_get_stage_budget(self._h, ctypes.byref(n)), "dd_batch_get_stage_budget")
+        return n.value
 
     def set_profiling(self, level=1):
         """0 off, 1 leaf-hash kernel events, 2 events around every stage."""

@@ -337,13 +337,14 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
 enum Stage { S_LEAF = 0, S_INSL, S_TILES, S_NODEBLK, S_STILE, S_GATHER, S_TILE0 };
 
 // Batch-wide workgroup budget of an incremental grid-stride stage of
-// 256-lane workgroups (DD_STAGE_BUDGET overrides it): 8192 is 32 per CU of
-// the 256 CUs, four rounds of the eight such workgroups a CU holds at once.
-static uint64_t stage_budget() {
+// 256-lane workgroups: DD_STAGE_BUDGET when set (0 otherwise), else the one
+// the tables of a tuning key settled on (stage_budget_for); 8192 is 32 per CU
+// of the 256 CUs, four rounds of the eight such workgroups a CU holds at once.
+static uint64_t forced_stage_budget() {
   static const uint64_t b = [] {
     const char* e = std::getenv("DD_STAGE_BUDGET");
     const long long v = e ? std::atoll(e) : 0;
-    return v > 0 ? (uint64_t)v : (uint64_t)8192;
+    return v > 0 ? (uint64_t)v : (uint64_t)0;
   }();
   return b;
 }
@@ -353,6 +354,8 @@ Pipeline::~Pipeline() {
   if (host_table_) (void)hipHostFree(host_table_);
   if (host_ctr) (void)hipHostFree(host_ctr);
   for (hipEvent_t e : trial_ev_)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : bev_)
     if (e) (void)hipEventDestroy(e);
 }
 
@@ -377,8 +380,10 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
     aligned16_ = aligned16_ && al;
     vec16_ = vec16_ && al;
   }
+  std::vector<uint64_t> key;
+  bool incremental = true;  // a baseline's launches are not comparable with an incremental's
+  for (const JobDev& J : jobs) incremental = incremental && !J.is_baseline;
   {
-    std::vector<uint64_t> key;
     std::vector<int> cand;
     leaf_map_ = leaf_mapping_for(jobs, aligned16_, hash_kind_, s, &key, &cand);
     trial_.current = -1;
@@ -386,8 +391,6 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
       trial_ = LeafTrial{};
     } else {
       if (trial_.key != key || trial_.cand != cand) trial_ = LeafTrial{key, cand, std::vector<std::vector<float>>(cand.size()), -1};
-      bool incremental = true;  // a baseline's leaf launch writes every digest: not comparable
-      for (const JobDev& J : jobs) incremental = incremental && !J.is_baseline;
       if (incremental) {
         size_t i = 0;
         for (size_t k = 1; k < cand.size(); k++)
@@ -415,31 +418,62 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   uint32_t* hprefix = reinterpret_cast<uint32_t*>(host_table_ + jobs_bytes);
   off_.assign(nstages, 0);
   grid_.assign(nstages, 0);
+  // Workgroups of stage st for job J before the batch-wide budget.
+  auto stage_wgs = [&](size_t st, const JobDev& J) -> uint64_t {
+    uint64_t n = 0;
+    switch (st) {
+      case S_LEAF: n = leaf_workgroups(leaf_map_, J.C); break;
+      // A baseline visits every leaf block and tile (one workgroup each);
+      // an incremental checkpoint strides over its work lists.
+      // The insert stages take one list entry per lane: a baseline lists
+      // (nearly) every leaf and interior node.
+      case S_INSL:
+      case S_TILES: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
+      case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kRootWG); break;
+      case S_STILE: n = std::min<uint64_t>(ceil_div(J.C, kSTile), 512); break;  // grid-stride over sort tiles
+      case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
+      default: {
+        const uint64_t tg = tiles_in_group(J.D, (int)st - S_TILE0);
+        n = J.is_baseline ? tg : std::min<uint64_t>(tg, kMaxTileWG);
+        break;
+      }
+    }
+    return n;
+  };
+  // The budget: forced, settled for the key, or (an incremental table whose
+  // stages the smallest candidate would cap, while the key is unsettled and
+  // no leaf-mapping trial runs) the candidate with the fewest timed launches.
+  budget_ = kBudgetCand[0];
+  btrial_.current = -1;
+  if (forced_stage_budget()) {
+    budget_ = forced_stage_budget();
+  } else if (!key.empty()) {
+    bool settled = true;
+    budget_ = stage_budget_for(key, &settled);
+    bool binds = false;
+    for (size_t st = 1; st < nstages && !binds; st++) {
+      uint64_t incr = 0;
+      for (const JobDev& J : jobs)
+        if (!J.is_baseline) incr += stage_wgs(st, J);
+      binds = incr > (st >= S_TILE0 ? 4 : 1) * kBudgetCand[kBudgetCands - 1];
+    }
+    if (!settled && incremental && binds && trial_.cand.empty()) {
+      if (btrial_.key != key) btrial_ = BudgetTrial{key, std::vector<std::vector<float>>(kBudgetCands), -1};
+      size_t i = 0;
+      for (size_t k = 1; k < kBudgetCands; k++)
+        if (btrial_.ms[k].size() < btrial_.ms[i].size()) i = k;
+      btrial_.current = (int)i;
+      budget_ = kBudgetCand[i];
+    }
+  }
   std::vector<uint64_t> nwg(nj);
-  const uint64_t budget = stage_budget();
+  const uint64_t budget = budget_;
   for (size_t st = 0; st < nstages; st++) {
     off_[st] = (uint32_t)(st * (nj + 1));
     uint64_t incr = 0;  // workgroups of the incremental jobs
     for (size_t j = 0; j < nj; j++) {
       const JobDev& J = jobs[j];
-      uint64_t n = 0;
-      switch (st) {
-        case S_LEAF: n = leaf_workgroups(leaf_map_, J.C); break;
-        // A baseline visits every leaf block and tile (one workgroup each);
-        // an incremental checkpoint strides over its work lists.
-        // The insert stages take one list entry per lane: a baseline lists
-        // (nearly) every leaf and interior node.
-        case S_INSL:
-        case S_TILES: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
-        case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kRootWG); break;
-        case S_STILE: n = std::min<uint64_t>(ceil_div(J.C, kSTile), 512); break;  // grid-stride over sort tiles
-        case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
-        default: {
-          const uint64_t tg = tiles_in_group(J.D, (int)st - S_TILE0);
-          n = J.is_baseline ? tg : std::min<uint64_t>(tg, kMaxTileWG);
-          break;
-        }
-      }
+      const uint64_t n = stage_wgs(st, J);
       nwg[j] = n;
       if (!J.is_baseline && st != S_LEAF) incr += n;
     }
@@ -494,6 +528,33 @@ std::vector<uint64_t> Pipeline::signature() const {
 }
 
 void Pipeline::trial_done() {
+  if (btrial_.current >= 0) {
+    float ms = 0;
+    const size_t i = (size_t)btrial_.current;
+    btrial_.current = -1;
+    if (btrial_end_ && hipEventElapsedTime(&ms, bev_[0], bev_[1]) == hipSuccess && ms > 0) {
+      btrial_.ms[i].push_back(ms);
+      bool all = true;
+      for (const auto& v : btrial_.ms) all = all && v.size() >= kTrialRounds;
+      if (all) {
+        size_t best = 0;
+        float best_ms = 0;
+        for (size_t k = 0; k < kBudgetCands; k++) {
+          std::vector<float> v = btrial_.ms[k];
+          std::sort(v.begin(), v.end());
+          if (k == 0 || v[v.size() / 2] < best_ms) {
+            best_ms = v[v.size() / 2];
+            best = k;
+          }
+        }
+        stage_budget_settle(btrial_.key, kBudgetCand[best]);
+        btrial_ = BudgetTrial{};
+      }
+    } else {
+      (void)hipGetLastError();
+    }
+    btrial_end_ = false;
+  }
   if (trial_.current < 0) return;
   float ms = 0;
   const size_t i = (size_t)trial_.current;
@@ -540,6 +601,13 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   };
   const bool leaf_events = stage_ev && profile_level_ == 1;
   if (part != PART_TAIL) {
+    if (rerun) btrial_.current = -1;  // a re-run's launches are not a whole checkpoint
+    if (btrial_.current >= 0) {
+      for (hipEvent_t& e : bev_)
+        if (!e) DD_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+      DD_HIP(hipEventRecord(bev_[0], s));
+      btrial_end_ = false;
+    }
     if (rerun || begin) launch_begin(J, nullptr, nj, rerun, s);
     DD_TRY(mark(0));
     if (rerun) trial_.current = -1;  // no leaf kernel in a re-run
@@ -586,7 +654,16 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
   DD_TRY(mark(6));
   launch_gather(J, P(S_GATHER), nj, grid_[S_GATHER], vec16_, s);
   DD_TRY(mark(7));
+  if (part == PART_ALL) DD_TRY(trial_mark_end(s));
   DD_HIP(hipGetLastError());
+  return DD_OK;
+}
+
+dd_status Pipeline::trial_mark_end(hipStream_t s) {
+  if (btrial_.current >= 0 && bev_[1]) {
+    DD_HIP(hipEventRecord(bev_[1], s));
+    btrial_end_ = true;
+  }
   return DD_OK;
 }
 
@@ -1424,6 +1501,7 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
       b->sig = sig;
     }
     DD_HIP(hipGraphLaunch(b->exec, b->stream));
+    DD_TRY(b->pipe.trial_mark_end(b->stream));
   } else {
     DD_TRY(b->pipe.launch_all(b->stream, false, begin, nullptr, pev));
   }
@@ -1472,6 +1550,18 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
 dd_status dd_get_leaf_mapping(dd_dedup* h, int* mapping) {
   if (!h || !mapping) return DD_EINVAL;
   *mapping = h->pipe.leaf_mapping();
+  return DD_OK;
+}
+
+dd_status dd_get_stage_budget(dd_dedup* h, uint64_t* workgroups) {
+  if (!h || !workgroups) return DD_EINVAL;
+  *workgroups = h->pipe.stage_budget();
+  return DD_OK;
+}
+
+dd_status dd_batch_get_stage_budget(dd_batch* b, uint64_t* workgroups) {
+  if (!b || !workgroups) return DD_EINVAL;
+  *workgroups = b->pipe.stage_budget();
   return DD_OK;
 }
 

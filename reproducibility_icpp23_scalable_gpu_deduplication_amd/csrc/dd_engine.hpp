@@ -143,13 +143,29 @@ class Pipeline {
   hipEvent_t trial_ev_[2] = {nullptr, nullptr};
   hipEvent_t trial_used_[2] = {nullptr, nullptr};  // the events on the leaf dispatch in flight
   static constexpr size_t kTrialRounds = 3;
+  // The same for the incremental stages' batch-wide workgroup budget, once
+  // the leaf mapping is settled: each candidate's whole-checkpoint time
+  // (events before the first and after the last launch).
+  struct BudgetTrial {
+    std::vector<uint64_t> key;
+    std::vector<std::vector<float>> ms;
+    int current = -1;
+  } btrial_;
+  hipEvent_t bev_[2] = {nullptr, nullptr};
+  bool btrial_end_ = false;  // the end event of the launch in flight is recorded
+  uint64_t budget_ = 8192;
 
  public:
   int leaf_mapping() const { return leaf_map_; }
   uint64_t device_bytes() const { return dev_table_.bytes; }
   // After the stream has completed a launch_all: records the trial launch's
-  // leaf time and settles the mapping once every candidate has its rounds.
+  // leaf time and settles the mapping once every candidate has its rounds
+  // (then the same for the stage budget).
   void trial_done();
+  // Records the end of a checkpoint's launches for a budget trial; launch_all
+  // does it itself for PART_ALL, the caller after a graph's launch.
+  dd_status trial_mark_end(hipStream_t s);
+  uint64_t stage_budget() const { return budget_; }
 };
 
 uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
@@ -164,6 +180,13 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool aligned16, int hash_k
                      std::vector<uint64_t>* key = nullptr, std::vector<int>* cand = nullptr);
 // Settles a key's mapping after its comparison on checkpoints.
 void leaf_mapping_settle(const std::vector<uint64_t>& key, int mapping);
+// Candidate batch-wide workgroup budgets of the incremental stages, the
+// default first.
+constexpr uint64_t kBudgetCand[] = {8192, 16384, 4096};
+constexpr size_t kBudgetCands = sizeof(kBudgetCand) / sizeof(kBudgetCand[0]);
+// The budget a key settled on (the default while `*settled` is false).
+uint64_t stage_budget_for(const std::vector<uint64_t>& key, bool* settled);
+void stage_budget_settle(const std::vector<uint64_t>& key, uint64_t budget);
 
 }  // namespace dd
 

@@ -297,6 +297,21 @@ int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipSt
   return publish(best, ok && best_ms >= 0, cand);
 }
 
+// Stage budgets settled per key (absent: still to be compared).
+std::map<Key, uint64_t> g_budget;
+
+uint64_t stage_budget_for(const std::vector<uint64_t>& key, bool* settled) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_budget.find(key_of(key));
+  *settled = it != g_budget.end();
+  return *settled ? it->second : kBudgetCand[0];
+}
+
+void stage_budget_settle(const std::vector<uint64_t>& key, uint64_t b) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_budget.emplace(key_of(key), b);
+}
+
 void leaf_mapping_settle(const std::vector<uint64_t>& key, int m) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_choice.find(key_of(key));

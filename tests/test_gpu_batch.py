@@ -135,3 +135,46 @@ def test_batch_equals_plain_path(torch_gpu):
             t = torch.from_numpy(x).to("cuda")
             torch.cuda.synchronize()
             assert g.checkpoint(t.data_ptr(), t.numel(), k == 0) == bdiffs[i][k]
+
+
+def test_batch_tuning_trials_keep_bytes(torch_gpu):
+    # 64 buffers of 4 MiB: the incremental stages' workgroups add up past the
+    # smallest candidate budget, so the first incremental calls compare leaf
+    # mappings and then stage budgets (timed on the calls themselves) before
+    # both settle.  Every call's diffs equal the plain path's throughout, and
+    # the settled budget is one of the candidates and stays.
+    torch = torch_gpu
+    nb, size, cs, nk = 64, (4 << 20) + 1024, 128, 26
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(77)
+    versions = [torch.randint(0, 256, (nb, size), dtype=torch.uint8, device="cuda", generator=gen)]
+    for v in range(1, 4):  # each version changes 2 % of the chunks of every buffer
+        x = versions[-1].clone()
+        C = size // cs
+        idx = torch.randint(0, C, (nb, C // 50), device="cuda", generator=gen)
+        rows = torch.arange(nb, device="cuda").unsqueeze(1).expand_as(idx)
+        x.view(nb, -1, cs)[rows, idx] = torch.randint(0, 256, (cs,), dtype=torch.uint8, device="cuda", generator=gen)
+        versions.append(x)
+    cap = dd.max_diff_size(size, cs, nk + 1)
+    outs_b = torch.empty((nb, cap), dtype=torch.uint8, device="cuda")
+    outs_p = torch.empty((nb, cap), dtype=torch.uint8, device="cuda")
+    hb = [dd.TreeDeduplicator(cs) for _ in range(nb)]
+    hp = [dd.TreeDeduplicator(cs) for _ in range(nb)]
+    batch = dd.Batch(hb, graph=True)
+    torch.cuda.synchronize()
+    budgets = []
+    for k in range(nk):
+        v = versions[k % len(versions)]
+        ptrs = [v[i].data_ptr() for i in range(nb)]
+        lb = batch.checkpoint_device(ptrs, [size] * nb, [k == 0] * nb, [outs_b[i].data_ptr() for i in range(nb)],
+                                     [cap] * nb)
+        budgets.append(batch.stage_budget())
+        for i in range(nb):
+            lp = hp[i].checkpoint_device(ptrs[i], size, outs_p[i].data_ptr(), cap, k == 0)
+            assert lp == lb[i], (k, i)
+        torch.cuda.synchronize()
+        for i in range(nb):
+            assert torch.equal(outs_b[i, :lb[i]], outs_p[i, :lb[i]]), (k, i)
+    assert set(budgets) <= {4096, 8192, 16384}
+    assert len(set(budgets[-4:])) == 1, budgets
+    batch.close()
