@@ -102,6 +102,11 @@ dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void*
  * like any other (src/tree_approach.cpp:887-925).  Default 0: the
  * reference's bytes. */
 #define DD_OPT_EMIT_ROOT_SHIFT 1u
+/* DD_OPT_DENSE_BASELINE: a baseline builds its digest map with one map
+ * insert per leaf and per first-occurrence interior node, as an incremental
+ * checkpoint does, instead of partitioning the entries by map bucket and
+ * writing each bucket once.  The diff bytes are the same either way. */
+#define DD_OPT_DENSE_BASELINE 2u
 dd_status dd_set_options(dd_dedup* h, uint32_t options);
 dd_status dd_get_options(dd_dedup* h, uint32_t* options);
 /* The method a handle was created with. */

@@ -18,6 +18,7 @@ NOALLOC_MAX_CHUNKS = 10 << 20
 HASH_MURMUR3 = 0
 HASH_MD5 = 1
 OPT_EMIT_ROOT_SHIFT = 1
+OPT_DENSE_BASELINE = 2
 METHOD_TREE = 0
 METHOD_LIST = 1
 METHOD_TREE_LOW_ROOT = 2
@@ -160,7 +161,7 @@ class TreeDeduplicator:
 
     @property
     def options(self):
-        """DD_OPT_* flags (OPT_EMIT_ROOT_SHIFT)."""
+        """DD_OPT_* flags (OPT_EMIT_ROOT_SHIFT, OPT_DENSE_BASELINE)."""
         o = ctypes.c_uint32()
         _check(load().dd_get_options(self._h, ctypes.byref(o)), "dd_get_options")
         return o.value

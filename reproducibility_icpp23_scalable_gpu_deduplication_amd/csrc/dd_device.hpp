@@ -25,7 +25,19 @@ enum : int { HASH_MURMUR3 = 0, HASH_MD5 = 1 };
 
 // Map / pipeline status flags.
 // FLAG_BAD_DIFF: a restart's parse found a diff inconsistent.
-enum : uint32_t { FLAG_NEED_GROW = 1u, FLAG_MAP_FULL = 2u, FLAG_BAD_SOURCE = 4u, FLAG_BAD_DIFF = 8u };
+// FLAG_BULK_OFF: a baseline's bulk map build found a bucket with more leaves
+// than its workgroup holds; the job's map is built by the dense inserts.
+// FLAG_BULK_DUPS: a bulk build without the leaf duplicate search found a
+// repeated leaf digest (raised with FLAG_MAP_FULL, which stops the later
+// stages); the host runs the job again with the search.
+enum : uint32_t {
+  FLAG_NEED_GROW = 1u,
+  FLAG_MAP_FULL = 2u,
+  FLAG_BAD_SOURCE = 4u,
+  FLAG_BAD_DIFF = 8u,
+  FLAG_BULK_OFF = 16u,
+  FLAG_BULK_DUPS = 32u
+};
 
 struct Digest {
   uint64_t k0, k1;
@@ -502,10 +514,19 @@ __host__ __device__ inline uint64_t run_leaf(const LeafRuns& r, uint64_t idx) {
 // ---------------------------------------------------------------------------
 // Digest map probing.
 // ---------------------------------------------------------------------------
+// The map is split into buckets of kBucketSlots consecutive slots (the whole
+// map when it is smaller): a probe sequence starts at the home slot and wraps
+// around inside the home slot's bucket, so each bucket is an independent
+// linear-probing table.  A baseline builds every bucket at once in LDS and
+// writes it out whole (k_bulk_build).
+constexpr uint32_t kBucketLog = 12;
+constexpr uint32_t kBucketSlots = 1u << kBucketLog;
+
 template <template <class> class P>
 struct MapT {
   P<Slot> slots;
-  uint64_t mask;
+  uint64_t mask;   // capacity - 1
+  uint64_t bmask;  // slots per bucket - 1
   uint32_t shift;  // 64 - log2(capacity)
 };
 using MapView = MapT<HostPtr>;  // as the host fills it
@@ -515,6 +536,12 @@ static_assert(sizeof(MapView) == sizeof(MapG), "one layout for both pointer kind
 template <class M>
 __host__ __device__ inline uint64_t map_start(const M& m, const Digest& k) {
   return ((k.k0 ^ (k.k1 * 0x9e3779b97f4a7c15ull)) * 0xff51afd7ed558ccdull) >> m.shift;
+}
+
+// Slot e places after s in s's bucket (wrapping inside the bucket).
+template <class M>
+__host__ __device__ inline uint64_t map_next(const M& m, uint64_t s, uint64_t e) {
+  return (s & ~m.bmask) | ((s + e) & m.bmask);
 }
 
 }  // namespace dd

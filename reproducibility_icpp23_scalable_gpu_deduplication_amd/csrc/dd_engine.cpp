@@ -4,6 +4,7 @@
 // (src/tree_approach.cpp:1577-1689) and restart_chkpt (:712-1128).
 #include "dd_engine.hpp"
 
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
@@ -61,7 +62,7 @@ uint64_t resident_bytes(const dd_dedup* h) {
   const StreamState& st = h->st;
   const DevBuf* bufs[] = {&st.tree, &st.labels, &st.slot_of, &st.lr_new, &st.map, &st.blk, &st.f_list, &st.f_off, &st.s_node,
                           &st.s_prev, &st.s_tree, &st.ss_node, &st.ss_prev, &st.s_hist, &st.ptab, &st.leafblk,
-                          &st.pend_list, &st.pend_dig, &st.tiles, &st.tile_epoch, &st.lists, &st.diff, &st.ctr,
+                          &st.pend_list, &st.pend_dig, &st.tiles, &st.tile_epoch, &st.lists, &st.diff, &st.ctr, &st.bulk,
                           &h->cover, &h->ref, &h->pend, &h->rs_hdr, &h->rs_arena};
   uint64_t n = h->pipe.device_bytes();
   for (const DevBuf* b : bufs) n += b->bytes;
@@ -79,6 +80,7 @@ static MapView make_view(void* slots, uint64_t cap) {
   MapView m;
   m.slots = reinterpret_cast<Slot*>(slots);
   m.mask = cap - 1;
+  m.bmask = std::min<uint64_t>(cap, kBucketSlots) - 1;
   m.shift = 64 - ilog2_64(cap);
   return m;
 }
@@ -133,6 +135,61 @@ dd_status StreamState::grow_map(uint64_t need, hipStream_t s) {
   return DD_OK;
 }
 
+// Leaf records per bucket above which a bulk job falls back to the dense
+// inserts (a bucket holds 1024 leaves on average at most), and records per
+// bucket the final build takes (leaves and interior nodes, 2048 on average
+// at most).  DD_BULK_CAPS="leaf,build" overrides both (at most kBulkRec).
+static void bulk_caps(uint32_t* cap_l, uint32_t* cap_b) {
+  *cap_l = 1600;
+  *cap_b = kBulkRec;
+  if (const char* e = std::getenv("DD_BULK_CAPS")) {
+    unsigned a = 0, b = 0;
+    if (sscanf(e, "%u,%u", &a, &b) == 2) {
+      *cap_l = std::min<uint32_t>(a, kBulkRec);
+      *cap_b = std::min<uint32_t>(b, kBulkRec);
+    }
+  }
+}
+
+bool StreamState::uses_bulk(uint64_t Cn, uint64_t cap) const {
+  if (method != DD_METHOD_TREE || (options & DD_OPT_DENSE_BASELINE)) return false;
+  return (cap >> kBucketLog) <= kBulkMaxBuckets && ceil_div(Cn, kPartTile) <= kBulkMaxTiles;
+}
+
+// Sizes the bulk build's arrays.  None needs clearing: the partition
+// kernels write every count they later read.
+dd_status StreamState::prepare_bulk(uint64_t Cn, uint64_t cap, hipStream_t) {
+  const uint64_t nb = std::max<uint64_t>(cap >> kBucketLog, 1);
+  const uint64_t nt = ceil_div(Cn, kPartTile);
+  const uint64_t ng = ceil_div(nb, kColBuckets);
+  const uint64_t nrec = 2 * Cn - 1;
+  auto a256 = [](uint64_t x) { return (x + 255) & ~255ull; };
+  bk_off[0] = 0;                                         // count matrices: leaves, interior nodes
+  bk_off[1] = bk_off[0] + a256(2 * nt * nb * 4);         // bucket totals
+  bk_off[2] = bk_off[1] + a256(2 * nb * 4);              // group totals
+  bk_off[3] = bk_off[2] + a256(2 * ng * 4);              // entries per bucket
+  bk_off[4] = bk_off[3] + a256(nb * 4);                  // records
+  const uint64_t total = bk_off[4] + nrec * 32;
+  DD_TRY(bulk.ensure(total));
+  bk_nb = nb;
+  bk_nt = nt;
+  return DD_OK;
+}
+
+void StreamState::fill_bulk(JobDev& job) const {
+  uint8_t* base = bulk.as<uint8_t>();
+  job.bulk = 1;
+  job.nbuckets = (uint32_t)bk_nb;
+  job.ntiles_b = (uint32_t)bk_nt;
+  bulk_caps(&job.bk_cap_l, &job.bk_cap_b);
+  job.bk_mat = reinterpret_cast<uint32_t*>(base + bk_off[0]);
+  job.bk_tot = reinterpret_cast<uint32_t*>(base + bk_off[1]);
+  job.bk_grp = reinterpret_cast<uint32_t*>(base + bk_off[2]);
+  job.bk_claim = reinterpret_cast<uint32_t*>(base + bk_off[3]);
+  job.bk_rec = reinterpret_cast<u32x4_t*>(base + bk_off[4]);
+  job.bulk_dedup = bulk_dups ? 1 : 0;
+}
+
 dd_status StreamState::validate(uint64_t len, bool baseline, const uint8_t* out, uint64_t out_cap) const {
   const uint64_t Cn = ceil_div(len, chunk_size);
   if (len == 0 || Cn > 0x7fffffffull) return DD_EINVAL;
@@ -161,7 +218,11 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     if (map.bytes > 4 * cap * sizeof(Slot)) map.release();
     DD_TRY(map.ensure(cap * sizeof(Slot)));
     map_cap = cap;
-    launch_map_clear(map.as<Slot>(), cap, s);
+    // The bulk build writes every slot (or clears its buckets when it falls
+    // back to the dense inserts); the dense inserts need a cleared map.
+    bulk_now = uses_bulk(Cn, cap);
+    if (bulk_now) DD_TRY(prepare_bulk(Cn, cap, s));
+    else launch_map_clear(map.as<Slot>(), cap, s);
     map_count = 0;
     have_tree = true;
     data_len = len;
@@ -253,6 +314,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   static const char* rounds = std::getenv("DD_INSERT_ROUNDS");
   if (!baseline && rounds) job.wide_probe = strcmp(rounds, "wide") == 0 ? 1 : 0;
   job.root_shift = (options & DD_OPT_EMIT_ROOT_SHIFT) ? 1 : 0;
+  if (baseline && bulk_now) fill_bulk(job);
   job.low_root = method == DD_METHOD_TREE_LOW_ROOT ? 1 : 0;
   job.lr_new = job.low_root ? lr_new.as<uint8_t>() : nullptr;
   labels_fixed = false;  // until this checkpoint completes
@@ -334,7 +396,9 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
 // ---------------------------------------------------------------------------
 // Pipeline
 // ---------------------------------------------------------------------------
-enum Stage { S_LEAF = 0, S_INSL, S_TILES, S_NODEBLK, S_STILE, S_GATHER, S_TILE0 };
+// S_BULK: partition tiles of the bulk build, S_BSCAN: its column-scan groups,
+// S_BUCKET: its map buckets (none for jobs without the bulk build).
+enum Stage { S_LEAF = 0, S_INSL, S_TILES, S_NODEBLK, S_STILE, S_GATHER, S_BULK, S_BSCAN, S_BUCKET, S_TILE0 };
 
 // Batch-wide workgroup budget of an incremental grid-stride stage of
 // 256-lane workgroups: DD_STAGE_BUDGET when set (0 otherwise), else the one
@@ -427,8 +491,16 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
       // an incremental checkpoint strides over its work lists.
       // The insert stages take one list entry per lane: a baseline lists
       // (nearly) every leaf and interior node.
+      // A bulk baseline runs the dense inserts only when it falls back, over
+      // fewer grid-stride workgroups.
       case S_INSL:
-      case S_TILES: n = J.is_baseline ? ceil_div(J.C, 256) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG); break;
+      case S_TILES:
+        n = J.is_baseline ? (J.bulk ? std::min<uint64_t>(ceil_div(J.C, 256), 2048) : ceil_div(J.C, 256))
+                          : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG);
+        break;
+      case S_BULK: n = J.bulk ? J.ntiles_b : 0; break;
+      case S_BSCAN: n = J.bulk ? ceil_div(J.nbuckets, kColBuckets) : 0; break;
+      case S_BUCKET: n = J.bulk ? J.nbuckets : 0; break;
       case S_NODEBLK: n = std::min<uint64_t>(J.nblk, kRootWG); break;
       case S_STILE: n = std::min<uint64_t>(ceil_div(J.C, kSTile), 512); break;  // grid-stride over sort tiles
       case S_GATHER: n = std::min<uint64_t>(ceil_div(J.C, 256), 4096); break;
@@ -629,6 +701,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
       return DD_OK;
     }
   }
+  launch_bulk(J, nj, false, P(S_BULK), grid_[S_BULK], P(S_BSCAN), grid_[S_BSCAN], P(S_BUCKET), grid_[S_BUCKET], s);
   launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], kinds_, s);
   if (low_root_) launch_low_root(J, nj, max_c_, (uint32_t)ilog2_64(2 * max_c_ - 1), s);
   DD_TRY(mark(2));
@@ -641,6 +714,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
     for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
     DD_TRY(mark(3));
     launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], kinds_, s);
+    launch_bulk(J, nj, true, P(S_BULK), grid_[S_BULK], P(S_BSCAN), grid_[S_BSCAN], P(S_BUCKET), grid_[S_BUCKET], s);
     DD_TRY(mark(4));
     for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
   }
@@ -685,6 +759,29 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A bulk baseline run without the leaf duplicate search that found a repeated
+// leaf digest runs again from the leaf kernel with the search (and so do the
+// handle's later baselines).
+dd_status rerun_bulk_dedup(StreamState& st, JobDev& job, Pipeline& p, hipStream_t s, JobCounters* c) {
+  st.bulk_dups = true;
+  job.bulk_dedup = 1;
+  DD_TRY(p.run(std::vector<JobDev>(1, job), s, false, true));
+  DD_TRY(p.fetch_counters(std::vector<JobDev>(1, job), s));
+  *c = p.host_ctr[0];
+  return DD_OK;
+}
+
+// A bulk baseline whose final build did not fit one of its buckets runs again
+// from the leaf kernel with the dense inserts, on a cleared map.
+dd_status rerun_dense_baseline(StreamState& st, JobDev& job, Pipeline& p, hipStream_t s, JobCounters* c) {
+  job.bulk = 0;
+  launch_map_clear(st.map.as<Slot>(), st.map_cap, s);
+  DD_TRY(p.run(std::vector<JobDev>(1, job), s, false, true));
+  DD_TRY(p.fetch_counters(std::vector<JobDev>(1, job), s));
+  *c = p.host_ctr[0];
+  return DD_OK;
+}
+
 // One checkpoint of a handle: runs the pipeline, grows the map and re-runs the
 // inserts when the map would pass its load limit, and commits the state.
 dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline, uint8_t* out,
@@ -715,6 +812,14 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
   DD_TRY(h->pipe.fetch_counters(jobs, h->stream));
   h->pipe.trial_done();
   JobCounters c = h->pipe.host_ctr[0];
+  if (jobs[0].bulk && (c.flags & FLAG_BULK_DUPS)) {
+    DD_TRY(rerun_bulk_dedup(st, jobs[0], h->pipe, h->stream, &c));
+    DD_HIP(hipEventRecord(h->ev[2], h->stream));
+  }
+  if (jobs[0].bulk && (c.flags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
+    DD_TRY(rerun_dense_baseline(st, jobs[0], h->pipe, h->stream, &c));
+    DD_HIP(hipEventRecord(h->ev[2], h->stream));
+  }
   for (int attempt = 0; (c.flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) && attempt < 4; attempt++) {
     const uint64_t need = c.map_count + 2 * c.pending;
     DD_TRY(st.grow_map(need, h->stream));
@@ -1261,7 +1366,7 @@ dd_status dd_create_method(uint32_t chunk_size, int hash_kind, int method, void*
 }
 
 dd_status dd_set_options(dd_dedup* h, uint32_t options) {
-  if (!h || (options & ~(uint32_t)DD_OPT_EMIT_ROOT_SHIFT)) return DD_EINVAL;
+  if (!h || (options & ~(uint32_t)(DD_OPT_EMIT_ROOT_SHIFT | DD_OPT_DENSE_BASELINE))) return DD_EINVAL;
   h->st.options = options;
   return DD_OK;
 }
@@ -1515,6 +1620,12 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
     }
   }
   std::vector<JobCounters> c(b->pipe.host_ctr, b->pipe.host_ctr + n);
+  for (size_t i = 0; i < n; i++) {
+    if (jobs[i].bulk && (c[i].flags & FLAG_BULK_DUPS))
+      DD_TRY(rerun_bulk_dedup(b->hs[i]->st, jobs[i], b->pipe_rerun, b->stream, &c[i]));
+    if (jobs[i].bulk && (c[i].flags & (FLAG_NEED_GROW | FLAG_MAP_FULL)))
+      DD_TRY(rerun_dense_baseline(b->hs[i]->st, jobs[i], b->pipe_rerun, b->stream, &c[i]));
+  }
   for (int attempt = 0; attempt < 4; attempt++) {
     std::vector<size_t> redo;
     for (size_t i = 0; i < n; i++)

@@ -56,6 +56,14 @@ struct StreamState {
   uint64_t tile_epoch_n = 0;
   DevBuf diff;  // internal diff buffer (host-output path)
   DevBuf ctr;   // JobCounters
+  // Bulk map build of a baseline: count matrices, bucket and group totals,
+  // entry counts and the records (digest, node) of the leaves and interior
+  // nodes.
+  DevBuf bulk;
+  uint64_t bk_off[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t bk_nb = 0, bk_nt = 0;
+  bool bulk_now = false;  // the baseline being prepared uses the bulk build
+  bool bulk_dups = false;  // a baseline of this handle had a repeated leaf digest
 
   uint64_t nfirst = 0, nshift = 0, last_diff_len = 0;
   dd_ckpt_stats stats{};  // of the last completed checkpoint
@@ -79,6 +87,11 @@ struct StreamState {
   void commit(const JobCounters& c, bool baseline);
   // Grows the map so that `need` entries stay under the load limit.
   dd_status grow_map(uint64_t need, hipStream_t s);
+  // Bulk map build: whether a baseline of Cn chunks and map capacity `cap`
+  // uses it, its arrays, and its job fields.
+  bool uses_bulk(uint64_t Cn, uint64_t cap) const;
+  dd_status prepare_bulk(uint64_t Cn, uint64_t cap, hipStream_t s);
+  void fill_bulk(JobDev& job) const;
 };
 
 // Runs the checkpoint kernels over a table of jobs.  Owns the device copies

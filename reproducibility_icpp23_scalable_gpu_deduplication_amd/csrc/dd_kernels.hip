@@ -658,24 +658,25 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
   uint64_t s = map_start(m, k) & m.mask;
   InsertResult r;
   const Digest none{0, 0};
-  uint64_t probed = 0;  // slots examined (bounds the walk of a full map)
-  while (probed <= m.mask) {
+  uint64_t probed = 0;  // slots examined (bounds the walk of a full bucket)
+  while (probed <= m.bmask) {
     uint64_t cand = ~0ull;  // EMPTY slot to claim
-    // Round: slots base+lo .. base+hi-1 (the capacity is a power of two
-    // >= 1024, so base+e wraps with the mask).
+    // Round: slots base+lo .. base+hi-1, wrapping inside the bucket (a
+    // bucket holds at least 1024 slots, so a pair or a line never straddles
+    // its end).
     uint64_t base = Wide ? (s & ~(uint64_t)(kLineSlots - 1)) : (s & ~1ull);
     uint32_t lo = Wide ? (uint32_t)(s & (kLineSlots - 1)) : (uint32_t)(s & 1);
     uint32_t hi = Wide ? kLineSlots : RW;
-    for (; probed <= m.mask; probed += hi - lo, base = (base + hi) & m.mask, lo = 0, hi = RW) {
+    for (; probed <= m.bmask; probed += hi - lo, base = map_next(m, base, hi), lo = 0, hi = RW) {
       uint4 kk[RW];
       unsigned long long vv[RW];
 #pragma unroll
       for (uint32_t e = 0; e < RW; e++)
-        if (e >= lo && e < hi) load_slot(m, (base + e) & m.mask, kk[e], vv[e]);
+        if (e >= lo && e < hi) load_slot(m, map_next(m, base, e), kk[e], vv[e]);
 #pragma unroll
       for (uint32_t e = 0; e < RW; e++) {
         if (e < lo || e >= hi) continue;
-        const uint64_t se = (base + e) & m.mask;
+        const uint64_t se = map_next(m, base, e);
         if (vv[e] == SLOT_EMPTY) {
           cand = se;
           break;
@@ -690,7 +691,7 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
     if (v == SLOT_EMPTY) return InsertResult{true, true, cand, SLOT_EMPTY};
     // The loaded key predates the claim: settle with the slot's key.
     if (map_settle<Leaf>(J, k, cand, v, none, false, mine, r)) return r;
-    s = (cand + 1) & m.mask;
+    s = map_next(m, cand, 1);
     probed += 1;
   }
   return InsertResult{false, false, 0, 0};
@@ -749,7 +750,8 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
   } else {
     __syncthreads();
   }
-  if (sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) {
+  // A bulk job's map is built by k_bulk_build unless it fell back.
+  if ((sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) || (dense && J.bulk && !(sflags & FLAG_BULK_OFF))) {
     if (threadIdx.x == 0) J.blk_claim[wg] = 0;
     return;
   }
@@ -1013,7 +1015,9 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
   if (threadIdx.x == 0) sflags = J.ctr->flags;
   if (Mode != JOBS_DENSE) load_list_prefix(J.dirty_cnt, pre);
   else __syncthreads();
-  if (!(sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL))) {
+  // A bulk job's map is built by k_bulk_build unless it fell back.
+  const bool bulk_built = Mode == JOBS_DENSE && J.bulk && !(sflags & FLAG_BULK_OFF);
+  if (!(sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) && !bulk_built) {
     const InsertCtx ic = insert_ctx(J);
     const GlobalPtr<uint32_t> flags = &J.ctr->flags;
     if (Mode == JOBS_DENSE) {
@@ -1052,6 +1056,401 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
   }
   const uint32_t n = block_sum<uint32_t>(claims, sh);
   if (threadIdx.x == 0) J.int_claim[wg] = n;
+}
+
+// ---------------------------------------------------------------------------
+// Bulk map build of a baseline.  A baseline starts from an empty map, so
+// instead of a clear of the whole map and one random read-modify-write of a
+// 128-byte line per insert, its entries are partitioned by map bucket and
+// every bucket is built in LDS and written once, its slots in order.  Per
+// part (the leaves after the leaf kernel; the first-occurrence interior
+// nodes after pass 1):
+// - k_bulk_hist: each partition workgroup counts its tile's records per
+//   bucket in LDS and writes its row of the (tile x bucket) count matrix;
+// - k_bulk_colscan: per bucket, exclusive prefixes down the tiles (in place),
+//   the bucket's total, and totals per group of kColBuckets buckets;
+// - k_bulk_scatter: each partition workgroup turns its row into LDS cursors
+//   (bucket start + its prefix) and writes every record (digest, node) at a
+//   cursor bumped with an LDS atomic.  No global atomics: device-scope
+//   atomics from the eight XCDs meet at memory, and a count with one per
+//   record took longer than the random-line inserts it replaces.
+// - k_bulk_build, one workgroup per bucket: the bucket's records go into LDS
+//   and are inserted into an LDS image of the bucket with the map's own
+//   probing (home slot, then the next slots, wrapping inside the bucket).
+//   Equal digests share a slot, whose entry is the one the dense inserts
+//   keep: the lowest leaf, and among interior nodes the highest
+//   (include/reference_impl.hpp:49-82).  The leaf part labels every leaf of
+//   a digest but the lowest SHIFT (pass 1 needs the leaf labels); the final
+//   part writes the bucket's slots and the map slot of each such leaf.
+// A bucket with more leaf records than a workgroup holds (a digest repeated
+// thousands of times, as zero chunks are) switches its job to the dense
+// inserts (FLAG_BULK_OFF) before anything is written; a final bucket that
+// does not fit raises FLAG_MAP_FULL, and the host runs the job again without
+// the bulk build.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t bulk_bucket(const MapG& m, const Digest& d) {
+  return (uint32_t)((map_start(m, d) & m.mask) >> kBucketLog);
+}
+
+__device__ __forceinline__ bool bulk_skip(const JobG& J, uint32_t flags) {
+  return !J.bulk || (flags & (FLAG_BULK_OFF | FLAG_NEED_GROW | FLAG_MAP_FULL | FLAG_BULK_DUPS));
+}
+
+// Batch `b` of a partition lane: items w * kPartTile + (b * kPartBatch + k) *
+// kPartThreads + lane (k < kPartBatch) of the part, consecutive lanes on
+// consecutive items.  Bit k of the result: item k is a record (every leaf;
+// the first-occurrence interior nodes), with its digest in d[k].  The labels
+// and then the digests are loaded together.
+constexpr uint32_t kPartBatch = 8;
+template <bool Int>
+__device__ __forceinline__ uint32_t bulk_load(const JobG& J, uint32_t w, uint32_t b, Digest d[kPartBatch],
+                                              uint64_t item[kPartBatch]) {
+  const uint64_t n = Int ? J.C - 1 : J.C;
+  const uint64_t node0 = Int ? 0 : J.C - 1;
+  uint8_t lab[kPartBatch];
+#pragma unroll
+  for (uint32_t k = 0; k < kPartBatch; k++) {
+    item[k] = (uint64_t)w * kPartTile + (uint64_t)(b * kPartBatch + k) * kPartThreads + threadIdx.x;
+    lab[k] = FIRST_OCUR;
+    if (Int) lab[k] = J.labels[item[k] < n ? item[k] : 0];
+  }
+  uint32_t valid = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kPartBatch; k++)
+    valid |= (item[k] < n && lab[k] == FIRST_OCUR ? 1u : 0u) << k;
+#pragma unroll
+  for (uint32_t k = 0; k < kPartBatch; k++)
+    d[k] = load_digest(J.tree + (((valid >> k) & 1u) ? node0 + item[k] : 0));
+  return valid;
+}
+
+// Part p's arrays: the count matrix (tiles x buckets), the bucket totals and
+// the group totals.
+__device__ __forceinline__ GlobalPtr<uint32_t> bulk_mat(const JobG& J, bool Int) {
+  return J.bk_mat + (Int ? (uint64_t)J.ntiles_b * J.nbuckets : 0);
+}
+__device__ __forceinline__ GlobalPtr<uint32_t> bulk_tot(const JobG& J, bool Int) {
+  return J.bk_tot + (Int ? J.nbuckets : 0);
+}
+__device__ __forceinline__ GlobalPtr<uint32_t> bulk_grp(const JobG& J, bool Int) {
+  return J.bk_grp + (Int ? (J.nbuckets + kColBuckets - 1) / kColBuckets : 0);
+}
+
+// One workgroup per tile: the tile's records per bucket -> its matrix row.
+template <bool Int>
+__global__ __launch_bounds__(kPartThreads) void k_bulk_hist(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobG& J = jobs[j];
+  __shared__ uint32_t hist[kBulkMaxBuckets];
+  __shared__ uint32_t sflags;
+  if (threadIdx.x == 0) sflags = J.ctr->flags;
+  const uint32_t nb = J.nbuckets;
+  for (uint32_t b = threadIdx.x; b < nb; b += kPartThreads) hist[b] = 0;
+  __syncthreads();
+  if (bulk_skip(J, sflags)) return;
+  const uint32_t w = blockIdx.x - prefix[j];
+  for (uint32_t b = 0; b < kPartItems / kPartBatch; b++) {
+    Digest d[kPartBatch];
+    uint64_t item[kPartBatch];
+    const uint32_t valid = bulk_load<Int>(J, w, b, d, item);
+#pragma unroll
+    for (uint32_t k = 0; k < kPartBatch; k++)
+      if ((valid >> k) & 1u) atomicAdd(&hist[bulk_bucket(J.map, d[k])], 1u);
+  }
+  __syncthreads();
+  const GlobalPtr<uint32_t> row = bulk_mat(J, Int) + (uint64_t)w * nb;
+  for (uint32_t b = threadIdx.x; b < nb; b += kPartThreads) row[b] = hist[b];
+}
+
+// One workgroup per group of kColBuckets buckets; lane (segment s, bucket
+// bl) holds rows s*RS .. s*RS+RS-1 of its bucket's column (RS <= 16, all
+// loaded together).  The column becomes exclusive prefixes down the tiles;
+// the bucket's total and the group's total are written; the leaf part
+// switches the job to the dense inserts when a bucket has more than
+// bk_cap_l leaf records.
+template <bool Int>
+__global__ __launch_bounds__(kColBuckets * kColSegs) void k_bulk_colscan(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobG& J = jobs[j];
+  __shared__ uint32_t seg[kColSegs][kColBuckets];
+  __shared__ uint32_t sh[kColSegs + 1];
+  __shared__ uint32_t sflags;
+  if (threadIdx.x == 0) sflags = J.ctr->flags;
+  __syncthreads();
+  if (bulk_skip(J, sflags)) return;
+  const uint32_t g = blockIdx.x - prefix[j];
+  const uint32_t nb = J.nbuckets, W = J.ntiles_b;
+  const uint32_t bl = threadIdx.x % kColBuckets, s = threadIdx.x / kColBuckets;
+  const uint32_t b = g * kColBuckets + bl;
+  const bool inb = b < nb;
+  const uint32_t RS = (W + kColSegs - 1) / kColSegs;  // at most 16 (kBulkMaxTiles)
+  const GlobalPtr<uint32_t> mat = bulk_mat(J, Int);
+  uint32_t v[16];
+#pragma unroll
+  for (uint32_t r = 0; r < 16; r++) {
+    const uint32_t w = s * RS + r;
+    v[r] = (inb && r < RS && w < W) ? mat[(uint64_t)w * nb + b] : 0u;
+  }
+  uint32_t run = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < 16; r++) {
+    const uint32_t x = v[r];
+    v[r] = run;
+    run += x;
+  }
+  seg[s][bl] = run;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  for (uint32_t q = 0; q < kColSegs; q++) {
+    const uint32_t x = seg[q][bl];
+    pre += q < s ? x : 0u;
+    tot += x;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < 16; r++) {
+    const uint32_t w = s * RS + r;
+    if (inb && r < RS && w < W) mat[(uint64_t)w * nb + b] = pre + v[r];
+  }
+  if (s == 0 && inb) {
+    bulk_tot(J, Int)[b] = tot;
+    if (!Int && tot > J.bk_cap_l) g_or(&J.ctr->flags, FLAG_BULK_OFF);
+  }
+  const uint32_t gt = block_sum<uint32_t>(s == 0 && inb ? tot : 0u, sh);
+  if (threadIdx.x == 0) bulk_grp(J, Int)[g] = gt;
+}
+
+// First record of bucket b within its part: the totals of the groups before
+// its group and of the buckets before it in its group, summed by one
+// wavefront (every lane calls it).
+__device__ __forceinline__ uint32_t bulk_start(const JobG& J, bool Int, uint32_t b) {
+  const uint32_t g = b / kColBuckets;
+  const GlobalPtr<const uint32_t> grp = bulk_grp(J, Int), tot = bulk_tot(J, Int);
+  uint32_t s = 0;
+  for (uint32_t i = lane_id(); i < g; i += 64) s += grp[i];
+  const uint32_t i = g * kColBuckets + lane_id();
+  if (i < b) s += tot[i];
+  return wave_sum(s);
+}
+
+// One workgroup per tile, the same items as k_bulk_hist: LDS cursors (bucket
+// start + the tile's prefix in the bucket), then one LDS atomic per record.
+template <bool Int>
+__global__ __launch_bounds__(kPartThreads) void k_bulk_scatter(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobG& J = jobs[j];
+  __shared__ uint32_t cur[kBulkMaxBuckets];
+  __shared__ uint32_t sh[kPartThreads / 64 + 1];
+  __shared__ uint32_t sflags;
+  if (threadIdx.x == 0) sflags = J.ctr->flags;
+  const uint32_t nb = J.nbuckets;
+  const uint32_t w = blockIdx.x - prefix[j];
+  // Bucket starts: an exclusive scan of the bucket totals, kPerLane buckets
+  // per lane, loaded with the tile's row.
+  constexpr uint32_t kPerLane = kBulkMaxBuckets / kPartThreads;
+  const GlobalPtr<const uint32_t> tot = bulk_tot(J, Int);
+  const GlobalPtr<const uint32_t> row = bulk_mat(J, Int) + (uint64_t)w * nb;
+  uint32_t t[kPerLane], r[kPerLane];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kPerLane; i++) {
+    const uint32_t b = threadIdx.x * kPerLane + i;
+    t[i] = b < nb ? tot[b] : 0u;
+    r[i] = b < nb ? row[b] : 0u;
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < kPerLane; i++) sum += t[i];
+  uint32_t total;
+  uint32_t base = block_excl_scan<kPartThreads / 64>(sum, sh, total);
+#pragma unroll
+  for (uint32_t i = 0; i < kPerLane; i++) {
+    const uint32_t b = threadIdx.x * kPerLane + i;
+    if (b < nb) cur[b] = base + r[i];
+    base += t[i];
+  }
+  __syncthreads();
+  if (bulk_skip(J, sflags)) return;
+  const uint64_t roff = Int ? J.C : 0;  // first record of the part
+  const uint64_t node0 = Int ? 0 : J.C - 1;
+  for (uint32_t b = 0; b < kPartItems / kPartBatch; b++) {
+    Digest d[kPartBatch];
+    uint64_t item[kPartBatch];
+    const uint32_t valid = bulk_load<Int>(J, w, b, d, item);
+#pragma unroll
+    for (uint32_t k = 0; k < kPartBatch; k++) {
+      if ((valid >> k) & 1u) {
+        const uint64_t pos = roff + atomicAdd(&cur[bulk_bucket(J.map, d[k])], 1u);
+        // One 32-byte record (digest, node): a single sector per record
+        // where records land in lines that other workgroups also write.
+        const GlobalPtr<u32x4_t> rec = (GlobalPtr<u32x4_t>)(J.bk_rec + 2 * pos);
+        rec[0] = u32x4_t{(uint32_t)d[k].k0, (uint32_t)(d[k].k0 >> 32), (uint32_t)d[k].k1, (uint32_t)(d[k].k1 >> 32)};
+        rec[1] = u32x4_t{(uint32_t)(node0 + item[k]), 0u, 0u, 0u};
+      }
+    }
+  }
+}
+
+// One workgroup per bucket.  Leaf part (Final = false): the bucket's leaf
+// records; every leaf whose digest has a lower leaf in the bucket is labelled
+// SHIFT (a FIXED|PENDING leaf keeps its label, as in the dense inserts).  A
+// job that fell back to the dense inserts clears the bucket instead.  Final
+// part: the bucket's leaf and interior records; the bucket's slots are
+// written in order (EMPTY where no record landed), each shifted leaf gets its
+// digest's slot, and the entry count goes to bk_claim.
+//
+// Probing in LDS: a record claims an empty slot of the LDS image with a CAS
+// (the slot then names it, its digest's first record); a slot naming a
+// record with the same digest is the digest's slot.  Each record lowers the
+// rank kept for its slot's first record with an atomicMin: ranks are unique
+// (a leaf's index, below every interior node, which rank by descending id),
+// so the record whose rank is the minimum is the entry.
+template <bool Final>
+__global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
+  const uint32_t j = find_job(prefix, njobs, blockIdx.x);
+  const JobG& J = jobs[j];
+  __shared__ uint32_t owner[kBucketSlots];  // record that claimed the slot, ~0 = empty
+  __shared__ uint32_t best[kBulkRec];       // per first record: lowest rank of its digest
+  __shared__ uint4 rdig[kBulkRec];
+  __shared__ uint32_t rnode[kBulkRec];
+  __shared__ uint16_t rslot[kBulkRec];
+  __shared__ uint32_t sh[kBuildThreads / 64 + 1];
+  __shared__ uint32_t sflags, sbase[2], sn[2];
+  const uint32_t b = blockIdx.x - prefix[j];
+  const uint64_t bs = J.map.bmask + 1;
+  const GlobalPtr<Slot> slots = J.map.slots + (uint64_t)b * bs;
+  // The flags and the bucket's record ranges: wavefront p (0 = leaves, 1 =
+  // interior nodes) finds its part's range.
+  if (threadIdx.x == 0) sflags = J.ctr->flags;
+  const uint32_t w = threadIdx.x >> 6;
+  if (w < (Final ? 2u : 1u)) {
+    const bool Int = w == 1;
+    const uint32_t st = bulk_start(J, Int, b);
+    if (lane_id() == 0) {
+      sbase[w] = (uint32_t)((Int ? J.C : 0) + st);
+      sn[w] = bulk_tot(J, Int)[b];
+    }
+  }
+  __syncthreads();
+  const uint32_t flags = sflags;
+  if (!J.bulk || (flags & (FLAG_NEED_GROW | FLAG_MAP_FULL | FLAG_BULK_DUPS))) return;
+  // Without the duplicate search (bulk_dedup 0: a job whose last baseline
+  // had no repeated leaf digest) the leaf part only clears the buckets of a
+  // job that fell back.
+  if (!Final && !J.bulk_dedup && !(flags & FLAG_BULK_OFF)) return;
+  if (flags & FLAG_BULK_OFF) {
+    if (!Final) {  // the dense inserts fill the cleared map
+      for (uint64_t q = threadIdx.x; q < bs; q += blockDim.x) {
+        const GlobalPtr<u32x4_t> p = (GlobalPtr<u32x4_t>)&slots[q];
+        p[0] = u32x4_t{0, 0, 0, 0};
+        p[1] = u32x4_t{0xffffffffu, 0xffffffffu, 0, 0};
+      }
+    }
+    return;
+  }
+  const uint32_t nl = sn[0], ni = Final ? sn[1] : 0u, n = nl + ni;
+  if (n > J.bk_cap_b || n > kBulkRec || n >= bs) {
+    // Does not fit: the host runs the job again without the bulk build.
+    if (threadIdx.x == 0) g_or(&J.ctr->flags, FLAG_MAP_FULL);
+    return;
+  }
+  // Every record of the lane is loaded before any is stored to LDS.
+  constexpr uint32_t kRecPerLane = (kBulkRec + kBuildThreads - 1) / kBuildThreads;
+  {
+    u32x4_t rd[kRecPerLane];
+    uint32_t rn[kRecPerLane];
+#pragma unroll
+    for (uint32_t i = 0; i < kRecPerLane; i++) {
+      const uint32_t r = threadIdx.x + i * kBuildThreads;
+      const uint64_t e = r < nl ? (uint64_t)sbase[0] + r : (uint64_t)sbase[1] + (r - nl);
+      const GlobalPtr<const u32x4_t> rec = (GlobalPtr<const u32x4_t>)(J.bk_rec + 2 * (r < n ? e : 0));
+      rd[i] = rec[0];
+      rn[i] = rec[1].x;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kRecPerLane; i++) {
+      const uint32_t r = threadIdx.x + i * kBuildThreads;
+      if (r < n) {
+        rdig[r] = make_uint4(rd[i].x, rd[i].y, rd[i].z, rd[i].w);
+        rnode[r] = rn[i];
+        best[r] = 0xffffffffu;
+      }
+    }
+  }
+  for (uint64_t q = threadIdx.x; q < bs; q += blockDim.x) owner[q] = 0xffffffffu;
+  __syncthreads();
+  const uint32_t C1 = (uint32_t)(J.C - 1);
+  const uint32_t bm = (uint32_t)J.map.bmask;
+  auto rank_of = [&](uint32_t r) -> uint32_t {
+    return r < nl ? rnode[r] - C1 : 0x80000000u | (0x7fffffffu - rnode[r]);
+  };
+  for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
+    const uint4 dv = rdig[r];
+    const Digest d{dv.x | ((uint64_t)dv.y << 32), dv.z | ((uint64_t)dv.w << 32)};
+    uint32_t q = (uint32_t)(map_start(J.map, d) & bm);
+    uint32_t first = r;
+    for (uint64_t it = 0; it < bs; it++, q = (q + 1) & bm) {
+      uint32_t cur = __hip_atomic_load(&owner[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (cur == 0xffffffffu) {
+        cur = atomicCAS(&owner[q], 0xffffffffu, r);
+        if (cur == 0xffffffffu) break;  // claimed: this record is the digest's first
+      }
+      const uint4 ov = rdig[cur];
+      if (ov.x == dv.x && ov.y == dv.y && ov.z == dv.z && ov.w == dv.w) {
+        first = cur;
+        break;
+      }
+    }
+    rslot[r] = (uint16_t)q;
+    atomicMin(&best[first], rank_of(r));
+  }
+  __syncthreads();
+  if (!Final) {
+    for (uint32_t r = threadIdx.x; r < nl; r += blockDim.x) {
+      const uint32_t q = rslot[r];
+      if (best[owner[q]] != rank_of(r) && J.labels[rnode[r]] == PENDING) J.labels[rnode[r]] = SHIFT_DUPL;
+    }
+    return;
+  }
+  // The entry of each slot: its digest's lowest-ranked record.  A leaf that
+  // is not its digest's entry is a repeated leaf digest: without the leaf
+  // part's duplicate search pass 1 took it for a first occurrence, so the
+  // host runs the job again with the search (FLAG_BULK_DUPS).
+  bool win[kRecPerLane];
+  bool dup = false;
+#pragma unroll
+  for (uint32_t i = 0; i < kRecPerLane; i++) {
+    const uint32_t r = threadIdx.x + i * kBuildThreads;
+    win[i] = r < n && best[owner[rslot[r]]] == rank_of(r);
+    dup = dup || (r < nl && !win[i]);
+  }
+  if (!J.bulk_dedup && __syncthreads_or(dup)) {
+    if (threadIdx.x == 0) g_or(&J.ctr->flags, FLAG_BULK_DUPS | FLAG_MAP_FULL);  // later stages stop
+    return;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t i = 0; i < kRecPerLane; i++) {
+    const uint32_t r = threadIdx.x + i * kBuildThreads;
+    if (r < n && win[i]) owner[rslot[r]] = r;
+    if (r < nl && !win[i]) J.slot_of[rnode[r]] = (uint32_t)((uint64_t)b * bs + rslot[r]);
+  }
+  __syncthreads();
+  uint32_t used = 0;
+  const uint32_t K = J.ckpt_id;
+  for (uint64_t q = threadIdx.x; q < bs; q += blockDim.x) {
+    const uint32_t o = owner[q];
+    const GlobalPtr<u32x4_t> p = (GlobalPtr<u32x4_t>)&slots[q];
+    if (o == 0xffffffffu) {
+      p[0] = u32x4_t{0, 0, 0, 0};
+      p[1] = u32x4_t{0xffffffffu, 0xffffffffu, 0, 0};
+    } else {
+      const uint4 dg = rdig[o];
+      p[0] = u32x4_t{dg.x, dg.y, dg.z, dg.w};
+      p[1] = u32x4_t{rnode[o], K, 0, 0};
+      used++;
+    }
+  }
+  const uint32_t tot = block_sum<uint32_t>(used, sh);
+  if (threadIdx.x == 0) J.bk_claim[b] = tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -1371,14 +1770,14 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
     todo[n] = need[n];
     s[n] = map_start(m, key[n]) & m.mask;
   }
-  for (uint64_t it = 0; it <= m.mask; it += 2) {
+  for (uint64_t it = 0; it <= m.bmask; it += 2) {
     uint4 k[NK][2];
     unsigned long long v[NK][2];
 #pragma unroll
     for (int n = 0; n < NK; n++) {
       if (!todo[n]) continue;
       load_slot(m, s[n], k[n][0], v[n][0]);
-      load_slot(m, (s[n] + 1) & m.mask, k[n][1], v[n][1]);
+      load_slot(m, map_next(m, s[n], 1), k[n][1], v[n][1]);
     }
     bool any = false;
 #pragma unroll
@@ -1392,11 +1791,11 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
         if (val == SLOT_EMPTY) {
           todo[n] = false;
         } else if (k0 == key[n].k0 && k1 == key[n].k1) {
-          out[n] = (s[n] + e) & m.mask;
+          out[n] = map_next(m, s[n], e);
           todo[n] = false;
         }
       }
-      s[n] = (s[n] + 2) & m.mask;
+      s[n] = map_next(m, s[n], 2);
       any |= todo[n];
     }
     if (!any) break;
@@ -1902,6 +2301,9 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ job
   unsigned long long claims = 0;
   for (uint32_t i = threadIdx.x; i < J.nwg_ins_leaf; i += blockDim.x) claims += J.blk_claim[i];
   for (uint32_t i = threadIdx.x; i < J.nwg_ins_int; i += blockDim.x) claims += J.int_claim[i];
+  __syncthreads();  // sflags
+  if (J.bulk && !(sflags & (FLAG_BULK_OFF | FLAG_NEED_GROW | FLAG_MAP_FULL)))
+    for (uint32_t i = threadIdx.x; i < J.nbuckets; i += blockDim.x) claims += J.bk_claim[i];
   claims = block_sum(claims, shl);  // its barriers publish the fields above
   const uint64_t pending = J.is_baseline ? J.C : spend;
   if (sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) {
@@ -2194,14 +2596,14 @@ __global__ void k_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView m,
     const Digest k{o.k0, o.k1};
     uint64_t s = map_start(m, k) & m.mask;
     bool done = false;
-    for (uint64_t it = 0; it <= m.mask; it++) {
+    for (uint64_t it = 0; it <= m.bmask; it++) {
       if (atomicCAS(&m.slots[s].val, SLOT_EMPTY, o.val) == SLOT_EMPTY) {
         m.slots[s].k0 = k.k0;
         m.slots[s].k1 = k.k1;
         done = true;
         break;
       }
-      s = (s + 1) & m.mask;
+      s = map_next(m, s, 1);
     }
     if (!done) atomicOr(flags, FLAG_MAP_FULL);
   }
@@ -2799,6 +3201,23 @@ void launch_gather(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs,
   if (!grid) return;
   if (vec16) DD_LAUNCH(k_gather<true>, grid, 256, 0, s, jobs, prefix, njobs);
   else DD_LAUNCH(k_gather<false>, grid, 256, 0, s, jobs, prefix, njobs);
+}
+
+void launch_bulk(const JobDev* jobs_h, uint32_t njobs, bool final_part, const uint32_t* ppre, uint32_t pgrid,
+                 const uint32_t* spre, uint32_t sgrid, const uint32_t* bpre, uint32_t bgrid, hipStream_t s) {
+  const JobG* jobs = gview(jobs_h);
+  if (!pgrid || !sgrid || !bgrid) return;
+  if (!final_part) {
+    DD_LAUNCH(k_bulk_hist<false>, pgrid, kPartThreads, 0, s, jobs, ppre, njobs);
+    DD_LAUNCH(k_bulk_colscan<false>, sgrid, kColBuckets * kColSegs, 0, s, jobs, spre, njobs);
+    DD_LAUNCH(k_bulk_scatter<false>, pgrid, kPartThreads, 0, s, jobs, ppre, njobs);
+    DD_LAUNCH(k_bulk_build<false>, bgrid, kBuildThreads, 0, s, jobs, bpre, njobs);
+  } else {
+    DD_LAUNCH(k_bulk_hist<true>, pgrid, kPartThreads, 0, s, jobs, ppre, njobs);
+    DD_LAUNCH(k_bulk_colscan<true>, sgrid, kColBuckets * kColSegs, 0, s, jobs, spre, njobs);
+    DD_LAUNCH(k_bulk_scatter<true>, pgrid, kPartThreads, 0, s, jobs, ppre, njobs);
+    DD_LAUNCH(k_bulk_build<true>, bgrid, kBuildThreads, 0, s, jobs, bpre, njobs);
+  }
 }
 
 void launch_map_clear(Slot* slots, uint64_t cap, hipStream_t s) {

@@ -25,6 +25,22 @@ constexpr uint32_t kTileInterior = 255;  // interior nodes of a tile of 8 levels
 // 256-byte block: counters that share a cache line serialize their atomics
 // in one L2 channel (one atomic per wavefront with a changed chunk).
 constexpr uint32_t kCntStride = 64;
+// Bulk map build of a baseline: partition workgroups of kPartTile records
+// (a row of per-bucket counts each), column scans over groups of
+// kColBuckets buckets, one build workgroup per bucket holding at most
+// kBulkRec records.  A job has at most kBulkMaxBuckets buckets (LDS
+// counters of a partition workgroup: maps of at most 2^25 slots) and
+// kBulkMaxTiles tiles (16 rows for each of the kColSegs segments of a
+// column scan).
+constexpr uint32_t kPartThreads = 1024;
+constexpr uint32_t kPartItems = 32;  // records per lane
+constexpr uint32_t kPartTile = kPartThreads * kPartItems;
+constexpr uint32_t kBulkMaxBuckets = 8192;
+constexpr uint32_t kColBuckets = 64;
+constexpr uint32_t kColSegs = 16;
+constexpr uint32_t kBulkMaxTiles = 16 * kColSegs;
+constexpr uint32_t kBuildThreads = 512;
+constexpr uint32_t kBulkRec = 2400;
 
 // Per-job counters written by the device and read back by the host.
 struct JobCounters {
@@ -73,6 +89,27 @@ struct JobT {
   // 1: low-root first occurrences (DD_METHOD_TREE_LOW_ROOT); lr_new holds N
   // new-node flags of this checkpoint.
   uint32_t low_root;
+  // Bulk map build of a baseline (k_bulk_*): 1 for a tree-method baseline
+  // whose map is built bucket by bucket instead of by the dense inserts.
+  // The leaves, and after pass 1 the first-occurrence interior nodes, are
+  // partitioned by map bucket into record arrays (digest, node); each bucket
+  // is then built in LDS and written out whole.
+  uint32_t bulk;
+  // 1: the leaf part searches each bucket's leaf records for repeated
+  // digests and labels them SHIFT before pass 1; 0: the job is expected to
+  // have none (its last baseline had none), and the final build checks.
+  uint32_t bulk_dedup;
+  uint32_t nbuckets;    // map buckets (capacity / kBucketSlots, at least 1)
+  uint32_t ntiles_b;    // partition workgroups: ceil(C / kPartTile)
+  uint32_t bk_cap_l;    // leaf records per bucket above which the job falls back to the dense inserts
+  uint32_t bk_cap_b;    // records per bucket above which the final build gives up (FLAG_MAP_FULL)
+  P<uint32_t> bk_mat;   // per part (leaves, then interior nodes): ntiles_b x nbuckets counts, then column prefixes
+  P<uint32_t> bk_tot;   // per part: records per bucket
+  P<uint32_t> bk_grp;   // per part: records per group of kColBuckets buckets
+  P<uint32_t> bk_claim;  // per bucket: entries written by the final build
+  // Records of 32 bytes (digest; node id and zero padding): C leaves, then
+  // C-1 interior nodes.
+  P<u32x4_t> bk_rec;
   // persistent state
   P<Digest> tree;        // N digests, heap order
   P<uint8_t> labels;     // N labels (padded for 16-byte reads)
@@ -258,6 +295,16 @@ void launch_sbin_scatter(const JobDev* jobs, const uint32_t* prefix, uint32_t nj
 void launch_gather(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, bool vec16,
                    hipStream_t s);
 
+// Bulk map build of the bulk jobs of a table (JobT::bulk).  Leaf part,
+// between the leaf kernel and the leaf inserts: counts, column scan, partition,
+// then the per-bucket duplicate search that labels every leaf but the lowest
+// of each digest SHIFT (a bucket of a job that fell back to the dense
+// inserts is cleared instead).  Final part, after pass 1: the same for the
+// first-occurrence interior nodes, then the per-bucket build that writes the
+// map.  `pgrid`/`ppre`: the partition tiles, `sgrid`/`spre`: the column-scan
+// groups, `bgrid`/`bpre`: one workgroup per bucket.
+void launch_bulk(const JobDev* jobs, uint32_t njobs, bool final_part, const uint32_t* ppre, uint32_t pgrid,
+                 const uint32_t* spre, uint32_t sgrid, const uint32_t* bpre, uint32_t bgrid, hipStream_t s);
 void launch_map_clear(Slot* slots, uint64_t cap, hipStream_t s);
 void launch_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView nmap, uint32_t* flags, hipStream_t s);
 

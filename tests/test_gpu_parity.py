@@ -143,7 +143,7 @@ def test_whole_buffer_repeat_root_shift_option(torch_gpu):
         assert struct.unpack_from("<IIQIIII", gd[2])[4:] == (0, 1, 1)
         assert gpu_restart(torch_gpu, g, gd, 2, len(datas[2])) == bytes(datas[2])
     with pytest.raises(dd.DedupError):
-        g.options = 2
+        g.options = 1 << 8  # no such option
 
 
 def test_config1_shape_64mib(torch_gpu):
