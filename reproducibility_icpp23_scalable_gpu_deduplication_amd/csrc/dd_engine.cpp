@@ -1296,7 +1296,10 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
   bool any_shift = false;  // the pending list stays empty until a checkpoint has shifted entries
   for (size_t k = 0; k < M; k++) {
     const uint32_t st = base + 1 + (uint32_t)k;
-    launch_restart_regions(ddesc + k, g, cover, base, st, ref, pend, ctr, out, len, cs, piece, s);
+    if (piece == 16 && hdrs[k].num_first_ocur == 1 && hdrs[k].num_shift_dupl == 0)
+      launch_restart_whole(ddesc + k, g, cover, base, st, ctr, out, len, cs, s);
+    else
+      launch_restart_regions(ddesc + k, g, cover, base, st, ref, pend, ctr, out, len, cs, piece, s);
     any_shift = any_shift || hdrs[k].num_shift_dupl > 0;
     if (any_shift) launch_restart_pending(ddesc + k, g, cover, st, ref, pend, ctr, out, len, cs, piece, s);
   }

@@ -2987,6 +2987,83 @@ __global__ __launch_bounds__(256) void k_restart_regions(const RestartCkpt* __re
   if (lane_id() == 0 && tot) g_add((GlobalPtr<unsigned long long>)&ctr->claimed, (unsigned long long)tot);
 }
 
+// Step of a checkpoint whose diff holds one first-occurrence region and no
+// shifted one (a baseline without repeated chunks).  The region's leaves are
+// at most two runs of consecutive chunks, stored in that order in the data
+// section, so a round of 256 leaves reads one contiguous range; no region
+// search and no root load per leaf, and every piece a lane copies in a
+// round is loaded before any is stored.  Cover entries and claims as in
+// k_restart_regions (region 0).
+__global__ __launch_bounds__(256) void k_restart_whole(const RestartCkpt* __restrict__ rcp, TreeGeom g,
+                                                       uint2* __restrict__ cover, uint32_t base, uint32_t stamp,
+                                                       RestartCtr* __restrict__ ctr, uint8_t* __restrict__ out,
+                                                       uint64_t len, uint32_t cs) {
+  const RestartCkpt rc = *rcp;
+  if (((GlobalPtr<const uint32_t>)&ctr->flags)[0] & FLAG_BAD_DIFF) return;
+  __shared__ uint32_t s_dst[256];  // chunk of round leaf i; ~0 when a newer checkpoint covers it
+  const uint64_t root = (uint64_t)rs_load32(rc.diff + rc.first_off) + rc.node_bias;
+  const LeafRuns runs = subtree_runs(g, root);
+  const uint64_t nl = min(rc.nfl, runs.n0 + runs.n1);  // equal once the parse accepted the diff
+  const GlobalPtr<u32x2_t> cv = (GlobalPtr<u32x2_t>)cover;
+  const GlobalPtr<uint8_t> o = (GlobalPtr<uint8_t>)out;
+  const uint8_t* data = rc.diff + rc.data_off;
+  const uint32_t ppc = cs / 16;
+  const bool pow2 = (ppc & (ppc - 1)) == 0;
+  const uint32_t sh = pow2 ? __builtin_ctz(ppc) : 0;
+  constexpr uint32_t U = 8;
+  uint32_t claimed = 0;
+  for (uint64_t t = blockIdx.x; t * 256 < nl; t += gridDim.x) {
+    const uint64_t e = t * 256 + threadIdx.x;
+    uint32_t c = 0xffffffffu;
+    const uint64_t leaf = e < nl ? run_leaf(runs, e) : 0;
+    if (e < nl && leaf >= g.C - 1 && leaf < g.N) {
+      const uint32_t cc = (uint32_t)(leaf - (g.C - 1));
+      const u32x2_t old = cv[cc];
+      cv[cc] = u32x2_t{stamp, 0u};
+      if (old.x <= base) {  // no newer checkpoint of this restart covered the chunk
+        claimed++;
+        c = cc;
+      }
+    }
+    s_dst[threadIdx.x] = c;
+    __syncthreads();
+    const uint32_t nq = (uint32_t)min((uint64_t)256, nl - t * 256);
+    const uint32_t total = nq * ppc;
+    const uint8_t* src0 = data + t * 256 * (uint64_t)cs;
+    for (uint32_t w0 = threadIdx.x; w0 < total; w0 += 256 * U) {
+      u32x4_t v[U];
+      uint64_t dst[U];
+      bool full[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        const uint32_t w = w0 + 256 * u;
+        full[u] = false;
+        dst[u] = 0;
+        if (w >= total) continue;
+        const uint32_t i = pow2 ? (w >> sh) : (w / ppc);
+        const uint32_t cd = s_dst[i];
+        if (cd == 0xffffffffu) continue;
+        const uint32_t pi = w - i * ppc;
+        dst[u] = (uint64_t)cd * cs + (uint64_t)pi * 16;
+        const uint8_t* in = src0 + (uint64_t)w * 16;
+        if (dst[u] + 16 <= len) {
+          const u32x4a4 x = *(GlobalPtr<const u32x4a4>)in;  // 4-byte aligned source
+          v[u] = u32x4_t{x.x, x.y, x.z, x.w};
+          full[u] = true;
+        } else {
+          copy_chunk_piece<16>(out, dst[u], in, len);  // the tail of the buffer
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++)
+        if (full[u]) *(GlobalPtr<u32x4_t>)(o + dst[u]) = v[u];
+    }
+    __syncthreads();
+  }
+  const uint32_t tot = wave_sum(claimed);
+  if (lane_id() == 0 && tot) g_add((GlobalPtr<unsigned long long>)&ctr->claimed, (unsigned long long)tot);
+}
+
 template <int PIECE>
 __global__ __launch_bounds__(256) void k_restart_pending(const RestartCkpt* __restrict__ rcp, TreeGeom g,
                                                          const uint2* __restrict__ cover, uint32_t stamp,
@@ -3262,6 +3339,13 @@ void launch_restart_regions(const RestartCkpt* rc, TreeGeom g, uint2* cover, uin
   if (piece == 16) DD_LAUNCH(k_restart_regions<16>, b, 256, 0, s, rc, g, cover, base, stamp, ref, pend, ctr, out, len, cs);
   else if (piece == 4) DD_LAUNCH(k_restart_regions<4>, b, 256, 0, s, rc, g, cover, base, stamp, ref, pend, ctr, out, len, cs);
   else DD_LAUNCH(k_restart_regions<1>, b, 256, 0, s, rc, g, cover, base, stamp, ref, pend, ctr, out, len, cs);
+}
+
+void launch_restart_whole(const RestartCkpt* rc, TreeGeom g, uint2* cover, uint32_t base, uint32_t stamp,
+                          RestartCtr* ctr, uint8_t* out, uint64_t len, uint32_t cs, hipStream_t s) {
+  uint64_t blocks = (g.C + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  DD_LAUNCH(k_restart_whole, (uint32_t)blocks, 256, 0, s, rc, g, cover, base, stamp, ctr, out, len, cs);
 }
 
 void launch_restart_pending(const RestartCkpt* rc, TreeGeom g, const uint2* cover, uint32_t stamp, uint64_t* ref,

@@ -329,6 +329,10 @@ void launch_restart_regions(const RestartCkpt* rc, TreeGeom g, uint2* cover, uin
 // Resolves the pending chunks whose source is in this checkpoint (read from
 // the cover entries the step wrote), copying their data or moving them to an
 // older checkpoint.
+// Step of a checkpoint with one first-occurrence region and no shifted one,
+// 16-byte pieces (same results as launch_restart_regions).
+void launch_restart_whole(const RestartCkpt* rc, TreeGeom g, uint2* cover, uint32_t base, uint32_t stamp,
+                          RestartCtr* ctr, uint8_t* out, uint64_t len, uint32_t cs, hipStream_t s);
 void launch_restart_pending(const RestartCkpt* rc, TreeGeom g, const uint2* cover, uint32_t stamp, uint64_t* ref,
                             const uint32_t* pend, RestartCtr* ctr, uint8_t* out, uint64_t len, uint32_t chunk_size,
                             int piece, hipStream_t s);
