@@ -26,7 +26,8 @@ enum : int { HASH_MURMUR3 = 0, HASH_MD5 = 1 };
 // Map / pipeline status flags.
 // FLAG_BAD_DIFF: a restart's parse found a diff inconsistent.
 // FLAG_BULK_OFF: a baseline's bulk map build found a bucket with more leaves
-// than its workgroup holds; the job's map is built by the dense inserts.
+// than its workgroup holds (raised with FLAG_MAP_FULL); the host runs the job
+// again with the dense inserts.
 // FLAG_BULK_DUPS: a bulk build without the leaf duplicate search found a
 // repeated leaf digest (raised with FLAG_MAP_FULL, which stops the later
 // stages); the host runs the job again with the search.

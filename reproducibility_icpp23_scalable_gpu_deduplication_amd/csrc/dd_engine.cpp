@@ -152,7 +152,7 @@ static void bulk_caps(uint32_t* cap_l, uint32_t* cap_b) {
 }
 
 bool StreamState::uses_bulk(uint64_t Cn, uint64_t cap) const {
-  if (method != DD_METHOD_TREE || (options & DD_OPT_DENSE_BASELINE)) return false;
+  if (method != DD_METHOD_TREE || (options & DD_OPT_DENSE_BASELINE) || bulk_heavy) return false;
   return (cap >> kBucketLog) <= kBulkMaxBuckets && ceil_div(Cn, kPartTile) <= kBulkMaxTiles;
 }
 
@@ -434,9 +434,11 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   low_root_ = false;
   max_c_ = 0;
   kinds_ = 0;
+  bulk_dedup_ = false;
   for (const JobDev& J : jobs) {
     if (J.hash_kind != hash_kind_ || (J.list_mode != 0) != list_mode_) return DD_EINVAL;
-    kinds_ |= J.is_baseline ? JOBS_DENSE : J.wide_probe ? JOBS_LIST_WIDE : JOBS_LIST_NARROW;
+    kinds_ |= J.is_baseline ? (J.bulk ? 0u : JOBS_DENSE) : J.wide_probe ? JOBS_LIST_WIDE : JOBS_LIST_NARROW;
+    bulk_dedup_ = bulk_dedup_ || (J.bulk && J.bulk_dedup);
     low_root_ = low_root_ || J.low_root;
     max_c_ = std::max<uint64_t>(max_c_, J.C);
     max_groups_ = std::max(max_groups_, groups_of(J.D));
@@ -491,12 +493,11 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
       // an incremental checkpoint strides over its work lists.
       // The insert stages take one list entry per lane: a baseline lists
       // (nearly) every leaf and interior node.
-      // A bulk baseline runs the dense inserts only when it falls back, over
-      // fewer grid-stride workgroups.
+      // A bulk baseline has no dense inserts (a fallback runs it again
+      // without the bulk build).
       case S_INSL:
       case S_TILES:
-        n = J.is_baseline ? (J.bulk ? std::min<uint64_t>(ceil_div(J.C, 256), 2048) : ceil_div(J.C, 256))
-                          : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG);
+        n = J.is_baseline ? (J.bulk ? 0 : ceil_div(J.C, 256)) : std::min<uint64_t>(ceil_div(J.C, 256), kMaxStageWG);
         break;
       case S_BULK: n = J.bulk ? J.ntiles_b : 0; break;
       case S_BSCAN: n = J.bulk ? ceil_div(J.nbuckets, kColBuckets) : 0; break;
@@ -590,7 +591,7 @@ std::vector<uint64_t> Pipeline::signature() const {
   sig.push_back((uint64_t)max_groups_);
   sig.push_back((uint64_t)aligned16_ | ((uint64_t)vec16_ << 1));
   sig.push_back((uint64_t)hash_kind_ | ((uint64_t)list_mode_ << 8) | ((uint64_t)kinds_ << 16) |
-                ((uint64_t)low_root_ << 24));
+                ((uint64_t)low_root_ << 24) | ((uint64_t)bulk_dedup_ << 25));
   sig.push_back(low_root_ ? max_c_ : 0);
   sig.push_back(reinterpret_cast<uintptr_t>(dev_table_.p));
   for (uint32_t o : off_) sig.push_back(o);
@@ -701,7 +702,8 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
       return DD_OK;
     }
   }
-  launch_bulk(J, nj, false, P(S_BULK), grid_[S_BULK], P(S_BSCAN), grid_[S_BSCAN], P(S_BUCKET), grid_[S_BUCKET], s);
+  launch_bulk(J, nj, false, bulk_dedup_, P(S_BULK), grid_[S_BULK], P(S_BSCAN), grid_[S_BSCAN], P(S_BUCKET),
+              grid_[S_BUCKET], s);
   launch_insert_leaves(J, P(S_INSL), nj, grid_[S_INSL], kinds_, s);
   if (low_root_) launch_low_root(J, nj, max_c_, (uint32_t)ilog2_64(2 * max_c_ - 1), s);
   DD_TRY(mark(2));
@@ -714,7 +716,8 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
     for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
     DD_TRY(mark(3));
     launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], kinds_, s);
-    launch_bulk(J, nj, true, P(S_BULK), grid_[S_BULK], P(S_BSCAN), grid_[S_BSCAN], P(S_BUCKET), grid_[S_BUCKET], s);
+    launch_bulk(J, nj, true, bulk_dedup_, P(S_BULK), grid_[S_BULK], P(S_BSCAN), grid_[S_BSCAN], P(S_BUCKET),
+                grid_[S_BUCKET], s);
     DD_TRY(mark(4));
     for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
   }
@@ -771,9 +774,12 @@ dd_status rerun_bulk_dedup(StreamState& st, JobDev& job, Pipeline& p, hipStream_
   return DD_OK;
 }
 
-// A bulk baseline whose final build did not fit one of its buckets runs again
-// from the leaf kernel with the dense inserts, on a cleared map.
+// A bulk baseline that stopped (a bucket with more leaf records than the
+// build holds, or a final bucket that does not fit) runs again from the leaf
+// kernel with the dense inserts, on a cleared map; after a bucket of repeated
+// leaf digests the handle's later baselines go straight to the dense inserts.
 dd_status rerun_dense_baseline(StreamState& st, JobDev& job, Pipeline& p, hipStream_t s, JobCounters* c) {
+  if (c->flags & FLAG_BULK_OFF) st.bulk_heavy = true;
   job.bulk = 0;
   launch_map_clear(st.map.as<Slot>(), st.map_cap, s);
   DD_TRY(p.run(std::vector<JobDev>(1, job), s, false, true));

@@ -64,6 +64,7 @@ struct StreamState {
   uint64_t bk_nb = 0, bk_nt = 0;
   bool bulk_now = false;  // the baseline being prepared uses the bulk build
   bool bulk_dups = false;  // a baseline of this handle had a repeated leaf digest
+  bool bulk_heavy = false;  // ... one repeated so often that a bucket overflowed: no bulk build
 
   uint64_t nfirst = 0, nshift = 0, last_diff_len = 0;
   dd_ckpt_stats stats{};  // of the last completed checkpoint
@@ -142,6 +143,7 @@ class Pipeline {
   bool low_root_ = false;   // some job uses the low-root rule
   uint64_t max_c_ = 0;      // chunks of the largest job
   uint32_t kinds_ = 0;      // insert modes (JOBS_DENSE | JOBS_LIST_NARROW | JOBS_LIST_WIDE) in the table
+  bool bulk_dedup_ = false;  // some bulk job runs the leaf duplicate search
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
   // Comparison of the tuner's candidate mappings on this pipeline's own
   // incremental checkpoints: each launch of a table with no baseline job

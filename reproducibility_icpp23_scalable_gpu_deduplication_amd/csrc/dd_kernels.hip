@@ -750,8 +750,9 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
   } else {
     __syncthreads();
   }
-  // A bulk job's map is built by k_bulk_build unless it fell back.
-  if ((sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) || (dense && J.bulk && !(sflags & FLAG_BULK_OFF))) {
+  // A bulk job's map is built by k_bulk_build (such a job has no workgroups
+  // here; a fallback runs the job again without it).
+  if ((sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) || (dense && J.bulk)) {
     if (threadIdx.x == 0) J.blk_claim[wg] = 0;
     return;
   }
@@ -1015,8 +1016,8 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
   if (threadIdx.x == 0) sflags = J.ctr->flags;
   if (Mode != JOBS_DENSE) load_list_prefix(J.dirty_cnt, pre);
   else __syncthreads();
-  // A bulk job's map is built by k_bulk_build unless it fell back.
-  const bool bulk_built = Mode == JOBS_DENSE && J.bulk && !(sflags & FLAG_BULK_OFF);
+  // A bulk job's map is built by k_bulk_build.
+  const bool bulk_built = Mode == JOBS_DENSE && J.bulk;
   if (!(sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)) && !bulk_built) {
     const InsertCtx ic = insert_ctx(J);
     const GlobalPtr<uint32_t> flags = &J.ctr->flags;
@@ -1083,10 +1084,9 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
 //   a digest but the lowest SHIFT (pass 1 needs the leaf labels); the final
 //   part writes the bucket's slots and the map slot of each such leaf.
 // A bucket with more leaf records than a workgroup holds (a digest repeated
-// thousands of times, as zero chunks are) switches its job to the dense
-// inserts (FLAG_BULK_OFF) before anything is written; a final bucket that
-// does not fit raises FLAG_MAP_FULL, and the host runs the job again without
-// the bulk build.
+// thousands of times, as zero chunks are; FLAG_BULK_OFF) or a final bucket
+// that does not fit stops the job (FLAG_MAP_FULL), and the host runs it
+// again with the dense inserts.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t bulk_bucket(const MapG& m, const Digest& d) {
   return (uint32_t)((map_start(m, d) & m.mask) >> kBucketLog);
@@ -1213,7 +1213,7 @@ __global__ __launch_bounds__(kColBuckets * kColSegs) void k_bulk_colscan(const J
   }
   if (s == 0 && inb) {
     bulk_tot(J, Int)[b] = tot;
-    if (!Int && tot > J.bk_cap_l) g_or(&J.ctr->flags, FLAG_BULK_OFF);
+    if (!Int && tot > J.bk_cap_l) g_or(&J.ctr->flags, FLAG_BULK_OFF | FLAG_MAP_FULL);  // later stages stop
   }
   const uint32_t gt = block_sum<uint32_t>(s == 0 && inb ? tot : 0u, sh);
   if (threadIdx.x == 0) bulk_grp(J, Int)[g] = gt;
@@ -1291,8 +1291,7 @@ __global__ __launch_bounds__(kPartThreads) void k_bulk_scatter(const JobG* __res
 
 // One workgroup per bucket.  Leaf part (Final = false): the bucket's leaf
 // records; every leaf whose digest has a lower leaf in the bucket is labelled
-// SHIFT (a FIXED|PENDING leaf keeps its label, as in the dense inserts).  A
-// job that fell back to the dense inserts clears the bucket instead.  Final
+// SHIFT (a FIXED|PENDING leaf keeps its label, as in the dense inserts).  Final
 // part: the bucket's leaf and interior records; the bucket's slots are
 // written in order (EMPTY where no record landed), each shifted leaf gets its
 // digest's slot, and the entry count goes to bk_claim.
@@ -1332,20 +1331,9 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
   __syncthreads();
   const uint32_t flags = sflags;
   if (!J.bulk || (flags & (FLAG_NEED_GROW | FLAG_MAP_FULL | FLAG_BULK_DUPS))) return;
-  // Without the duplicate search (bulk_dedup 0: a job whose last baseline
-  // had no repeated leaf digest) the leaf part only clears the buckets of a
-  // job that fell back.
-  if (!Final && !J.bulk_dedup && !(flags & FLAG_BULK_OFF)) return;
-  if (flags & FLAG_BULK_OFF) {
-    if (!Final) {  // the dense inserts fill the cleared map
-      for (uint64_t q = threadIdx.x; q < bs; q += blockDim.x) {
-        const GlobalPtr<u32x4_t> p = (GlobalPtr<u32x4_t>)&slots[q];
-        p[0] = u32x4_t{0, 0, 0, 0};
-        p[1] = u32x4_t{0xffffffffu, 0xffffffffu, 0, 0};
-      }
-    }
-    return;
-  }
+  // Without the duplicate search (bulk_dedup 0: a job whose baselines had no
+  // repeated leaf digest) the leaf part has nothing to do.
+  if (!Final && !J.bulk_dedup) return;
   const uint32_t nl = sn[0], ni = Final ? sn[1] : 0u, n = nl + ni;
   if (n > J.bk_cap_b || n > kBulkRec || n >= bs) {
     // Does not fit: the host runs the job again without the bulk build.
@@ -2302,7 +2290,7 @@ __global__ __launch_bounds__(1024) void k_sbin_scan(const JobG* __restrict__ job
   for (uint32_t i = threadIdx.x; i < J.nwg_ins_leaf; i += blockDim.x) claims += J.blk_claim[i];
   for (uint32_t i = threadIdx.x; i < J.nwg_ins_int; i += blockDim.x) claims += J.int_claim[i];
   __syncthreads();  // sflags
-  if (J.bulk && !(sflags & (FLAG_BULK_OFF | FLAG_NEED_GROW | FLAG_MAP_FULL)))
+  if (J.bulk && !(sflags & (FLAG_NEED_GROW | FLAG_MAP_FULL)))
     for (uint32_t i = threadIdx.x; i < J.nbuckets; i += blockDim.x) claims += J.bk_claim[i];
   claims = block_sum(claims, shl);  // its barriers publish the fields above
   const uint64_t pending = J.is_baseline ? J.C : spend;
@@ -3280,15 +3268,16 @@ void launch_gather(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs,
   else DD_LAUNCH(k_gather<false>, grid, 256, 0, s, jobs, prefix, njobs);
 }
 
-void launch_bulk(const JobDev* jobs_h, uint32_t njobs, bool final_part, const uint32_t* ppre, uint32_t pgrid,
-                 const uint32_t* spre, uint32_t sgrid, const uint32_t* bpre, uint32_t bgrid, hipStream_t s) {
+void launch_bulk(const JobDev* jobs_h, uint32_t njobs, bool final_part, bool dedup, const uint32_t* ppre,
+                 uint32_t pgrid, const uint32_t* spre, uint32_t sgrid, const uint32_t* bpre, uint32_t bgrid,
+                 hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
   if (!pgrid || !sgrid || !bgrid) return;
   if (!final_part) {
     DD_LAUNCH(k_bulk_hist<false>, pgrid, kPartThreads, 0, s, jobs, ppre, njobs);
     DD_LAUNCH(k_bulk_colscan<false>, sgrid, kColBuckets * kColSegs, 0, s, jobs, spre, njobs);
     DD_LAUNCH(k_bulk_scatter<false>, pgrid, kPartThreads, 0, s, jobs, ppre, njobs);
-    DD_LAUNCH(k_bulk_build<false>, bgrid, kBuildThreads, 0, s, jobs, bpre, njobs);
+    if (dedup) DD_LAUNCH(k_bulk_build<false>, bgrid, kBuildThreads, 0, s, jobs, bpre, njobs);
   } else {
     DD_LAUNCH(k_bulk_hist<true>, pgrid, kPartThreads, 0, s, jobs, ppre, njobs);
     DD_LAUNCH(k_bulk_colscan<true>, sgrid, kColBuckets * kColSegs, 0, s, jobs, spre, njobs);

@@ -296,15 +296,16 @@ void launch_gather(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, u
                    hipStream_t s);
 
 // Bulk map build of the bulk jobs of a table (JobT::bulk).  Leaf part,
-// between the leaf kernel and the leaf inserts: counts, column scan, partition,
-// then the per-bucket duplicate search that labels every leaf but the lowest
-// of each digest SHIFT (a bucket of a job that fell back to the dense
-// inserts is cleared instead).  Final part, after pass 1: the same for the
-// first-occurrence interior nodes, then the per-bucket build that writes the
-// map.  `pgrid`/`ppre`: the partition tiles, `sgrid`/`spre`: the column-scan
-// groups, `bgrid`/`bpre`: one workgroup per bucket.
-void launch_bulk(const JobDev* jobs, uint32_t njobs, bool final_part, const uint32_t* ppre, uint32_t pgrid,
-                 const uint32_t* spre, uint32_t sgrid, const uint32_t* bpre, uint32_t bgrid, hipStream_t s);
+// between the leaf kernel and pass 1: counts, column scan, partition, then
+// (`dedup`: some job has bulk_dedup set) the per-bucket duplicate search
+// that labels every leaf but the lowest of each digest SHIFT.  Final part,
+// after pass 1: the same for the first-occurrence interior nodes, then the
+// per-bucket build that writes the map.  `pgrid`/`ppre`: the partition
+// tiles, `sgrid`/`spre`: the column-scan groups, `bgrid`/`bpre`: one
+// workgroup per bucket.
+void launch_bulk(const JobDev* jobs, uint32_t njobs, bool final_part, bool dedup, const uint32_t* ppre,
+                 uint32_t pgrid, const uint32_t* spre, uint32_t sgrid, const uint32_t* bpre, uint32_t bgrid,
+                 hipStream_t s);
 void launch_map_clear(Slot* slots, uint64_t cap, hipStream_t s);
 void launch_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView nmap, uint32_t* flags, hipStream_t s);
 
