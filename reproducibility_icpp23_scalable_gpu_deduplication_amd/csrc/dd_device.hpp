@@ -519,24 +519,34 @@ __host__ __device__ inline uint64_t run_leaf(const LeafRuns& r, uint64_t idx) {
 // map when it is smaller): a probe sequence starts at the home slot and wraps
 // around inside the home slot's bucket, so each bucket is an independent
 // linear-probing table.  A baseline builds every bucket at once in LDS and
-// writes it out whole (k_bulk_build).
+// writes it out whole (k_bulk_build).  The capacity is a multiple of
+// kBucketSlots (a power of two of 1024 .. 4096 slots for small maps), not a
+// power of two: the bucket of a digest comes from a multiply-high of its
+// mixed key by the bucket count.
 constexpr uint32_t kBucketLog = 12;
 constexpr uint32_t kBucketSlots = 1u << kBucketLog;
 
 template <template <class> class P>
 struct MapT {
   P<Slot> slots;
-  uint64_t mask;   // capacity - 1
-  uint64_t bmask;  // slots per bucket - 1
-  uint32_t shift;  // 64 - log2(capacity)
+  uint64_t nbuckets;  // capacity / (bmask + 1)
+  uint64_t bmask;     // slots per bucket - 1
 };
 using MapView = MapT<HostPtr>;  // as the host fills it
 using MapG = MapT<GlobalPtr>;   // as the kernels read it
 static_assert(sizeof(MapView) == sizeof(MapG), "one layout for both pointer kinds");
 
+// Home slot of a digest: bucket from the high bits of the mixed key, slot in
+// the bucket from its bits 20 and up.
 template <class M>
 __host__ __device__ inline uint64_t map_start(const M& m, const Digest& k) {
-  return ((k.k0 ^ (k.k1 * 0x9e3779b97f4a7c15ull)) * 0xff51afd7ed558ccdull) >> m.shift;
+  const uint64_t h = (k.k0 ^ (k.k1 * 0x9e3779b97f4a7c15ull)) * 0xff51afd7ed558ccdull;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint64_t b = __umul64hi(h, m.nbuckets);
+#else
+  const uint64_t b = (uint64_t)(((unsigned __int128)h * m.nbuckets) >> 64);
+#endif
+  return b * (m.bmask + 1) + ((h >> 20) & m.bmask);
 }
 
 // Slot e places after s in s's bucket (wrapping inside the bucket).

@@ -79,10 +79,17 @@ uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id) {
 static MapView make_view(void* slots, uint64_t cap) {
   MapView m;
   m.slots = reinterpret_cast<Slot*>(slots);
-  m.mask = cap - 1;
-  m.bmask = std::min<uint64_t>(cap, kBucketSlots) - 1;
-  m.shift = 64 - ilog2_64(cap);
+  const uint64_t bs = std::min<uint64_t>(cap, kBucketSlots);
+  m.bmask = bs - 1;
+  m.nbuckets = cap / bs;
   return m;
+}
+
+// Map capacity for `slots` slots: a power of two of at least 1024 up to one
+// bucket, else a whole number of buckets.
+static uint64_t map_capacity(uint64_t slots) {
+  if (slots <= kBucketSlots) return next_pow2(std::max<uint64_t>(slots, 1024));
+  return ceil_div(slots, kBucketSlots) * kBucketSlots;
 }
 
 static uint64_t map_limit_of(uint64_t cap) { return cap / 2 + cap / 4; }
@@ -113,12 +120,12 @@ dd_status StreamState::grow_map(uint64_t need, hipStream_t s) {
   // baseline, under the load limit: a chain that adds many entries per
   // checkpoint grows in fewer, larger steps (each growth rewrites the whole
   // table).
-  uint64_t cap = next_pow2(std::max<uint64_t>(2 * need, 1024));
+  uint64_t cap = map_capacity(2 * need);
   if (ckpts_since_base > 0 && map_count > base_map_count) {
     const uint64_t rate = (map_count - base_map_count) / ckpts_since_base;
-    cap = std::max(cap, next_pow2((need + kGrowHorizon * rate) / 3 * 4));
+    cap = std::max(cap, map_capacity((need + kGrowHorizon * rate) / 3 * 4));
   }
-  if (cap <= map_cap) cap = map_cap * 2;
+  if (cap <= map_cap) cap = map_capacity(map_cap * 2);
   DevBuf fresh;
   DD_TRY(fresh.ensure(cap * sizeof(Slot)));
   launch_map_clear(fresh.as<Slot>(), cap, s);
@@ -214,7 +221,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     DD_TRY(tree.ensure(Nn * sizeof(Digest)));
     DD_TRY(labels.ensure(ceil_div(Nn, kNodesPerBlock) * kNodesPerBlock + 64));  // root scans read 16 at a time
     DD_TRY(slot_of.ensure(Nn * 4));
-    const uint64_t cap = next_pow2(std::max<uint64_t>(2 * Nn, 1024));
+    const uint64_t cap = map_capacity(2 * Nn);
     if (map.bytes > 4 * cap * sizeof(Slot)) map.release();
     DD_TRY(map.ensure(cap * sizeof(Slot)));
     map_cap = cap;

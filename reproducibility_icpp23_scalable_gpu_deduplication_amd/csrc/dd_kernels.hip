@@ -655,7 +655,7 @@ __device__ inline InsertResult map_insert(const InsertCtx& J, const Digest& k, u
   constexpr uint32_t RW = Wide ? 2 * kLineSlots : 2;  // slots per round after the first
   const MapG& m = J.m;
   const unsigned long long mine = pack_node(J.ckpt, node);
-  uint64_t s = map_start(m, k) & m.mask;
+  uint64_t s = map_start(m, k);
   InsertResult r;
   const Digest none{0, 0};
   uint64_t probed = 0;  // slots examined (bounds the walk of a full bucket)
@@ -1089,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_insert_interior(const JobG* __restrict_
 // again with the dense inserts.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t bulk_bucket(const MapG& m, const Digest& d) {
-  return (uint32_t)((map_start(m, d) & m.mask) >> kBucketLog);
+  return (uint32_t)(map_start(m, d) >> kBucketLog);
 }
 
 __device__ __forceinline__ bool bulk_skip(const JobG& J, uint32_t flags) {
@@ -1756,7 +1756,7 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
   for (int n = 0; n < NK; n++) {
     out[n] = ~0ull;
     todo[n] = need[n];
-    s[n] = map_start(m, key[n]) & m.mask;
+    s[n] = map_start(m, key[n]);
   }
   for (uint64_t it = 0; it <= m.bmask; it += 2) {
     uint4 k[NK][2];
@@ -2582,7 +2582,7 @@ __global__ void k_map_rehash(const Slot* old_slots, uint64_t old_cap, MapView m,
     const Slot& o = old_slots[i];
     if (o.val == SLOT_EMPTY) continue;
     const Digest k{o.k0, o.k1};
-    uint64_t s = map_start(m, k) & m.mask;
+    uint64_t s = map_start(m, k);
     bool done = false;
     for (uint64_t it = 0; it <= m.bmask; it++) {
       if (atomicCAS(&m.slots[s].val, SLOT_EMPTY, o.val) == SLOT_EMPTY) {

@@ -64,8 +64,9 @@ bool groupable(const Request* a, const Request* b) {
 // smallest work lists).
 uint64_t state_bytes(uint64_t len, uint32_t chunk) {
   const uint64_t C = (len + chunk - 1) / chunk, N = 2 * C - 1;
-  uint64_t cap = 1;
-  while (cap < 2 * N) cap <<= 1;
+  uint64_t cap = 1024;  // the map's capacity rule: a power of two up to 4096 slots, else whole buckets of 4096
+  while (cap < 2 * N && cap < 4096) cap <<= 1;
+  if (2 * N > 4096) cap = (2 * N + 4095) / 4096 * 4096;
   const uint64_t tiles = (C + 32767) / 32768, buckets = (cap + 4095) / 4096;
   return 21 * N + 32 * cap + 44 * C + 32 * N + 8 * tiles * buckets + 16 * buckets + (128u << 10);
 }
