@@ -1550,20 +1550,44 @@ __device__ __forceinline__ void tile_labels(const JobG& J, int g, const TileLane
   for (int k = 0; k < 2; k++) rawu[k] = vu[k] ? u[k] : (uint8_t)0xff;
 }
 
-// Digests of the nodes labelled `want` among a lane's bottom nodes and
-// level-1 leaves (zero elsewhere), loaded together from clamped addresses.
+// Digests of the nodes labelled `want` that a lane combines: a pair of
+// bottom nodes when both are labelled `want` (their parent's digest is the
+// hash of the two), a level-1 leaf when its level-1 sibling may be `want`
+// too (a leaf so labelled, or a node whose two bottom children are); zero
+// elsewhere.  A node whose sibling differs never reaches a parent digest.
+// The loads are issued only when some lane of the wavefront needs one, all
+// together from clamped addresses: a tile of isolated changes (most tiles
+// when changes are spread thin) skips the round trip.
 __device__ __forceinline__ void tile_digests(GlobalPtr<const Digest> tree, const TileLane& T, const uint8_t bl[4],
-                                             const uint8_t ul[2], uint8_t want, Digest bd[4], Digest ud[2]) {
+                                             const uint8_t ul[2], const bool vu[2], uint8_t want, Digest bd[4],
+                                             Digest ud[2]) {
+  bool nb[4], nu[2], m1[2];
 #pragma unroll
-  for (int k = 0; k < 4; k++) bd[k] = load_digest(tree + (bl[k] == want ? T.xb + k : 0));
+  for (int k = 0; k < 2; k++) {
+    const bool pair = bl[2 * k] == want && bl[2 * k + 1] == want;
+    nb[2 * k] = nb[2 * k + 1] = pair;
+    m1[k] = vu[k] ? ul[k] == want : (T.v1[k] && pair);
+  }
 #pragma unroll
-  for (int k = 0; k < 2; k++) ud[k] = load_digest(tree + (ul[k] == want ? T.x1 + k : 0));
+  for (int k = 0; k < 2; k++) nu[k] = vu[k] && ul[k] == want && m1[k ^ 1];
+  const bool any = nb[0] || nb[2] || nu[0] || nu[1];
+  if (!__ballot(any)) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) bd[k] = Digest{0, 0};
+#pragma unroll
+    for (int k = 0; k < 2; k++) ud[k] = Digest{0, 0};
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) bd[k] = load_digest(tree + (nb[k] ? T.xb + k : 0));
+#pragma unroll
+  for (int k = 0; k < 2; k++) ud[k] = load_digest(tree + (nu[k] ? T.x1 + k : 0));
 #pragma unroll
   for (int k = 0; k < 4; k++)
-    if (bl[k] != want) bd[k] = Digest{0, 0};
+    if (!nb[k]) bd[k] = Digest{0, 0};
 #pragma unroll
   for (int k = 0; k < 2; k++)
-    if (ul[k] != want) ud[k] = Digest{0, 0};
+    if (!nu[k]) ud[k] = Digest{0, 0};
 }
 
 // Tiles of one workgroup of a tile pass: iterations n = 0 .. niter-1 take
@@ -1634,9 +1658,9 @@ __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__
     for (int k = 0; k < 4; k++) bl[k] = g == 0 && T.vb[k] ? resolved_label(raw[k]) : raw[k];
 #pragma unroll
     for (int k = 0; k < 2; k++) ul[k] = vu[k] ? resolved_label(rawu[k]) : (uint8_t)0xff;
-    // ... then the digests of the first occurrences among them.
+    // ... then the digests of the first-occurrence pairs among them.
     Digest bd[4], ud[2];
-    tile_digests(tree, T, bl, ul, FIRST_OCUR, bd, ud);
+    tile_digests(tree, T, bl, ul, vu, FIRST_OCUR, bd, ud);
     // Levels 1 and 2 in registers.
     uint8_t l1[2];
     Digest d1[2];
@@ -1815,12 +1839,12 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
   for (uint32_t n = 0; n < niter; n++) {
     const TileLane T = tile_lane(tile_ref(J, g, tile_at(J, listed, pre, ids, wg, nwg, niter, n)), N, i);
     // Loads: labels (leaves were resolved by pass 1), then the digests of
-    // the shifted ones.
+    // the shifted pairs.
     uint8_t bl[4], ul[2];
     bool vu[2];
     tile_labels(J, g, T, C, bl, ul, vu, FIXED_DUPL);
     Digest bd[4], ud[2];
-    tile_digests(tree, T, bl, ul, SHIFT_DUPL, bd, ud);
+    tile_digests(tree, T, bl, ul, vu, SHIFT_DUPL, bd, ud);
     // Level 1: both lookups of the lane at once.
     bool look[2];
     Digest d1[2];
