@@ -138,7 +138,9 @@ dd_status dd_checkpoint_header(dd_dedup* h, dd_header* header, const void* dev_d
  * output buffer_d before its D2H copy, src/tree_approach.cpp:1668-1684): the
  * diff is written to `dev_out` (capacity `out_cap`, at least
  * dd_max_diff_size(len, chunk_size, current id)) and its length to *out_len.
- * Advances the checkpoint id. */
+ * Bytes of `dev_out` past *out_len, within that bound, may be overwritten
+ * (a baseline's leaf kernel copies every chunk into the data section before
+ * the labels decide the diff's layout).  Advances the checkpoint id. */
 dd_status dd_checkpoint_device(dd_dedup* h, const void* dev_data, uint64_t len, int make_baseline,
                                void* dev_out, uint64_t out_cap, uint64_t* out_len);
 
@@ -209,7 +211,9 @@ dd_status dd_batch_create(dd_dedup* const* handles, uint32_t n, dd_batch** out);
 dd_status dd_batch_destroy(dd_batch* b);
 dd_status dd_batch_use_graph(dd_batch* b, int enable);
 /* Arrays have one entry per handle.  dev_out[i] must hold at least
- * dd_max_diff_size(lens[i], chunk_size, current id of handle i) bytes. */
+ * dd_max_diff_size(lens[i], chunk_size, current id of handle i) bytes; as in
+ * dd_checkpoint_device, bytes past out_lens[i] within that bound may be
+ * overwritten. */
 dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, const uint64_t* lens,
                                      const int* make_baseline, void* const* dev_out, const uint64_t* out_caps,
                                      uint64_t* out_lens);

@@ -481,6 +481,15 @@ struct LeafRuns {
   uint64_t a0, n0, a1, n1;
 };
 
+// Rank of chunk k among the root's leaves in left-to-right order (its place
+// in the data section of a diff whose only region is the root).  The deepest
+// level holds chunks kd..C-1 (kd = 2^D - C) and comes first, then the level
+// above with chunks 0..kd-1: subtree_runs(g, 0) as a rotation.
+__host__ __device__ inline uint64_t root_rank(const TreeGeom& g, uint64_t k) {
+  const uint64_t kd = (1ull << g.D) - g.C;
+  return k >= kd ? k - kd : k + (g.C - kd);
+}
+
 __host__ __device__ inline LeafRuns subtree_runs(const TreeGeom& g, uint64_t v) {
   LeafRuns r{0, 0, 0, 0};
   if (v >= g.C - 1) {  // a leaf

@@ -371,6 +371,11 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.ptab = ptab.as<uint32_t>();
   job.diff = out;
   job.diff_cap = out_cap;
+  // A tree baseline's leaf kernel copies the chunks into the data section
+  // when the handle's last baseline had every chunk as a first occurrence
+  // (the pipeline keeps the request only with a leaf mapping that stages
+  // the chunks; out_cap >= max_diff_size covers the whole copy).
+  job.write_data = (baseline && method != DD_METHOD_LIST && !data_miss) ? 1 : 0;
   return DD_OK;
 }
 
@@ -398,6 +403,9 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
   }
   last_diff_len = c.diff_len;
   last_map_ckpt = current_id;
+  // The root alone as the first-occurrence region: the data section is every
+  // chunk in the root's leaf order.
+  if (baseline) data_miss = !(c.nfirst == 1 && c.nprior == 0 && c.nshift == 0 && c.dchunks == C);
 }
 
 // ---------------------------------------------------------------------------
@@ -473,6 +481,9 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
       }
     }
   }
+  if (!incremental) leaf_map_ = leaf_mapping_with_copy(jobs, aligned16_, hash_kind_, leaf_map_);
+  const bool copy_data = leaf_writes_data(leaf_map_);
+  copy_ = false;
   const size_t nstages = S_TILE0 + max_groups_;
   const size_t need_prefix = nstages * (nj + 1);
   const size_t jobs_bytes = (nj * sizeof(JobDev) + 255) & ~size_t(255);
@@ -578,6 +589,8 @@ dd_status Pipeline::upload(const std::vector<JobDev>& jobs, hipStream_t s) {
   for (size_t j = 0; j < nj; j++) {
     hjobs[j] = jobs[j];
     hjobs[j].host_ctr = &host_ctr[j];
+    hjobs[j].write_data = copy_data ? jobs[j].write_data : 0;
+    copy_ = copy_ || hjobs[j].write_data;
     hjobs[j].nwg_ins_leaf = hprefix[off_[S_INSL] + j + 1] - hprefix[off_[S_INSL] + j];
     // A list job launches no interior inserts, so it has no interior claims
     // for sbin_scan to add to its entry count.
@@ -701,7 +714,7 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
       }
       trial_used_[0] = e0;
       trial_used_[1] = e1;
-      launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s, e0, e1);
+      launch_leaf(J, P(S_LEAF), nj, grid_[S_LEAF], leaf_map_, aligned16_, hash_kind_, s, e0, e1, copy_);
     }
     DD_TRY(mark(1));
     if (part == PART_HEAD) {

@@ -65,6 +65,9 @@ struct StreamState {
   bool bulk_now = false;  // the baseline being prepared uses the bulk build
   bool bulk_dups = false;  // a baseline of this handle had a repeated leaf digest
   bool bulk_heavy = false;  // ... one repeated so often that a bucket overflowed: no bulk build
+  // The last baseline's diff had another layout than the root as its only
+  // region: the next baseline's leaf kernel copies no data (JobT::write_data).
+  bool data_miss = false;
 
   uint64_t nfirst = 0, nshift = 0, last_diff_len = 0;
   dd_ckpt_stats stats{};  // of the last completed checkpoint
@@ -145,6 +148,7 @@ class Pipeline {
   uint32_t kinds_ = 0;      // insert modes (JOBS_DENSE | JOBS_LIST_NARROW | JOBS_LIST_WIDE) in the table
   bool bulk_dedup_ = false;  // some bulk job runs the leaf duplicate search
   int leaf_map_ = LEAF_DIRECT;  // leaf kernel mapping (LeafMapping), chosen by leaf_mapping_for
+  bool copy_ = false;  // some job of the table has write_data: the leaf kernel's copy variant
   // Comparison of the tuner's candidate mappings on this pipeline's own
   // incremental checkpoints: each launch of a table with no baseline job
   // runs the candidate with the fewest samples, timed with events on the
@@ -193,6 +197,13 @@ uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
 // tuner's sample first).
 int leaf_mapping_for(const std::vector<JobDev>& jobs, bool aligned16, int hash_kind, hipStream_t s,
                      std::vector<uint64_t>* key = nullptr, std::vector<int>* cand = nullptr);
+// The mapping for a table whose baselines ask for the data-section copy
+// (JobT::write_data): `current` when it makes the copy, is forced, or the
+// asking jobs hold less than half of the table's bytes; else DMA, or LDS64
+// where chunks are not a multiple of 128 bytes (a copy in the leaf kernel
+// saves the gather a second read of the buffer, which costs more than the
+// mapping's difference in hashing speed).
+int leaf_mapping_with_copy(const std::vector<JobDev>& jobs, bool aligned16, int hash_kind, int current);
 // Settles a key's mapping after its comparison on checkpoints.
 void leaf_mapping_settle(const std::vector<uint64_t>& key, int mapping);
 // Candidate batch-wide workgroup budgets of the incremental stages, the

@@ -111,6 +111,54 @@ __global__ void k_begin(const JobG* __restrict__ jobs, uint32_t njobs, int rerun
 }
 
 // ---------------------------------------------------------------------------
+typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// One piece of chunk data at `src` (16 bytes, 16-byte aligned, when Vec;
+// else one byte, in .x), zero at and past `avail` bytes.
+template <bool Vec>
+__device__ __forceinline__ uint4 load_piece(GlobalPtr<const uint8_t> src, uint64_t avail) {
+  if (!Vec) return make_uint4(avail > 0 ? *src : 0u, 0, 0, 0);
+  if (avail >= 16) return gload16((GlobalPtr<const void>)src);
+  uint32_t w[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const uint64_t o = 4 * k + b;
+      if (o < avail) x |= (uint32_t)src[o] << (8 * b);
+    }
+    w[k] = x;
+  }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Stores a piece to the 4-byte aligned `dst`.
+template <bool Vec>
+__device__ __forceinline__ void store_piece(GlobalPtr<uint8_t> dst, const uint4& v) {
+  if (Vec) *(GlobalPtr<u32x4a4>)dst = u32x4a4{v.x, v.y, v.z, v.w};
+  else *dst = (uint8_t)v.x;
+}
+
+// Data-section copy of a tree baseline (JobT::write_data): byte 0 of chunk
+// k's place when the root is the diff's only region, after the 32-byte
+// header and the root's 4-byte node id.
+__device__ __forceinline__ GlobalPtr<uint8_t> root_data_dst(const JobG& J, const TreeGeom& g, uint64_t k) {
+  return J.diff + 36 + root_rank(g, k) * (uint64_t)J.chunk_size;
+}
+
+// The partial last chunk's copy, zero padded to a whole chunk as the gather
+// writes it, by the wavefront that holds chunk nfull (16 bytes per lane per
+// step; the buffer is 16-byte aligned and the chunk size a multiple of 16).
+__device__ __forceinline__ void copy_partial_chunk(const JobG& J, const TreeGeom& g, uint32_t lane) {
+  const uint64_t cs = J.chunk_size;
+  const uint64_t off = J.nfull * cs;
+  const uint64_t nb = J.len - off;
+  const GlobalPtr<uint8_t> dst = root_data_dst(J, g, J.nfull);
+  for (uint64_t o = (uint64_t)lane * 16; o < cs; o += 1024)
+    store_piece<true>(dst + o, load_piece<true>(J.data + off + o, o < nb ? nb - o : 0));
+}
+
 // leaf_hash: one lane per chunk.  Leaf digests are written in place over the
 // previous checkpoint's digests, only when they change.
 // Reference: "Checkpoint K: Leaves", src/tree_approach.cpp:305-351 and
@@ -316,8 +364,10 @@ __device__ __forceinline__ bool leaf_finish(const JobG& J, uint64_t k, const Dig
 // 16-byte aligned buffer; the partial last chunk is hashed from global memory.
 // PF: the loads of round r+1 go to registers before round r is hashed, and
 // are stored to LDS at the start of the next round (long chunks: a wavefront
-// keeps a round of loads in flight while it hashes).
-template <int Kind, int SEG, bool PF>
+// keeps a round of loads in flight while it hashes).  Copy: the variant for
+// tables with a data-section copy (JobT::write_data); it holds more
+// registers, so the other tables launch the one without.
+template <int Kind, int SEG, bool PF, bool Copy>
 __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   constexpr int PITCH = SEG + 16;
   constexpr int VPC = SEG / 16;        // 16-byte vectors per chunk per round
@@ -333,6 +383,8 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
   uint8_t* L = lds[wave];
   const GlobalPtr<const uint8_t> base = J.data + k0 * cs;
   const uint32_t rounds = (uint32_t)(cs / SEG);
+  const bool wr = Copy && J.write_data != 0;
+  const TreeGeom g = make_geom(J.C);
   Digest prev{0, 0};  // loaded first: its latency hides behind the chunk loads
   if (k < J.C && !J.is_baseline) prev = load_digest(J.tree + (J.C - 1 + k));
   Murmur m;
@@ -349,6 +401,11 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
       nx[i] = k0 + c < full ? gload16(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16) : make_uint4(0, 0, 0, 0);
     }
   };
+  // A baseline's data-section copy of what the wavefront loaded: chunk c's
+  // piece goes where the gather would put it (contiguous 16-byte stores).
+  auto copy_out = [&](uint32_t c, uint32_t piece, uint32_t r, const uint4& x) {
+    store_piece<true>(root_data_dst(J, g, k0 + c) + (uint64_t)r * SEG + piece * 16, x);
+  };
   if (PF && rounds > 0) load_round(0);
   for (uint32_t r = 0; r < rounds; r++) {
     if (PF) {
@@ -356,6 +413,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
       for (int i = 0; i < LOADS; i++) {
         const uint32_t v = i * 64 + lane;
         *reinterpret_cast<uint4*>(L + (v / VPC) * PITCH + (v % VPC) * 16) = nx[i];
+        if (wr && k0 + v / VPC < full) copy_out(v / VPC, v % VPC, r, nx[i]);
       }
       __syncthreads();
       if (r + 1 < rounds) load_round(r + 1);
@@ -367,6 +425,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
         if (k0 + c < full) {
           const uint4 x = gload16(base + (uint64_t)c * cs + (uint64_t)r * SEG + piece * 16);
           *reinterpret_cast<uint4*>(L + c * PITCH + piece * 16) = x;
+          if (wr) copy_out(c, piece, r, x);
         }
       }
       __syncthreads();
@@ -415,6 +474,7 @@ __global__ __launch_bounds__(256) void k_leaf_hash_lds(const JobG* __restrict__ 
     d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
     pending = leaf_finish(J, k, d, prev);
   }
+  if (wr && full < J.C && k0 <= full && full - k0 < 64) copy_partial_chunk(J, g, lane);
   record_pending(J, k, pending, d);
 }
 
@@ -446,8 +506,8 @@ __device__ __forceinline__ uint32_t dma_slot(uint32_t i, uint32_t j, uint32_t p)
 // has one 8 KiB round of loads in flight in LDS, so the LDS of a CU (160 KB)
 // bounds the bytes in flight; with long chunks (many rounds per wavefront,
 // few wavefronts) one-wavefront workgroups let 20 rather than 16 wavefronts
-// share a CU.
-template <int Kind, int WPG>
+// share a CU.  Copy: as in k_leaf_hash_lds.
+template <int Kind, int WPG, bool Copy>
 __global__ __launch_bounds__(64 * WPG) void k_leaf_hash_dma(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   __shared__ __attribute__((aligned(16))) uint8_t lds[WPG][8192];
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
@@ -474,6 +534,11 @@ __global__ __launch_bounds__(64 * WPG) void k_leaf_hash_dma(const JobG* __restri
   }
   const bool have = k0 < full;  // uniform: some chunk of this wave is complete
   const uint32_t rounds = (uint32_t)(cs >> 7);  // chunk_size is a multiple of 128
+  // A baseline's data-section copy: slot `lane` of block i is piece p of
+  // chunk c (as loaded above), stored from there to the chunk's place, so
+  // each store instruction covers 128 contiguous bytes of 8 chunks.
+  const bool wr = Copy && J.write_data != 0;
+  const TreeGeom g = make_geom(J.C);
   const uint32_t bi = lane >> 3, bj = lane & 7;  // this lane's chunk: block bi, position bj
   Murmur m;
   uint32_t st[4];
@@ -485,15 +550,27 @@ __global__ __launch_bounds__(64 * WPG) void k_leaf_hash_dma(const JobG* __restri
   }
   for (uint32_t r = 0; have && r < rounds; r++) {
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
-    uint4 v[8];
+    uint4 v[8], w[8];
 #pragma unroll
     for (uint32_t p = 0; p < 8; p++) v[p] = *reinterpret_cast<const uint4*>(L + bi * 1024 + dma_slot(bi, bj, p) * 16);
+    if (wr) {
+#pragma unroll
+      for (uint32_t i = 0; i < 8; i++) w[i] = *reinterpret_cast<const uint4*>(L + i * 1024 + lane * 16);
+    }
     __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // the reads are done before the region is refilled
     if (r + 1 < rounds) {
 #pragma unroll
       for (uint32_t i = 0; i < 8; i++)
         __builtin_amdgcn_global_load_lds((global_void_t*)(src[i] + (uint64_t)(r + 1) * 128), (lds_void_t*)(L + i * 1024),
                                          16, 0, 0);
+    }
+    if (wr) {
+#pragma unroll
+      for (uint32_t i = 0; i < 8; i++) {
+        const uint32_t p = (i & 1) ? po : pe, jj = (i & 1) ? jo : je;
+        const uint64_t c = k0 + i * 8 + jj;
+        if (c < full) store_piece<true>(root_data_dst(J, g, c) + (uint64_t)r * 128 + p * 16, w[i]);
+      }
     }
     if (k < full) {
       if (Kind == HASH_MD5) {
@@ -532,6 +609,7 @@ __global__ __launch_bounds__(64 * WPG) void k_leaf_hash_dma(const JobG* __restri
     d = (Kind == HASH_MD5) ? md5_range<true>(p, nb) : murmur3_aligned(p, nb);
     pending = leaf_finish(J, k, d, prev);
   }
+  if (wr && full < J.C && k0 <= full && full - k0 < 64) copy_partial_chunk(J, g, lane);
   record_pending(J, k, pending, d);
 }
 
@@ -2497,35 +2575,6 @@ __device__ __forceinline__ uint32_t region_of(Ptr f_off, uint32_t F, uint64_t q)
   return lo;
 }
 
-typedef uint32_t u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
-
-// One piece of chunk data at `src` (16 bytes, 16-byte aligned, when Vec;
-// else one byte, in .x), zero at and past `avail` bytes.
-template <bool Vec>
-__device__ __forceinline__ uint4 load_piece(GlobalPtr<const uint8_t> src, uint64_t avail) {
-  if (!Vec) return make_uint4(avail > 0 ? *src : 0u, 0, 0, 0);
-  if (avail >= 16) return gload16((GlobalPtr<const void>)src);
-  uint32_t w[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    uint32_t x = 0;
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const uint64_t o = 4 * k + b;
-      if (o < avail) x |= (uint32_t)src[o] << (8 * b);
-    }
-    w[k] = x;
-  }
-  return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-// Stores a piece to the 4-byte aligned `dst`.
-template <bool Vec>
-__device__ __forceinline__ void store_piece(GlobalPtr<uint8_t> dst, const uint4& v) {
-  if (Vec) *(GlobalPtr<u32x4a4>)dst = u32x4a4{v.x, v.y, v.z, v.w};
-  else *dst = (uint8_t)v.x;
-}
-
 // The data section: groups of 256 chunks per workgroup iteration.
 template <bool Vec>
 __device__ __forceinline__ void gather_data(const JobG& J, uint32_t F, uint32_t P, uint32_t S, uint64_t Dq,
@@ -2579,7 +2628,10 @@ __global__ __launch_bounds__(256) void k_gather(const JobG* __restrict__ jobs, c
   const uint32_t F = J.ctr->nfirst, P = J.ctr->nprior, S = J.ctr->nshift;
   const uint64_t Dq = J.ctr->dchunks;
   const uint32_t wg = blockIdx.x - prefix[j], nblk = prefix[j + 1] - prefix[j];
-  gather_data<Vec>(J, F, P, S, Dq, wg, nblk);
+  // The leaf kernel already copied the data section when the root is the
+  // only region (the one region that holds all C chunks).
+  const bool copied = J.write_data && F == 1 && P == 0 && S == 0 && Dq == J.C;
+  if (!copied) gather_data<Vec>(J, F, P, S, Dq, wg, nblk);
   // Stores only from here on: the metadata, and (the roots are written) the
   // reset of the labels of the tiles this checkpoint visited.
   write_meta(J, F, P, S, wg * blockDim.x + threadIdx.x, nblk * blockDim.x);
@@ -3189,18 +3241,25 @@ void launch_begin(const JobDev* jobs_h, const uint32_t*, uint32_t njobs, bool re
 #define DD_LEAF_LAUNCH(kern) DD_LEAF_LAUNCH_T(kern, 256)
 
 void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping, bool al16,
-                 int kind, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
+                 int kind, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop, bool copy) {
   const JobG* jobs = gview(jobs_h);
   if (grid == 0) return;
   if (!al16) mapping = LEAF_DIRECT;  // the other mappings need 16-byte aligned buffers
+  copy = copy && leaf_writes_data(mapping);
   if (kind == HASH_MD5) {
     switch (mapping) {
-      case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64, false>));
-      case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, false>));
-      case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256, false>));
-      case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, true>));
-      case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5, 4>));
-      case LEAF_DMA1: DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MD5, 1>), 64);
+      case LEAF_LDS64:
+        if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64, false, true>));
+        DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64, false, false>));
+      case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, false, false>));
+      case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256, false, false>));
+      case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, true, false>));
+      case LEAF_DMA:
+        if (copy) DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5, 4, true>));
+        DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5, 4, false>));
+      case LEAF_DMA1:
+        if (copy) DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MD5, 1, true>), 64);
+        DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MD5, 1, false>), 64);
       default:
         if (al16) DD_LEAF_LAUNCH((k_leaf_hash<HASH_MD5, true, LEAF_DIRECT>));
         DD_LEAF_LAUNCH((k_leaf_hash<HASH_MD5, false, LEAF_DIRECT>));
@@ -3209,12 +3268,18 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
   switch (mapping) {
     case LEAF_DEEP: DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DEEP>));
     case LEAF_SHORT: DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_SHORT>));
-    case LEAF_LDS64: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64, false>));
-    case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, false>));
-    case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256, false>));
-    case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, true>));
-    case LEAF_DMA: DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3, 4>));
-    case LEAF_DMA1: DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MURMUR3, 1>), 64);
+    case LEAF_LDS64:
+      if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64, false, true>));
+      DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64, false, false>));
+    case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, false, false>));
+    case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256, false, false>));
+    case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, true, false>));
+    case LEAF_DMA:
+      if (copy) DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3, 4, true>));
+      DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3, 4, false>));
+    case LEAF_DMA1:
+      if (copy) DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MURMUR3, 1, true>), 64);
+      DD_LEAF_LAUNCH_T((k_leaf_hash_dma<HASH_MURMUR3, 1, false>), 64);
     default:
       if (al16) DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, true, LEAF_DIRECT>));
       DD_LEAF_LAUNCH((k_leaf_hash<HASH_MURMUR3, false, LEAF_DIRECT>));

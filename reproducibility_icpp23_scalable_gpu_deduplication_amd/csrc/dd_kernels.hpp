@@ -166,6 +166,12 @@ struct JobT {
   // output
   P<uint8_t> diff;
   uint64_t diff_cap;
+  // 1: a tree baseline whose leaf kernel also copies every chunk into the
+  // data section of the diff, at the place it takes when the root is the
+  // only region (every chunk a first occurrence); the gather then copies no
+  // data unless the labels found another layout, which overwrites it.
+  uint32_t write_data;
+  uint32_t pad_;
 };
 using JobDev = JobT<HostPtr>;
 using JobG = JobT<GlobalPtr>;
@@ -263,12 +269,17 @@ enum LeafMapping : int {
   LEAF_NMAP = 9
 };
 // ev_start/ev_stop (optional) are recorded at the start and the end of the
-// leaf kernel's own dispatch.
+// leaf kernel's own dispatch.  `copy`: some job of the table has write_data
+// set (the mapping must be one with leaf_writes_data).
 void launch_leaf(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int mapping,
                  bool all_aligned16, int hash_kind, hipStream_t s, hipEvent_t ev_start = nullptr,
-                 hipEvent_t ev_stop = nullptr);
+                 hipEvent_t ev_stop = nullptr, bool copy = false);
 // Workgroups of the leaf stage for a job of C chunks.
 inline uint64_t leaf_workgroups(int mapping, uint64_t C) { return mapping == LEAF_DMA1 ? (C + 63) / 64 : (C + 255) / 256; }
+// Mappings with a kernel variant that copies the chunks into a baseline's
+// data section (JobT::write_data): each stages whole rounds of 64 chunks, so
+// its stores are contiguous.
+inline bool leaf_writes_data(int mapping) { return mapping == LEAF_DMA || mapping == LEAF_DMA1 || mapping == LEAF_LDS64; }
 // Insert modes of the jobs in a table (bit mask): the insert stages launch
 // one kernel variant per mode present.  Dense: a baseline (every leaf and
 // interior node, narrow probe rounds); list, narrow or wide rounds: an

@@ -109,6 +109,19 @@ double time_mapping(const JobDev* dj, const uint32_t* dprefix, uint32_t njobs, u
 
 }  // namespace
 
+int leaf_mapping_with_copy(const std::vector<JobDev>& jobs, bool al16, int kind, int current) {
+  if (leaf_writes_data(current) || forced_mapping() >= 0) return current;
+  uint64_t copy = 0, total = 0;
+  for (const JobDev& J : jobs) {
+    total += J.len;
+    if (J.write_data) copy += J.len;
+  }
+  if (2 * copy < total) return current;
+  for (int m : {LEAF_DMA, LEAF_LDS64})
+    if (mapping_valid(m, al16, kind, jobs)) return m;
+  return current;
+}
+
 int leaf_mapping_for(const std::vector<JobDev>& jobs, bool al16, int kind, hipStream_t s, std::vector<uint64_t>* key_out,
                      std::vector<int>* cand_out) {
   if (cand_out) cand_out->clear();
