@@ -91,10 +91,12 @@ def traffic_from_profiles(kernel, workload):
 
 
 def leaf_bytes(st, chunk):
-    """Algorithmic bytes of one leaf-kernel pass over a buffer (ckpt_stats)."""
+    """Algorithmic bytes of one leaf-kernel pass over a buffer (ckpt_stats).
+    A baseline's leaf kernel that also wrote the diff's data section
+    (leaf_copy) moved a chunk more per chunk."""
     C = st["chunks"]
     if st["baseline"]:
-        return C * (chunk + 33)
+        return C * (chunk + 33) + (C * chunk if st.get("leaf_copy") else 0)
     return C * (chunk + 16) + 17 * st["changed"]
 
 

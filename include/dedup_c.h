@@ -303,7 +303,7 @@ typedef struct dd_ckpt_stats {
   uint64_t first_chunks;  /* chunks copied into the diff's data section */
   uint64_t diff_bytes;    /* bytes of the diff */
   uint32_t baseline;      /* 1 when it ran as a baseline */
-  uint32_t pad;
+  uint32_t leaf_copy;     /* 1 when the leaf kernel also wrote the data section (a baseline) */
 } dd_ckpt_stats;
 dd_status dd_get_ckpt_stats(dd_dedup* h, dd_ckpt_stats* out);
 /* Device bytes the handle holds now: tree, digest map, scratch arrays and

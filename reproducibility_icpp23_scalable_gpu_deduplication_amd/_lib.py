@@ -58,7 +58,7 @@ class dd_header(ctypes.Structure):
 
 class dd_ckpt_stats(ctypes.Structure):
     _fields_ = [("chunks", ctypes.c_uint64), ("changed", ctypes.c_uint64), ("first_chunks", ctypes.c_uint64),
-                ("diff_bytes", ctypes.c_uint64), ("baseline", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+                ("diff_bytes", ctypes.c_uint64), ("baseline", ctypes.c_uint32), ("leaf_copy", ctypes.c_uint32)]
 
 
 class dd_request_result(ctypes.Structure):
@@ -285,10 +285,11 @@ class TreeDeduplicator:
     def ckpt_stats(self):
         """Sizes of the last completed checkpoint: chunks, changed leaves
         (every leaf at a baseline), first-occurrence chunks copied, diff
-        bytes, baseline flag."""
+        bytes, baseline flag, and whether the leaf kernel wrote the data
+        section (leaf_copy)."""
         st = dd_ckpt_stats()
         _check(load().dd_get_ckpt_stats(self._h, ctypes.byref(st)), "dd_get_ckpt_stats")
-        return {k: getattr(st, k) for k, _ in dd_ckpt_stats._fields_ if k != "pad"}
+        return {k: getattr(st, k) for k, _ in dd_ckpt_stats._fields_}
 
     def device_bytes(self):
         """Device bytes the handle holds (state, scratch, restart buffers)."""

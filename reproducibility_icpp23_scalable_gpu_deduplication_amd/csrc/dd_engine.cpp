@@ -374,12 +374,20 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   // A tree baseline's leaf kernel copies the chunks into the data section
   // when the handle's last baseline had every chunk as a first occurrence
   // (the pipeline keeps the request only with a leaf mapping that stages
-  // the chunks; out_cap >= max_diff_size covers the whole copy).
-  job.write_data = (baseline && method != DD_METHOD_LIST && !data_miss) ? 1 : 0;
+  // the chunks; out_cap >= max_diff_size covers the whole copy).  Only for
+  // chunks of at most 128 bytes: a round of a wavefront's 64 chunks is then
+  // one contiguous range, read and written whole; with longer chunks the
+  // round's pieces lie a chunk apart, and the copy's stores in that pattern
+  // cost more than the gather's contiguous copy (1 GiB chains: +2-5 % at
+  // 128 B, -1 % at 512 B, -4 to -5 % at 4096 B).  DD_LEAF_COPY=0 turns the
+  // copy off (the gather then copies every chunk).
+  static const bool copy_on = !(std::getenv("DD_LEAF_COPY") && strcmp(std::getenv("DD_LEAF_COPY"), "0") == 0);
+  job.write_data =
+      (copy_on && baseline && method != DD_METHOD_LIST && !data_miss && chunk_size <= 128) ? 1 : 0;
   return DD_OK;
 }
 
-void StreamState::commit(const JobCounters& c, bool baseline) {
+void StreamState::commit(const JobCounters& c, bool baseline, bool leaf_copy) {
   if (baseline) baseline_id = current_id;
   need_baseline = false;
   // The gather kernel of a completed tree checkpoint reset every label of
@@ -392,6 +400,7 @@ void StreamState::commit(const JobCounters& c, bool baseline) {
   stats.first_chunks = c.dchunks;
   stats.diff_bytes = c.diff_len;
   stats.baseline = baseline ? 1 : 0;
+  stats.leaf_copy = leaf_copy ? 1 : 0;
   nfirst = c.nfirst;
   nshift = c.nshift;
   map_count = c.map_count;
@@ -875,7 +884,7 @@ dd_status run_checkpoint(dd_dedup* h, const void* dev_data, uint64_t len, int ma
       h->stage_ms[i] = ms;
     }
   }
-  st.commit(c, baseline);
+  st.commit(c, baseline, h->pipe.job_copied(0));
   st.ctr_armed = true;  // k_sbin_scan zeroed the next checkpoint's counter set
   *result = c;
   return DD_OK;
@@ -1679,7 +1688,7 @@ dd_status dd_batch_checkpoint_device(dd_batch* b, const void* const* dev_data, c
   }
   for (size_t i = 0; i < n; i++) {
     StreamState& st = b->hs[i]->st;
-    st.commit(c[i], base[i]);
+    st.commit(c[i], base[i], b->pipe.job_copied(i));
     st.ctr_armed = true;
     out_lens[i] = c[i].diff_len;
     st.current_id += 1;

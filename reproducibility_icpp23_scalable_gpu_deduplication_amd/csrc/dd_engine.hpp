@@ -87,8 +87,9 @@ struct StreamState {
   // handle needs a baseline.
   dd_status prepare(const uint8_t* data, uint64_t len, bool baseline, uint8_t* out, uint64_t out_cap, JobDev& job,
                     hipStream_t s);
-  // Called after the checkpoint's counters are back on the host.
-  void commit(const JobCounters& c, bool baseline);
+  // Called after the checkpoint's counters are back on the host;
+  // `leaf_copy`: its leaf kernel wrote the data section.
+  void commit(const JobCounters& c, bool baseline, bool leaf_copy);
   // Grows the map so that `need` entries stay under the load limit.
   dd_status grow_map(uint64_t need, hipStream_t s);
   // Bulk map build: whether a baseline of Cn chunks and map capacity `cap`
@@ -176,6 +177,11 @@ class Pipeline {
 
  public:
   int leaf_mapping() const { return leaf_map_; }
+  // Job j of the last uploaded table had its chunks copied into the data
+  // section by the leaf kernel.
+  bool job_copied(size_t j) const {
+    return j < njobs_ && reinterpret_cast<const JobDev*>(host_table_)[j].write_data != 0;
+  }
   uint64_t device_bytes() const { return dev_table_.bytes; }
   // After the stream has completed a launch_all: records the trial launch's
   // leaf time and settles the mapping once every candidate has its rounds
@@ -198,8 +204,9 @@ uint64_t max_diff_size(uint64_t len, uint32_t chunk_size, uint32_t ckpt_id);
 int leaf_mapping_for(const std::vector<JobDev>& jobs, bool aligned16, int hash_kind, hipStream_t s,
                      std::vector<uint64_t>* key = nullptr, std::vector<int>* cand = nullptr);
 // The mapping for a table whose baselines ask for the data-section copy
-// (JobT::write_data): `current` when it makes the copy, is forced, or the
-// asking jobs hold less than half of the table's bytes; else DMA, or LDS64
+// (JobT::write_data): `current` when it is forced, makes the copy (DMA
+// becomes DMA1, whose copy variant runs more wavefronts per SIMD), or the
+// asking jobs hold less than half of the table's bytes; else DMA1, or LDS64
 // where chunks are not a multiple of 128 bytes (a copy in the leaf kernel
 // saves the gather a second read of the buffer, which costs more than the
 // mapping's difference in hashing speed).

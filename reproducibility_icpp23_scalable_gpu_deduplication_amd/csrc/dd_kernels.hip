@@ -550,12 +550,21 @@ __global__ __launch_bounds__(64 * WPG) void k_leaf_hash_dma(const JobG* __restri
   }
   for (uint32_t r = 0; have && r < rounds; r++) {
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
-    uint4 v[8], w[8];
+    uint4 v[8];
 #pragma unroll
     for (uint32_t p = 0; p < 8; p++) v[p] = *reinterpret_cast<const uint4*>(L + bi * 1024 + dma_slot(bi, bj, p) * 16);
     if (wr) {
+      // One slot at a time (read, store): the compiler barrier keeps the
+      // reads from being hoisted together, which would hold 32 more
+      // registers and cost the kernel a wavefront per SIMD.
 #pragma unroll
-      for (uint32_t i = 0; i < 8; i++) w[i] = *reinterpret_cast<const uint4*>(L + i * 1024 + lane * 16);
+      for (uint32_t i = 0; i < 8; i++) {
+        const uint4 w = *reinterpret_cast<const uint4*>(L + i * 1024 + lane * 16);
+        const uint32_t p = (i & 1) ? po : pe, jj = (i & 1) ? jo : je;
+        const uint64_t c = k0 + i * 8 + jj;
+        if (c < full) store_piece<true>(root_data_dst(J, g, c) + (uint64_t)r * 128 + p * 16, w);
+        __asm__ volatile("" ::: "memory");
+      }
     }
     __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // the reads are done before the region is refilled
     if (r + 1 < rounds) {
@@ -563,14 +572,6 @@ __global__ __launch_bounds__(64 * WPG) void k_leaf_hash_dma(const JobG* __restri
       for (uint32_t i = 0; i < 8; i++)
         __builtin_amdgcn_global_load_lds((global_void_t*)(src[i] + (uint64_t)(r + 1) * 128), (lds_void_t*)(L + i * 1024),
                                          16, 0, 0);
-    }
-    if (wr) {
-#pragma unroll
-      for (uint32_t i = 0; i < 8; i++) {
-        const uint32_t p = (i & 1) ? po : pe, jj = (i & 1) ? jo : je;
-        const uint64_t c = k0 + i * 8 + jj;
-        if (c < full) store_piece<true>(root_data_dst(J, g, c) + (uint64_t)r * 128 + p * 16, w[i]);
-      }
     }
     if (k < full) {
       if (Kind == HASH_MD5) {
@@ -3251,9 +3252,15 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
       case LEAF_LDS64:
         if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64, false, true>));
         DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 64, false, false>));
-      case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, false, false>));
-      case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256, false, false>));
-      case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, true, false>));
+      case LEAF_LDS128:
+        if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, false, true>));
+        DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, false, false>));
+      case LEAF_LDS256:
+        if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256, false, true>));
+        DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 256, false, false>));
+      case LEAF_LDS128P:
+        if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, true, true>));
+        DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MD5, 128, true, false>));
       case LEAF_DMA:
         if (copy) DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5, 4, true>));
         DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MD5, 4, false>));
@@ -3271,9 +3278,15 @@ void launch_leaf(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, u
     case LEAF_LDS64:
       if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64, false, true>));
       DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 64, false, false>));
-    case LEAF_LDS128: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, false, false>));
-    case LEAF_LDS256: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256, false, false>));
-    case LEAF_LDS128P: DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, true, false>));
+    case LEAF_LDS128:
+      if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, false, true>));
+      DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, false, false>));
+    case LEAF_LDS256:
+      if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256, false, true>));
+      DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 256, false, false>));
+    case LEAF_LDS128P:
+      if (copy) DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, true, true>));
+      DD_LEAF_LAUNCH((k_leaf_hash_lds<HASH_MURMUR3, 128, true, false>));
     case LEAF_DMA:
       if (copy) DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3, 4, true>));
       DD_LEAF_LAUNCH((k_leaf_hash_dma<HASH_MURMUR3, 4, false>));

@@ -279,7 +279,10 @@ inline uint64_t leaf_workgroups(int mapping, uint64_t C) { return mapping == LEA
 // Mappings with a kernel variant that copies the chunks into a baseline's
 // data section (JobT::write_data): each stages whole rounds of 64 chunks, so
 // its stores are contiguous.
-inline bool leaf_writes_data(int mapping) { return mapping == LEAF_DMA || mapping == LEAF_DMA1 || mapping == LEAF_LDS64; }
+inline bool leaf_writes_data(int mapping) {
+  return mapping == LEAF_DMA || mapping == LEAF_DMA1 || mapping == LEAF_LDS64 || mapping == LEAF_LDS128 ||
+         mapping == LEAF_LDS256 || mapping == LEAF_LDS128P;
+}
 // Insert modes of the jobs in a table (bit mask): the insert stages launch
 // one kernel variant per mode present.  Dense: a baseline (every leaf and
 // interior node, narrow probe rounds); list, narrow or wide rounds: an

@@ -110,14 +110,18 @@ double time_mapping(const JobDev* dj, const uint32_t* dprefix, uint32_t njobs, u
 }  // namespace
 
 int leaf_mapping_with_copy(const std::vector<JobDev>& jobs, bool al16, int kind, int current) {
-  if (leaf_writes_data(current) || forced_mapping() >= 0) return current;
+  if (forced_mapping() >= 0) return current;
+  // DMA's copy variant holds more registers than DMA1's (a wavefront per
+  // SIMD fewer); both read the chunks the same way.
+  if (current == LEAF_DMA) return LEAF_DMA1;
+  if (leaf_writes_data(current)) return current;
   uint64_t copy = 0, total = 0;
   for (const JobDev& J : jobs) {
     total += J.len;
     if (J.write_data) copy += J.len;
   }
   if (2 * copy < total) return current;
-  for (int m : {LEAF_DMA, LEAF_LDS64})
+  for (int m : {LEAF_DMA1, LEAF_LDS64})
     if (mapping_valid(m, al16, kind, jobs)) return m;
   return current;
 }

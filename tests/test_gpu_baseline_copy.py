@@ -103,3 +103,23 @@ def test_batch_mixed_layouts(torch_gpu):
             od, _ = oracle_chain(cs, [first[i], second[i]][:k + 1])
             got = outs[i][:lens[i]].cpu().numpy().tobytes()
             assert got == od[k], f"buffer {i} ckpt {k}: {first_mismatch(got, od[k])}"
+
+
+@pytest.mark.parametrize("cs,copies", [(128, True), (64, True), (48, False), (512, False)])
+def test_copy_follows_last_baseline(torch_gpu, cs, copies):
+    # The handle asks for the copy while its last baseline had the root as
+    # its only region; ckpt_stats reports whether the leaf kernel made it.
+    rng = np.random.default_rng(cs)
+    n = 4096 * cs
+    datas = [unique(rng, n), repeated(rng, n, cs), unique(rng, n), unique(rng, n)]
+    inc = datas[-1].copy()
+    inc[:cs] ^= 1
+    datas.append(inc)
+    want = [copies, copies, False, copies, False]
+    od, _ = oracle_chain(cs, datas, baselines=(0, 1, 2, 3))
+    g = dd.TreeDeduplicator(cs)
+    for k, x in enumerate(datas):
+        t = to_dev(torch_gpu, x)
+        d = g.checkpoint(t.data_ptr(), t.numel(), k < 4)
+        assert d == od[k], f"ckpt {k}: {first_mismatch(d, od[k])}"
+        assert bool(g.ckpt_stats()["leaf_copy"]) == want[k], f"ckpt {k}"
