@@ -1271,6 +1271,15 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
       }
     }
   }
+  // Copy width: 16-byte pieces need 16-byte chunks and output and 4-byte
+  // aligned data sections (16-byte loads at 4-byte alignment).
+  bool src4 = true;
+  for (size_t k = 0; k < M; k++) src4 = src4 && ((reinterpret_cast<uintptr_t>(diff[k]) + 32 + meta_bytes[k]) & 3) == 0;
+  const uintptr_t op = reinterpret_cast<uintptr_t>(dev_out);
+  const int piece = (src4 && cs % 16 == 0 && op % 16 == 0) ? 16 : (src4 && cs % 4 == 0 && op % 4 == 0) ? 4 : 1;
+  // A diff with one first-occurrence region and no shifted one takes the
+  // whole-region step.
+  auto whole_step = [&](size_t k) { return piece == 16 && hdrs[k].num_first_ocur == 1 && hdrs[k].num_shift_dupl == 0; };
   // 4. Upload block.
   memset(hst, 0, up_lists);
   RestartCkpt* desc = reinterpret_cast<RestartCkpt*>(hst + up_desc);
@@ -1301,6 +1310,8 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
       rc.nshift = hd.num_shift_dupl;
       rc.ckpt_id = hd.chkpt_id;
       rc.node_bias = bias;
+      // The walk's last step takes k_restart_whole without cover writes.
+      rc.no_cover = (k + 1 == M && whole_step(k)) ? 1u : 0u;
       rc.first_off = 32;
       rc.shift_off = 32 + 4ull * hd.num_first_ocur + 8ull * hd.num_prior_chkpts;
       rc.data_off = 32 + meta_bytes[k];
@@ -1315,12 +1326,6 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
   launch_restart_parse(ddesc, reinterpret_cast<const RestartList*>(ar + up_lists), (uint32_t)(2 * M),
                        reinterpret_cast<const uint32_t*>(ar + up_lblk), (uint32_t)nblk,
                        reinterpret_cast<uint32_t*>(ar + blk_at), g, cs, ctr, s);
-  // Copy width: 16-byte pieces need 16-byte chunks and output and 4-byte
-  // aligned data sections (16-byte loads at 4-byte alignment).
-  bool src4 = true;
-  for (size_t k = 0; k < M; k++) src4 = src4 && ((reinterpret_cast<uintptr_t>(diff[k]) + 32 + meta_bytes[k]) & 3) == 0;
-  const uintptr_t op = reinterpret_cast<uintptr_t>(dev_out);
-  const int piece = (src4 && cs % 16 == 0 && op % 16 == 0) ? 16 : (src4 && cs % 4 == 0 && op % 4 == 0) ? 4 : 1;
   // 5. The walk, newest first: two launches per checkpoint.
   if ((uint64_t)*stamp + M + 1 > 0xfffffff0ull) {  // stamps would wrap: start over from a cleared table
     launch_fill_u32(reinterpret_cast<uint32_t*>(cover), 2 * C, 0, s);
@@ -1331,7 +1336,7 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
   bool any_shift = false;  // the pending list stays empty until a checkpoint has shifted entries
   for (size_t k = 0; k < M; k++) {
     const uint32_t st = base + 1 + (uint32_t)k;
-    if (piece == 16 && hdrs[k].num_first_ocur == 1 && hdrs[k].num_shift_dupl == 0)
+    if (whole_step(k))
       launch_restart_whole(ddesc + k, g, cover, base, st, ctr, out, len, cs, s);
     else
       launch_restart_regions(ddesc + k, g, cover, base, st, ref, pend, ctr, out, len, cs, piece, s);

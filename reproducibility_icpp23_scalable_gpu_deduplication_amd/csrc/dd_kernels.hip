@@ -3056,9 +3056,12 @@ __global__ __launch_bounds__(256) void k_restart_regions(const RestartCkpt* __re
 // shifted one (a baseline without repeated chunks).  The region's leaves are
 // at most two runs of consecutive chunks, stored in that order in the data
 // section, so a round of 256 leaves reads one contiguous range; no region
-// search and no root load per leaf, and every piece a lane copies in a
-// round is loaded before any is stored.  Cover entries and claims as in
-// k_restart_regions (region 0).
+// search and no root load per leaf.  The round's pieces are loaded together
+// with its cover entries, before the claims are known (a claimed chunk's
+// pieces are read and dropped), so a round costs one memory round trip and
+// the stores.  Cover entries and claims as in k_restart_regions (region 0);
+// the walk's last step (no_cover) writes no cover entries: no older step
+// reads them, and its pending step tests membership in region 0 instead.
 __global__ __launch_bounds__(256) void k_restart_whole(const RestartCkpt* __restrict__ rcp, TreeGeom g,
                                                        uint2* __restrict__ cover, uint32_t base, uint32_t stamp,
                                                        RestartCtr* __restrict__ ctr, uint8_t* __restrict__ out,
@@ -3075,58 +3078,74 @@ __global__ __launch_bounds__(256) void k_restart_whole(const RestartCkpt* __rest
   const uint32_t ppc = cs / 16;
   const bool pow2 = (ppc & (ppc - 1)) == 0;
   const uint32_t sh = pow2 ? __builtin_ctz(ppc) : 0;
+  const bool keep = rc.no_cover == 0;
   constexpr uint32_t U = 8;
   uint32_t claimed = 0;
   for (uint64_t t = blockIdx.x; t * 256 < nl; t += gridDim.x) {
     const uint64_t e = t * 256 + threadIdx.x;
-    uint32_t c = 0xffffffffu;
     const uint64_t leaf = e < nl ? run_leaf(runs, e) : 0;
-    if (e < nl && leaf >= g.C - 1 && leaf < g.N) {
-      const uint32_t cc = (uint32_t)(leaf - (g.C - 1));
-      const u32x2_t old = cv[cc];
-      cv[cc] = u32x2_t{stamp, 0u};
-      if (old.x <= base) {  // no newer checkpoint of this restart covered the chunk
-        claimed++;
-        c = cc;
-      }
-    }
-    s_dst[threadIdx.x] = c;
-    __syncthreads();
+    const bool valid = e < nl && leaf >= g.C - 1 && leaf < g.N;
+    const uint32_t cc = valid ? (uint32_t)(leaf - (g.C - 1)) : 0u;
+    const u32x2_t old = valid ? (u32x2_t)cv[cc] : u32x2_t{0xffffffffu, 0u};
     const uint32_t nq = (uint32_t)min((uint64_t)256, nl - t * 256);
     const uint32_t total = nq * ppc;
     const uint8_t* src0 = data + t * 256 * (uint64_t)cs;
-    for (uint32_t w0 = threadIdx.x; w0 < total; w0 += 256 * U) {
-      u32x4_t v[U];
-      uint64_t dst[U];
-      bool full[U];
+    u32x4_t v[U];
+    auto load_pass = [&](uint32_t w0) {
 #pragma unroll
       for (uint32_t u = 0; u < U; u++) {
         const uint32_t w = w0 + 256 * u;
-        full[u] = false;
-        dst[u] = 0;
+        v[u] = u32x4_t{0u, 0u, 0u, 0u};
+        if (w < total) {
+          const u32x4a4 x = *(GlobalPtr<const u32x4a4>)(src0 + (uint64_t)w * 16);  // 4-byte aligned source
+          v[u] = u32x4_t{x.x, x.y, x.z, x.w};
+        }
+      }
+    };
+    auto store_pass = [&](uint32_t w0) {
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        const uint32_t w = w0 + 256 * u;
         if (w >= total) continue;
         const uint32_t i = pow2 ? (w >> sh) : (w / ppc);
         const uint32_t cd = s_dst[i];
         if (cd == 0xffffffffu) continue;
-        const uint32_t pi = w - i * ppc;
-        dst[u] = (uint64_t)cd * cs + (uint64_t)pi * 16;
-        const uint8_t* in = src0 + (uint64_t)w * 16;
-        if (dst[u] + 16 <= len) {
-          const u32x4a4 x = *(GlobalPtr<const u32x4a4>)in;  // 4-byte aligned source
-          v[u] = u32x4_t{x.x, x.y, x.z, x.w};
-          full[u] = true;
-        } else {
-          copy_chunk_piece<16>(out, dst[u], in, len);  // the tail of the buffer
-        }
+        const uint64_t dst = (uint64_t)cd * cs + (uint64_t)(w - i * ppc) * 16;
+        if (dst + 16 <= len) *(GlobalPtr<u32x4_t>)(o + dst) = v[u];
+        else copy_chunk_piece<16>(out, dst, src0 + (uint64_t)w * 16, len);  // the tail of the buffer
       }
-#pragma unroll
-      for (uint32_t u = 0; u < U; u++)
-        if (full[u]) *(GlobalPtr<u32x4_t>)(o + dst[u]) = v[u];
+    };
+    // Passes of 256 x U pieces (one for chunks of at most 128 bytes); the
+    // first one's loads are in flight together with the cover entry.
+    const uint32_t npass = (total + 256 * U - 1) / (256 * U);
+    for (uint32_t p = 0; p < npass; p++) {  // uniform
+      const uint32_t w0 = threadIdx.x + p * 256 * U;
+      load_pass(w0);
+      if (p == 0) {
+        const bool own = valid && old.x <= base;  // no newer checkpoint of this restart covered the chunk
+        if (valid && keep) cv[cc] = u32x2_t{stamp, 0u};
+        claimed += own ? 1u : 0u;
+        s_dst[threadIdx.x] = own ? cc : 0xffffffffu;
+        __syncthreads();
+      }
+      store_pass(w0);
     }
     __syncthreads();
   }
   const uint32_t tot = wave_sum(claimed);
   if (lane_id() == 0 && tot) g_add((GlobalPtr<unsigned long long>)&ctr->claimed, (unsigned long long)tot);
+}
+
+// Cover entry of chunk x for a pending step: the table's, or for a step that
+// wrote none (no_cover: one first-occurrence region, r0 its leaf runs)
+// membership in that region.
+__device__ __forceinline__ u32x2_t pending_cover(const RestartCkpt& rc, const TreeGeom& g,
+                                                 GlobalPtr<const u32x2_t> cv, uint32_t stamp, const LeafRuns& r0,
+                                                 uint64_t x) {
+  if (!rc.no_cover) return cv[x];
+  const uint64_t leaf = x + g.C - 1;
+  const bool in = (leaf >= r0.a0 && leaf - r0.a0 < r0.n0) || (leaf >= r0.a1 && leaf - r0.a1 < r0.n1);
+  return in ? u32x2_t{stamp, 0u} : u32x2_t{0u, 0u};
 }
 
 template <int PIECE>
@@ -3146,6 +3165,8 @@ __global__ __launch_bounds__(256) void k_restart_pending(const RestartCkpt* __re
   const uint32_t cur = rc.ckpt_id;
   const GlobalPtr<const u32x2_t> cv = (GlobalPtr<const u32x2_t>)cover;
   const GlobalPtr<uint64_t> rf = (GlobalPtr<uint64_t>)ref;
+  LeafRuns r0{0, 0, 0, 0};
+  if (rc.no_cover) r0 = subtree_runs(g, (uint64_t)rs_load32(rc.diff + rc.first_off) + rc.node_bias);
   uint32_t done = 0, bad = 0;
   for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < n; i0 += (uint64_t)gridDim.x * 256) {
     const uint64_t i = i0 + threadIdx.x;
@@ -3157,7 +3178,7 @@ __global__ __launch_bounds__(256) void k_restart_pending(const RestartCkpt* __re
       if (r != kRefDone && (uint32_t)(r >> 32) == cur) {
         const bool same = ((uint32_t)r & kRefSameCkpt) != 0;
         const uint64_t x = (uint32_t)r & ~kRefSameCkpt;
-        const u32x2_t e = cv[x];
+        const u32x2_t e = pending_cover(rc, g, cv, stamp, r0, x);
         if (e.x == stamp && !(e.y & kCoverShift)) {
           so = first_src(rc, g, e.y, x, cs);
         } else if (e.x == stamp && !same) {
@@ -3167,7 +3188,7 @@ __global__ __launch_bounds__(256) void k_restart_pending(const RestartCkpt* __re
           if (s2 < cur) {
             rf[c] = ((uint64_t)s2 << 32) | (uint32_t)x2;
           } else {  // a source in this checkpoint must be a first occurrence here
-            const u32x2_t e2 = cv[x2];
+            const u32x2_t e2 = pending_cover(rc, g, cv, stamp, r0, x2);
             if (e2.x == stamp && !(e2.y & kCoverShift)) so = first_src(rc, g, e2.y, x2, cs);
             else bad = 1;
           }

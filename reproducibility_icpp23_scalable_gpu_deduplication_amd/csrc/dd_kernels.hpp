@@ -193,7 +193,7 @@ struct RestartCkpt {
   uint32_t nfirst, nshift;
   uint32_t ckpt_id;
   uint32_t node_bias;       // added to every node id of the diff (C-1 for list diffs: chunk -> leaf)
-  uint32_t pad;
+  uint32_t no_cover;        // 1: the walk's last step, by k_restart_whole: it writes no cover entries
   uint64_t first_off;       // byte offset of the first-occurrence list
   uint64_t shift_off;       // byte offset of the shifted list
   uint64_t data_off;        // byte offset of the data section
