@@ -1385,11 +1385,22 @@ template <bool Final>
 __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
   const JobG& J = jobs[j];
-  __shared__ uint32_t owner[kBucketSlots];  // record that claimed the slot, ~0 = empty
-  __shared__ uint32_t best[kBulkRec];       // per first record: lowest rank of its digest
-  __shared__ uint4 rdig[kBulkRec];
-  __shared__ uint32_t rnode[kBulkRec];
+  // LDS holds 32 bits of each record's digest (the full digests stay in the
+  // registers of the lane that loaded the record, and in the record array
+  // for the rare equal keys): about 50 KB, three workgroups per CU.  The
+  // probing arrays are reused as the staging area of the slot writes.
+  constexpr uint32_t kHalf = kBucketSlots / 2;
+  constexpr uint32_t kPool = (kBucketSlots + 3 * kBulkRec) * 4;
+  static_assert(kPool >= kHalf * 20, "a half bucket of entries fits in the probing arrays");
+  __shared__ __attribute__((aligned(16))) uint8_t pool[kPool];
+  uint32_t* const owner = reinterpret_cast<uint32_t*>(pool);  // record that claimed the slot, ~0 = empty
+  uint32_t* const best = owner + kBucketSlots;                 // per first record: lowest rank of its digest
+  uint32_t* const rkey = best + kBulkRec;                      // digest bits 0..31 of each record
+  uint32_t* const rnode = rkey + kBulkRec;
+  uint4* const sdig = reinterpret_cast<uint4*>(pool);          // staging: entries of half a bucket
+  uint32_t* const snode = reinterpret_cast<uint32_t*>(pool + kHalf * 16);
   __shared__ uint16_t rslot[kBulkRec];
+  __shared__ uint32_t occ[kBucketSlots / 32];  // bit q: slot q holds an entry
   __shared__ uint32_t sh[kBuildThreads / 64 + 1];
   __shared__ uint32_t sflags, sbase[2], sn[2];
   const uint32_t b = blockIdx.x - prefix[j];
@@ -1419,16 +1430,22 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
     if (threadIdx.x == 0) g_or(&J.ctr->flags, FLAG_MAP_FULL);
     return;
   }
-  // Every record of the lane is loaded before any is stored to LDS.
+  // Record r's entry in the record array.
+  auto rec_of = [&](uint32_t r) -> GlobalPtr<const u32x4_t> {
+    const uint64_t e = r < nl ? (uint64_t)sbase[0] + r : (uint64_t)sbase[1] + (r - nl);
+    return (GlobalPtr<const u32x4_t>)(J.bk_rec + 2 * e);
+  };
+  // Every record of the lane is loaded before any is stored to LDS; lane
+  // threadIdx.x keeps records threadIdx.x + i * kBuildThreads in registers.
   constexpr uint32_t kRecPerLane = (kBulkRec + kBuildThreads - 1) / kBuildThreads;
+  u32x4_t rd[kRecPerLane];
   {
-    u32x4_t rd[kRecPerLane];
     uint32_t rn[kRecPerLane];
 #pragma unroll
     for (uint32_t i = 0; i < kRecPerLane; i++) {
       const uint32_t r = threadIdx.x + i * kBuildThreads;
-      const uint64_t e = r < nl ? (uint64_t)sbase[0] + r : (uint64_t)sbase[1] + (r - nl);
-      const GlobalPtr<const u32x4_t> rec = (GlobalPtr<const u32x4_t>)(J.bk_rec + 2 * (r < n ? e : 0));
+      // Past the bucket's records: the array's first record (in bounds).
+      const GlobalPtr<const u32x4_t> rec = r < n ? rec_of(r) : (GlobalPtr<const u32x4_t>)J.bk_rec;
       rd[i] = rec[0];
       rn[i] = rec[1].x;
     }
@@ -1436,7 +1453,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
     for (uint32_t i = 0; i < kRecPerLane; i++) {
       const uint32_t r = threadIdx.x + i * kBuildThreads;
       if (r < n) {
-        rdig[r] = make_uint4(rd[i].x, rd[i].y, rd[i].z, rd[i].w);
+        rkey[r] = rd[i].x;
         rnode[r] = rn[i];
         best[r] = 0xffffffffu;
       }
@@ -1449,8 +1466,11 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
   auto rank_of = [&](uint32_t r) -> uint32_t {
     return r < nl ? rnode[r] - C1 : 0x80000000u | (0x7fffffffu - rnode[r]);
   };
-  for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
-    const uint4 dv = rdig[r];
+#pragma unroll
+  for (uint32_t i = 0; i < kRecPerLane; i++) {
+    const uint32_t r = threadIdx.x + i * kBuildThreads;
+    if (r >= n) continue;
+    const u32x4_t dv = rd[i];
     const Digest d{dv.x | ((uint64_t)dv.y << 32), dv.z | ((uint64_t)dv.w << 32)};
     uint32_t q = (uint32_t)(map_start(J.map, d) & bm);
     uint32_t first = r;
@@ -1460,10 +1480,12 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
         cur = atomicCAS(&owner[q], 0xffffffffu, r);
         if (cur == 0xffffffffu) break;  // claimed: this record is the digest's first
       }
-      const uint4 ov = rdig[cur];
-      if (ov.x == dv.x && ov.y == dv.y && ov.z == dv.z && ov.w == dv.w) {
-        first = cur;
-        break;
+      if (rkey[cur] == dv.x) {  // equal keys: compare the whole digests
+        const u32x4_t ov = rec_of(cur)[0];
+        if (ov.x == dv.x && ov.y == dv.y && ov.z == dv.z && ov.w == dv.w) {
+          first = cur;
+          break;
+        }
       }
     }
     rslot[r] = (uint16_t)q;
@@ -1493,28 +1515,48 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
     if (threadIdx.x == 0) g_or(&J.ctr->flags, FLAG_BULK_DUPS | FLAG_MAP_FULL);  // later stages stop
     return;
   }
-  __syncthreads();
+  // Slots, written in slot order half a bucket at a time: each digest's
+  // entry is staged in LDS by the lane that holds its record (the probing
+  // arrays are free by then), EMPTY where no record landed.
+  uint32_t node[kRecPerLane], slot[kRecPerLane];
+  uint32_t used = 0;
 #pragma unroll
   for (uint32_t i = 0; i < kRecPerLane; i++) {
     const uint32_t r = threadIdx.x + i * kBuildThreads;
-    if (r < n && win[i]) owner[rslot[r]] = r;
-    if (r < nl && !win[i]) J.slot_of[rnode[r]] = (uint32_t)((uint64_t)b * bs + rslot[r]);
+    node[i] = r < n ? rnode[r] : 0u;
+    slot[i] = r < n ? rslot[r] : 0u;
+    if (r < nl && !win[i]) J.slot_of[node[i]] = (uint32_t)((uint64_t)b * bs + slot[i]);
+    used += (r < n && win[i]) ? 1u : 0u;
   }
+  for (uint32_t x = threadIdx.x; x < kBucketSlots / 32; x += blockDim.x) occ[x] = 0;
   __syncthreads();
-  uint32_t used = 0;
+  for (uint32_t q = threadIdx.x; q < bs; q += blockDim.x)
+    if (owner[q] != 0xffffffffu) atomicOr(&occ[q >> 5], 1u << (q & 31));
+  __syncthreads();  // every read of the probing arrays is done
   const uint32_t K = J.ckpt_id;
-  for (uint64_t q = threadIdx.x; q < bs; q += blockDim.x) {
-    const uint32_t o = owner[q];
-    const GlobalPtr<u32x4_t> p = (GlobalPtr<u32x4_t>)&slots[q];
-    if (o == 0xffffffffu) {
-      p[0] = u32x4_t{0, 0, 0, 0};
-      p[1] = u32x4_t{0xffffffffu, 0xffffffffu, 0, 0};
-    } else {
-      const uint4 dg = rdig[o];
-      p[0] = u32x4_t{dg.x, dg.y, dg.z, dg.w};
-      p[1] = u32x4_t{rnode[o], K, 0, 0};
-      used++;
+  for (uint32_t h0 = 0; h0 < bs; h0 += kHalf) {
+#pragma unroll
+    for (uint32_t i = 0; i < kRecPerLane; i++) {
+      const uint32_t r = threadIdx.x + i * kBuildThreads;
+      if (r < n && win[i] && slot[i] >= h0 && slot[i] < h0 + kHalf) {
+        sdig[slot[i] - h0] = make_uint4(rd[i].x, rd[i].y, rd[i].z, rd[i].w);
+        snode[slot[i] - h0] = node[i];
+      }
     }
+    __syncthreads();
+    const uint32_t hn = (uint32_t)min((uint64_t)kHalf, bs - h0);
+    for (uint32_t q = threadIdx.x; q < hn; q += blockDim.x) {
+      const GlobalPtr<u32x4_t> p = (GlobalPtr<u32x4_t>)&slots[h0 + q];
+      if ((occ[(h0 + q) >> 5] >> ((h0 + q) & 31)) & 1u) {
+        const uint4 dg = sdig[q];
+        p[0] = u32x4_t{dg.x, dg.y, dg.z, dg.w};
+        p[1] = u32x4_t{snode[q], K, 0, 0};
+      } else {
+        p[0] = u32x4_t{0, 0, 0, 0};
+        p[1] = u32x4_t{0xffffffffu, 0xffffffffu, 0, 0};
+      }
+    }
+    __syncthreads();  // the staging area is refilled for the next half
   }
   const uint32_t tot = block_sum<uint32_t>(used, sh);
   if (threadIdx.x == 0) J.bk_claim[b] = tot;
