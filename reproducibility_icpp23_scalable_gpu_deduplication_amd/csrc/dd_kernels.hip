@@ -1739,6 +1739,26 @@ __device__ __forceinline__ uint64_t tile_at(const JobG& J, bool listed, const ui
   return ids[n & 63];
 }
 
+// Level-2 positions 2j and 2j+1 of a tile (lanes 2j, 2j+1) are the children
+// of level-3 node j: true when some such pair has both bits set in `m`.
+__device__ __forceinline__ bool sibling_pair(uint64_t m) { return (m & (m >> 1) & 0x5555555555555555ull) != 0; }
+
+// Label of LDS position q (levels 3 .. H, level-major from the tile root) when
+// no node above level 2 is shifted: FIXED when every level-2 node below it is
+// FIXED, FIRST when every one is FIRST, else DONE (two children with equal
+// labels pass theirs on, different ones make DONE).  `mx`, `mr`: the level-2
+// FIXED and FIRST masks (bit = lane = level-2 position).
+__device__ __forceinline__ uint8_t upper_label(uint32_t q, int H, uint64_t mx, uint64_t mr) {
+  const uint32_t l = 31 - __clz(q + 1);  // level below the tile root
+  const uint32_t k = q + 1 - (1u << l);  // position within that level
+  const uint32_t span = 1u << (H - (int)l - 2);  // level-2 nodes below it
+  const uint32_t start = k * span;
+  const uint64_t lm = span >= 64 ? ~0ull : ((1ull << span) - 1);
+  if (((mx >> start) & lm) == lm) return FIXED_DUPL;
+  if (((mr >> start) & lm) == lm) return FIRST_OCUR;
+  return DONE;
+}
+
 // Pass 1, the first-occurrence forest (include/reference_impl.hpp:73-82).
 // Group 0 visits the dirty tiles only; the groups above visit every tile.
 // In an incremental checkpoint the first-occurrence interior nodes of a tile
@@ -1805,8 +1825,12 @@ __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__
       l2 = both ? (uint8_t)FIRST_OCUR : (uint8_t)DONE;
       if (both) d2 = hash_pair(kind, d1[0], d1[1]);
     }
-    // Levels 3 .. H in LDS (level-2 results at positions nq .. 2 nq).
-    if (T.nq) {
+    // Levels 3 .. H in LDS (level-2 results at positions nq .. 2 nq).  A node
+    // is FIRST only when both children are: with no two sibling level-2
+    // nodes FIRST every node above level 2 is DONE, and the LDS levels (a
+    // barrier each) are skipped.
+    const bool deep = T.nq && sibling_pair(__ballot(T.v2 && l2 == FIRST_OCUR));  // uniform
+    if (deep) {
       if (T.v2) {
         tlab[T.nq + i] = l2;
         tdig[T.nq + i] = d2;
@@ -1827,8 +1851,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__
     Digest dq{0, 0};
     uint64_t xq = 0;
     if (i < T.nq) {
-      lq = tlab[i];
-      dq = tdig[i];
+      lq = deep ? tlab[i] : (uint8_t)DONE;
+      if (deep) dq = tdig[i];
       xq = tile_node(T.r, i);
     }
     // Positions of the first-occurrence interior nodes in the tile's fint
@@ -1993,8 +2017,12 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
       l2 = dup_label(l1[0], l1[1], f2 != ~0ull);
       if (l2 != SHIFT_DUPL) d2 = Digest{0, 0};
     }
-    // Levels 3 .. H in LDS (level-2 results at positions nq .. 2 nq).
-    if (T.nq) {
+    // Levels 3 .. H in LDS (level-2 results at positions nq .. 2 nq).  With
+    // no two sibling level-2 nodes shifted no node above level 2 is looked
+    // up or shifted, and its label follows from the level-2 labels below it
+    // (upper_label): the LDS levels run only for tiles with such a pair.
+    const bool deep = T.nq && sibling_pair(__ballot(T.v2 && l2 == SHIFT_DUPL));  // uniform
+    if (deep) {
       if (T.v2) {
         tlab[T.nq + i] = l2;
         tdig[T.nq + i] = d2;
@@ -2040,9 +2068,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
         slot_of[T.x2] = (uint32_t)f2;
       }
     }
+    const uint64_t mx2 = __ballot(T.v2 && l2 == FIXED_DUPL), mr2 = __ballot(T.v2 && l2 == FIRST_OCUR);
     if (i < T.nq) {
       const uint64_t x = tile_node(T.r, i);
-      const uint8_t L = tlab[i];
+      const uint8_t L = deep ? tlab[i] : upper_label(i, T.H, mx2, mr2);
       labels[x] = L;
       if (L == SHIFT_DUPL) {
         store_digest(tree + x, tdig[i]);
