@@ -742,13 +742,14 @@ dd_status Pipeline::launch_all(hipStream_t s, bool rerun, bool begin, hipEvent_t
     DD_TRY(mark(3));
     DD_TRY(mark(4));
   } else {
-    for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
+    const bool narrow = 2 * max_c_ - 1 < (1ull << 31);  // every tree below 2^31 nodes
+    for (int g = 0; g < max_groups_; g++) launch_pass1(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, narrow, s);
     DD_TRY(mark(3));
     launch_insert_interior(J, P(S_TILES), nj, grid_[S_TILES], kinds_, s);
     launch_bulk(J, nj, true, bulk_dedup_, P(S_BULK), grid_[S_BULK], P(S_BSCAN), grid_[S_BSCAN], P(S_BUCKET),
                 grid_[S_BUCKET], s);
     DD_TRY(mark(4));
-    for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, s);
+    for (int g = 0; g < max_groups_; g++) launch_pass2(J, P(S_TILE0 + g), nj, grid_[S_TILE0 + g], g, narrow, s);
   }
   DD_TRY(mark(5));
   if (ev_dedup) DD_HIP(hipEventRecord(ev_dedup, s));

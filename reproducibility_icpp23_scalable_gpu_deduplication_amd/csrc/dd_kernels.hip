@@ -1613,26 +1613,31 @@ constexpr uint32_t kTopNodes = 63;  // tile nodes on levels 3 .. 8 above the bot
 // 2i, 2i+1), level-2 node x2 (position i); positions past the level's width
 // do not exist.  LDS position q < 2^(H-2) - 1 holds tile node q in
 // level-major order from the tile root (levels 3 .. H above the bottom).
+// Idx: the node index type, 32 bits when the tree has fewer than 2^31 nodes
+// (no heap index of a tile position then passes 2^32), so that index
+// arithmetic and label addresses take one instruction instead of two.
+template <class Idx>
 struct TileLane {
   int H;
-  uint64_t r;
-  uint32_t width;       // bottom positions, 2^H
-  uint64_t xb, x1, x2;  // first heap node of this lane on levels 0, 1, 2
-  bool vb[4];           // bottom node exists
-  bool v1[2], v2;       // level-1 / level-2 position exists
-  uint32_t nq;          // LDS positions (levels >= 3)
+  Idx r;
+  uint32_t width;  // bottom positions, 2^H
+  Idx xb, x1, x2;  // first heap node of this lane on levels 0, 1, 2
+  bool vb[4];      // bottom node exists
+  bool v1[2], v2;  // level-1 / level-2 position exists
+  uint32_t nq;     // LDS positions (levels >= 3)
 };
 
-__device__ __forceinline__ TileLane tile_lane(const TileRef& tr, uint64_t N, uint32_t i) {
-  TileLane L;
+template <class Idx>
+__device__ __forceinline__ TileLane<Idx> tile_lane(const TileRef& tr, uint64_t N, uint32_t i) {
+  TileLane<Idx> L;
   L.H = tr.H;
-  L.r = tr.r;
+  L.r = (Idx)tr.r;
   L.width = 1u << tr.H;
-  L.xb = ((tr.r + 1) << tr.H) - 1 + 4 * i;
-  L.x1 = tr.H >= 1 ? ((tr.r + 1) << (tr.H - 1)) - 1 + 2 * i : 0;
-  L.x2 = tr.H >= 2 ? ((tr.r + 1) << (tr.H - 2)) - 1 + i : 0;
+  L.xb = ((L.r + 1) << tr.H) - 1 + 4 * i;
+  L.x1 = tr.H >= 1 ? ((L.r + 1) << (tr.H - 1)) - 1 + 2 * i : 0;
+  L.x2 = tr.H >= 2 ? ((L.r + 1) << (tr.H - 2)) - 1 + i : 0;
 #pragma unroll
-  for (int k = 0; k < 4; k++) L.vb[k] = 4 * i + k < L.width && L.xb + k < N;
+  for (int k = 0; k < 4; k++) L.vb[k] = 4 * i + k < L.width && L.xb + k < (Idx)N;
 #pragma unroll
   for (int k = 0; k < 2; k++) L.v1[k] = tr.H >= 1 && 2 * i + k < (L.width >> 1);
   L.v2 = tr.H >= 2 && i < (L.width >> 2);
@@ -1640,12 +1645,19 @@ __device__ __forceinline__ TileLane tile_lane(const TileRef& tr, uint64_t N, uin
   return L;
 }
 
+template <class Idx>
+__device__ __forceinline__ Idx tile_node_t(Idx r, uint32_t i) {
+  const uint32_t q = 31 - __clz(i + 1);
+  return ((r + 1) << q) - 1 + (i + 1 - (1u << q));
+}
+
 // Labels of a lane's bottom nodes (0xff where absent; a clean child tile's
 // root reads as the pass's clean label: DONE in pass 1, FIXED in pass 2,
 // chosen by the caller through `clean`) and of its level-1 leaves (vu).  The
 // loads are straight-line, from clamped addresses, with the results selected
 // afterwards, so that all of them are in flight together.
-__device__ __forceinline__ void tile_labels(const JobG& J, int g, const TileLane& T, uint64_t C, uint8_t raw[4],
+template <class Idx>
+__device__ __forceinline__ void tile_labels(const JobG& J, int g, const TileLane<Idx>& T, uint64_t C, uint8_t raw[4],
                                             uint8_t rawu[2], bool vu[2], uint8_t clean) {
   const GlobalPtr<const uint8_t> labels = J.labels;
   uint8_t b[4], u[2];
@@ -1653,12 +1665,12 @@ __device__ __forceinline__ void tile_labels(const JobG& J, int g, const TileLane
   for (int k = 0; k < 4; k++) b[k] = labels[T.vb[k] ? T.xb + k : 0];
 #pragma unroll
   for (int k = 0; k < 2; k++) {
-    vu[k] = T.v1[k] && T.x1 + k >= C - 1;
+    vu[k] = T.v1[k] && T.x1 + k >= (Idx)(C - 1);
     u[k] = labels[vu[k] ? T.x1 + k : 0];
   }
   bool cl[4] = {false, false, false, false};
   if (check_child_tiles(J, g)) {  // uniform
-    const uint64_t base = child_tile_base(J);
+    const Idx base = (Idx)child_tile_base(J);
     uint32_t e[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) e[k] = J.tile_epoch[T.vb[k] ? T.xb + k - base : 0];
@@ -1679,7 +1691,8 @@ __device__ __forceinline__ void tile_labels(const JobG& J, int g, const TileLane
 // The loads are issued only when some lane of the wavefront needs one, all
 // together from clamped addresses: a tile of isolated changes (most tiles
 // when changes are spread thin) skips the round trip.
-__device__ __forceinline__ void tile_digests(GlobalPtr<const Digest> tree, const TileLane& T, const uint8_t bl[4],
+template <class Idx>
+__device__ __forceinline__ void tile_digests(GlobalPtr<const Digest> tree, const TileLane<Idx>& T, const uint8_t bl[4],
                                              const uint8_t ul[2], const bool vu[2], uint8_t want, Digest bd[4],
                                              Digest ud[2]) {
   bool nb[4], nu[2], m1[2];
@@ -1765,6 +1778,7 @@ __device__ __forceinline__ uint8_t upper_label(uint32_t q, int H, uint64_t mx, u
 // are appended to the fint list for the interior map inserts (a baseline's
 // interior inserts visit every node instead).  Group-0 leaves left PENDING
 // by the inserts are resolved and written back here.
+template <class Idx>
 __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
                                                         int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
@@ -1790,7 +1804,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__
   const uint32_t niter = tile_iterations(wg, nwg, nitems);
   for (uint32_t n = 0; n < niter; n++) {
     const uint64_t t = tile_at(J, listed, pre, ids, wg, nwg, niter, n);
-    const TileLane T = tile_lane(tile_ref(J, g, t), N, i);
+    const TileLane<Idx> T = tile_lane<Idx>(tile_ref(J, g, t), N, i);
     // Loads: labels of the bottom nodes and of the level-1 leaves ...
     uint8_t raw[4], rawu[2], bl[4], ul[2];
     bool vu[2];
@@ -1849,11 +1863,11 @@ __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__
     }
     uint8_t lq = 0xff;
     Digest dq{0, 0};
-    uint64_t xq = 0;
+    Idx xq = 0;
     if (i < T.nq) {
       lq = deep ? tlab[i] : (uint8_t)DONE;
       if (deep) dq = tdig[i];
-      xq = tile_node(T.r, i);
+      xq = tile_node_t<Idx>(T.r, i);
     }
     // Positions of the first-occurrence interior nodes in the tile's fint
     // region (global tile id tbase + t).
@@ -1875,7 +1889,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass1(const JobG* __restrict__
     }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-      const uint64_t x = T.x1 + k;
+      const Idx x = T.x1 + k;
       if (vu[k]) {
         if (rawu[k] & PENDING) labels[x] = ul[k];
       } else if (T.v1[k]) {
@@ -1959,6 +1973,7 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
   }
 }
 
+template <class Idx>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pass2(const JobG* __restrict__ jobs, const uint32_t* __restrict__ prefix, uint32_t njobs,
                                                         int g) {
   const uint32_t j = find_job(prefix, njobs, blockIdx.x);
@@ -1982,7 +1997,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
   __shared__ uint32_t ids[64];
   const uint32_t niter = tile_iterations(wg, nwg, nitems);
   for (uint32_t n = 0; n < niter; n++) {
-    const TileLane T = tile_lane(tile_ref(J, g, tile_at(J, listed, pre, ids, wg, nwg, niter, n)), N, i);
+    const TileLane<Idx> T = tile_lane<Idx>(tile_ref(J, g, tile_at(J, listed, pre, ids, wg, nwg, niter, n)), N, i);
     // Loads: labels (leaves were resolved by pass 1), then the digests of
     // the shifted pairs.
     uint8_t bl[4], ul[2];
@@ -2053,7 +2068,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       if (T.v1[k] && !vu[k]) {
-        const uint64_t x = T.x1 + k;
+        const Idx x = T.x1 + k;
         labels[x] = l1[k];
         if (l1[k] == SHIFT_DUPL) {
           store_digest(tree + x, d1[k]);
@@ -2070,7 +2085,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
     }
     const uint64_t mx2 = __ballot(T.v2 && l2 == FIXED_DUPL), mr2 = __ballot(T.v2 && l2 == FIRST_OCUR);
     if (i < T.nq) {
-      const uint64_t x = tile_node(T.r, i);
+      const Idx x = tile_node_t<Idx>(T.r, i);
       const uint8_t L = deep ? tlab[i] : upper_label(i, T.H, mx2, mr2);
       labels[x] = L;
       if (L == SHIFT_DUPL) {
@@ -2100,50 +2115,53 @@ __device__ __forceinline__ void node_block_range(const JobG& J, uint32_t wg, uin
   b1 = min(J.nblk, b0 + per);
 }
 
-// True when some byte of the 16 labels is FIRST (0) or SHIFT (2).
-__device__ __forceinline__ bool any_root_candidate(const uint4 w) {
-  auto z = [](uint32_t x) {
-    x &= 0xfdfdfdfdu;  // FIRST and SHIFT become zero bytes
-    return ((x - 0x01010101u) & ~x & 0x80808080u) != 0;
-  };
-  return z(w.x) || z(w.y) || z(w.z) || z(w.w);
+// Bytes of x that are zero: bit 7 of each such byte (exact, no borrow
+// between bytes).
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {
+  return ~(((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u;
 }
 
-__device__ __forceinline__ uint8_t byte_of(const uint4 w, int t) {
-  const uint32_t x = t < 4 ? w.x : t < 8 ? w.y : t < 12 ? w.z : w.w;
-  return (uint8_t)(x >> (8 * (t & 3)));
+// Bit 7 of byte t -> bit t (t < 4).
+__device__ __forceinline__ uint32_t byte_bits(uint32_t m) {
+  return ((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u);
 }
 
 // Root kinds of the 16 nodes starting at v0 (a multiple of 16) as a bit mask
 // per kind (bit t: node v0 + t), from their labels `w` and the labels of
 // their parents (v - 1) / 2, which are v0/2 - 1 .. v0/2 + 7: the last byte of
-// `lo` and the bytes of `pw`.  Labels of nodes the checkpoint did not visit
-// are FIXED (tile reset after every checkpoint), so most groups have no
-// candidate.
+// `lo` for t = 0, then byte (t - 1) / 2 of `pw`, gathered into four dwords
+// matching `w` with byte permutes.  All 16 nodes are tested together, a
+// byte per node: the node is FIRST or SHIFT and its parent DONE.
 __device__ __forceinline__ void scan16(const JobG& J, uint64_t v0, const uint4 w, uint64_t lo, uint64_t pw,
                                        uint32_t& fmask, uint32_t& smask) {
   fmask = smask = 0;
-  if (v0 >= J.N || !any_root_candidate(w)) return;
-  const uint8_t p_first = (uint8_t)(lo >> 56);  // parent byte v0/2 - 1
+  if (v0 >= J.N) return;
+  const uint32_t lohi = (uint32_t)(lo >> 32), pl = (uint32_t)pw, ph = (uint32_t)(pw >> 32);
+  // __builtin_amdgcn_perm(s0, s1, sel): byte k of the result is byte sel_k of
+  // the 8 bytes {s1 (0..3), s0 (4..7)}.
+  const uint32_t par[4] = {__builtin_amdgcn_perm(pl, lohi, 0x05040403u),  // p_first, pw0, pw0, pw1
+                           __builtin_amdgcn_perm(pl, pl, 0x03020201u),    // pw1, pw2, pw2, pw3
+                           __builtin_amdgcn_perm(ph, pl, 0x05040403u),    // pw3, pw4, pw4, pw5
+                           __builtin_amdgcn_perm(ph, ph, 0x03020201u)};   // pw5, pw6, pw6, pw7
+  const uint32_t lab[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-  for (int t = 0; t < 16; t++) {
-    const uint64_t v = v0 + t;
-    if (v >= J.N) break;
-    const uint8_t L = byte_of(w, t);
-    if (L != FIRST_OCUR && L != SHIFT_DUPL) continue;
-    bool root;
-    if (v == 0) {
-      // A tree emits node 0 only as a first occurrence (or, with the
-      // root-shift option, also as a shifted root); in a list job node 0 is
-      // a candidate only when it is the single chunk.
-      root = L == FIRST_OCUR || J.list_mode || J.root_shift;
-    } else {
-      const uint8_t P = t == 0 ? p_first : (uint8_t)(pw >> (8 * ((t - 1) >> 1)));
-      root = P == DONE;
-    }
-    if (!root) continue;
-    if (L == FIRST_OCUR) fmask |= 1u << t;
-    else smask |= 1u << t;
+  for (int d = 0; d < 4; d++) {
+    const uint32_t pdone = zero_bytes(par[d] ^ (DONE * 0x01010101u));
+    fmask |= byte_bits(zero_bytes(lab[d] ^ (FIRST_OCUR * 0x01010101u)) & pdone) << (4 * d);
+    smask |= byte_bits(zero_bytes(lab[d] ^ (SHIFT_DUPL * 0x01010101u)) & pdone) << (4 * d);
+  }
+  if (v0 == 0) {
+    // A tree emits node 0 only as a first occurrence (or, with the
+    // root-shift option, also as a shifted root); in a list job node 0 is
+    // a candidate only when it is the single chunk.
+    const uint8_t L = (uint8_t)w.x;
+    fmask = (fmask & ~1u) | (L == FIRST_OCUR ? 1u : 0u);
+    smask = (smask & ~1u) | (L == SHIFT_DUPL && (J.list_mode || J.root_shift) ? 1u : 0u);
+  }
+  if (J.N - v0 < 16) {
+    const uint32_t keep = (1u << (uint32_t)(J.N - v0)) - 1;
+    fmask &= keep;
+    smask &= keep;
   }
 }
 
@@ -3419,9 +3437,12 @@ void launch_low_root(const JobDev* jobs_h, uint32_t njobs, uint64_t max_c, uint3
   hipLaunchKernelGGL(k_lr_key, leaf_grid, dim3(256), 0, s, jobs);
   hipLaunchKernelGGL(k_lr_select, leaf_grid, dim3(256), 0, s, jobs);
 }
-void launch_pass1(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
+void launch_pass1(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, bool narrow,
+                  hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
-  if (grid) DD_LAUNCH(k_pass1, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
+  if (!grid) return;
+  if (narrow) DD_LAUNCH(k_pass1<uint32_t>, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
+  else DD_LAUNCH(k_pass1<uint64_t>, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
 }
 void launch_insert_interior(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid,
                             uint32_t kinds, hipStream_t s) {
@@ -3431,9 +3452,12 @@ void launch_insert_interior(const JobDev* jobs_h, const uint32_t* prefix, uint32
   if (kinds & JOBS_LIST_NARROW) DD_LAUNCH(k_insert_interior<JOBS_LIST_NARROW>, grid, 256, 0, s, jobs, prefix, njobs);
   if (kinds & JOBS_LIST_WIDE) DD_LAUNCH(k_insert_interior<JOBS_LIST_WIDE>, grid, 256, 0, s, jobs, prefix, njobs);
 }
-void launch_pass2(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, hipStream_t s) {
+void launch_pass2(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int g, bool narrow,
+                  hipStream_t s) {
   const JobG* jobs = gview(jobs_h);
-  if (grid) DD_LAUNCH(k_pass2, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
+  if (!grid) return;
+  if (narrow) DD_LAUNCH(k_pass2<uint32_t>, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
+  else DD_LAUNCH(k_pass2<uint64_t>, grid, kTileThreads, 0, s, jobs, prefix, njobs, g);
 }
 void launch_roots_count(const JobDev* jobs_h, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s) {
   const JobG* jobs = gview(jobs_h);

@@ -297,10 +297,14 @@ void launch_list_resolve(const JobDev* jobs, const uint32_t* prefix, uint32_t nj
 // occurrence to the leaf whose highest new ancestor has the smallest node id
 // (jobs without low_root are skipped).  `max_c` / `max_depth`: over the jobs.
 void launch_low_root(const JobDev* jobs, uint32_t njobs, uint64_t max_c, uint32_t max_depth, hipStream_t s);
-void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
+// `narrow`: every job's tree has fewer than 2^31 nodes (32-bit node
+// indices in the tile passes).
+void launch_pass1(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, bool narrow,
+                  hipStream_t s);
 void launch_insert_interior(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, uint32_t kinds,
                             hipStream_t s);
-void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, hipStream_t s);
+void launch_pass2(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, int group, bool narrow,
+                  hipStream_t s);
 void launch_roots_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_roots_write(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
 void launch_sbin_count(const JobDev* jobs, const uint32_t* prefix, uint32_t njobs, uint32_t grid, hipStream_t s);
