@@ -240,7 +240,9 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   const uint64_t nblk = ceil_div(N, kNodesPerBlock);
   // Per root-scan workgroup (at most one per node block): three counts; then
   // the root masks of every group of 16 nodes.
-  DD_TRY(blk.ensure((3 + kNodesPerBlock / 16) * nblk * 4));
+  // Block counts, root masks, then (8-byte aligned) the label masks.
+  const uint64_t lmask_off = ((3 + kNodesPerBlock / 16) * nblk * 4 + 7) & ~7ull;
+  DD_TRY(blk.ensure(lmask_off + nblk * (kNodesPerBlock / 16 / 64) * 8));
   DD_TRY(f_list.ensure((C + 1) * 4));
   DD_TRY(f_off.ensure((C + 1) * 4));
   DD_TRY(s_node.ensure(C * 4));
@@ -360,6 +362,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   job.blk_s = job.blk_fc + nblk;
   job.nblk = (uint32_t)nblk;
   job.rmask = job.blk_s + nblk;
+  job.lmask = reinterpret_cast<uint64_t*>(blk.as<uint8_t>() + lmask_off);
   job.f_list = f_list.as<uint32_t>();
   job.f_off = f_off.as<uint32_t>();
   job.s_node = s_node.as<uint32_t>();

@@ -2247,7 +2247,15 @@ __global__ __launch_bounds__(256) void k_roots_count(const JobG* __restrict__ jo
       f += __popc(fm);
       fc += first_chunks(g, v0, fm);
       s += __popc(sm);
-      if (b + q < b1) J.rmask[(uint64_t)(b + q) * (kNodesPerBlock / 16) + threadIdx.x] = fm | (sm << 16);
+      // Groups with a label other than FIXED (past the nodes the loads put
+      // FIXED bytes): the root writer resets them.
+      const uint4 w = lg[q].w;
+      const bool dirty = ((w.x ^ 0x01010101u) | (w.y ^ 0x01010101u) | (w.z ^ 0x01010101u) | (w.w ^ 0x01010101u)) != 0;
+      const uint64_t dm = __ballot(dirty);
+      if (b + q < b1) {  // uniform
+        J.rmask[(uint64_t)(b + q) * (kNodesPerBlock / 16) + threadIdx.x] = fm | (sm << 16);
+        if (lane_id() == 0) J.lmask[(uint64_t)(b + q) * (kNodesPerBlock / 16 / 64) + (threadIdx.x >> 6)] = dm;
+      }
     }
   }
   __shared__ uint32_t sh[12];
@@ -2289,15 +2297,29 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobG* __restrict__ jo
     J.ctr->dchunks = run_fc + J.blk_fc[wg];
     J.ctr->nshift = run_s + J.blk_s[wg];
   }
+  // Label hygiene of a tree checkpoint: every group of 16 nodes the count
+  // found with a label other than FIXED is reset to FIXED (16 bytes), which
+  // is what pass 2 computes for a subtree whose leaves are all fixed.  A
+  // later checkpoint that does not visit a tile then reads FIXED labels
+  // there, and its leaf kernel writes the labels of changed leaves only.  The
+  // count has read every label (and parent label) of the job by now; the
+  // list method keeps its labels.
+  const bool reset = !J.list_mode;
+  const GlobalPtr<u32x4_t> lab16 = (GlobalPtr<u32x4_t>)J.labels;
   for (uint32_t b = b0; b < b1; b += kRootBatch) {
     uint32_t rm[kRootBatch];
+    uint64_t lm[kRootBatch];
 #pragma unroll
-    for (uint32_t q = 0; q < kRootBatch; q++)
-      rm[q] = J.rmask[(uint64_t)(b + q < b1 ? b + q : b) * (kNodesPerBlock / 16) + threadIdx.x];
+    for (uint32_t q = 0; q < kRootBatch; q++) {
+      const uint64_t bq = b + q < b1 ? b + q : b;
+      rm[q] = J.rmask[bq * (kNodesPerBlock / 16) + threadIdx.x];
+      lm[q] = reset ? J.lmask[bq * (kNodesPerBlock / 16 / 64) + (threadIdx.x >> 6)] : 0ull;
+    }
 #pragma unroll
     for (uint32_t q = 0; q < kRootBatch; q++) {
       if (b + q >= b1) break;  // uniform
       const uint64_t v0 = (uint64_t)(b + q) * kNodesPerBlock + threadIdx.x * 16;
+      if ((lm[q] >> lane_id()) & 1ull) lab16[v0 / 16] = u32x4_t{0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
       uint32_t fm = rm[q] & 0xffffu, sm = rm[q] >> 16;
       if (!__syncthreads_or((fm | sm) != 0)) continue;  // no root in this node block (uniform)
       const uint32_t f = __popc(fm), fc = first_chunks(g, v0, fm), s = __popc(sm);
@@ -2333,48 +2355,6 @@ __global__ __launch_bounds__(256) void k_roots_write(const JobG* __restrict__ jo
 // s_hist is bin-major: entry (bin, tile) at bin * ntiles + tile.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t stiles_of(const JobG& J) { return (J.ctr->nshift + kSTile - 1) / kSTile; }
-
-// Label hygiene, run by k_gather once the roots are written: every label of
-// every group-0 tile visited by this checkpoint (leaves, interior nodes, tile
-// root) is reset to FIXED, which is what pass 2 computes for a tile whose
-// leaves are all fixed.  A later checkpoint that does not visit the tile then
-// reads FIXED labels there, so the root scan needs no per-tile check, and its
-// leaf kernel writes the labels of changed leaves only.
-__device__ __forceinline__ void reset_visited_tiles(const JobG& J, uint32_t wg, uint32_t nwg) {
-  const uint32_t Lr0 = J.D > (uint32_t)kTileLevels ? J.D - kTileLevels : 0;
-  const uint32_t ntiles0 = 1u << Lr0;
-  const uint32_t H = J.D - Lr0;
-  __shared__ uint32_t pre[kPendSpread + 1];
-  __shared__ uint32_t tids[256];
-  uint32_t nitems = ntiles0;
-  if (!J.is_baseline) {
-    load_list_prefix(J.dirty_cnt, pre);
-    nitems = pre[kPendSpread];
-  }
-  // The tile ids of up to 256 iterations are loaded together, then their
-  // labels stored: a list load between stores would wait for them.
-  const uint32_t niter = wg < nitems ? (nitems - wg + nwg - 1) / nwg : 0;
-  for (uint32_t n0 = 0; n0 < niter; n0 += 256) {
-    const uint32_t nb = min(256u, niter - n0);
-    __syncthreads();  // the previous block's ids are read
-    if (threadIdx.x < nb) {
-      const uint32_t it = wg + (n0 + threadIdx.x) * nwg;
-      tids[threadIdx.x] = J.is_baseline ? it : list_item(J.dirty_list, J.dirty_cap, pre, it);
-    }
-    __syncthreads();
-    for (uint32_t b = 0; b < nb; b++) {
-      const uint64_t r = ((uint64_t)1 << Lr0) - 1 + tids[b];
-      if (threadIdx.x + 1 < (1u << H)) {  // levels above the bottom (256 threads, at most 255 positions)
-        const uint64_t x = tile_node(r, threadIdx.x);
-        if (x < J.N) J.labels[x] = FIXED_DUPL;
-      }
-      if (threadIdx.x < (1u << H)) {  // the bottom level
-        const uint64_t x = ((r + 1) << H) - 1 + threadIdx.x;
-        if (x < J.N) J.labels[x] = FIXED_DUPL;
-      }
-    }
-  }
-}
 
 // Workgroups stride over the sort tiles (kSTile shifted roots each, kSTile /
 // 256 per lane).  Each lane loads its roots' nodes, then their map slots, then
@@ -2722,10 +2702,8 @@ __global__ __launch_bounds__(256) void k_gather(const JobG* __restrict__ jobs, c
   // only region (the one region that holds all C chunks).
   const bool copied = J.write_data && F == 1 && P == 0 && S == 0 && Dq == J.C;
   if (!copied) gather_data<Vec>(J, F, P, S, Dq, wg, nblk);
-  // Stores only from here on: the metadata, and (the roots are written) the
-  // reset of the labels of the tiles this checkpoint visited.
+  // Stores only from here on: the metadata.
   write_meta(J, F, P, S, wg * blockDim.x + threadIdx.x, nblk * blockDim.x);
-  if (!J.list_mode) reset_visited_tiles(J, wg, nblk);
 }
 
 // ---------------------------------------------------------------------------

@@ -154,6 +154,9 @@ struct JobT {
   P<uint32_t> blk_s;    // per root-scan workgroup: shifted roots (prefix)
   uint32_t nblk;      // node blocks = ceil(N / kNodesPerBlock)
   P<uint32_t> rmask;  // per group of 16 nodes: first-root bits | shifted-root bits << 16
+  // Per 64 groups of 16 nodes (a wavefront of the root count): bit g set when
+  // group g holds a label other than FIXED, which the root writer resets.
+  P<uint64_t> lmask;
   P<uint32_t> f_list;   // first roots, ascending
   P<uint32_t> f_off;    // chunk offset of each first region in the data section
   P<uint32_t> s_node;   // shifted roots, ascending
