@@ -393,8 +393,8 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
 void StreamState::commit(const JobCounters& c, bool baseline, bool leaf_copy) {
   if (baseline) baseline_id = current_id;
   need_baseline = false;
-  // The gather kernel of a completed tree checkpoint reset every label of
-  // the tiles it visited; the list method keeps its labels.
+  // The root writer of a completed tree checkpoint reset every label that
+  // was not FIXED; the list method keeps its labels.
   labels_fixed = method != DD_METHOD_LIST;
   cnt_par ^= 1u;  // this checkpoint's k_sbin_scan zeroed the other counter set
   if (!baseline) last_pending = c.pending;

@@ -75,7 +75,7 @@ struct JobT {
   uint32_t list_mode;
   uint32_t node_bias;
   // 0: every leaf label is FIXED on entry (the previous tree checkpoint of
-  // the handle completed and reset the labels of the tiles it visited), so
+  // the handle completed and its root writer reset every other label), so
   // the leaf kernel writes the labels of changed leaves only.
   uint32_t write_all_labels;
   // 1: the map inserts probe in wide rounds (few inserts expected, each
