@@ -220,7 +220,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
     if (method == DD_METHOD_TREE_LOW_ROOT) DD_TRY(lr_new.ensure(Nn));
     DD_TRY(tree.ensure(Nn * sizeof(Digest)));
     DD_TRY(labels.ensure(ceil_div(Nn, kNodesPerBlock) * kNodesPerBlock + 64));  // root scans read 16 at a time
-    DD_TRY(slot_of.ensure(Nn * 4));
+    DD_TRY(slot_of.ensure(Nn * 8));
     const uint64_t cap = map_capacity(2 * Nn);
     if (map.bytes > 4 * cap * sizeof(Slot)) map.release();
     DD_TRY(map.ensure(cap * sizeof(Slot)));
@@ -329,7 +329,7 @@ dd_status StreamState::prepare(const uint8_t* data, uint64_t len, bool baseline,
   labels_fixed = false;  // until this checkpoint completes
   job.tree = tree.as<Digest>();
   job.labels = labels.as<uint8_t>();
-  job.slot_of = slot_of.as<uint32_t>();
+  job.slot_of = slot_of.as<uint64_t>();
   job.map = make_view(map.p, map_cap);
   job.map_count_at_start = map_count;
   job.map_limit = map_limit_of(map_cap);

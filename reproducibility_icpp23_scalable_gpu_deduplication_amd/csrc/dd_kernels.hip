@@ -849,8 +849,9 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
   const uint32_t K = J.ckpt_id;
   const InsertCtx ic = insert_ctx(J);
   const GlobalPtr<uint8_t> labels = J.labels;
-  const GlobalPtr<uint32_t> slot_of = J.slot_of;
+  const GlobalPtr<uint64_t> slot_of = J.slot_of;
   const GlobalPtr<uint32_t> flags = &J.ctr->flags;
+  const bool by_slot = J.low_root != 0;  // low-root kernels rewrite entries through their slots
   const uint32_t lane = lane_id();
   // One item per lane: leaf `leaf` with label `own` and digest `k`.  Called
   // by every lane of a wavefront together (uniform loop bounds), since it
@@ -861,17 +862,20 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
     const int pact = __shfl_up((int)act, 1, 64);
     const bool head = !(act && lane > 0 && pact && pk0 == k.k0 && pk1 == k.k1 && pleaf < leaf);
     bool ok = true;
-    uint64_t slot = 0;
+    uint64_t ref = 0;
     if (act && head) {
       const InsertResult r = map_insert<true, Wide>(ic, k, (uint32_t)leaf);
       ok = r.ok;
-      slot = r.slot;
+      ref = kRefSlot | r.slot;
       if (!r.ok) {
         g_or(flags, FLAG_MAP_FULL);
       } else if (r.claimed) {
         claims++;
       } else {
         const uint32_t pt = (uint32_t)(r.prev >> 32), pn = (uint32_t)r.prev;
+        // An entry of an earlier checkpoint is final: the shifted-root
+        // lookup then needs no map read.
+        if (pt != K && !by_slot && !ref_is_slot(r.prev)) ref = r.prev;
         bool self_shift = true;
         if (pt == K) {
           if (pn > leaf) {
@@ -892,13 +896,13 @@ __global__ __launch_bounds__(256) void k_insert_leaves(const JobG* __restrict__ 
     // Each lane's run head: the highest head lane at or below it.
     const unsigned long long heads = __ballot(head);
     const uint32_t hl = 63 - __clzll(heads & ((2ull << lane) - 1ull));
-    const unsigned long long hslot = __shfl((unsigned long long)slot, hl, 64);
+    const unsigned long long href = __shfl((unsigned long long)ref, hl, 64);
     const int hok = __shfl((int)ok, hl, 64);
     if (act && !head && hok) {
-      slot_of[leaf] = (uint32_t)hslot;
+      slot_of[leaf] = href;
       if (own == PENDING) labels[leaf] = SHIFT_DUPL;  // a lower leaf of this run holds the digest
     } else if (act && head && ok) {
-      slot_of[leaf] = (uint32_t)slot;
+      slot_of[leaf] = ref;
     }
   };
   if (dense) {
@@ -970,9 +974,9 @@ __global__ __launch_bounds__(256) void k_lr_leaves(const JobG* __restrict__ jobs
     uint8_t nw = 0;
     if (lab == PENDING) {
       nw = 1;
-      *(GlobalPtr<uint64_t>)&J.map.slots[J.slot_of[leaf]].pad = ~0ull;
+      *(GlobalPtr<uint64_t>)&J.map.slots[(uint32_t)J.slot_of[leaf]].pad = ~0ull;
     } else if (lab == SHIFT_DUPL) {
-      nw = (uint32_t)(J.map.slots[J.slot_of[leaf]].val >> 32) == J.ckpt_id ? 1 : 0;
+      nw = (uint32_t)(J.map.slots[(uint32_t)J.slot_of[leaf]].val >> 32) == J.ckpt_id ? 1 : 0;
     }
     J.lr_new[leaf] = nw;
   }
@@ -998,7 +1002,7 @@ __global__ __launch_bounds__(256) void k_lr_key(const JobG* __restrict__ jobs) {
     uint64_t t = leaf;
     while (t > 0 && J.lr_new[(t - 1) >> 1]) t = (t - 1) >> 1;
     const unsigned long long key = ((unsigned long long)t << 32) | (uint32_t)leaf;
-    const GlobalPtr<unsigned long long> pad = (GlobalPtr<unsigned long long>)&J.map.slots[J.slot_of[leaf]].pad;
+    const GlobalPtr<unsigned long long> pad = (GlobalPtr<unsigned long long>)&J.map.slots[(uint32_t)J.slot_of[leaf]].pad;
     if (key < *pad) g_min(pad, key);  // a key already below cannot change
   }
 }
@@ -1011,7 +1015,7 @@ __global__ __launch_bounds__(256) void k_lr_select(const JobG* __restrict__ jobs
   for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < J.C; k += (uint64_t)gridDim.x * 256) {
     const uint64_t leaf = J.C - 1 + k;
     if (!J.lr_new[leaf]) continue;
-    const GlobalPtr<Slot> sl = &J.map.slots[J.slot_of[leaf]];
+    const GlobalPtr<Slot> sl = &J.map.slots[(uint32_t)J.slot_of[leaf]];
     const unsigned long long key = *(GlobalPtr<unsigned long long>)&sl->pad;
     if ((uint32_t)key == (uint32_t)leaf) {
       J.labels[leaf] = PENDING;
@@ -1518,6 +1522,7 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
   // Slots, written in slot order half a bucket at a time: each digest's
   // entry is staged in LDS by the lane that holds its record (the probing
   // arrays are free by then), EMPTY where no record landed.
+  const uint32_t K = J.ckpt_id;
   uint32_t node[kRecPerLane], slot[kRecPerLane];
   uint32_t used = 0;
 #pragma unroll
@@ -1525,7 +1530,13 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
     const uint32_t r = threadIdx.x + i * kBuildThreads;
     node[i] = r < n ? rnode[r] : 0u;
     slot[i] = r < n ? rslot[r] : 0u;
-    if (r < nl && !win[i]) J.slot_of[node[i]] = (uint32_t)((uint64_t)b * bs + slot[i]);
+    if (r < nl && !win[i]) {
+      // A shifted leaf: its digest's entry, final (the build decides it):
+      // this checkpoint and the winner's node, decoded from its rank.
+      const uint32_t wr = best[owner[slot[i]]];
+      const uint32_t wn = wr < 0x80000000u ? wr + C1 : 0x7fffffffu - (wr & 0x7fffffffu);
+      J.slot_of[node[i]] = K != 0xffffffffu ? (((uint64_t)K << 32) | wn) : (kRefSlot | ((uint64_t)b * bs + slot[i]));
+    }
     used += (r < n && win[i]) ? 1u : 0u;
   }
   for (uint32_t x = threadIdx.x; x < kBucketSlots / 32; x += blockDim.x) occ[x] = 0;
@@ -1533,7 +1544,6 @@ __global__ __launch_bounds__(kBuildThreads) void k_bulk_build(const JobG* __rest
   for (uint32_t q = threadIdx.x; q < bs; q += blockDim.x)
     if (owner[q] != 0xffffffffu) atomicOr(&occ[q >> 5], 1u << (q & 31));
   __syncthreads();  // every read of the probing arrays is done
-  const uint32_t K = J.ckpt_id;
   for (uint32_t h0 = 0; h0 < bs; h0 += kHalf) {
 #pragma unroll
     for (uint32_t i = 0; i < kRecPerLane; i++) {
@@ -1926,18 +1936,21 @@ __device__ __forceinline__ uint8_t dup_label(uint8_t la, uint8_t lb, bool found)
   return found ? (uint8_t)SHIFT_DUPL : (uint8_t)DONE;
 }
 
-// Lookups of up to NK keys (need[n] false: skipped, out[n] = ~0).  Each
+// Lookups of up to NK keys: out[n] = the matching slot, outv[n] = its value
+// (~0 for both when need[n] is false or the key is absent).  Each
 // probe round loads two consecutive slots of every key still searching
 // before comparing any of them, so a lookup that ends within two slots of
 // its home slot (most of them at the map's load factor, found or not) costs
 // one memory round trip.
 template <int NK>
-__device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, const bool* need, uint64_t* out) {
+__device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, const bool* need, uint64_t* out,
+                                           uint64_t* outv) {
   uint64_t s[NK];
   bool todo[NK];
 #pragma unroll
   for (int n = 0; n < NK; n++) {
     out[n] = ~0ull;
+    outv[n] = ~0ull;
     todo[n] = need[n];
     s[n] = map_start(m, key[n]);
   }
@@ -1963,6 +1976,7 @@ __device__ __forceinline__ void map_find_n(const MapG& m, const Digest* key, con
           todo[n] = false;
         } else if (k0 == key[n].k0 && k1 == key[n].k1) {
           out[n] = map_next(m, s[n], e);
+          outv[n] = val;
           todo[n] = false;
         }
       }
@@ -1984,13 +1998,16 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
   __shared__ uint32_t pre[kPendSpread + 1];
   __shared__ uint8_t tlab[kTopNodes + 64];
   __shared__ Digest tdig[kTopNodes + 64];
-  __shared__ uint32_t tslot[kTopNodes];
+  __shared__ uint64_t tref[kTopNodes];
   const uint32_t nitems = group_items(J, g, pre);
   const uint32_t i = threadIdx.x;
   const uint64_t N = J.N, C = J.C;
   const GlobalPtr<uint8_t> labels = J.labels;
   const GlobalPtr<Digest> tree = J.tree;
-  const GlobalPtr<uint32_t> slot_of = J.slot_of;
+  const GlobalPtr<uint64_t> slot_of = J.slot_of;
+  // A shifted node's map reference: the entry's value (final: every insert
+  // of the checkpoint is done), or its slot for the reserved value.
+  auto ref_of = [](uint64_t slot, uint64_t val) { return ref_is_slot(val) ? kRefSlot | slot : val; };
   const int kind = J.hash_kind;
   const MapG map = J.map;
   const bool listed = g == 0 && !J.is_baseline;
@@ -2008,29 +2025,31 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
     // Level 1: both lookups of the lane at once.
     bool look[2];
     Digest d1[2];
-    uint64_t f1[2];
+    uint64_t f1[2], e1[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       look[k] = !vu[k] && T.v1[k] && bl[2 * k] == SHIFT_DUPL && bl[2 * k + 1] == SHIFT_DUPL;
       d1[k] = vu[k] ? ud[k] : look[k] ? hash_pair(kind, bd[2 * k], bd[2 * k + 1]) : Digest{0, 0};
     }
-    map_find_n<2>(map, d1, look, f1);
+    map_find_n<2>(map, d1, look, f1, e1);
     uint8_t l1[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       l1[k] = vu[k] ? ul[k] : T.v1[k] ? dup_label(bl[2 * k], bl[2 * k + 1], f1[k] != ~0ull) : (uint8_t)0xff;
       if (!vu[k] && l1[k] != SHIFT_DUPL) d1[k] = Digest{0, 0};
+      if (!vu[k] && l1[k] == SHIFT_DUPL) slot_of[T.x1 + k] = ref_of(f1[k], e1[k]);  // stored at once
     }
     // Level 2.
     uint8_t l2 = 0xff;
     Digest d2{0, 0};
-    uint64_t f2 = ~0ull;
+    uint64_t f2 = ~0ull, e2 = ~0ull;
     if (T.v2) {
       const bool lk = l1[0] == SHIFT_DUPL && l1[1] == SHIFT_DUPL;
       if (lk) d2 = hash_pair(kind, d1[0], d1[1]);
-      map_find_n<1>(map, &d2, &lk, &f2);
+      map_find_n<1>(map, &d2, &lk, &f2, &e2);
       l2 = dup_label(l1[0], l1[1], f2 != ~0ull);
       if (l2 != SHIFT_DUPL) d2 = Digest{0, 0};
+      if (l2 == SHIFT_DUPL) slot_of[T.x2] = ref_of(f2, e2);
     }
     // Levels 3 .. H in LDS (level-2 results at positions nq .. 2 nq).  With
     // no two sibling level-2 nodes shifted no node above level 2 is looked
@@ -2050,38 +2069,32 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
           const uint8_t la = tlab[qc], lb = tlab[qc + 1];
           const bool lk = la == SHIFT_DUPL && lb == SHIFT_DUPL;
           Digest d{0, 0};
-          uint64_t f = ~0ull;
+          uint64_t f = ~0ull, fv = ~0ull;
           if (lk) {
             d = hash_pair(kind, tdig[qc], tdig[qc + 1]);
-            map_find_n<1>(map, &d, &lk, &f);
+            map_find_n<1>(map, &d, &lk, &f, &fv);
           }
           const uint8_t L = dup_label(la, lb, f != ~0ull);
           tlab[q0 + i] = L;
           tdig[q0 + i] = L == SHIFT_DUPL ? d : Digest{0, 0};
-          tslot[q0 + i] = (uint32_t)f;
+          tref[q0 + i] = ref_of(f, fv);
         }
       }
       __syncthreads();
     }
-    // Stores: labels of the interior nodes, digest and map slot of the
-    // shifted ones.
+    // Stores: labels of the interior nodes, digest (and, above level 2, map
+    // reference) of the shifted ones.
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       if (T.v1[k] && !vu[k]) {
         const Idx x = T.x1 + k;
         labels[x] = l1[k];
-        if (l1[k] == SHIFT_DUPL) {
-          store_digest(tree + x, d1[k]);
-          slot_of[x] = (uint32_t)f1[k];
-        }
+        if (l1[k] == SHIFT_DUPL) store_digest(tree + x, d1[k]);
       }
     }
     if (T.v2) {
       labels[T.x2] = l2;
-      if (l2 == SHIFT_DUPL) {
-        store_digest(tree + T.x2, d2);
-        slot_of[T.x2] = (uint32_t)f2;
-      }
+      if (l2 == SHIFT_DUPL) store_digest(tree + T.x2, d2);
     }
     const uint64_t mx2 = __ballot(T.v2 && l2 == FIXED_DUPL), mr2 = __ballot(T.v2 && l2 == FIRST_OCUR);
     if (i < T.nq) {
@@ -2090,7 +2103,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6,
       labels[x] = L;
       if (L == SHIFT_DUPL) {
         store_digest(tree + x, tdig[i]);
-        slot_of[x] = tslot[i];
+        slot_of[x] = tref[i];
       }
     }
     __syncthreads();  // the LDS levels are reused by the next tile
@@ -2372,9 +2385,11 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobG* __restrict__ job
   const uint32_t S = J.ctr->nshift;
   for (uint32_t t = wg; t < nt; t += nwg) {
     const uint32_t e0 = t * kSTile, e1 = min(S, e0 + kSTile);
-    // The roots' sources: the map entries their digests matched (slots
-    // recorded by pass 2, or by the leaf inserts for shifted leaves).
-    uint32_t node[Q], slot[Q];
+    // The roots' sources: the map entries their digests matched, as the
+    // leaf inserts, pass 2 or the bulk build recorded them: the entry's value
+    // when it was final, else its slot (an entry of this checkpoint met by a
+    // leaf insert, which a later insert could still lower), read now.
+    uint32_t node[Q];
     unsigned long long val[Q];
 #pragma unroll
     for (uint32_t q = 0; q < Q; q++) {
@@ -2384,12 +2399,12 @@ __global__ __launch_bounds__(256) void k_sbin_count(const JobG* __restrict__ job
 #pragma unroll
     for (uint32_t q = 0; q < Q; q++) {
       const uint32_t e = e0 + q * 256 + threadIdx.x;
-      slot[q] = e < e1 ? J.slot_of[node[q]] : 0;
+      val[q] = e < e1 ? J.slot_of[node[q]] : 0;
     }
 #pragma unroll
     for (uint32_t q = 0; q < Q; q++) {
       const uint32_t e = e0 + q * 256 + threadIdx.x;
-      val[q] = e < e1 ? J.map.slots[slot[q]].val : 0;
+      if (e < e1 && ref_is_slot(val[q])) val[q] = J.map.slots[(uint32_t)val[q]].val;
     }
     __syncthreads();  // the previous tile's atomics are done
     for (uint32_t bn = threadIdx.x; bn < nbins; bn += blockDim.x) J.s_hist[(uint64_t)bn * nt + t] = 0;

@@ -113,7 +113,12 @@ struct JobT {
   // persistent state
   P<Digest> tree;        // N digests, heap order
   P<uint8_t> labels;     // N labels (padded for 16-byte reads)
-  P<uint32_t> slot_of;   // N: map slot of pending leaves and shifted interior nodes
+  // N: the map entry of a pending leaf or shifted node: its value (source
+  // checkpoint << 32 | source node) when it could not change any more when
+  // the node met it (an entry of an earlier checkpoint, or one met after this
+  // checkpoint's inserts), else kRefSlot | its slot (low-root jobs: always
+  // the slot, which their kernels rewrite entries through).
+  P<uint64_t> slot_of;
   P<uint8_t> lr_new;     // N (low-root jobs only)
   MapT<P> map;
   uint64_t map_count_at_start;  // entries before this checkpoint
@@ -176,6 +181,10 @@ struct JobT {
   uint32_t write_data;
   uint32_t pad_;
 };
+// slot_of entries that name a map slot rather than an entry's value.
+constexpr uint64_t kRefSlot = 0xffffffff00000000ull;
+__host__ __device__ inline bool ref_is_slot(uint64_t ref) { return (ref >> 32) == 0xffffffffu; }
+
 using JobDev = JobT<HostPtr>;
 using JobG = JobT<GlobalPtr>;
 static_assert(sizeof(JobDev) == sizeof(JobG), "one layout for both pointer kinds");

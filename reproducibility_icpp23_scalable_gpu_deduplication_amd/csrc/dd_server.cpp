@@ -55,7 +55,7 @@ bool groupable(const Request* a, const Request* b) {
 }
 
 // Device bytes a handle keeps for a buffer of `len` bytes: tree (16 B/node),
-// labels and slot index (5 B/node), the digest map (2N slots of 32 B), the
+// labels and map references (9 B/node), the digest map (2N slots of 32 B), the
 // per-chunk scratch arrays (root lists, work lists with their digests, tile
 // regions: about 42 B/chunk, counted as 44) and the bulk map build's records
 // (32 B/node), count matrices (8 B per tile of 32768 chunks and map bucket of
@@ -68,7 +68,7 @@ uint64_t state_bytes(uint64_t len, uint32_t chunk) {
   while (cap < 2 * N && cap < 4096) cap <<= 1;
   if (2 * N > 4096) cap = (2 * N + 4095) / 4096 * 4096;
   const uint64_t tiles = (C + 32767) / 32768, buckets = (cap + 4095) / 4096;
-  return 21 * N + 32 * cap + 44 * C + 32 * N + 8 * tiles * buckets + 16 * buckets + (128u << 10);
+  return 25 * N + 32 * cap + 44 * C + 32 * N + 8 * tiles * buckets + 16 * buckets + (128u << 10);
 }
 
 }  // namespace
