@@ -942,8 +942,9 @@ uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 class RestartSpace {
  public:
   virtual ~RestartSpace() = default;
-  // Device staging for kHdrWindow headers.
-  virtual dd_status headers(uint8_t** dev) = 0;
+  // Device staging for a header round trip of `want` bytes; *cap is what it
+  // holds (at least kHdrWindow headers).
+  virtual dd_status headers(uint64_t want, uint8_t** dev, uint64_t* cap) = 0;
   // Cover table, pending sources and pending list for C chunks, and the
   // stamp counter of that cover table.  A cover table that is new (or whose
   // contents are unknown) reads as all zero, with the counter at zero.
@@ -961,9 +962,11 @@ constexpr uint32_t kHdrWindow = 64;
 class HandleSpace : public RestartSpace {
  public:
   HandleSpace(dd_dedup* h) : h_(h) {}
-  dd_status headers(uint8_t** dev) override {
-    DD_TRY(h_->rs_hdr.ensure(kHdrWindow * 32));
+  dd_status headers(uint64_t want, uint8_t** dev, uint64_t* cap) override {
+    // Grows to at most 1 MiB: windows of very long chains fetch headers only.
+    DD_TRY(h_->rs_hdr.ensure(std::max<uint64_t>(std::min<uint64_t>(want, 1 << 20), kHdrWindow * 32)));
     *dev = h_->rs_hdr.as<uint8_t>();
+    *cap = h_->rs_hdr.bytes;
     return DD_OK;
   }
   dd_status tables(uint64_t C, uint2** cover, uint64_t** ref, uint32_t** pend, uint32_t** stamp) override {
@@ -1033,8 +1036,9 @@ NoAllocDevice na_dev[kNaDevices];
 class NoAllocSpace : public RestartSpace {
  public:
   explicit NoAllocSpace(NoAllocDevice& d) : d_(d) {}
-  dd_status headers(uint8_t** dev) override {
+  dd_status headers(uint64_t, uint8_t** dev, uint64_t* cap) override {
     *dev = d_.ptr;
+    *cap = kNaCover;
     return DD_OK;
   }
   dd_status tables(uint64_t C, uint2** cover, uint64_t** ref, uint32_t** pend, uint32_t** stamp) override {
@@ -1091,17 +1095,21 @@ dd_status walk_headers(uint32_t chkpt_id, uint64_t len, HdrOf hdr_of, std::vecto
 
 // Copies device byte ranges into one device buffer with k_fetch launches
 // (kFetchRanges per launch), then into host memory; waits for the copy.
+// With `lim` (diff lengths), each range is a diff whose header and prior
+// table are fetched into n[i] bytes.
 dd_status fetch_to_host(const std::vector<const uint8_t*>& src, const std::vector<uint64_t>& n,
                         const std::vector<uint64_t>& off, uint8_t* dev_dst, uint8_t* host_dst, uint64_t total,
-                        hipStream_t s) {
+                        hipStream_t s, const std::vector<uint64_t>* lim = nullptr) {
   for (size_t b = 0; b < src.size(); b += kFetchRanges) {
     FetchArgs a;
     memset(&a, 0, sizeof a);
+    a.tables = lim ? 1u : 0u;
     uint64_t mx = 0;
     for (size_t i = b; i < src.size() && i < b + kFetchRanges; i++) {
       a.src[a.n] = src[i];
       a.len[a.n] = n[i];
       a.dst_off[a.n] = off[i];
+      a.lim[a.n] = lim ? (*lim)[i] : 0;
       mx = std::max(mx, n[i]);
       a.n++;
     }
@@ -1125,9 +1133,12 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
     return (host_chain ? host_chain[i].data != nullptr : dev_chain[i] != nullptr) && total_of(i) >= 32;
   };
   // 1. Headers of the visited checkpoints (device diffs: read in windows of
-  // kHdrWindow ids per round trip).
+  // kHdrWindow ids per round trip, with their prior tables where the staging
+  // holds them).
   std::vector<uint32_t> ids;
   std::vector<dd_header> hdrs;
+  std::vector<std::vector<uint8_t>> tabs;  // device diffs: prior table of id i, when fetched whole
+  std::vector<char> have_tab;
   if (host_chain) {
     DD_TRY(walk_headers(chkpt_id, len, [&](uint32_t i, dd_header& hd) -> dd_status {
       if (!valid(i)) return DD_ECORRUPT;
@@ -1135,30 +1146,52 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
       return DD_OK;
     }, ids, hdrs));
   } else {
-    uint8_t* stage = nullptr;
-    DD_TRY(sp.headers(&stage));
     std::vector<dd_header> known((size_t)chkpt_id + 1);
     std::vector<char> have((size_t)chkpt_id + 1, 0);
+    tabs.resize((size_t)chkpt_id + 1);
+    have_tab.assign((size_t)chkpt_id + 1, 0);
     DD_TRY(walk_headers(chkpt_id, len, [&](uint32_t i, dd_header& hd) -> dd_status {
       if (!valid(i)) return DD_ECORRUPT;
       if (!have[i]) {
         const uint32_t lo = i + 1 >= kHdrWindow ? i + 1 - kHdrWindow : 0;
         std::vector<const uint8_t*> src;
-        std::vector<uint64_t> n, off;
+        std::vector<uint64_t> n, off, lim;
         std::vector<uint32_t> got;
+        uint64_t total = 0;
         for (uint32_t k = lo; k <= i; k++) {
           if (have[k] || !valid(k)) continue;
           src.push_back(static_cast<const uint8_t*>(dev_chain[k]));
-          n.push_back(32);
-          off.push_back(32ull * got.size());
+          // A valid diff k lists at most k + 1 prior checkpoints.
+          n.push_back(32 + 8ull * (k + 1));
+          off.push_back(total);
+          lim.push_back(dev_lens[k]);
+          total += align256(n.back());
           got.push_back(k);
         }
+        uint8_t* stage = nullptr;
+        uint64_t cap = 0;
+        DD_TRY(sp.headers(total, &stage, &cap));
+        const bool with_tab = total <= cap;
+        if (!with_tab) {  // headers only
+          for (size_t k = 0; k < got.size(); k++) {
+            n[k] = 32;
+            off[k] = 32ull * k;
+          }
+          total = 32ull * got.size();
+        }
         uint8_t* hst = nullptr;
-        DD_TRY(sp.host(32ull * got.size(), &hst));
-        DD_TRY(fetch_to_host(src, n, off, stage, hst, 32ull * got.size(), s));
+        DD_TRY(sp.host(total, &hst));
+        DD_TRY(fetch_to_host(src, n, off, stage, hst, total, s, with_tab ? &lim : nullptr));
         for (size_t k = 0; k < got.size(); k++) {
-          memcpy(&known[got[k]], hst + 32 * k, 32);
+          dd_header& kh = known[got[k]];
+          memcpy(&kh, hst + off[k], 32);
           have[got[k]] = 1;
+          // The table is whole when it fitted its staging and the diff.
+          const uint64_t at = 32 + 4ull * kh.num_first_ocur, tb = 8ull * kh.num_prior_chkpts;
+          if (with_tab && tb <= n[k] - 32 && at + tb <= dev_lens[got[k]]) {
+            tabs[got[k]].assign(hst + off[k] + 32, hst + off[k] + 32 + tb);
+            have_tab[got[k]] = 1;
+          }
         }
       }
       hd = known[i];
@@ -1208,19 +1241,30 @@ dd_status restart_run(RestartSpace& sp, hipStream_t s, int method, const dd_buf*
   {
     uint8_t* stage = nullptr;
     uint8_t* hst = nullptr;
-    if (dev_chain && tab_total) {  // no prior table in the walk (a baseline alone): no round trip
-      DD_TRY(sp.arena(align256(tab_total) + 4096, &stage));
-      DD_TRY(sp.host(tab_total, &hst));
-      std::vector<const uint8_t*> src(M);
-      for (size_t k = 0; k < M; k++)
-        src[k] = static_cast<const uint8_t*>(dev_chain[ids[k]]) + 32 + 4ull * hdrs[k].num_first_ocur;
-      DD_TRY(fetch_to_host(src, tab_bytes, tab_off, stage, hst, tab_total, s));
+    // Device diffs: the tables the header round trips did not bring, in one
+    // more (none when the walk has no prior table: a baseline alone).
+    std::vector<const uint8_t*> src;
+    std::vector<uint64_t> n, off;
+    uint64_t miss = 0;
+    for (size_t k = 0; dev_chain && k < M; k++) {
+      if (!tab_bytes[k] || have_tab[ids[k]]) continue;
+      src.push_back(static_cast<const uint8_t*>(dev_chain[ids[k]]) + 32 + 4ull * hdrs[k].num_first_ocur);
+      n.push_back(tab_bytes[k]);
+      off.push_back(miss);
+      tab_off[k] = miss;
+      miss += tab_bytes[k];
+    }
+    if (miss) {
+      DD_TRY(sp.arena(align256(miss) + 4096, &stage));
+      DD_TRY(sp.host(miss, &hst));
+      DD_TRY(fetch_to_host(src, n, off, stage, hst, miss, s));
     }
     for (size_t k = 0; k < M; k++) {
       const dd_header& hd = hdrs[k];
-      const uint8_t* tb = !tab_bytes[k] ? nullptr
-                          : dev_chain ? hst + tab_off[k]
-                                      : host_chain[ids[k]].data + 32 + 4ull * hd.num_first_ocur;
+      const uint8_t* tb = !tab_bytes[k]             ? nullptr
+                          : !dev_chain              ? host_chain[ids[k]].data + 32 + 4ull * hd.num_first_ocur
+                          : have_tab[ids[k]]        ? tabs[ids[k]].data()
+                                                    : hst + tab_off[k];
       std::vector<uint64_t> cnt((size_t)hd.chkpt_id + 1, 0);
       for (uint32_t i = 0; i < hd.num_prior_chkpts; i++) {
         const uint32_t ck = rd32(tb + 8ull * i), n = rd32(tb + 8ull * i + 4);

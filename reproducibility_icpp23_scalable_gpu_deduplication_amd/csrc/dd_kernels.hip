@@ -3296,20 +3296,40 @@ __global__ __launch_bounds__(256) void k_restart_pending(const RestartCkpt* __re
   if (bad) g_or((GlobalPtr<uint32_t>)&ctr->flags, FLAG_BAD_SOURCE);
 }
 
-// Copies up to kFetchRanges device byte ranges into dst (blockIdx.y: range).
-__global__ __launch_bounds__(256) void k_fetch(FetchArgs a, uint8_t* __restrict__ dst) {
-  const uint32_t r = blockIdx.y;
-  if (r >= a.n) return;
-  const GlobalPtr<const uint8_t> src = (GlobalPtr<const uint8_t>)a.src[r];
-  const GlobalPtr<uint8_t> d = (GlobalPtr<uint8_t>)dst + a.dst_off[r];
-  const uint64_t n = a.len[r];
+// Copies n bytes from src to d with the launch's threads of one range.
+__device__ __forceinline__ void fetch_copy(GlobalPtr<const uint8_t> src, GlobalPtr<uint8_t> d, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * 256;
-  if (((reinterpret_cast<uintptr_t>(a.src[r]) | a.dst_off[r] | n) & 3) == 0) {
+  if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(d) | n) & 3) == 0) {
     for (uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x; w < n / 4; w += stride)
       ((GlobalPtr<uint32_t>)d)[w] = ((GlobalPtr<const uint32_t>)src)[w];
   } else {
     for (uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x; b < n; b += stride) d[b] = src[b];
   }
+}
+
+// Copies up to kFetchRanges device byte ranges into dst (blockIdx.y: range);
+// in table mode, each diff's header and prior table.
+__global__ __launch_bounds__(256) void k_fetch(FetchArgs a, uint8_t* __restrict__ dst) {
+  const uint32_t r = blockIdx.y;
+  if (r >= a.n) return;
+  const GlobalPtr<const uint8_t> src = (GlobalPtr<const uint8_t>)a.src[r];
+  const GlobalPtr<uint8_t> d = (GlobalPtr<uint8_t>)dst + a.dst_off[r];
+  if (!a.tables) {
+    fetch_copy(src, d, a.len[r]);
+    return;
+  }
+  // Header words 5 and 6: num_first_ocur, num_prior_chkpts (byte loads: a
+  // diff need not be 4-byte aligned).
+  auto word = [&](uint32_t i) {
+    return (uint32_t)src[4 * i] | ((uint32_t)src[4 * i + 1] << 8) | ((uint32_t)src[4 * i + 2] << 16) |
+           ((uint32_t)src[4 * i + 3] << 24);
+  };
+  const uint64_t at = 32 + 4ull * word(5);
+  uint64_t tb = 8ull * word(6);
+  tb = min(tb, a.len[r] - 32);
+  tb = at < a.lim[r] ? min(tb, a.lim[r] - at) : 0;
+  fetch_copy(src, d, 32);
+  fetch_copy(src + at, d + 32, tb);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {

@@ -245,14 +245,18 @@ constexpr uint64_t kRefDone = ~0ull;
 constexpr uint32_t kRefSameCkpt = 0x80000000u;
 
 // Up to kFetchRanges device byte ranges copied into one buffer by one launch
-// (headers and metadata of the diffs of a chain).
+// (headers and metadata of the diffs of a chain).  With `tables` set, range r
+// is a diff of lim[r] bytes: its 32-byte header goes to dst_off[r] and its
+// prior table (8 x num_prior_chkpts bytes after the first-occurrence list),
+// cut to len[r] - 32 bytes and to the diff's end, right after it.
 constexpr int kFetchRanges = 32;
 struct FetchArgs {
   uint32_t n;
-  uint32_t pad;
+  uint32_t tables;
   const uint8_t* src[kFetchRanges];
   uint64_t len[kFetchRanges];
   uint64_t dst_off[kFetchRanges];
+  uint64_t lim[kFetchRanges];
 };
 
 // Launch helpers (dd_kernels.hip) -------------------------------------------
